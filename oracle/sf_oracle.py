@@ -1,0 +1,281 @@
+"""ctypes binding of the CPU oracle (libsf_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg, as the checker.  The product package (spearfish_amd) never imports it.
+
+The functions mirror crates/map/src/align.rs (the reference, Rust) one to one; see sf_oracle.h for
+the line citations.  Sequences are bytes / bytearray / numpy uint8 arrays of base codes (0..3).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsf_oracle.so")
+
+INFIX, PREFIX, GLOBAL = 0, 1, 2
+NEG = -(2**31) // 4
+NONE = 0xFFFFFFFF
+UMAX = 2**64 - 1
+
+
+class Scoring(C.Structure):
+    _fields_ = [("mat", C.c_int32), ("mis", C.c_int32), ("open", C.c_int32), ("ext", C.c_int32)]
+
+
+ONT = (2, 4, 4, 2)  # align.rs:677-683
+HIFI = (1, 4, 6, 2)  # align.rs:684-690
+
+
+class Dp(C.Structure):
+    _fields_ = [
+        ("score", C.c_int32),
+        ("score_col", C.c_uint64),
+        ("clip_best", C.c_int32),
+        ("clip_i", C.c_uint64),
+        ("clip_j", C.c_uint64),
+        ("text_read", C.c_uint64),
+        ("last_row", C.c_uint64),
+        ("cut_best", C.c_int32),
+        ("cut_score", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        u8p, u64, i32, u32 = C.c_char_p, C.c_uint64, C.c_int32, C.c_uint32
+        sc = C.POINTER(Scoring)
+        L.sfo_distance.argtypes = [u8p, u64, u8p, u64, C.c_int, u32]
+        L.sfo_distance.restype = u32
+        L.sfo_batch_distances.argtypes = [u8p, u64, C.POINTER(C.c_char_p), C.POINTER(u64), u64, C.c_int, u32,
+                                          C.POINTER(u32)]
+        L.sfo_locate.argtypes = [u8p, u64, u8p, u64, C.c_int, u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+        L.sfo_affine_dp.argtypes = [u8p, u64, u8p, u64, C.c_int, sc, u64, u64, C.c_int, i32, C.c_int, u64,
+                                    C.POINTER(Dp)]
+        L.sfo_affine.argtypes = [u8p, u64, u8p, u64, C.c_int, sc, u64]
+        L.sfo_affine.restype = i32
+        L.sfo_affine_extend.argtypes = [u8p, u64, u8p, u64, sc, u64, i32, i32]
+        L.sfo_affine_extend.restype = i32
+        L.sfo_affine_extend_window.argtypes = [u8p, u64, u8p, u64, u64, sc, u64, i32, i32, C.POINTER(C.c_int64)]
+        L.sfo_affine_extend_cut_bonus.argtypes = [u8p, u64, u8p, u64, sc, u64, i32, i32, C.c_int, u64, u64, C.c_int,
+                                                  C.c_int, C.POINTER(C.c_int64)]
+        L.sfo_local_band_ends.argtypes = [u8p, u64, u8p, u64, sc, C.c_int64, C.c_int64, i32, i32]
+        L.sfo_local_band_ends.restype = i32
+        L.sfo_align.argtypes = [u8p, u64, u8p, u64, C.c_int, sc, u64, C.POINTER(u32), u64, C.POINTER(i32),
+                                C.POINTER(u64), C.POINTER(u64)]
+        L.sfo_align.restype = C.c_int64
+        L.sfo_naive_edit.argtypes = [u8p, u64, u8p, u64, C.c_int]
+        L.sfo_naive_edit.restype = u32
+        L.sfo_naive_edit_lastrow.argtypes = [u8p, u64, u8p, u64, C.c_int, C.POINTER(u32)]
+        L.sfo_naive_affine.argtypes = [u8p, u64, u8p, u64, C.c_int, sc]
+        L.sfo_naive_affine.restype = i32
+        L.sfo_naive_extend.argtypes = [u8p, u64, u8p, u64, sc]
+        L.sfo_naive_extend.restype = i32
+        L.sfo_naive_local.argtypes = [u8p, u64, u8p, u64, sc, i32, i32]
+        L.sfo_naive_local.restype = i32
+        L.sfo_naive_local_band.argtypes = [u8p, u64, u8p, u64, sc, C.c_int64, C.c_int64, i32, i32]
+        L.sfo_naive_local_band.restype = i32
+        L.sfo_affine_cells.argtypes = [u64, u64, C.c_int, u64]
+        L.sfo_affine_cells.restype = u64
+        vp = C.c_void_p
+        L.sfo_batch_myers.argtypes = [vp, vp, vp, vp, vp, vp, u64, C.c_int, u32, vp, C.c_int]
+        L.sfo_batch_affine.argtypes = [vp, vp, vp, vp, vp, vp, u64, C.c_int, sc, vp, vp, vp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _b(s) -> bytes:
+    if isinstance(s, bytes):
+        return s
+    if isinstance(s, np.ndarray):
+        return s.astype(np.uint8, copy=False).tobytes()
+    return bytes(s)
+
+
+def _sc(sc) -> Scoring:
+    if isinstance(sc, Scoring):
+        return sc
+    return Scoring(*sc)
+
+
+def _band(band) -> int:
+    return UMAX if band is None else int(band)
+
+
+# ---- Myers ----
+
+def distance(p, t, mode, k=NONE):
+    """align.rs:87-90 `distance`; None for Option::None."""
+    p, t = _b(p), _b(t)
+    d = lib().sfo_distance(p, len(p), t, len(t), mode, k)
+    return None if d == NONE else d
+
+
+def infix_distance(p, t):
+    """align.rs:655-658."""
+    return distance(p, t, INFIX, NONE)
+
+
+def batch_distances(p, texts, mode, k=NONE):
+    """align.rs:170-200 with the 8-lane union band of batch_lanes (align.rs:552-653)."""
+    p = _b(p)
+    ts = [_b(t) for t in texts]
+    n = len(ts)
+    arr = (C.c_char_p * max(n, 1))(*ts)
+    lens = (C.c_uint64 * max(n, 1))(*[len(t) for t in ts])
+    out = (C.c_uint32 * max(n, 1))()
+    lib().sfo_batch_distances(p, len(p), arr, lens, n, mode, k, out)
+    return [None if out[i] == NONE else out[i] for i in range(n)]
+
+
+def locate(p, t, mode, k=NONE):
+    """Own definition: (distance, end, start); Nones when over k."""
+    p, t = _b(p), _b(t)
+    d, e, s = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    lib().sfo_locate(p, len(p), t, len(t), mode, k, C.byref(d), C.byref(e), C.byref(s))
+    f = lambda x: None if x.value == NONE else x.value
+    return f(d), f(e), f(s)
+
+
+# ---- Affine ----
+
+def affine_dp(p, t, mode, sc=ONT, band=None, geom=None, clip=None, cut=None) -> Dp:
+    """align.rs:2183-2208 `affine_dp` / run_core; clip = (xdrop, cell) or None."""
+    p, t = _b(p), _b(t)
+    out = Dp()
+    has, xd, cell = (0, 0, 0) if clip is None else (1, clip[0], int(clip[1]))
+    lib().sfo_affine_dp(p, len(p), t, len(t), mode, C.byref(_sc(sc)), _band(band),
+                        len(t) if geom is None else geom, has, xd, cell, UMAX if cut is None else cut, C.byref(out))
+    return out
+
+
+def affine(p, t, mode, sc=ONT, band=None) -> int:
+    p, t = _b(p), _b(t)
+    return lib().sfo_affine(p, len(p), t, len(t), mode, C.byref(_sc(sc)), _band(band))
+
+
+def affine_extend(p, t, sc=ONT, band=None, end_bonus=0, xdrop=2**31 - 1) -> int:
+    p, t = _b(p), _b(t)
+    return lib().sfo_affine_extend(p, len(p), t, len(t), C.byref(_sc(sc)), _band(band), end_bonus, xdrop)
+
+
+def affine_extend_window(p, t, window, sc=ONT, band=None, end_bonus=0, xdrop=2**31 - 1):
+    p, t = _b(p), _b(t)
+    out = (C.c_int64 * 4)()
+    lib().sfo_affine_extend_window(p, len(p), t, len(t), window, C.byref(_sc(sc)), _band(band), end_bonus, xdrop, out)
+    return tuple(out)
+
+
+def affine_extend_cut_bonus(p, t, sc, band, end_bonus, xdrop, diagonal, window, cut, track_cut=True,
+                            bonus_within_cut=False):
+    p, t = _b(p), _b(t)
+    out = (C.c_int64 * 3)()
+    lib().sfo_affine_extend_cut_bonus(p, len(p), t, len(t), C.byref(_sc(sc)), _band(band), end_bonus, xdrop,
+                                      int(diagonal), window, UMAX if cut is None else cut, int(track_cut),
+                                      int(bonus_within_cut), out)
+    return tuple(out)
+
+
+def local_band_ends(read, t, sc, dlo, dhi, bonus):
+    r, t = _b(read), _b(t)
+    return lib().sfo_local_band_ends(r, len(r), t, len(t), C.byref(_sc(sc)), dlo, dhi, bonus[0], bonus[1])
+
+
+def local_band(read, t, sc, dlo, dhi, end_bonus):
+    """align.rs:735-737."""
+    return local_band_ends(read, t, sc, dlo, dhi, (end_bonus, end_bonus))
+
+
+def align(p, t, mode, sc=ONT, band=None):
+    """Own traceback definition: (score, score_col, tstart, cigar list) or None if the pass stopped."""
+    p, t = _b(p), _b(t)
+    cap = len(p) + len(t) + 2
+    buf = (C.c_uint32 * cap)()
+    s, col, ts = C.c_int32(), C.c_uint64(), C.c_uint64()
+    r = lib().sfo_align(p, len(p), t, len(t), mode, C.byref(_sc(sc)), _band(band), buf, cap, C.byref(s),
+                        C.byref(col), C.byref(ts))
+    if r == -2:
+        return None
+    assert r >= 0
+    return s.value, col.value, ts.value, [buf[i] for i in range(r)]
+
+
+def naive_edit(p, t, mode):
+    p, t = _b(p), _b(t)
+    return lib().sfo_naive_edit(p, len(p), t, len(t), mode)
+
+
+def naive_edit_lastrow(p, t, mode):
+    p, t = _b(p), _b(t)
+    out = (C.c_uint32 * (len(t) + 1))()
+    lib().sfo_naive_edit_lastrow(p, len(p), t, len(t), mode, out)
+    return list(out)
+
+
+def naive_affine(p, t, mode, sc=ONT):
+    p, t = _b(p), _b(t)
+    return lib().sfo_naive_affine(p, len(p), t, len(t), mode, C.byref(_sc(sc)))
+
+
+def naive_extend(p, t, sc=ONT):
+    p, t = _b(p), _b(t)
+    return lib().sfo_naive_extend(p, len(p), t, len(t), C.byref(_sc(sc)))
+
+
+def naive_local(p, t, sc, bonus):
+    p, t = _b(p), _b(t)
+    return lib().sfo_naive_local(p, len(p), t, len(t), C.byref(_sc(sc)), bonus[0], bonus[1])
+
+
+def naive_local_band(p, t, sc, dlo, dhi, bonus):
+    p, t = _b(p), _b(t)
+    return lib().sfo_naive_local_band(p, len(p), t, len(t), C.byref(_sc(sc)), dlo, dhi, bonus[0], bonus[1])
+
+
+def affine_cells(m, n, mode, band=None) -> int:
+    return lib().sfo_affine_cells(m, n, mode, _band(band))
+
+
+# ---- batch drivers (CPU baseline) ----
+
+def batch_myers(pat, poff, plen, txt, toff, tlen, mode, k=NONE, threads=1):
+    n = len(plen)
+    out = np.empty(n, np.uint32)
+    a = [np.ascontiguousarray(x) for x in (pat, poff, plen, txt, toff, tlen)]
+    lib().sfo_batch_myers(*[x.ctypes.data for x in a], n, mode, k, out.ctypes.data, threads)
+    return out
+
+
+def batch_affine(pat, poff, plen, txt, toff, tlen, mode, sc, band, threads=1):
+    n = len(plen)
+    score = np.empty(n, np.int32)
+    col = np.empty(n, np.uint64)
+    band = np.ascontiguousarray(np.broadcast_to(np.asarray(band, np.uint64), (n,)))
+    a = [np.ascontiguousarray(x) for x in (pat, poff, plen, txt, toff, tlen)]
+    lib().sfo_batch_affine(*[x.ctypes.data for x in a], n, mode, C.byref(_sc(sc)), band.ctypes.data,
+                           score.ctypes.data, col.ctypes.data, threads)
+    return score, col
+
+
+# ---- CIGAR helpers ----
+
+OPS = {0: "M", 1: "I", 2: "D", 7: "=", 8: "X"}
+
+
+def cigar_str(cig) -> str:
+    return "".join(f"{c >> 4}{OPS[c & 15]}" for c in cig)
