@@ -1,0 +1,147 @@
+/*
+ * sf_align.h -- C ABI of the MI355X (gfx950) alignment library, libsf_align.so.
+ *
+ * A drop-in for the base-level alignment step of spearfish (crates/map/src/align.rs, v0.3.2):
+ * Myers bit-parallel edit distance and banded affine-gap (Gotoh) alignment, computed by HIP
+ * kernels with results bit-identical to the reference's CPU kernels.  Every entry point is a
+ * batch form of one or more of the reference's functions; the comment on each cites the
+ * function(s) it replaces.  The reference binds it from Rust through a thin `-sys` crate
+ * (see INTEGRATION.md in the source tree).
+ *
+ * Conventions
+ *   - Plain C types only; the caller owns every buffer it passes, the library keeps no pointer
+ *     past the call's return (sf_batch objects own their device copies).
+ *   - Sequences are base codes A=0 C=1 G=2 T=3 (crates/lcp-core/src/encode.rs), one per byte and
+ *     masked with `& 3` as the reference does (align.rs:78, 120, 1054, 2557), or 2-bit packed:
+ *     16 bases per little-endian u32 word, base k at bits 2*(k % 16) of word k / 16 -- the same
+ *     bit order as the index's u64 packing (crates/index/src/lib.rs:45-151).
+ *   - Every call returns an int status: SF_OK or an SF_E* code; sf_last_error() gives the
+ *     calling thread's last message.  No call aborts the process.
+ *   - Thread-safe: calls from several host threads may overlap; each has its own stream.
+ */
+#ifndef SF_ALIGN_H
+#define SF_ALIGN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SF_ABI_VERSION 1
+
+/* Status codes. */
+#define SF_OK 0
+#define SF_EINVAL 1  /* invalid argument */
+#define SF_ENOMEM 2  /* host or device memory exhausted */
+#define SF_EDEVICE 3 /* HIP runtime or kernel failure */
+#define SF_ENODEV 4  /* no gfx950 device, or the library was built without device code */
+#define SF_ERANGE 5  /* an input exceeds this build's limits, or an output buffer is too small */
+
+/* align.rs:40-45 `Mode`. */
+#define SF_INFIX 0
+#define SF_PREFIX 1
+#define SF_GLOBAL 2
+
+#define SF_NEG (INT32_MIN / 4) /* align.rs:693 `NEG`: unreachable / stopped pass */
+#define SF_NONE UINT32_MAX     /* Option<u32>::None (Myers distance above k) */
+#define SF_BAND_EXACT UINT64_MAX /* `band = usize::MAX`: exact (unbanded) pass */
+
+/* align.rs:660-691 `Affine`: +mat per match, -mis per mismatch, -(open + ext * l) per gap of l. */
+typedef struct {
+    int32_t mat, mis, open, ext;
+} sf_scoring;
+
+/* A batch of (pattern, text) pairs.  Pattern i is bases [pat_off[i], pat_off[i] + pat_len[i])
+ * of `pat`; text i likewise of `txt`.  `packed` = 0: one code per byte; 1: 2-bit words (offsets
+ * are still in bases).  pat_bases / txt_bases bound the buffers (in bases) for checking. */
+typedef struct {
+    const void *pat;
+    const uint64_t *pat_off;
+    const uint32_t *pat_len;
+    const void *txt;
+    const uint64_t *txt_off;
+    const uint32_t *txt_len;
+    uint64_t n;
+    int32_t packed;
+    uint64_t pat_bases;
+    uint64_t txt_bases;
+} sf_pairs;
+
+/* Per-kernel device time of the last run on a batch (HIP events on the batch's stream). */
+typedef struct {
+    double myers_ms;     /* Myers distance (+ end) kernels */
+    double locate_ms;    /* reversed Myers pass for start coordinates */
+    double affine_ms;    /* banded affine DP kernels (with traceback bits when asked) */
+    double walk_ms;      /* traceback walk + CIGAR compaction */
+    uint64_t myers_cells;  /* sum of m * n over the pairs run (the GCUPS numerator) */
+    uint64_t affine_cells; /* DP cells as the reference's DP_WORK counts them (align.rs:2573-2578) */
+    uint64_t affine_launches;
+} sf_timing;
+
+/* ---- library ---- */
+const char *sf_strerror(int status);
+const char *sf_last_error(void);
+int sf_abi_version(void);
+/* Selects the device for this process (default: device 0 on first use).  SF_ENODEV if it is
+ * not a gfx950. */
+int sf_init(int device);
+
+/* ---- one-shot batch calls (host buffers in, host buffers out) ---- */
+
+/* Replaces align.rs:87-168 `distance` / `distance_with`, align.rs:170-200 `batch_distances`
+ * (equal to one distance_with per text, align.rs:170-172, 3373-3432) and align.rs:655-658
+ * `infix_distance`: dist[i] = edit distance of pair i in `mode`, or SF_NONE if above k.
+ * `end` (nullable): smallest text end column j with D[m][j] = dist (n for Global).
+ * `start` (nullable, needs end): start column of the shortest text substring at that end with the
+ * same distance (0 for Prefix and Global).  end/start have no reference counterpart. */
+int sf_myers(const sf_pairs *pairs, int mode, uint32_t k, uint32_t *dist, uint32_t *end, uint32_t *start);
+
+/* Replaces align.rs:699-701 `affine` and align.rs:854-861 `affine_sorted` (equal to separate
+ * calls, align.rs:2210-2213): score[i] of pair i in `mode` with band[i] (SF_BAND_EXACT for
+ * exact; band == NULL: all exact), SF_NEG where the pass stops (align.rs:2543-2556).
+ * score_col (nullable): the text column where the score ends (align.rs:2643-2650). */
+int sf_affine(const sf_pairs *pairs, int mode, const sf_scoring *sc, const uint64_t *band, int32_t *score,
+              uint32_t *score_col);
+
+/* sf_affine plus a traceback (no reference counterpart: the reference returns scores only,
+ * align.rs:2111-2129).  CIGARs are BAM-encoded u32 (len << 4 | op; '=' 7, 'X' 8, 'I' 1 pattern
+ * base only, 'D' 2 text base only), pair i's at cigar[cigar_off[i] .. cigar_off[i + 1]);
+ * cigar_off has n + 1 entries.  tstart[i] (nullable): the text column where the alignment
+ * starts (0 unless Infix).  SF_ERANGE if cigar_cap is too small (sum of m + n + 1 always fits).
+ * A pair whose pass stopped gets an empty CIGAR. */
+int sf_affine_cigar(const sf_pairs *pairs, int mode, const sf_scoring *sc, const uint64_t *band, int32_t *score,
+                    uint32_t *score_col, uint32_t *tstart, uint32_t *cigar, uint64_t cigar_cap, uint64_t *cigar_off);
+
+/* ---- device-resident batches ---- */
+/* A batch packs its pairs 2-bit into HBM once, bins them by shape, and can then be run many
+ * times; runs are asynchronous on the batch's own HIP stream until sf_batch_sync. */
+typedef struct sf_batch sf_batch;
+
+int sf_batch_create(const sf_pairs *pairs, sf_batch **out);
+void sf_batch_destroy(sf_batch *b);
+uint64_t sf_batch_size(const sf_batch *b);
+/* Enqueues Myers over every pair (results stay on the device).  locate: 0 distance only,
+ * 1 + end, 2 + end + start. */
+int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate);
+/* Enqueues the banded affine pass over every pair; band as in sf_affine; cigar != 0 adds the
+ * traceback. */
+int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t *band, int cigar);
+int sf_batch_sync(sf_batch *b);
+/* Copies results of the last runs out (implies sync).  Nullable outputs are skipped. */
+int sf_batch_myers_results(sf_batch *b, uint32_t *dist, uint32_t *end, uint32_t *start);
+int sf_batch_affine_results(sf_batch *b, int32_t *score, uint32_t *score_col, uint32_t *tstart, uint32_t *cigar,
+                            uint64_t cigar_cap, uint64_t *cigar_off);
+/* Device time of the runs since the last sf_batch_reset_timing (implies sync). */
+int sf_batch_timing(sf_batch *b, sf_timing *t);
+void sf_batch_reset_timing(sf_batch *b);
+
+/* Diagnostic (no reference counterpart): runs a one-wave kernel that records the lane moves of
+ * the DPP and shuffle encodings the kernels rely on, 5 x 64 values into `out` (320 int32). */
+int sf_selftest_lanes(int32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

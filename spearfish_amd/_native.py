@@ -1,0 +1,104 @@
+"""Loads libsf_align.so (the HIP library) and declares its C ABI (include/sf_align.h).
+
+There is no CPU fallback: if the library is missing or was built for another device, every
+call fails with NativeError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ALIGN_LIB = os.path.join(HERE, "libsf_align.so")
+WORKLOAD_LIB = os.path.join(HERE, "libsf_workload.so")
+
+SF_OK, SF_EINVAL, SF_ENOMEM, SF_EDEVICE, SF_ENODEV, SF_ERANGE = range(6)
+
+
+class NativeError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"sf status {status}: {msg}")
+        self.status = status
+
+
+class Scoring(C.Structure):
+    """sf_scoring: align.rs:660-691 `Affine` (mat, mis, open, ext)."""
+
+    _fields_ = [("mat", C.c_int32), ("mis", C.c_int32), ("open", C.c_int32), ("ext", C.c_int32)]
+
+
+class Pairs(C.Structure):
+    _fields_ = [
+        ("pat", C.c_void_p),
+        ("pat_off", C.c_void_p),
+        ("pat_len", C.c_void_p),
+        ("txt", C.c_void_p),
+        ("txt_off", C.c_void_p),
+        ("txt_len", C.c_void_p),
+        ("n", C.c_uint64),
+        ("packed", C.c_int32),
+        ("pat_bases", C.c_uint64),
+        ("txt_bases", C.c_uint64),
+    ]
+
+
+class Timing(C.Structure):
+    _fields_ = [
+        ("myers_ms", C.c_double),
+        ("locate_ms", C.c_double),
+        ("affine_ms", C.c_double),
+        ("walk_ms", C.c_double),
+        ("myers_cells", C.c_uint64),
+        ("affine_cells", C.c_uint64),
+        ("affine_launches", C.c_uint64),
+    ]
+
+
+# Every symbol include/sf_align.h declares, with its ctypes signature.
+_vp, _u32, _u64, _i32, _int = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_int
+SIGNATURES = {
+    "sf_strerror": ([_int], C.c_char_p),
+    "sf_last_error": ([], C.c_char_p),
+    "sf_abi_version": ([], _int),
+    "sf_init": ([_int], _int),
+    "sf_myers": ([C.POINTER(Pairs), _int, _u32, _vp, _vp, _vp], _int),
+    "sf_affine": ([C.POINTER(Pairs), _int, C.POINTER(Scoring), _vp, _vp, _vp], _int),
+    "sf_affine_cigar": ([C.POINTER(Pairs), _int, C.POINTER(Scoring), _vp, _vp, _vp, _vp, _vp, _u64, _vp], _int),
+    "sf_batch_create": ([C.POINTER(Pairs), C.POINTER(_vp)], _int),
+    "sf_batch_destroy": ([_vp], None),
+    "sf_batch_size": ([_vp], _u64),
+    "sf_batch_myers": ([_vp, _int, _u32, _int], _int),
+    "sf_batch_affine": ([_vp, _int, C.POINTER(Scoring), _vp, _int], _int),
+    "sf_batch_sync": ([_vp], _int),
+    "sf_batch_myers_results": ([_vp, _vp, _vp, _vp], _int),
+    "sf_batch_affine_results": ([_vp, _vp, _vp, _vp, _vp, _u64, _vp], _int),
+    "sf_batch_timing": ([_vp, C.POINTER(Timing)], _int),
+    "sf_batch_reset_timing": ([_vp], None),
+    "sf_selftest_lanes": ([_vp], _int),
+}
+
+_lib = None
+
+
+def load(path: str = ALIGN_LIB) -> C.CDLL:
+    """The HIP library; raises NativeError(SF_ENODEV) when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NativeError(SF_ENODEV, f"{path} is missing: run `python -m spearfish_amd.build` (no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (args, res) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    if lib.sf_abi_version() != 1:
+        raise NativeError(SF_ENODEV, "ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != SF_OK:
+        lib = load()
+        raise NativeError(status, (lib.sf_last_error() or b"").decode() or lib.sf_strerror(status).decode())

@@ -1,0 +1,139 @@
+// internal.hpp -- shared between the kernel translation units and the runtime of libsf_align.so.
+//
+// Device layout of a batch (HBM):
+//   seq   : all sequences 2-bit packed, 16 bases per u32 (base k at bits 2*(k%16) of word k/16),
+//           each sequence starting on a word boundary.
+//   poff/toff (u64) : word offset of pair i's pattern / text in `seq`;  plen/tlen (u32): lengths.
+//   order (u32)     : pair indices grouped by kernel shape class ("bins"); a launch covers one bin.
+// Loads past a sequence's last word are clamped to it, so no padding words are needed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sf {
+
+constexpr int32_t kNeg = INT32_MIN / 4;       // align.rs:693
+constexpr int32_t kScanIdent = INT32_MIN / 2; // below every value a cell or gap can take
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr int kInfix = 0, kPrefix = 1, kGlobal = 2;
+
+struct PairsView {
+    const uint32_t *seq;
+    const uint64_t *poff;
+    const uint64_t *toff;
+    const uint32_t *plen;
+    const uint32_t *tlen;
+};
+
+struct MyersArgs {
+    PairsView pv;
+    const uint32_t *order;
+    uint32_t count;
+    int mode;
+    uint32_t k;
+    int reverse; // 0: forward pass (dist, end); 1: reversed Prefix pass for `start`
+    uint32_t *dist;
+    uint32_t *end;
+    uint32_t *start;
+};
+
+struct AffineArgs {
+    PairsView pv;
+    const uint32_t *order;
+    const uint64_t *below_above; // per pair: below | (above << 32); full mode is (m, n)
+    uint32_t count;
+    int mode;
+    int32_t mat, mis, open, ext;
+    int32_t *score;
+    uint32_t *score_col;
+    uint32_t *tb;           // traceback bits (chunk base); null for score only
+    const uint64_t *tb_off; // per order slot of the launch: word offset into tb
+};
+
+struct WalkArgs {
+    PairsView pv;
+    const uint32_t *order;
+    const uint64_t *below_above;
+    uint32_t count;
+    int mode;
+    int K;
+    const int32_t *score;
+    const uint32_t *score_col;
+    const uint32_t *tb;
+    const uint64_t *tb_off;  // per order slot
+    uint32_t *runs;          // reversed runs per pair, at runs_off[pair]
+    const uint64_t *runs_off; // per pair
+    uint32_t *nruns;          // per pair
+    uint32_t *tstart;         // per pair
+};
+
+// ---- device helpers ----
+
+__device__ __forceinline__ uint32_t ldw(const uint32_t *S, int64_t w, int64_t nw) {
+    if (nw <= 0) return 0;
+    w = w < 0 ? 0 : (w >= nw ? nw - 1 : w);
+    return S[w];
+}
+
+// 16 codes starting at base position s (any integer): code (s + k) at bits 2k.  Positions
+// outside the sequence give unspecified codes.
+__device__ __forceinline__ uint32_t codes16(const uint32_t *S, int64_t s, int64_t nw) {
+    int64_t w = s >> 4; // floor for negative s
+    uint32_t sh = (uint32_t)(s & 15) * 2;
+    uint32_t a = ldw(S, w, nw), b = ldw(S, w + 1, nw);
+    return sh ? (a >> sh) | (b << (32 - sh)) : a;
+}
+
+// Reverses the order of the 16 two-bit codes of a word.
+__device__ __forceinline__ uint32_t reverse2(uint32_t x) {
+    x = __builtin_bitreverse32(x);
+    return ((x >> 1) & 0x55555555u) | ((x << 1) & 0xAAAAAAAAu);
+}
+
+// The 16 even bits of x, compacted to the low half.
+__device__ __forceinline__ uint32_t even16(uint32_t x) {
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    x = (x | (x >> 8)) & 0x0000FFFFu;
+    return x;
+}
+
+__device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+__device__ __forceinline__ int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+
+// DPP lane moves (gfx9 encodings).  Lanes without a source keep `fill`.
+template <int CTRL, int ROWMASK = 0xf>
+__device__ __forceinline__ int32_t dpp(int32_t fill, int32_t v) {
+    return __builtin_amdgcn_update_dpp(fill, v, CTRL, ROWMASK, 0xf, false);
+}
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kWaveShl1 = 0x130, kWaveShr1 = 0x138;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+
+// Exclusive max-scan across the 64 lanes of a wave (lane 0 gets `ident`).
+__device__ __forceinline__ int32_t wave_excl_max(int32_t v, int32_t ident) {
+    int32_t x = v;
+    x = imax(x, dpp<kRowShr1>(ident, x));
+    x = imax(x, dpp<kRowShr2>(ident, x));
+    x = imax(x, dpp<kRowShr4>(ident, x));
+    x = imax(x, dpp<kRowShr8>(ident, x));
+    x = imax(x, dpp<kRowBcast15, 0xa>(ident, x));
+    x = imax(x, dpp<kRowBcast31, 0xc>(ident, x));
+    return dpp<kWaveShr1>(ident, x);
+}
+
+} // namespace sf
+
+// Launchers (defined in the kernel TUs).  Each returns hipSuccess or the launch error.
+hipError_t sf_launch_myers(const sf::MyersArgs &a, int maxb, hipStream_t s);
+hipError_t sf_launch_affine(const sf::AffineArgs &a, int K, bool tb, hipStream_t s);
+hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
+hipError_t sf_launch_compact(const uint32_t *order, uint32_t count, const uint32_t *runs, const uint64_t *runs_off,
+                             const uint32_t *nruns, const uint64_t *out_off, uint32_t *out, hipStream_t s);
+hipError_t sf_launch_probe(int32_t *out, hipStream_t s);
+
+// Supported shape classes.
+int sf_myers_maxb_for(uint32_t nblocks);     // 0 if unsupported
+int sf_affine_k_for(uint64_t width);         // 0 if unsupported

@@ -1,0 +1,717 @@
+// runtime.hip -- the C ABI of libsf_align.so (include/sf_align.h): device selection, batches
+// resident in HBM, shape binning, traceback scratch, per-kernel event timing, errors.
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/sf_align.h"
+#include "internal.hpp"
+
+using namespace sf;
+
+// ------------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------------
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return fail(SF_EDEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+std::mutex g_dev_mu;
+int g_device = -1;
+
+int ensure_device() {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (g_device >= 0) {
+        HIP_TRY(hipSetDevice(g_device));
+        return SF_OK;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(SF_ENODEV, "no HIP device visible");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, 0));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SF_ENODEV, "device 0 is %s, this library is built for gfx950", prop.gcnArchName);
+    HIP_TRY(hipSetDevice(0));
+    g_device = 0;
+    return SF_OK;
+}
+
+unsigned host_threads() {
+    unsigned t = std::thread::hardware_concurrency();
+    if (const char *e = std::getenv("SF_HOST_THREADS")) t = (unsigned)std::atoi(e);
+    return std::max(1u, std::min(t ? t : 1u, 16u));
+}
+
+template <class F>
+void parallel_for(uint64_t n, F f) {
+    unsigned nt = host_threads();
+    if (n < 4096 || nt == 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    uint64_t chunk = (n + nt - 1) / nt;
+    for (unsigned q = 0; q < nt; ++q) {
+        uint64_t a = q * chunk, b = std::min(n, a + chunk);
+        if (a >= b) break;
+        th.emplace_back([=] { f(a, b); });
+    }
+    for (auto &t : th) t.join();
+}
+
+template <class T>
+int dalloc(T **p, uint64_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SF_ENOMEM, "hipMalloc of %llu bytes failed: %s", (unsigned long long)(count * sizeof(T)),
+                    hipGetErrorString(e));
+    }
+    return SF_OK;
+}
+
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+uint64_t tb_budget_words() {
+    uint64_t mb = 16384;
+    if (const char *e = std::getenv("SF_TB_BUDGET_MB")) mb = std::strtoull(e, nullptr, 10);
+    return std::max<uint64_t>(mb, 16) * (1ull << 20) / 4;
+}
+
+__global__ void scan_u32_kernel(const uint32_t *in, uint64_t n, uint64_t *out) {
+    // Exclusive prefix sum of n u32 into n + 1 u64, one block of 1024 threads.
+    __shared__ uint64_t part[1024];
+    const uint64_t per = (n + 1023) / 1024;
+    const uint64_t a = threadIdx.x * per, b = a + per < n ? a + per : n;
+    uint64_t s = 0;
+    for (uint64_t i = a; i < b; ++i) s += in[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t acc = 0;
+        for (int q = 0; q < 1024; ++q) {
+            uint64_t v = part[q];
+            part[q] = acc;
+            acc += v;
+        }
+        out[n] = acc;
+    }
+    __syncthreads();
+    uint64_t acc = part[threadIdx.x];
+    for (uint64_t i = a; i < b; ++i) {
+        out[i] = acc;
+        acc += in[i];
+    }
+}
+
+} // namespace
+
+// ------------------------------------------------------------------------------------------
+// batch
+// ------------------------------------------------------------------------------------------
+struct sf_batch {
+    hipStream_t stream = nullptr;
+    uint64_t n = 0;
+    std::vector<uint32_t> plen, tlen;
+    uint32_t *d_seq = nullptr;
+    uint64_t *d_poff = nullptr, *d_toff = nullptr;
+    uint32_t *d_plen = nullptr, *d_tlen = nullptr;
+    // Myers
+    struct Bin {
+        int cls;
+        uint32_t begin, end;
+    };
+    std::vector<Bin> mbins;
+    uint32_t *d_morder = nullptr;
+    uint32_t *d_dist = nullptr, *d_end = nullptr, *d_start = nullptr;
+    std::string myers_unsupported;
+    // affine
+    std::vector<Bin> abins;
+    std::vector<uint32_t> aorder; // host copy of the affine order
+    uint64_t akey = 0;
+    bool akey_valid = false;
+    uint64_t *d_ba = nullptr;
+    uint32_t *d_aorder = nullptr;
+    int32_t *d_score = nullptr;
+    uint32_t *d_col = nullptr, *d_tstart = nullptr, *d_nruns = nullptr, *d_runs = nullptr, *d_cig = nullptr;
+    uint64_t *d_runs_off = nullptr, *d_out_off = nullptr, *d_tboff = nullptr;
+    uint32_t *d_tb = nullptr;
+    uint64_t tb_words = 0;
+    bool cigar_valid = false;
+    uint64_t affine_cells_cur = 0;
+    // timing
+    struct Ev {
+        int cat;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[4] = {0, 0, 0, 0};
+    uint64_t myers_cells = 0, affine_cells = 0, affine_launches = 0;
+    uint64_t myers_cells_per_run = 0;
+
+    hipEvent_t ev() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    void collect() {
+        for (auto &e : pending) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, e.a, e.b) == hipSuccess) ms[e.cat] += t;
+            pool.push_back(e.a);
+            pool.push_back(e.b);
+        }
+        pending.clear();
+    }
+    ~sf_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        collect();
+        for (auto e : pool) (void)hipEventDestroy(e);
+        dfree(d_seq), dfree(d_poff), dfree(d_toff), dfree(d_plen), dfree(d_tlen), dfree(d_morder);
+        dfree(d_dist), dfree(d_end), dfree(d_start), dfree(d_ba), dfree(d_aorder), dfree(d_score), dfree(d_col);
+        dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
+        dfree(d_tboff), dfree(d_tb);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+PairsView view(const sf_batch *b) { return PairsView{b->d_seq, b->d_poff, b->d_toff, b->d_plen, b->d_tlen}; }
+
+int check_pairs(const sf_pairs *p) {
+    if (!p) return fail(SF_EINVAL, "pairs is null");
+    if (p->n == 0) return SF_OK;
+    if (!p->pat || !p->txt || !p->pat_off || !p->pat_len || !p->txt_off || !p->txt_len)
+        return fail(SF_EINVAL, "null sequence or offset pointer");
+    if (p->packed != 0 && p->packed != 1) return fail(SF_EINVAL, "packed must be 0 or 1");
+    for (uint64_t i = 0; i < p->n; ++i) {
+        if (p->pat_off[i] + p->pat_len[i] > p->pat_bases || p->pat_off[i] + p->pat_len[i] < p->pat_off[i])
+            return fail(SF_EINVAL, "pattern %llu lies outside the pattern buffer", (unsigned long long)i);
+        if (p->txt_off[i] + p->txt_len[i] > p->txt_bases || p->txt_off[i] + p->txt_len[i] < p->txt_off[i])
+            return fail(SF_EINVAL, "text %llu lies outside the text buffer", (unsigned long long)i);
+    }
+    return SF_OK;
+}
+
+inline uint32_t code_at(const void *buf, int packed, uint64_t k) {
+    if (!packed) return static_cast<const uint8_t *>(buf)[k] & 3u;
+    return (static_cast<const uint32_t *>(buf)[k >> 4] >> (2 * (k & 15))) & 3u;
+}
+
+void pack_into(uint32_t *dst, const void *buf, int packed, uint64_t off, uint32_t len) {
+    const uint64_t nw = (len + 15) / 16;
+    for (uint64_t w = 0; w < nw; ++w) {
+        uint32_t v = 0;
+        const uint64_t k0 = w * 16;
+        const uint32_t cnt = (uint32_t)std::min<uint64_t>(16, len - k0);
+        for (uint32_t k = 0; k < cnt; ++k) v |= code_at(buf, packed, off + k0 + k) << (2 * k);
+        dst[w] = v;
+    }
+}
+
+int timed_begin(sf_batch *b, int cat, hipEvent_t *a) {
+    *a = b->ev();
+    HIP_TRY(hipEventRecord(*a, b->stream));
+    (void)cat;
+    return SF_OK;
+}
+
+int timed_end(sf_batch *b, int cat, hipEvent_t a) {
+    hipEvent_t e = b->ev();
+    HIP_TRY(hipEventRecord(e, b->stream));
+    b->pending.push_back({cat, a, e});
+    return SF_OK;
+}
+
+// Exact-equivalent band geometry of pair (m, n): (below, above) of align.rs:2468-2481, with any
+// band that keeps every row at [0, n] (Infix, usize::MAX, or band >= max(m, n)) as (m, n).
+inline void band_geom(uint32_t m, uint32_t n, int mode, uint64_t band, uint64_t *below, uint64_t *above) {
+    const uint64_t mx = std::max(m, n);
+    if (mode == SF_INFIX || band == SF_BAND_EXACT || band >= mx) {
+        *below = m;
+        *above = n;
+    } else {
+        *below = band + (m > n ? m - n : 0);
+        *above = band + (n > m ? n - m : 0);
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+const char *sf_strerror(int s) {
+    switch (s) {
+    case SF_OK: return "ok";
+    case SF_EINVAL: return "invalid argument";
+    case SF_ENOMEM: return "out of memory";
+    case SF_EDEVICE: return "device error";
+    case SF_ENODEV: return "no usable device";
+    case SF_ERANGE: return "out of range";
+    default: return "unknown status";
+    }
+}
+
+const char *sf_last_error(void) { return g_err.c_str(); }
+int sf_abi_version(void) { return SF_ABI_VERSION; }
+
+int sf_init(int device) {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(SF_ENODEV, "no HIP device visible");
+    if (device < 0 || device >= count) return fail(SF_EINVAL, "device %d out of range (%d devices)", device, count);
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SF_ENODEV, "device %d is %s, this library is built for gfx950", device, prop.gcnArchName);
+    HIP_TRY(hipSetDevice(device));
+    g_device = device;
+    return SF_OK;
+}
+
+uint64_t sf_batch_size(const sf_batch *b) { return b ? b->n : 0; }
+
+void sf_batch_destroy(sf_batch *b) { delete b; }
+
+int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
+    if (!out) return fail(SF_EINVAL, "out is null");
+    *out = nullptr;
+    int st = check_pairs(pairs);
+    if (st) return st;
+    if ((st = ensure_device())) return st;
+    for (uint64_t i = 0; i < pairs->n; ++i)
+        if (pairs->pat_len[i] >= (1u << 30) || pairs->txt_len[i] >= (1u << 30))
+            return fail(SF_ERANGE, "pair %llu: sequences of 2^30 bases or more", (unsigned long long)i);
+    auto *b = new sf_batch();
+    b->n = pairs->n;
+    const uint64_t n = pairs->n;
+    b->plen.assign(pairs->pat_len, pairs->pat_len + n);
+    b->tlen.assign(pairs->txt_len, pairs->txt_len + n);
+    std::vector<uint64_t> poff(n), toff(n);
+    uint64_t words = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        poff[i] = words;
+        words += (b->plen[i] + 15) / 16;
+        toff[i] = words;
+        words += (b->tlen[i] + 15) / 16;
+    }
+    std::vector<uint32_t> seq(std::max<uint64_t>(words, 1));
+    parallel_for(n, [&](uint64_t a, uint64_t e) {
+        for (uint64_t i = a; i < e; ++i) {
+            pack_into(&seq[poff[i]], pairs->pat, pairs->packed, pairs->pat_off[i], b->plen[i]);
+            pack_into(&seq[toff[i]], pairs->txt, pairs->packed, pairs->txt_off[i], b->tlen[i]);
+        }
+    });
+    auto bail = [&](int s) {
+        delete b;
+        return s;
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(SF_EDEVICE, "hipStreamCreate failed"));
+    if ((st = dalloc(&b->d_seq, seq.size())) || (st = dalloc(&b->d_poff, n)) || (st = dalloc(&b->d_toff, n)) ||
+        (st = dalloc(&b->d_plen, n)) || (st = dalloc(&b->d_tlen, n)) || (st = dalloc(&b->d_dist, n)) ||
+        (st = dalloc(&b->d_end, n)) || (st = dalloc(&b->d_start, n)) || (st = dalloc(&b->d_morder, n)))
+        return bail(st);
+    // Myers bins: by blocks per pattern (the lane kernel's register class), then text length.
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        uint32_t bx = (b->plen[x] + 63) / 64, by = (b->plen[y] + 63) / 64;
+        if (bx != by) return bx < by;
+        return b->tlen[x] < b->tlen[y];
+    });
+    for (uint64_t q = 0; q < n;) {
+        int cls = sf_myers_maxb_for((b->plen[order[q]] + 63) / 64);
+        uint64_t e = q;
+        while (e < n && sf_myers_maxb_for((b->plen[order[e]] + 63) / 64) == cls) ++e;
+        if (cls == 0 && b->myers_unsupported.empty()) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "pattern of %u bases: Myers supports up to 2048", b->plen[order[q]]);
+            b->myers_unsupported = msg;
+        }
+        b->mbins.push_back({cls, (uint32_t)q, (uint32_t)e});
+        q = e;
+    }
+    for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];
+    hipError_t e1 = hipMemcpy(b->d_seq, seq.data(), seq.size() * 4, hipMemcpyHostToDevice);
+    hipError_t e2 = hipMemcpy(b->d_poff, poff.data(), n * 8, hipMemcpyHostToDevice);
+    hipError_t e3 = hipMemcpy(b->d_toff, toff.data(), n * 8, hipMemcpyHostToDevice);
+    hipError_t e4 = hipMemcpy(b->d_plen, b->plen.data(), n * 4, hipMemcpyHostToDevice);
+    hipError_t e5 = hipMemcpy(b->d_tlen, b->tlen.data(), n * 4, hipMemcpyHostToDevice);
+    hipError_t e6 = hipMemcpy(b->d_morder, order.data(), n * 4, hipMemcpyHostToDevice);
+    if (e1 || e2 || e3 || e4 || e5 || e6) return bail(fail(SF_EDEVICE, "upload failed"));
+    *out = b;
+    return SF_OK;
+}
+
+int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
+    if (!b) return fail(SF_EINVAL, "batch is null");
+    if (mode < SF_INFIX || mode > SF_GLOBAL) return fail(SF_EINVAL, "mode %d", mode);
+    if (locate < 0 || locate > 2) return fail(SF_EINVAL, "locate %d", locate);
+    if (!b->myers_unsupported.empty()) return fail(SF_ERANGE, "%s", b->myers_unsupported.c_str());
+    int st;
+    if ((st = ensure_device())) return st;
+    MyersArgs a{};
+    a.pv = view(b);
+    a.mode = mode;
+    a.k = k;
+    a.dist = b->d_dist;
+    a.end = locate ? b->d_end : nullptr;
+    a.start = locate == 2 ? b->d_start : nullptr;
+    hipEvent_t e0;
+    if ((st = timed_begin(b, 0, &e0))) return st;
+    for (auto &bin : b->mbins) {
+        a.order = b->d_morder + bin.begin;
+        a.count = bin.end - bin.begin;
+        a.reverse = 0;
+        HIP_TRY(sf_launch_myers(a, bin.cls, b->stream));
+    }
+    if ((st = timed_end(b, 0, e0))) return st;
+    b->myers_cells += b->myers_cells_per_run;
+    if (locate == 2 && mode == SF_INFIX) {
+        if ((st = timed_begin(b, 1, &e0))) return st;
+        for (auto &bin : b->mbins) {
+            a.order = b->d_morder + bin.begin;
+            a.count = bin.end - bin.begin;
+            a.reverse = 1;
+            HIP_TRY(sf_launch_myers(a, bin.cls, b->stream));
+        }
+        if ((st = timed_end(b, 1, e0))) return st;
+    }
+    return SF_OK;
+}
+
+int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t *band, int cigar) {
+    if (!b) return fail(SF_EINVAL, "batch is null");
+    if (!sc) return fail(SF_EINVAL, "scoring is null");
+    if (mode < SF_INFIX || mode > SF_GLOBAL) return fail(SF_EINVAL, "mode %d", mode);
+    if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0)
+        return fail(SF_EINVAL, "scores must be non-negative (align.rs:2511 requires open, ext >= 0)");
+    if (sc->mat > 65535 || sc->mis > 65535 || sc->open > 65535 || sc->ext > 65535)
+        return fail(SF_ERANGE, "scores above 65535 are outside this build's bound");
+    int st;
+    if ((st = ensure_device())) return st;
+    const uint64_t n = b->n;
+    // Shape classes, cached across runs with the same mode and bands.
+    uint64_t key = 1469598103934665603ull ^ (uint64_t)mode;
+    if (band)
+        for (uint64_t i = 0; i < n; ++i) key = (key ^ band[i]) * 1099511628211ull;
+    else
+        key ^= 0x9e3779b97f4a7c15ull;
+    if (!b->akey_valid || b->akey != key) {
+        std::vector<uint64_t> ba(n);
+        std::vector<uint8_t> kcls(n);
+        uint64_t cells = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            uint64_t below, above;
+            const uint32_t m = b->plen[i], t = b->tlen[i];
+            band_geom(m, t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
+            const uint64_t width = below + above + 1;
+            const int K = sf_affine_k_for(width);
+            if (K == 0)
+                return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this build handles up to 2048",
+                            (unsigned long long)i, (unsigned long long)width);
+            // Values stay far above NEG: |score| <= (mat + mis + ext) * (m + n + width) + 2 open.
+            const uint64_t bound = (uint64_t)(sc->mat + sc->mis + sc->ext) * (m + t + width + 2) + 2ull * sc->open;
+            if (bound >= (1ull << 28)) return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
+            ba[i] = below | (above << 32);
+            kcls[i] = (uint8_t)K;
+            // DP_WORK cells (align.rs:2573-2578).
+            for (uint64_t r = 1; r <= m; ++r) {
+                const uint64_t lo = r > below ? r - below : 0;
+                const uint64_t hi = std::min<uint64_t>(r + above, t);
+                if (lo > hi) break;
+                cells += hi + 1 - lo;
+            }
+        }
+        std::vector<uint32_t> order(n);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            if (kcls[x] != kcls[y]) return kcls[x] < kcls[y];
+            return b->plen[x] > b->plen[y];
+        });
+        b->abins.clear();
+        for (uint64_t q = 0; q < n;) {
+            uint64_t e = q;
+            while (e < n && kcls[order[e]] == kcls[order[q]]) ++e;
+            b->abins.push_back({kcls[order[q]], (uint32_t)q, (uint32_t)e});
+            q = e;
+        }
+        if (!b->d_ba && ((st = dalloc(&b->d_ba, n)) || (st = dalloc(&b->d_aorder, n)) || (st = dalloc(&b->d_score, n)) ||
+                         (st = dalloc(&b->d_col, n))))
+            return st;
+        HIP_TRY(hipMemcpy(b->d_ba, ba.data(), n * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d_aorder, order.data(), n * 4, hipMemcpyHostToDevice));
+        b->aorder = std::move(order);
+        b->akey = key;
+        b->akey_valid = true;
+        b->affine_cells_cur = cells;
+        b->tb_words = 0; // traceback offsets depend on the classes
+    }
+    AffineArgs a{};
+    a.pv = view(b);
+    a.below_above = b->d_ba;
+    a.mode = mode;
+    a.mat = sc->mat, a.mis = sc->mis, a.open = sc->open, a.ext = sc->ext;
+    a.score = b->d_score;
+    a.score_col = b->d_col;
+    hipEvent_t e0;
+    if (!cigar) {
+        if ((st = timed_begin(b, 2, &e0))) return st;
+        for (auto &bin : b->abins) {
+            a.order = b->d_aorder + bin.begin;
+            a.count = bin.end - bin.begin;
+            HIP_TRY(sf_launch_affine(a, bin.cls, false, b->stream));
+            ++b->affine_launches;
+        }
+        if ((st = timed_end(b, 2, e0))) return st;
+        b->affine_cells += b->affine_cells_cur;
+        b->cigar_valid = false;
+        return SF_OK;
+    }
+    // Traceback: chunks of each bin whose bits fit the scratch budget.
+    struct Chunk {
+        int K;
+        uint32_t begin, end;
+    };
+    std::vector<Chunk> chunks;
+    std::vector<uint64_t> tboff(n);
+    const uint64_t budget = tb_budget_words();
+    uint64_t need_max = 0;
+    for (auto &bin : b->abins) {
+        uint64_t used = 0;
+        uint32_t cb = bin.begin;
+        for (uint32_t q = bin.begin; q < bin.end; ++q) {
+            const uint64_t words = ((uint64_t)b->plen[b->aorder[q]] * 8 * bin.cls + 63) / 64 * 64;
+            if (used + words > budget && q > cb) {
+                chunks.push_back({bin.cls, cb, q});
+                need_max = std::max(need_max, used);
+                used = 0;
+                cb = q;
+            }
+            tboff[q] = used;
+            used += words;
+        }
+        if (cb < bin.end) chunks.push_back({bin.cls, cb, bin.end});
+        need_max = std::max(need_max, used);
+    }
+    if (need_max > b->tb_words) {
+        dfree(b->d_tb);
+        if ((st = dalloc(&b->d_tb, need_max))) return st;
+        b->tb_words = need_max;
+    }
+    if (!b->d_tboff) {
+        if ((st = dalloc(&b->d_tboff, n)) || (st = dalloc(&b->d_tstart, n)) || (st = dalloc(&b->d_nruns, n)) ||
+            (st = dalloc(&b->d_runs_off, n)) || (st = dalloc(&b->d_out_off, n + 1)))
+            return st;
+        std::vector<uint64_t> roff(n);
+        uint64_t tot = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            roff[i] = tot;
+            tot += (uint64_t)b->plen[i] + b->tlen[i] + 1;
+        }
+        if ((st = dalloc(&b->d_runs, tot)) || (st = dalloc(&b->d_cig, tot))) return st;
+        HIP_TRY(hipMemcpy(b->d_runs_off, roff.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemcpy(b->d_tboff, tboff.data(), n * 8, hipMemcpyHostToDevice));
+    a.tb = b->d_tb;
+    WalkArgs w{};
+    w.pv = view(b);
+    w.below_above = b->d_ba;
+    w.mode = mode;
+    w.score = b->d_score;
+    w.score_col = b->d_col;
+    w.tb = b->d_tb;
+    w.runs = b->d_runs;
+    w.runs_off = b->d_runs_off;
+    w.nruns = b->d_nruns;
+    w.tstart = b->d_tstart;
+    for (auto &c : chunks) {
+        a.order = b->d_aorder + c.begin;
+        a.count = c.end - c.begin;
+        a.tb_off = b->d_tboff + c.begin;
+        if ((st = timed_begin(b, 2, &e0))) return st;
+        HIP_TRY(sf_launch_affine(a, c.K, true, b->stream));
+        if ((st = timed_end(b, 2, e0))) return st;
+        ++b->affine_launches;
+        w.order = a.order;
+        w.count = a.count;
+        w.K = c.K;
+        w.tb_off = a.tb_off;
+        if ((st = timed_begin(b, 3, &e0))) return st;
+        HIP_TRY(sf_launch_walk(w, b->stream));
+        if ((st = timed_end(b, 3, e0))) return st;
+    }
+    if ((st = timed_begin(b, 3, &e0))) return st;
+    hipLaunchKernelGGL(scan_u32_kernel, dim3(1), dim3(1024), 0, b->stream, b->d_nruns, n, b->d_out_off);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(sf_launch_compact(b->d_aorder, (uint32_t)n, b->d_runs, b->d_runs_off, b->d_nruns, b->d_out_off, b->d_cig,
+                              b->stream));
+    if ((st = timed_end(b, 3, e0))) return st;
+    b->affine_cells += b->affine_cells_cur;
+    b->cigar_valid = true;
+    return SF_OK;
+}
+
+int sf_batch_sync(sf_batch *b) {
+    if (!b) return fail(SF_EINVAL, "batch is null");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return SF_OK;
+}
+
+int sf_batch_myers_results(sf_batch *b, uint32_t *dist, uint32_t *end, uint32_t *start) {
+    int st;
+    if ((st = sf_batch_sync(b))) return st;
+    const uint64_t n = b->n;
+    if (dist) HIP_TRY(hipMemcpy(dist, b->d_dist, n * 4, hipMemcpyDeviceToHost));
+    if (end) HIP_TRY(hipMemcpy(end, b->d_end, n * 4, hipMemcpyDeviceToHost));
+    if (start) HIP_TRY(hipMemcpy(start, b->d_start, n * 4, hipMemcpyDeviceToHost));
+    return SF_OK;
+}
+
+int sf_batch_affine_results(sf_batch *b, int32_t *score, uint32_t *score_col, uint32_t *tstart, uint32_t *cigar,
+                            uint64_t cigar_cap, uint64_t *cigar_off) {
+    int st;
+    if ((st = sf_batch_sync(b))) return st;
+    const uint64_t n = b->n;
+    if (!b->d_score) return fail(SF_EINVAL, "no affine run on this batch");
+    if (score) HIP_TRY(hipMemcpy(score, b->d_score, n * 4, hipMemcpyDeviceToHost));
+    if (score_col) HIP_TRY(hipMemcpy(score_col, b->d_col, n * 4, hipMemcpyDeviceToHost));
+    if (tstart || cigar || cigar_off) {
+        if (!b->cigar_valid) return fail(SF_EINVAL, "the last affine run had no traceback");
+        std::vector<uint32_t> ts(n);
+        HIP_TRY(hipMemcpy(ts.data(), b->d_tstart, n * 4, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < n; ++i)
+            if (ts[i] == kNone) return fail(SF_EDEVICE, "pair %llu: traceback left the band", (unsigned long long)i);
+        if (tstart) std::memcpy(tstart, ts.data(), n * 4);
+        std::vector<uint64_t> off(n + 1);
+        HIP_TRY(hipMemcpy(off.data(), b->d_out_off, (n + 1) * 8, hipMemcpyDeviceToHost));
+        if (cigar_off) std::memcpy(cigar_off, off.data(), (n + 1) * 8);
+        if (cigar) {
+            if (off[n] > cigar_cap)
+                return fail(SF_ERANGE, "cigar buffer holds %llu ops, %llu needed", (unsigned long long)cigar_cap,
+                            (unsigned long long)off[n]);
+            HIP_TRY(hipMemcpy(cigar, b->d_cig, off[n] * 4, hipMemcpyDeviceToHost));
+        }
+    }
+    return SF_OK;
+}
+
+int sf_batch_timing(sf_batch *b, sf_timing *t) {
+    int st;
+    if (!t) return fail(SF_EINVAL, "timing is null");
+    if ((st = sf_batch_sync(b))) return st;
+    b->collect();
+    t->myers_ms = b->ms[0];
+    t->locate_ms = b->ms[1];
+    t->affine_ms = b->ms[2];
+    t->walk_ms = b->ms[3];
+    t->myers_cells = b->myers_cells;
+    t->affine_cells = b->affine_cells;
+    t->affine_launches = b->affine_launches;
+    return SF_OK;
+}
+
+void sf_batch_reset_timing(sf_batch *b) {
+    if (!b) return;
+    (void)hipStreamSynchronize(b->stream);
+    b->collect();
+    for (double &x : b->ms) x = 0;
+    b->myers_cells = b->affine_cells = b->affine_launches = 0;
+}
+
+// ---- one-shot calls ----
+
+int sf_myers(const sf_pairs *pairs, int mode, uint32_t k, uint32_t *dist, uint32_t *end, uint32_t *start) {
+    if (!dist) return fail(SF_EINVAL, "dist is null");
+    if (start && !end) return fail(SF_EINVAL, "start needs end");
+    if (pairs && pairs->n == 0) return check_pairs(pairs);
+    sf_batch *b;
+    int st = sf_batch_create(pairs, &b);
+    if (st) return st;
+    st = sf_batch_myers(b, mode, k, start ? 2 : end ? 1 : 0);
+    if (!st) st = sf_batch_myers_results(b, dist, end, start);
+    sf_batch_destroy(b);
+    return st;
+}
+
+int sf_affine(const sf_pairs *pairs, int mode, const sf_scoring *sc, const uint64_t *band, int32_t *score,
+              uint32_t *score_col) {
+    if (!score) return fail(SF_EINVAL, "score is null");
+    if (pairs && pairs->n == 0) return check_pairs(pairs);
+    sf_batch *b;
+    int st = sf_batch_create(pairs, &b);
+    if (st) return st;
+    st = sf_batch_affine(b, mode, sc, band, 0);
+    if (!st) st = sf_batch_affine_results(b, score, score_col, nullptr, nullptr, 0, nullptr);
+    sf_batch_destroy(b);
+    return st;
+}
+
+int sf_affine_cigar(const sf_pairs *pairs, int mode, const sf_scoring *sc, const uint64_t *band, int32_t *score,
+                    uint32_t *score_col, uint32_t *tstart, uint32_t *cigar, uint64_t cigar_cap, uint64_t *cigar_off) {
+    if (!score || !cigar_off) return fail(SF_EINVAL, "score and cigar_off are required");
+    if (pairs && pairs->n == 0) {
+        if (cigar_off) cigar_off[0] = 0;
+        return check_pairs(pairs);
+    }
+    sf_batch *b;
+    int st = sf_batch_create(pairs, &b);
+    if (st) return st;
+    st = sf_batch_affine(b, mode, sc, band, 1);
+    if (!st) st = sf_batch_affine_results(b, score, score_col, tstart, cigar, cigar_cap, cigar_off);
+    sf_batch_destroy(b);
+    return st;
+}
+
+// Diagnostic: lane movements of the DPP encodings the kernels use (320 values).
+int sf_selftest_lanes(int32_t *out) {
+    int st;
+    if ((st = ensure_device())) return st;
+    int32_t *d = nullptr;
+    if ((st = dalloc(&d, 320))) return st;
+    hipError_t e = sf_launch_probe(d, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, d, 320 * 4, hipMemcpyDeviceToHost);
+    dfree(d);
+    if (e != hipSuccess) return fail(SF_EDEVICE, "probe: %s", hipGetErrorString(e));
+    return SF_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,182 @@
+"""GPU parity: the HIP kernels through the C ABI against the CPU oracle, bit for bit.
+
+Inputs are seeded: the parity-set workload (1) and hand-made edge cases.  Every comparison is
+exact (integer scores, distances, coordinates and CIGAR ops).
+"""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import sf_oracle as O
+from spearfish_amd import align as A
+from spearfish_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+MODES = (A.Mode.Infix, A.Mode.Prefix, A.Mode.Global)
+U32 = A.U32_MAX
+
+
+def opt(x):
+    return U32 if x is None else x
+
+
+def test_lane_moves():
+    import ctypes as C
+    from spearfish_amd import _native as N
+    out = (C.c_int32 * 320)()
+    N.check(N.load().sf_selftest_lanes(C.cast(out, C.c_void_p)))
+    v = list(out)
+    assert v[0:64] == [l + 1 for l in range(63)] + [-1], "wave_shl:1"
+    assert v[64:128] == [-1] + list(range(63)), "wave_shr:1"
+    assert v[128:192] == [(-1 if l % 16 == 0 else l - 1) for l in range(64)], "row_shr:1"
+    vals = [l * 3 % 17 for l in range(64)]
+    assert v[192:256] == [-100] + [max(vals[:l]) for l in range(1, 64)], "exclusive max scan"
+    assert v[256:320] == [l ^ 5 for l in range(64)], "shfl_xor"
+
+
+def edge_pairs():
+    rng = random.Random(21)
+    P, T = [], []
+    for m in (0, 1, 2, 63, 64, 65, 127, 128, 129, 500, 1000, 1024, 1025, 2048):
+        for n in (0, 1, 5, 64, 200, 1100):
+            p = [rng.randrange(4) for _ in range(m)]
+            t = [rng.randrange(4) for _ in range(n)]
+            if n >= m and m > 0 and rng.random() < 0.5:
+                s = rng.randrange(n - m + 1)
+                t[s:s + m] = p
+            P.append(p)
+            T.append(t)
+    # codes above 3 are masked (align.rs:78, 120)
+    P.append([4, 5, 6, 7, 0, 1]), T.append([0, 1, 2, 3, 8, 9, 10])
+    return P, T
+
+
+def check_myers(ps, mode, k):
+    dist, end, start = A.myers_batch(ps, mode, k, locate=2)
+    for i in range(len(ps)):
+        p, t = ps.pattern(i), ps.text(i)
+        d, e, s = O.locate(p, t, int(mode), k)
+        assert (int(dist[i]), int(end[i]), int(start[i])) == (opt(d), opt(e), opt(s)), (i, len(p), len(t), mode, k)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_myers_parity_set(mode):
+    ps, _ = W.generate(1, 1024, 0)
+    check_myers(ps, mode, U32)
+    check_myers(ps, mode, 40)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_myers_edges(mode):
+    P, T = edge_pairs()
+    ps = A.PairSet.from_seqs(P, T)
+    for k in (U32, 0, 3, 100):
+        check_myers(ps, mode, k)
+
+
+def test_myers_reference_examples():
+    # align.rs:2908-2926 through the device
+    assert A.infix_distance([0, 1, 2, 3], [3, 3, 0, 1, 2, 3, 3]) == 0
+    assert A.infix_distance([0, 1, 2, 3], [0, 1, 3]) == 1
+    assert A.infix_distance([0, 1, 2, 3], []) == 4
+    assert A.distance([0, 1, 2, 3], [3, 3, 0, 1, 2, 3, 3], A.Mode.Prefix, 10) == 2
+    assert A.distance([0, 1, 2, 3], [3, 3, 0, 1, 2, 3, 3], A.Mode.Global, 10) == 3
+    assert A.distance([0, 1, 2, 3], [3, 3, 0, 1, 2, 3, 3], A.Mode.Global, 2) is None
+    assert A.distance([], [0, 1], A.Mode.Global, 5) == 2
+    pat = A.Pattern([0, 1, 2, 3])
+    assert A.batch_distances(pat, [[3, 3, 0, 1, 2, 3, 3], [0, 1, 3], []], A.Mode.Infix, U32) == [0, 1, 4]
+
+
+def check_affine(ps, mode, sc, bands, cigar=True):
+    r = A.affine_batch(ps, mode, sc, bands, cigar=cigar)
+    bands = np.broadcast_to(np.asarray(A.USIZE_MAX if bands is None else bands, np.uint64), (len(ps),))
+    for i in range(len(ps)):
+        p, t = ps.pattern(i), ps.text(i)
+        b = int(bands[i])
+        bo = None if b == A.USIZE_MAX else b
+        dp = O.affine_dp(p, t, int(mode), (sc.mat, sc.mis, sc.open, sc.ext), bo)
+        assert (int(r["score"][i]), int(r["score_col"][i])) == (dp.score, dp.score_col), (i, len(p), len(t), mode, b)
+        if cigar:
+            want = O.align(p, t, int(mode), (sc.mat, sc.mis, sc.open, sc.ext), bo)
+            o0, o1 = int(r["cigar_off"][i]), int(r["cigar_off"][i + 1])
+            got = [int(x) for x in r["cigar"][o0:o1]]
+            if want is None:
+                assert got == []
+            else:
+                assert (int(r["tstart"][i]), got) == (want[2], want[3]), (i, len(p), len(t), mode, b,
+                                                                          O.cigar_str(got), O.cigar_str(want[3]))
+
+
+def test_affine_parity_set_global_band64():
+    ps, band = W.generate(1, 1024, 0)
+    check_affine(ps, A.Mode.Global, A.Affine.ONT, band)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("sc", [A.Affine.ONT, A.Affine.HIFI, A.Affine(1, 1, 0, 1), A.Affine(3, 2, 5, 1)])
+def test_affine_small_random(mode, sc):
+    rng = random.Random(31 + int(mode))
+    P, T, B = [], [], []
+    for _ in range(300):
+        n = rng.randrange(0, 300)
+        t = [rng.randrange(4) for _ in range(n)]
+        p = []
+        for x in t:
+            u = rng.random()
+            if u < 0.05:
+                p.append((x + 1) & 3)
+            elif u < 0.08:
+                p += [x, rng.randrange(4)]
+            elif u < 0.11:
+                pass
+            else:
+                p.append(x)
+        if rng.random() < 0.2:
+            p = p[rng.randrange(len(p) + 1):]
+        P.append(p)
+        T.append(t)
+        B.append(rng.choice([0, 1, 2, 5, 16, 64, 200, A.USIZE_MAX]))
+    ps = A.PairSet.from_seqs(P, T)
+    check_affine(ps, mode, sc, np.array(B, np.uint64), cigar=mode != A.Mode.Infix or True)
+
+
+def test_affine_edges():
+    P, T = edge_pairs()
+    ps = A.PairSet.from_seqs(P[:60], T[:60])
+    for mode in MODES:
+        for band in (0, 3, 64, None):
+            check_affine(ps, mode, A.Affine.ONT, band)
+
+
+def test_affine_reference_examples():
+    # align.rs:3185-3196 through the device
+    p = [0, 1, 2, 3, 0, 1, 2, 3]
+    assert A.affine(p, p, A.Mode.Global) == 16
+    t = [0, 1, 2, 3, 0, 1, 2, 3, 2, 2]
+    assert A.affine(p, t, A.Mode.Global) == 16 - 8
+    assert A.affine(p, t, A.Mode.Prefix) == 16
+    assert A.affine([], [0, 1], A.Mode.Global) == -8
+    assert A.affine([0, 1], [], A.Mode.Global) == -8
+    assert A.affine_sorted(p, [p, t], A.Mode.Global) == [16, 8]
+
+
+def test_batch_object_reruns_are_identical():
+    ps, band = W.generate(1, 512, 3)
+    b = A.Batch(ps)
+    b.myers(A.Mode.Infix, locate=2)
+    d1 = b.myers_results(2)
+    b.affine(A.Mode.Global, A.Affine.ONT, bands=band, cigar=True)
+    r1 = b.affine_results(cigar=True)
+    b.myers(A.Mode.Infix, locate=2)
+    b.affine(A.Mode.Global, A.Affine.ONT, bands=band, cigar=True)
+    d2 = b.myers_results(2)
+    r2 = b.affine_results(cigar=True)
+    for x, y in zip(d1, d2):
+        assert np.array_equal(x, y)
+    for key in r1:
+        assert np.array_equal(r1[key], r2[key])
+    t = b.timing()
+    assert t.myers_ms > 0 and t.affine_ms > 0 and t.affine_cells > 0
+    b.close()
