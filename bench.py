@@ -1,10 +1,13 @@
 """Benchmark of the alignment hot path on one MI355X.
 
 Workload (default): 1,000,000 synthetic pairs of 1,000 bp targets with queries at 6% ONT-like
-error, band 128 -- the uniform roofline batch.  One step aligns every pair:
+error, band 128 -- the uniform roofline batch.  One step aligns every pair as the reference's
+kernels do (scores only, one launch per kernel shape):
   1. Myers semi-global (Infix) edit distance with the end column, k unbounded;
-  2. banded affine Global alignment, ONT scoring (2, 4, 4, 2), band 128, with CIGAR.
-Inputs are packed 2-bit into HBM before the timed region; results stay in HBM.
+  2. banded affine Global score, ONT scoring (2, 4, 4, 2), band 128.
+The same step with the affine traceback (CIGAR, this project's addition) is timed after it and
+reported under "with_cigar".  Inputs are packed 2-bit into HBM before the timed region; results
+stay in HBM.
 
 value = DP cells of a step (sum of m*n for Myers plus the affine band cells, counted as the
 reference's DP_WORK does) / wall time per step, in GCUPS.
@@ -30,8 +33,9 @@ METRIC = "GCUPS (bit-exact) batched vs per-pair on 1 GPU; serve reads/s + p50/p9
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 i32 lanes/clk x 2.4 GHz
 HBM_PEAK_GBS = 8000.0
 # Algorithmic i32 ops per affine cell (F: 2 sub + 2 max; substitution: compare + select;
-# x: add + max; E/H: 2 max + 1 sub) plus the 4 traceback compares.
-AFFINE_OPS_PER_CELL = 11 + 4
+# x: add + max; E/H: 2 max + 1 sub); the traceback adds 4 compares.
+AFFINE_OPS_PER_CELL = 11
+AFFINE_TB_OPS_PER_CELL = 11 + 4
 
 
 def parse():
@@ -87,7 +91,7 @@ def per_pair_launch_gcups(pairs, bands, k):
     t0 = time.perf_counter()
     for i, s in enumerate(singles):
         s.myers(A.Mode.Infix, locate=1)
-        s.affine(A.Mode.Global, A.Affine.ONT, bands=bands[i:i + 1], cigar=True)
+        s.affine(A.Mode.Global, A.Affine.ONT, bands=bands[i:i + 1], cigar=False)
         s.sync()
     dt = time.perf_counter() - t0
     for i, s in enumerate(singles):
@@ -113,44 +117,39 @@ def main():
     batch = A.Batch(pairs)
     log(f"packed and uploaded in {time.perf_counter() - t0:.1f}s")
 
-    def step():
-        batch.myers(A.Mode.Infix, locate=1)
-        batch.affine(A.Mode.Global, A.Affine.ONT, bands=bands, cigar=True)
-
-    for w in range(args.warmup):
-        step()
+    def run(cigar, steps, warmup, label):
+        for w in range(warmup):
+            batch.myers(A.Mode.Infix, locate=1)
+            batch.affine(A.Mode.Global, A.Affine.ONT, bands=bands, cigar=cigar)
+            batch.sync()
+            log(f"{label}: warmup {w + 1}/{warmup} done")
+        batch.reset_timing()
         batch.sync()
-        log(f"warmup {w + 1}/{args.warmup} done")
-    batch.reset_timing()
-    batch.sync()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        step()
-    batch.sync()
-    wall = time.perf_counter() - t0
-    tm = batch.timing()
-    log(f"timed {args.steps} steps in {wall:.3f}s")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.myers(A.Mode.Infix, locate=1)
+            batch.affine(A.Mode.Global, A.Affine.ONT, bands=bands, cigar=cigar)
+        batch.sync()
+        wall = time.perf_counter() - t0
+        tm = batch.timing()
+        log(f"{label}: timed {steps} steps in {wall:.3f}s")
+        return wall, tm
+
+    def roofline(tm, ops, kernel):
+        per_launch_ms = tm.affine_ms / max(tm.affine_launches, 1)
+        cells_per_launch = tm.affine_cells / max(tm.affine_launches, 1)
+        achieved = cells_per_launch * ops / (per_launch_ms * 1e-3) / 1e12
+        return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 1),
+                "unit": "TOP/s (i32 VALU lane-ops)", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                "ops_per_cell": ops, "cells_per_launch": int(cells_per_launch),
+                "avg_launch_ms": round(per_launch_ms, 3)}
+
+    wall, tm = run(False, args.steps, args.warmup, "scores")
     cells = tm.myers_cells + tm.affine_cells
     value = cells / wall / 1e9
     ms_per_step = wall / args.steps * 1e3
-
     dist, _, _ = batch.myers_results(0)
     res = batch.affine_results(cigar=False)
-    affine_per_launch_ms = tm.affine_ms / max(tm.affine_launches, 1)
-    affine_cells_per_launch = tm.affine_cells / max(tm.affine_launches, 1)
-    achieved = affine_cells_per_launch * AFFINE_OPS_PER_CELL / (affine_per_launch_ms * 1e-3) / 1e12
-    roofline = {
-        "bound": "valu",
-        "kernel": "affine_wave_kernel<5,true> (banded affine DP + traceback bits)",
-        "achieved": round(achieved, 3),
-        "peak": round(VALU_PEAK_TOPS, 1),
-        "unit": "TOP/s (i32 VALU lane-ops)",
-        "frac": round(achieved / VALU_PEAK_TOPS, 4),
-        "traffic": None,
-        "ops_per_cell": AFFINE_OPS_PER_CELL,
-        "cells_per_launch": int(affine_cells_per_launch),
-        "avg_launch_ms": round(affine_per_launch_ms, 3),
-    }
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -166,12 +165,22 @@ def main():
         "data": "synthetic",
         "config": {"workload": "uniform-1e6x1000bp-6pct-band128", "pairs": len(pairs), "target_bp": 1000,
                    "error": "6% (3% sub, 1.5% ins, 1.5% del, geometric indels p=0.7)", "band": 128,
-                   "step": "myers infix distance+end, affine global ONT band 128 + CIGAR"},
+                   "step": "myers infix distance+end, affine global ONT band 128 score"},
         "myers_gcups": round(tm.myers_cells / (tm.myers_ms * 1e-3) / 1e9, 1) if tm.myers_ms else None,
-        "affine_gcups": round(tm.affine_cells / ((tm.affine_ms + tm.walk_ms) * 1e-3) / 1e9, 1) if tm.affine_ms else None,
-        "kernel_ms_per_step": {"myers": round(tm.myers_ms / args.steps, 2), "affine": round(tm.affine_ms / args.steps, 2),
-                               "walk": round(tm.walk_ms / args.steps, 2)},
-        "roofline": roofline,
+        "affine_gcups": round(tm.affine_cells / (tm.affine_ms * 1e-3) / 1e9, 1) if tm.affine_ms else None,
+        "kernel_ms_per_step": {"myers": round(tm.myers_ms / args.steps, 2),
+                               "affine": round(tm.affine_ms / args.steps, 2)},
+        "roofline": roofline(tm, AFFINE_OPS_PER_CELL, "affine_wave_kernel<5,false> (banded affine DP, scores)"),
+    }
+    wall_c, tm_c = run(True, args.steps, 1, "with cigar")
+    out["with_cigar"] = {
+        "value": round((tm_c.myers_cells + tm_c.affine_cells) / wall_c / 1e9, 2), "unit": "GCUPS",
+        "ms_per_step": round(wall_c / args.steps * 1e3, 2),
+        "affine_gcups": round(tm_c.affine_cells / ((tm_c.affine_ms + tm_c.walk_ms) * 1e-3) / 1e9, 1),
+        "kernel_ms_per_step": {"myers": round(tm_c.myers_ms / args.steps, 2),
+                               "affine_tb": round(tm_c.affine_ms / args.steps, 2),
+                               "walk_compact": round(tm_c.walk_ms / args.steps, 2)},
+        "roofline": roofline(tm_c, AFFINE_TB_OPS_PER_CELL, "affine_wave_kernel<5,true> (DP + traceback bits)"),
     }
     if args.per_pair:
         try:

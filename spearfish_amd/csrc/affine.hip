@@ -1,7 +1,7 @@
 // affine.hip -- banded affine-gap (Gotoh) DP on gfx950, one pair per wave, plus the traceback.
 //
 // Computes crates/map/src/align.rs:2446-2666 `run_core` (geom = n, no clip, no cut): the same
-// H and F in every band cell, hence the same score and score column.
+// H in every band cell, hence the same score and score column.
 //
 // Layout.  Row i covers text columns [i - below, i + above] clipped to [0, n] (align.rs:
 // 2468-2481).  The wave holds one row in diagonal coordinates: slot s = j - i + below, lane L
@@ -9,176 +9,196 @@
 // the vertical neighbour of slot s is slot s + 1 (a register, or lane L + 1's first slot via
 // DPP wave_shl:1) and the diagonal neighbour is slot s itself.
 //
-// A row is the reference's three passes (align.rs:1018-1072) fused: per cell
-//   F = max(Hup - go, Fup - ext, NEG),  x = max(Hdiag + s, F),  g = x - go + ext*(slot + 1),
-// then E at slot s = (max of g over slots < s) - ext*slot, as an in-lane running max and one
-// exclusive DPP max-scan across the 64 lanes, and H = max(x, E).  With open >= 0 this is
-// Gotoh's recurrence (align.rs:1060-1063), and all values are exact i32.
+// Values are kept biased by B(i, s) = ext*s + 2*ext*i:  Hb = H + B, Fb = F + B, xb = x + B.
+// The reference's passes (align.rs:1018-1072) then become, per cell,
+//   Fb = max(Ub_up, Fb_up)            with Ub = Hb + (ext - go), made by the row above
+//   xb = max(Hb_diag + s + 2 ext, Fb)
+//   Hb = max(xb, max{xb of the slots left of s in the row} + (ext - go))
+// because a horizontal gap from slot s' to s costs go + ext*(s - 1 - s'), which the bias turns
+// into the constant go - ext; the running max over the row is an in-lane max plus one exclusive
+// DPP max-scan across the 64 lanes.  With open >= 0 this is Gotoh's recurrence (align.rs:
+// 1060-1063); every value is exact i32.  The reference clamps F at NEG (align.rs:1053); that
+// clamp only acts where both vertical terms are below NEG, where H comes from the diagonal, so
+// it changes no H and is dropped.
 //
 // Cells outside the band.  The reference reads a NEG sentinel just right of each row's band
 // (align.rs:1047-1048) and starts the horizontal gap of a row beginning at lo > 0 from NEG.
-// Here: slots >= Wd (lanes' padding) are forced to H = NEG every row, which is the sentinel the
-// next row reads at its right edge (their F then stays NEG); a row starting at lo > 0 starts at
-// slot 0, whose scan carry is below NEG; slots left of column 0 (top rows) and right of column
-// n (bottom rows) hold values that stay within NEG + mat * m of NEG and are never read by a band
-// cell except as a horizontal-gap source or a diagonal that real band values dominate (the host
-// checks the score bound that guarantees it, as the reference's own `narrow` does for 16 bits).
+// Here slot Wd (just right of the widest row) is set to a sentinel each row -- the next row's
+// rightmost band cell reads it; a row starting at lo > 0 starts at slot 0, whose scan carry is
+// below every value; slots left of column 0 (top rows) and right of column n (bottom rows) hold
+// values that real band values dominate and that no band cell reads as a vertical or diagonal
+// neighbour.  The host rejects scorings for which band values could come within reach of NEG.
 //
-// Traceback bits (own definition; the reference has no traceback): per cell, ballots of
+// Traceback bits (own definition; the reference has no traceback), per cell
 //   nd: H != Hdiag + s,  ne: H != x,  fx: F > Hup - go,  ex: E > Hleft - go,
-// written as 64-lane masks: row i, slot-in-lane q, lane half h, bit b at word
-// (i-1)*8K + q*8 + h*4 + b.  The walk reads them back under the oracle's tie rules.
+// accumulated per lane, 8 slots per u32 (slot q's nibble nd,ne,fx,ex from high to low, the first
+// slot highest): row i, word wd of lane L at (i-1)*64*NWL + wd*64 + L, NWL = ceil(4K / 32).
+// The walk reads them back under the oracle's tie rules.
 #include "internal.hpp"
 
 using namespace sf;
 
 namespace {
 
+// Traceback words per lane per row: 4 bits per slot.
+template <int K>
+constexpr int kTbWords = (4 * K + 31) / 32;
+
+// acc = 2 * acc + (a != b) and 2 * acc + (a > b): the compare's VCC is the carry-in of one add.
+__device__ __forceinline__ uint32_t push_ne(uint32_t acc, int32_t a, int32_t b) {
+    asm("v_cmp_ne_u32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc" : "+v"(acc) : "v"(a), "v"(b) : "vcc");
+    return acc;
+}
+__device__ __forceinline__ uint32_t push_gt(uint32_t acc, int32_t a, int32_t b) {
+    asm("v_cmp_gt_i32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc" : "+v"(acc) : "v"(a), "v"(b) : "vcc");
+    return acc;
+}
+
 template <int K, bool TB>
 __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
-    constexpr int NW = (8 * K + 63) / 64;
+    constexpr int NWL = kTbWords<K>;
+    constexpr int REFRESH = 32 - K + 1; // rows served by one 32-code text window
     const int lane = threadIdx.x & 63;
     const uint32_t w = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (w >= a.count) return; // whole waves only
-    const uint32_t pi = a.order[w];
-    const int64_t m = a.pv.plen[pi], n = a.pv.tlen[pi];
+    const uint32_t pi = __builtin_amdgcn_readfirstlane(a.order[w]);
+    const int32_t m = __builtin_amdgcn_readfirstlane((int32_t)a.pv.plen[pi]);
+    const int32_t n = __builtin_amdgcn_readfirstlane((int32_t)a.pv.tlen[pi]);
     const uint64_t ba = a.below_above[pi];
-    const int64_t below = (int64_t)(uint32_t)ba, above = (int64_t)(ba >> 32);
-    const int64_t Wd = below + above + 1;
+    const int32_t below = __builtin_amdgcn_readfirstlane((int32_t)(uint32_t)ba);
+    const int32_t above = __builtin_amdgcn_readfirstlane((int32_t)(ba >> 32));
+    const int32_t Wd = below + above + 1;
     const uint32_t *P = a.pv.seq + a.pv.poff[pi];
     const uint32_t *T = a.pv.seq + a.pv.toff[pi];
-    const int64_t pnw = (m + 15) / 16, tnw = (n + 15) / 16;
+    const int32_t pnw = (m + 15) >> 4, tnw = (n + 15) >> 4;
     const int32_t mat = a.mat, mis = a.mis, ext = a.ext, go = a.open + a.ext;
+    const int32_t dlt = ext - go; // = -open
+    const int32_t scm = mat + 2 * ext, scx = 2 * ext - mis;
     const bool infix = a.mode == kInfix;
+    const int32_t s0 = lane * K;
 
-    int32_t Hp[K], Fp[K], GC[K], ES[K];
-    bool PAD[K];
-    const int64_t hi0 = above < n ? above : n;
+    int32_t Hb[K], Fb[K], Ub[K];
+    const int32_t hi0 = above < n ? above : n;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-        const int64_t s = (int64_t)lane * K + q;
-        const int64_t j = s - below;
-        Hp[q] = (j >= 0 && j <= hi0) ? ((j == 0 || infix) ? 0 : -(a.open + ext * (int32_t)j)) : kNeg;
-        Fp[q] = kNeg;
-        GC[q] = ext * (int32_t)(s + 1) - go;
-        ES[q] = ext * (int32_t)s;
-        PAD[q] = s >= Wd;
+        const int32_t s = s0 + q;
+        const int32_t j = s - below;
+        const int32_t h = (j >= 0 && j <= hi0) ? ((j == 0 || infix) ? 0 : -(a.open + ext * j)) : kNeg;
+        Hb[q] = h == kNeg ? kNeg : h + ext * s;
+        Fb[q] = kNeg;
+        Ub[q] = Hb[q] + dlt;
     }
+    // The sentinel slot Wd: lane and slot-in-lane (none when the band fills the wave).
+    const int32_t sent_lane = Wd < 64 * K ? Wd / K : -1;
+    const int32_t sent_q = Wd < 64 * K ? Wd % K : -1;
     uint64_t win = 0;
-    int avail = 0;
     uint32_t pw = 0;
     uint32_t *tbp = nullptr;
-    if constexpr (TB) tbp = a.tb + a.tb_off[w];
+    if constexpr (TB) tbp = a.tb + a.tb_off[w] + lane;
     bool stopped = false;
-    for (int64_t i = 1; i <= m; ++i) {
-        const int64_t b = i - below;
-        const int64_t lo = b > 0 ? b : 0;
-        const int64_t hi = (b + Wd - 1) < n ? (b + Wd - 1) : n;
+    int32_t refresh = 0;
+    for (int32_t i = 1; i <= m; ++i) {
+        const int32_t b = i - below;
+        const int32_t lo = b > 0 ? b : 0;
+        const int32_t hi = (b + Wd - 1) < n ? (b + Wd - 1) : n;
         if (lo > hi) {
             stopped = true; // align.rs:2543-2556
             break;
         }
-        if (((i - 1) & 15) == 0) pw = codes16(P, i - 1, pnw);
-        const uint32_t pc = __builtin_amdgcn_readfirstlane(pw & 3u);
+        if (((i - 1) & 15) == 0) pw = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)codes16(P, i - 1, pnw));
+        const uint32_t pc = pw & 3u;
         pw >>= 2;
-        if (avail < K) {
-            const int64_t pos = b + (int64_t)lane * K - 1; // text base of this lane's first slot
+        if (refresh == 0) {
+            const int32_t pos = b + s0 - 1; // text base of this lane's first slot
             win = (uint64_t)codes16(T, pos, tnw) | ((uint64_t)codes16(T, pos + 16, tnw) << 32);
-            avail = 32;
+            refresh = REFRESH;
         }
-        const int32_t Hn = dpp<kWaveShl1>(kNeg, Hp[0]);
-        const int32_t Fn = dpp<kWaveShl1>(kNeg, Fp[0]);
-        int32_t X[K], FQ[K], G[K];
-        int32_t D[K], U[K];
+        --refresh;
+        const int32_t Un = dpp<kWaveShl1>(kNeg, Ub[0]);
+        const int32_t Fn = dpp<kWaveShl1>(kNeg, Fb[0]);
+        int32_t X[K], FQ[K], G[K], D[K], Uu[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            const int32_t hu = q < K - 1 ? Hp[q + 1] : Hn;
-            const int32_t fu = q < K - 1 ? Fp[q + 1] : Fn;
+            const int32_t uu = q < K - 1 ? Ub[q + 1] : Un;
+            const int32_t fu = q < K - 1 ? Fb[q + 1] : Fn;
             const uint32_t code = (uint32_t)(win >> (2 * q)) & 3u;
-            const int32_t sc = code == pc ? mat : -mis;
-            const int32_t u = hu - go;
-            const int32_t f = imax(imax(u, fu - ext), kNeg);
-            const int32_t d = Hp[q] + sc;
+            const int32_t sc = code == pc ? scm : scx;
+            const int32_t f = imax(uu, fu);
+            const int32_t d = Hb[q] + sc;
             const int32_t x = imax(d, f);
             X[q] = x;
             FQ[q] = f;
             D[q] = d;
-            U[q] = u;
-            const int32_t g = x + GC[q];
+            Uu[q] = uu;
+            const int32_t g = x + dlt;
             G[q] = q ? imax(G[q - 1], g) : g;
         }
-        const int32_t C = wave_excl_max(G[K - 1], kScanIdent);
-        int32_t H[K], E[K];
+        const int32_t C = wave_excl_max_min(G[K - 1]);
+        int32_t E[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            const int32_t e = (q ? imax(C, G[q - 1]) : C) - ES[q];
-            const int32_t h = imax(X[q], e);
-            E[q] = e;
-            H[q] = PAD[q] ? kNeg : h;
+            E[q] = q ? imax(C, G[q - 1]) : C; // biased horizontal-gap value entering slot q
+            Hb[q] = imax(X[q], E[q]);
+            Fb[q] = FQ[q];
         }
+        if (sent_q >= 0) {
+#pragma unroll
+            for (int q = 0; q < K; ++q)
+                if (q == sent_q) {
+                    Hb[q] = lane == sent_lane ? kNeg : Hb[q];
+                    Fb[q] = lane == sent_lane ? kNeg : Fb[q];
+                }
+        }
+#pragma unroll
+        for (int q = 0; q < K; ++q) Ub[q] = Hb[q] + dlt;
         if constexpr (TB) {
-            const int32_t hl0 = dpp<kWaveShr1>(kNeg, H[K - 1]);
-            int32_t out[NW];
+            const int32_t ul0 = dpp<kWaveShr1>(kNeg, Ub[K - 1]);
+            uint32_t *row = tbp + (uint32_t)(i - 1) * (64 * NWL);
 #pragma unroll
-            for (int r = 0; r < NW; ++r) out[r] = 0;
+            for (int wd = 0; wd < NWL; ++wd) {
+                uint32_t acc = 0;
 #pragma unroll
-            for (int q = 0; q < K; ++q) {
-                const int32_t hl = q ? H[q - 1] : hl0;
-                const uint64_t bits[4] = {
-                    __builtin_amdgcn_ballot_w64(H[q] != D[q]),
-                    __builtin_amdgcn_ballot_w64(H[q] != X[q]),
-                    __builtin_amdgcn_ballot_w64(FQ[q] > U[q]),
-                    __builtin_amdgcn_ballot_w64(E[q] > hl - go),
-                };
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-#pragma unroll
-                    for (int bt = 0; bt < 4; ++bt) {
-                        const int idx = q * 8 + h * 4 + bt;
-                        const uint32_t v = (uint32_t)(bits[bt] >> (32 * h));
-                        out[idx / 64] = lane == idx % 64 ? (int32_t)v : out[idx / 64];
-                    }
+                for (int q = wd * 8; q < K && q < wd * 8 + 8; ++q) {
+                    const int32_t ul = q ? Ub[q - 1] : ul0;
+                    acc = push_ne(acc, Hb[q], D[q]);
+                    acc = push_ne(acc, Hb[q], X[q]);
+                    acc = push_gt(acc, FQ[q], Uu[q]);
+                    acc = push_gt(acc, E[q], ul);
+                }
+                row[wd * 64] = acc;
             }
-            uint32_t *row = tbp + (uint64_t)(i - 1) * (8 * K);
-#pragma unroll
-            for (int r = 0; r < NW; ++r)
-                if (r * 64 + lane < 8 * K) row[r * 64 + lane] = (uint32_t)out[r];
-        }
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            Hp[q] = H[q];
-            Fp[q] = FQ[q];
         }
         win >>= 2;
-        --avail;
     }
     int32_t score;
     uint32_t col;
+    const int32_t rowbias = 2 * ext * m;
     if (stopped) {
         score = kNeg;
         col = 0;
     } else if (a.mode == kGlobal) {
-        const int64_t sN = n - m + below; // slot of (m, n)
-        const int owner = (int)(sN / K), qq = (int)(sN % K);
-        int32_t v = Hp[0];
+        const int32_t sN = n - m + below; // slot of (m, n)
+        const int owner = sN / K, qq = sN % K;
+        int32_t v = Hb[0];
 #pragma unroll
-        for (int q = 1; q < K; ++q) v = q == qq ? Hp[q] : v;
-        score = __builtin_amdgcn_readlane(v, owner);
+        for (int q = 1; q < K; ++q) v = q == qq ? Hb[q] : v;
+        score = __builtin_amdgcn_readlane(v, owner) - ext * sN - rowbias;
         col = (uint32_t)n;
     } else {
         // align.rs:2643-2650: max of the last row, first argmax.
-        const int64_t b = m - below;
-        const int64_t lo = b > 0 ? b : 0;
-        const int64_t hi = (b + Wd - 1) < n ? (b + Wd - 1) : n;
-        const int64_t slo = lo - b, shi = hi - b;
+        const int32_t b = m - below;
+        const int32_t lo = b > 0 ? b : 0;
+        const int32_t hi = (b + Wd - 1) < n ? (b + Wd - 1) : n;
+        const int32_t slo = lo - b, shi = hi - b;
         int32_t bv = INT32_MIN;
         int32_t bs = INT32_MAX;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            const int64_t s = (int64_t)lane * K + q;
-            if (s >= slo && s <= shi && Hp[q] > bv) {
-                bv = Hp[q];
-                bs = (int32_t)s;
+            const int32_t s = s0 + q;
+            const int32_t h = Hb[q] - ext * s - rowbias;
+            if (s >= slo && s <= shi && h > bv) {
+                bv = h;
+                bs = s;
             }
         }
 #pragma unroll
@@ -231,17 +251,20 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
         }
     };
     bool bad = false;
-    auto bits_at = [&](int64_t ii, int64_t jj) -> uint4 {
+    const int64_t nwl = (4 * K + 31) / 32;
+    // The 4 bits (nd, ne, fx, ex) of cell (ii, jj) in bits 3..0 of the result.
+    auto bits_at = [&](int64_t ii, int64_t jj) -> uint32_t {
         const int64_t s = jj - (ii - below);
         if (ii < 1 || ii > m || s < 0 || s >= 64 * (int64_t)K) {
             // Off the band: cannot happen on a path the DP produced; stop rather than read out of
             // the pair's bits.
             bad = true;
-            return make_uint4(0, 0, 0, 0);
+            return 0;
         }
         const int64_t L = s / K, q = s % K;
-        const uint4 *p = reinterpret_cast<const uint4 *>(tb + (uint64_t)(ii - 1) * (8 * K) + q * 8 + (L >> 5) * 4);
-        return *p;
+        const int64_t wd = q / 8, nq = (K - wd * 8) < 8 ? (K - wd * 8) : 8;
+        const uint32_t word = tb[(uint64_t)(ii - 1) * (64 * nwl) + wd * 64 + L];
+        return (word >> (4 * (nq - 1 - (q % 8)))) & 15u;
     };
     for (;;) {
         if (bad) break;
@@ -255,30 +278,25 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
                 emit(1, (uint32_t)i);
                 break;
             }
-            const uint4 w = bits_at(i, j);
-            const uint32_t bit = (uint32_t)((j - (i - below)) / K) & 31u;
-            if (((w.x >> bit) & 1u) == 0) {
+            const uint32_t w = bits_at(i, j);
+            if ((w & 8u) == 0) {
                 const uint32_t pc = (ldw(P, (i - 1) >> 4, pnw) >> (2 * ((i - 1) & 15))) & 3u;
                 const uint32_t tc = (ldw(T, (j - 1) >> 4, tnw) >> (2 * ((j - 1) & 15))) & 3u;
                 emit(pc == tc ? 7u : 8u, 1);
                 --i;
                 --j;
-            } else if (((w.y >> bit) & 1u) == 0) {
+            } else if ((w & 4u) == 0) {
                 state = 1;
             } else {
                 state = 2;
             }
         } else if (state == 1) {
             emit(1, 1);
-            const uint4 w = bits_at(i, j);
-            const uint32_t bit = (uint32_t)((j - (i - below)) / K) & 31u;
-            state = ((w.z >> bit) & 1u) ? 1 : 0;
+            state = (bits_at(i, j) & 2u) ? 1 : 0;
             --i;
         } else {
             emit(2, 1);
-            const uint4 w = bits_at(i, j);
-            const uint32_t bit = (uint32_t)((j - (i - below)) / K) & 31u;
-            state = ((w.w >> bit) & 1u) ? 2 : 0;
+            state = (bits_at(i, j) & 1u) ? 2 : 0;
             --j;
         }
     }

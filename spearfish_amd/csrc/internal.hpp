@@ -112,6 +112,22 @@ constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0
 constexpr int kWaveShl1 = 0x130, kWaveShr1 = 0x138;
 constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
 
+// Exclusive max-scan across the 64 lanes of a wave, lane 0 getting INT32_MIN.  Signed values are
+// mapped to unsigned order (x ^ 0x80000000) so that DPP's zero fill of lanes without a source is
+// the identity and each step is one v_max_u32 with a DPP operand.
+__device__ __forceinline__ int32_t wave_excl_max_min(int32_t v) {
+    uint32_t x = (uint32_t)v ^ 0x80000000u;
+    auto step = [](uint32_t y, uint32_t z) { return y > z ? y : z; };
+    x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr1, 0xf, 0xf, true));
+    x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr2, 0xf, 0xf, true));
+    x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr4, 0xf, 0xf, true));
+    x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr8, 0xf, 0xf, true));
+    x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowBcast15, 0xa, 0xf, true));
+    x = step(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowBcast31, 0xc, 0xf, true));
+    x = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kWaveShr1, 0xf, 0xf, true);
+    return (int32_t)(x ^ 0x80000000u);
+}
+
 // Exclusive max-scan across the 64 lanes of a wave (lane 0 gets `ident`).
 __device__ __forceinline__ int32_t wave_excl_max(int32_t v, int32_t ident) {
     int32_t x = v;

@@ -158,6 +158,131 @@ __global__ __launch_bounds__(256) void myers_lane_kernel(MyersArgs a) {
     }
 }
 
+// Long patterns: G lanes per pair (G = 16: four pairs per wave, one DPP row each; G = 64: one
+// pair per wave), lane l holding R consecutive 64-row blocks.  The lanes form a pipeline over the
+// text: at step t lane l works column t - l, taking the horizontal delta out of lane l - 1's last
+// block (computed one step earlier) and lane l - 1's text base, both moved by DPP row_shr:1 /
+// wave_shr:1; lane 0 takes `top` and the next text base.  Blocks are placed so that the pattern's
+// last block is the last slot of lane G - 1 (positions below off = G*R - nb hold no rows and pass
+// the delta through), so one slot per lane needs the per-pair last-row bit.
+template <int G, int R>
+__global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
+    constexpr int SHR = G == 64 ? kWaveShr1 : kRowShr1;
+    const int lane = threadIdx.x & 63;
+    const int l = lane % G;
+    const uint32_t grp = (blockIdx.x * 256u + threadIdx.x) / G;
+    const bool valid = grp < a.count;
+    const uint32_t pi = valid ? a.order[grp] : 0;
+    const uint32_t m = valid ? a.pv.plen[pi] : 0;
+    const uint32_t tlen = valid ? a.pv.tlen[pi] : 0;
+    const uint32_t *P = a.pv.seq + (valid ? a.pv.poff[pi] : 0);
+    const uint32_t *T = a.pv.seq + (valid ? a.pv.toff[pi] : 0);
+    const int64_t pnw = (m + 15) / 16, tnw = (tlen + 15) / 16;
+    const bool rev = a.reverse != 0;
+    int mode = a.mode;
+    uint32_t n = tlen;
+    bool work = valid && m > 0;
+    if (work && rev) {
+        mode = kPrefix;
+        if (a.dist[pi] == kNone) work = false;
+        else n = a.end[pi];
+    }
+    if (!work) n = 0;
+    const int nb = work ? (int)((m + 63) / 64) : 0;
+    const int off = G * R - nb; // >= 0: the launcher's class holds every block
+    const unsigned last_bit = work ? (m - 1) & 63u : 63u;
+    uint64_t P1[R], P2[R], PV[R], MV[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        P1[r] = P2[r] = 0;
+        const int b = l * R + r - off;
+        if (work && b >= 0) load_planes(P, pnw, m, b, rev, P1[r], P2[r]);
+        PV[r] = ~0ull;
+        MV[r] = 0;
+    }
+    // Steps: the wave runs until its longest text has passed every lane.
+    int32_t steps = work ? (int32_t)n + G - 1 : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) steps = imax(steps, __shfl_xor(steps, o));
+    const int32_t topv = mode == kInfix ? 0 : 1; // packed delta: bit 0 +1, bit 1 -1
+    int32_t score = (int32_t)m, best = (int32_t)m;
+    uint32_t bestj = 0;
+    uint32_t tw = 0;
+    int32_t c = 0, hv = 0;
+    for (int32_t t = 0; t < steps; ++t) {
+        if ((t & 15) == 0 && l == 0 && work)
+            tw = rev ? reverse2(codes16(T, (int64_t)n - 16 - t, tnw)) : ldw(T, t >> 4, tnw);
+        const int32_t newc = (int32_t)(tw & 3u);
+        tw >>= 2;
+        c = __builtin_amdgcn_update_dpp(newc, c, SHR, 0xf, 0xf, false);
+        const int32_t hin = __builtin_amdgcn_update_dpp(topv, hv, SHR, 0xf, 0xf, false);
+        const int32_t j = t - l;
+        if (work && j >= 0 && j < (int32_t)n) {
+            const uint64_t M1 = (c & 1) ? 0ull : ~0ull;
+            const uint64_t M2 = (c & 2) ? 0ull : ~0ull;
+            uint64_t hp = (uint64_t)(hin & 1), hn = (uint64_t)((hin >> 1) & 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (l * R + r >= off) {
+                    const uint64_t eq = (P1[r] ^ M1) & (P2[r] ^ M2);
+                    const uint64_t p = PV[r], mm = MV[r];
+                    const uint64_t xv = eq | mm;
+                    const uint64_t eqn = eq | hn;
+                    const uint64_t xh = (((eqn & p) + p) ^ p) | eqn;
+                    uint64_t ph = mm | ~(xh | p);
+                    uint64_t mh = p & xh;
+                    uint64_t op, om;
+                    if (r < R - 1) {
+                        op = ph >> 63;
+                        om = mh >> 63;
+                    } else {
+                        const unsigned bit = l == G - 1 ? last_bit : 63u;
+                        op = (ph >> bit) & 1;
+                        om = (mh >> bit) & 1;
+                    }
+                    ph = (ph << 1) | hp;
+                    mh = (mh << 1) | hn;
+                    PV[r] = mh | ~(xv | ph);
+                    MV[r] = ph & xv;
+                    hp = op;
+                    hn = om;
+                }
+            }
+            hv = (int32_t)(hp | (hn << 1));
+            if (l == G - 1) {
+                score += (int32_t)hp - (int32_t)hn;
+                if (score < best) {
+                    best = score;
+                    bestj = (uint32_t)j + 1;
+                }
+            }
+        }
+    }
+    if (valid && !rev && m == 0) {
+        const uint32_t d = mode == kGlobal ? tlen : 0;
+        const bool ok = d <= a.k;
+        if (l == 0) {
+            a.dist[pi] = ok ? d : kNone;
+            if (a.end) a.end[pi] = ok ? (mode == kGlobal ? tlen : 0) : kNone;
+            if (a.start) a.start[pi] = ok ? 0 : kNone;
+        }
+        return;
+    }
+    if (!work || l != G - 1) return;
+    if (!rev) {
+        if (mode == kGlobal) {
+            best = score;
+            bestj = n;
+        }
+        const bool ok = (uint32_t)best <= a.k;
+        a.dist[pi] = ok ? (uint32_t)best : kNone;
+        if (a.end) a.end[pi] = ok ? bestj : kNone;
+        if (a.start && (mode != kInfix || !ok)) a.start[pi] = ok ? 0 : kNone;
+    } else {
+        a.start[pi] = n - bestj;
+    }
+}
+
 __global__ void probe_kernel(int32_t *out) {
     // Lane-movement check of the DPP encodings the kernels rely on: 5 x 64 values.
     const int32_t lane = threadIdx.x;
@@ -170,16 +295,41 @@ __global__ void probe_kernel(int32_t *out) {
 
 } // namespace
 
+// Shape classes: lane kernel MAXB (1..32), then pipelined classes encoded as 1000 * G + R.
 int sf_myers_maxb_for(uint32_t nblocks) {
-    static const int cls[] = {1, 2, 4, 8, 12, 16, 20, 24, 32};
-    for (int c : cls)
+    static const int lane_cls[] = {1, 2, 4, 8, 12, 16, 20, 24, 32};
+    for (int c : lane_cls)
         if (nblocks <= (uint32_t)c) return c;
+    static const int pipe16[] = {3, 4, 5, 6, 8, 10, 12, 16, 20, 24};
+    for (int r : pipe16)
+        if (nblocks <= (uint32_t)(16 * r)) return 16000 + r;
+    static const int pipe64[] = {8, 12, 16, 20, 24};
+    for (int r : pipe64)
+        if (nblocks <= (uint32_t)(64 * r)) return 64000 + r;
     return 0;
 }
 
 hipError_t sf_launch_myers(const MyersArgs &a, int maxb, hipStream_t s) {
     if (a.count == 0) return hipSuccess;
     dim3 grid((a.count + 255) / 256), block(256);
+    if (maxb >= 16000) {
+        const int G = maxb / 1000, R = maxb % 1000;
+        const uint64_t threads = (uint64_t)a.count * G;
+        dim3 pg((unsigned)((threads + 255) / 256));
+        switch (maxb) {
+#define PCASE(GG, RR) \
+    case GG * 1000 + RR: hipLaunchKernelGGL((myers_pipe_kernel<GG, RR>), pg, block, 0, s, a); break;
+            PCASE(16, 3) PCASE(16, 4) PCASE(16, 5) PCASE(16, 6) PCASE(16, 8) PCASE(16, 10) PCASE(16, 12)
+            PCASE(16, 16) PCASE(16, 20) PCASE(16, 24) PCASE(64, 8) PCASE(64, 12) PCASE(64, 16) PCASE(64, 20)
+            PCASE(64, 24)
+#undef PCASE
+        default:
+            return hipErrorInvalidValue;
+        }
+        (void)G;
+        (void)R;
+        return hipGetLastError();
+    }
     switch (maxb) {
 #define CASE(B) \
     case B: hipLaunchKernelGGL(myers_lane_kernel<B>, grid, block, 0, s, a); break;

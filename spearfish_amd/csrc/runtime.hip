@@ -362,7 +362,7 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
         while (e < n && sf_myers_maxb_for((b->plen[order[e]] + 63) / 64) == cls) ++e;
         if (cls == 0 && b->myers_unsupported.empty()) {
             char msg[160];
-            snprintf(msg, sizeof msg, "pattern of %u bases: Myers supports up to 2048", b->plen[order[q]]);
+            snprintf(msg, sizeof msg, "pattern of %u bases: Myers supports up to 98304", b->plen[order[q]]);
             b->myers_unsupported = msg;
         }
         b->mbins.push_back({cls, (uint32_t)q, (uint32_t)e});
@@ -518,7 +518,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         uint64_t used = 0;
         uint32_t cb = bin.begin;
         for (uint32_t q = bin.begin; q < bin.end; ++q) {
-            const uint64_t words = ((uint64_t)b->plen[b->aorder[q]] * 8 * bin.cls + 63) / 64 * 64;
+            const uint64_t words = (uint64_t)b->plen[b->aorder[q]] * 64 * ((4 * bin.cls + 31) / 32);
             if (used + words > budget && q > cb) {
                 chunks.push_back({bin.cls, cb, q});
                 need_max = std::max(need_max, used);

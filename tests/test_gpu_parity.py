@@ -180,3 +180,28 @@ def test_batch_object_reruns_are_identical():
     t = b.timing()
     assert t.myers_ms > 0 and t.affine_ms > 0 and t.affine_cells > 0
     b.close()
+
+
+def test_myers_long_patterns_class_boundaries():
+    # Patterns past the lane kernel (> 2048 bases) run 16 or 64 lanes per pair; lengths at the
+    # edges of the block classes, texts short enough for the oracle.
+    rng = random.Random(41)
+    P, T = [], []
+    for m in (2049, 2112, 3072, 3073, 4096, 5121, 16 * 64 * 24, 16 * 64 * 24 + 1, 30000):
+        n = rng.choice([0, 1, 150, 300])
+        t = [rng.randrange(4) for _ in range(n)]
+        p = [rng.randrange(4) for _ in range(m)]
+        if n and rng.random() < 0.5:
+            p[:n] = t
+        P.append(p)
+        T.append(t)
+    ps = A.PairSet.from_seqs(P, T)
+    for mode in MODES:
+        for k in (U32, 2100):
+            check_myers(ps, mode, k)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_myers_long_workload(mode):
+    ps, _ = W.generate(4, 24, 5)
+    check_myers(ps, mode, U32)
