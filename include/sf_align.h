@@ -128,6 +128,11 @@ int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate);
 /* Enqueues the banded affine pass over every pair; band as in sf_affine; cigar != 0 adds the
  * traceback. */
 int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t *band, int cigar);
+/* Restricts each pair's text to the window [start, end) located by the last sf_batch_myers run
+ * with locate = 2 (pairs with no distance keep their text), so that the next sf_batch_affine
+ * aligns each pattern to its best-matching substring; on = 0 restores the texts.  No reference
+ * counterpart: it chains T2 (end/start) into the affine pass for long segments. */
+int sf_batch_use_located(sf_batch *b, int on);
 int sf_batch_sync(sf_batch *b);
 /* Copies results of the last runs out (implies sync).  Nullable outputs are skipped. */
 int sf_batch_myers_results(sf_batch *b, uint32_t *dist, uint32_t *end, uint32_t *start);

@@ -69,6 +69,7 @@ SIGNATURES = {
     "sf_batch_size": ([_vp], _u64),
     "sf_batch_myers": ([_vp, _int, _u32, _int], _int),
     "sf_batch_affine": ([_vp, _int, C.POINTER(Scoring), _vp, _int], _int),
+    "sf_batch_use_located": ([_vp, _int], _int),
     "sf_batch_sync": ([_vp], _int),
     "sf_batch_myers_results": ([_vp, _vp, _vp, _vp], _int),
     "sf_batch_affine_results": ([_vp, _vp, _vp, _vp, _vp, _u64, _vp], _int),

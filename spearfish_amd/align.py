@@ -188,6 +188,10 @@ class Batch:
         scc = sc.c()
         N.check(self.lib.sf_batch_affine(self.h, int(mode), C.byref(scc), bands.ctypes.data, int(cigar)))
 
+    def use_located(self, on: bool = True):
+        """Restrict texts to the windows [start, end) of the last myers(locate=2) run (or restore)."""
+        N.check(self.lib.sf_batch_use_located(self.h, int(on)))
+
     def sync(self):
         N.check(self.lib.sf_batch_sync(self.h))
 

@@ -68,9 +68,8 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
     const int32_t below = __builtin_amdgcn_readfirstlane((int32_t)(uint32_t)ba);
     const int32_t above = __builtin_amdgcn_readfirstlane((int32_t)(ba >> 32));
     const int32_t Wd = below + above + 1;
-    const uint32_t *P = a.pv.seq + a.pv.poff[pi];
-    const uint32_t *T = a.pv.seq + a.pv.toff[pi];
-    const int32_t pnw = (m + 15) >> 4, tnw = (n + 15) >> 4;
+    const SeqRef P = seqref(a.pv.seq, a.pv.poff[pi], (uint32_t)m);
+    const SeqRef T = seqref(a.pv.seq, a.pv.toff[pi], (uint32_t)n);
     const int32_t mat = a.mat, mis = a.mis, ext = a.ext, go = a.open + a.ext;
     const int32_t dlt = ext - go; // = -open
     const int32_t scm = mat + 2 * ext, scx = 2 * ext - mis;
@@ -105,12 +104,12 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
             stopped = true; // align.rs:2543-2556
             break;
         }
-        if (((i - 1) & 15) == 0) pw = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)codes16(P, i - 1, pnw));
+        if (((i - 1) & 15) == 0) pw = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)codes16(P, i - 1));
         const uint32_t pc = pw & 3u;
         pw >>= 2;
         if (refresh == 0) {
             const int32_t pos = b + s0 - 1; // text base of this lane's first slot
-            win = (uint64_t)codes16(T, pos, tnw) | ((uint64_t)codes16(T, pos + 16, tnw) << 32);
+            win = (uint64_t)codes16(T, pos) | ((uint64_t)codes16(T, pos + 16) << 32);
             refresh = REFRESH;
         }
         --refresh;
@@ -232,9 +231,8 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
     }
     const int64_t m = a.pv.plen[pi], n = a.pv.tlen[pi];
     const int64_t below = (int64_t)(uint32_t)a.below_above[pi];
-    const uint32_t *P = a.pv.seq + a.pv.poff[pi];
-    const uint32_t *T = a.pv.seq + a.pv.toff[pi];
-    const int64_t pnw = (m + 15) / 16, tnw = (n + 15) / 16;
+    const SeqRef P = seqref(a.pv.seq, a.pv.poff[pi], (uint32_t)m);
+    const SeqRef T = seqref(a.pv.seq, a.pv.toff[pi], (uint32_t)n);
     const int K = a.K;
     const uint32_t *tb = a.tb + a.tb_off[t];
     int64_t i = m, j = a.mode == kGlobal ? n : (int64_t)a.score_col[pi];
@@ -280,8 +278,8 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
             }
             const uint32_t w = bits_at(i, j);
             if ((w & 8u) == 0) {
-                const uint32_t pc = (ldw(P, (i - 1) >> 4, pnw) >> (2 * ((i - 1) & 15))) & 3u;
-                const uint32_t tc = (ldw(T, (j - 1) >> 4, tnw) >> (2 * ((j - 1) & 15))) & 3u;
+                const uint32_t pc = code_at(P, i - 1);
+                const uint32_t tc = code_at(T, j - 1);
                 emit(pc == tc ? 7u : 8u, 1);
                 --i;
                 --j;

@@ -2,8 +2,9 @@
 //
 // Device layout of a batch (HBM):
 //   seq   : all sequences 2-bit packed, 16 bases per u32 (base k at bits 2*(k%16) of word k/16),
-//           each sequence starting on a word boundary.
-//   poff/toff (u64) : word offset of pair i's pattern / text in `seq`;  plen/tlen (u32): lengths.
+//           each sequence starting on a word boundary when the batch is made.
+//   poff/toff (u64) : base offset of pair i's pattern / text in `seq`;  plen/tlen (u32): lengths.
+//                     (Base offsets let a pair's text be restricted to any window in place.)
 //   order (u32)     : pair indices grouped by kernel shape class ("bins"); a launch covers one bin.
 // Loads past a sequence's last word are clamped to it, so no padding words are needed.
 #pragma once
@@ -69,19 +70,43 @@ struct WalkArgs {
 
 // ---- device helpers ----
 
-__device__ __forceinline__ uint32_t ldw(const uint32_t *S, int64_t w, int64_t nw) {
-    if (nw <= 0) return 0;
-    w = w < 0 ? 0 : (w >= nw ? nw - 1 : w);
-    return S[w];
+// One sequence inside the packed words: code k of the sequence is code (base + k) of `seq`.
+// Word loads are clamped to the sequence's own words, so reads just outside it stay in bounds
+// (their codes are unspecified) and nothing needs padding.
+struct SeqRef {
+    const uint32_t *seq;
+    int64_t base;
+    int64_t w0, w1; // first and last word holding the sequence (w1 < w0: empty)
+};
+
+__device__ __forceinline__ SeqRef seqref(const uint32_t *seq, uint64_t base, uint32_t len) {
+    SeqRef r;
+    r.seq = seq;
+    r.base = (int64_t)base;
+    r.w0 = (int64_t)(base >> 4);
+    r.w1 = len ? (int64_t)((base + len - 1) >> 4) : r.w0 - 1;
+    return r;
 }
 
-// 16 codes starting at base position s (any integer): code (s + k) at bits 2k.  Positions
+__device__ __forceinline__ uint32_t ldw(const SeqRef &r, int64_t w) {
+    if (r.w1 < r.w0) return 0;
+    w = w < r.w0 ? r.w0 : (w > r.w1 ? r.w1 : w);
+    return r.seq[w];
+}
+
+// 16 codes starting at sequence position s (any integer): code (s + k) at bits 2k.  Positions
 // outside the sequence give unspecified codes.
-__device__ __forceinline__ uint32_t codes16(const uint32_t *S, int64_t s, int64_t nw) {
-    int64_t w = s >> 4; // floor for negative s
-    uint32_t sh = (uint32_t)(s & 15) * 2;
-    uint32_t a = ldw(S, w, nw), b = ldw(S, w + 1, nw);
+__device__ __forceinline__ uint32_t codes16(const SeqRef &r, int64_t s) {
+    const int64_t abs = r.base + s;
+    const int64_t w = abs >> 4; // floor for negative positions
+    const uint32_t sh = (uint32_t)(abs & 15) * 2;
+    const uint32_t a = ldw(r, w), b = ldw(r, w + 1);
     return sh ? (a >> sh) | (b << (32 - sh)) : a;
+}
+
+__device__ __forceinline__ uint32_t code_at(const SeqRef &r, int64_t k) {
+    const int64_t abs = r.base + k;
+    return (ldw(r, abs >> 4) >> (2 * (abs & 15))) & 3u;
 }
 
 // Reverses the order of the 16 two-bit codes of a word.

@@ -19,18 +19,18 @@ using namespace sf;
 
 namespace {
 
-__device__ __forceinline__ void load_planes(const uint32_t *P, int64_t pnw, uint32_t m, int b, bool rev,
-                                            uint64_t &p1, uint64_t &p2) {
+__device__ __forceinline__ void load_planes(const SeqRef &P, uint32_t m, int b, bool rev, uint64_t &p1,
+                                            uint64_t &p2) {
     uint32_t w[4];
     if (!rev) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) w[q] = ldw(P, (int64_t)b * 4 + q, pnw);
+        for (int q = 0; q < 4; ++q) w[q] = codes16(P, 64 * (int64_t)b + 16 * q);
     } else {
         // Row i of the reversed pattern is P[m - 1 - i]: rows 64b.. are codes s.. reversed,
         // s = m - 64 - 64b.
         int64_t s = (int64_t)m - 64 - 64 * (int64_t)b;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) w[q] = reverse2(codes16(P, s + 16 * (3 - q), pnw));
+        for (int q = 0; q < 4; ++q) w[q] = reverse2(codes16(P, s + 16 * (3 - q)));
     }
     uint32_t lo1 = even16(w[0]) | (even16(w[1]) << 16), hi1 = even16(w[2]) | (even16(w[3]) << 16);
     uint32_t lo2 = even16(w[0] >> 1) | (even16(w[1] >> 1) << 16), hi2 = even16(w[2] >> 1) | (even16(w[3] >> 1) << 16);
@@ -45,9 +45,8 @@ __global__ __launch_bounds__(256) void myers_lane_kernel(MyersArgs a) {
     const uint32_t pi = valid ? a.order[t] : 0;
     uint32_t m = valid ? a.pv.plen[pi] : 0;
     const uint32_t tlen = valid ? a.pv.tlen[pi] : 0;
-    const uint32_t *P = a.pv.seq + (valid ? a.pv.poff[pi] : 0);
-    const uint32_t *T = a.pv.seq + (valid ? a.pv.toff[pi] : 0);
-    const int64_t pnw = (m + 15) / 16, tnw = (tlen + 15) / 16;
+    const SeqRef P = seqref(a.pv.seq, valid ? a.pv.poff[pi] : 0, m);
+    const SeqRef T = seqref(a.pv.seq, valid ? a.pv.toff[pi] : 0, tlen);
     const bool rev = a.reverse != 0;
     int mode = a.mode;
     uint32_t n = tlen;
@@ -84,7 +83,7 @@ __global__ __launch_bounds__(256) void myers_lane_kernel(MyersArgs a) {
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
         P1[b] = P2[b] = 0;
-        if (b < nb && work) load_planes(P, pnw, m, b, rev, P1[b], P2[b]);
+        if (b < nb && work) load_planes(P, m, b, rev, P1[b], P2[b]);
         PV[b] = ~0ull;
         MV[b] = 0;
     }
@@ -95,7 +94,7 @@ __global__ __launch_bounds__(256) void myers_lane_kernel(MyersArgs a) {
     uint32_t tw = 0;
     for (uint32_t j = 0; j < n; ++j) {
         if ((j & 15) == 0) {
-            tw = rev ? reverse2(codes16(T, (int64_t)n - 16 - (int64_t)j, tnw)) : ldw(T, j >> 4, tnw);
+            tw = rev ? reverse2(codes16(T, (int64_t)n - 16 - (int64_t)j)) : codes16(T, j);
         }
         const uint32_t c = tw & 3;
         tw >>= 2;
@@ -175,9 +174,8 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
     const uint32_t pi = valid ? a.order[grp] : 0;
     const uint32_t m = valid ? a.pv.plen[pi] : 0;
     const uint32_t tlen = valid ? a.pv.tlen[pi] : 0;
-    const uint32_t *P = a.pv.seq + (valid ? a.pv.poff[pi] : 0);
-    const uint32_t *T = a.pv.seq + (valid ? a.pv.toff[pi] : 0);
-    const int64_t pnw = (m + 15) / 16, tnw = (tlen + 15) / 16;
+    const SeqRef P = seqref(a.pv.seq, valid ? a.pv.poff[pi] : 0, m);
+    const SeqRef T = seqref(a.pv.seq, valid ? a.pv.toff[pi] : 0, tlen);
     const bool rev = a.reverse != 0;
     int mode = a.mode;
     uint32_t n = tlen;
@@ -196,7 +194,7 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
     for (int r = 0; r < R; ++r) {
         P1[r] = P2[r] = 0;
         const int b = l * R + r - off;
-        if (work && b >= 0) load_planes(P, pnw, m, b, rev, P1[r], P2[r]);
+        if (work && b >= 0) load_planes(P, m, b, rev, P1[r], P2[r]);
         PV[r] = ~0ull;
         MV[r] = 0;
     }
@@ -211,7 +209,7 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
     int32_t c = 0, hv = 0;
     for (int32_t t = 0; t < steps; ++t) {
         if ((t & 15) == 0 && l == 0 && work)
-            tw = rev ? reverse2(codes16(T, (int64_t)n - 16 - t, tnw)) : ldw(T, t >> 4, tnw);
+            tw = rev ? reverse2(codes16(T, (int64_t)n - 16 - t)) : codes16(T, t);
         const int32_t newc = (int32_t)(tw & 3u);
         tw >>= 2;
         c = __builtin_amdgcn_update_dpp(newc, c, SHR, 0xf, 0xf, false);

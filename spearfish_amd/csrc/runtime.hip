@@ -145,6 +145,11 @@ struct sf_batch {
     uint32_t *d_seq = nullptr;
     uint64_t *d_poff = nullptr, *d_toff = nullptr;
     uint32_t *d_plen = nullptr, *d_tlen = nullptr;
+    // The texts as made, while the working texts may be restricted to located windows.
+    uint64_t *d_toff0 = nullptr;
+    uint32_t *d_tlen0 = nullptr;
+    std::vector<uint32_t> tlen0;
+    bool located_ready = false; // the last Myers run produced start and end
     // Myers
     struct Bin {
         int cls;
@@ -205,7 +210,7 @@ struct sf_batch {
         dfree(d_seq), dfree(d_poff), dfree(d_toff), dfree(d_plen), dfree(d_tlen), dfree(d_morder);
         dfree(d_dist), dfree(d_end), dfree(d_start), dfree(d_ba), dfree(d_aorder), dfree(d_score), dfree(d_col);
         dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
-        dfree(d_tboff), dfree(d_tb);
+        dfree(d_tboff), dfree(d_tb), dfree(d_toff0), dfree(d_tlen0);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -269,6 +274,19 @@ inline void band_geom(uint32_t m, uint32_t n, int mode, uint64_t band, uint64_t 
     } else {
         *below = band + (m > n ? m - n : 0);
         *above = band + (n > m ? n - m : 0);
+    }
+}
+
+__global__ void restrict_kernel(uint64_t n, const uint32_t *dist, const uint32_t *start, const uint32_t *end,
+                                const uint64_t *toff0, const uint32_t *tlen0, uint64_t *toff, uint32_t *tlen) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (dist[i] == kNone) {
+        toff[i] = toff0[i];
+        tlen[i] = tlen0[i];
+    } else {
+        toff[i] = toff0[i] + start[i];
+        tlen[i] = end[i] - start[i];
     }
 }
 
@@ -346,7 +364,8 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
         return bail(fail(SF_EDEVICE, "hipStreamCreate failed"));
     if ((st = dalloc(&b->d_seq, seq.size())) || (st = dalloc(&b->d_poff, n)) || (st = dalloc(&b->d_toff, n)) ||
         (st = dalloc(&b->d_plen, n)) || (st = dalloc(&b->d_tlen, n)) || (st = dalloc(&b->d_dist, n)) ||
-        (st = dalloc(&b->d_end, n)) || (st = dalloc(&b->d_start, n)) || (st = dalloc(&b->d_morder, n)))
+        (st = dalloc(&b->d_end, n)) || (st = dalloc(&b->d_start, n)) || (st = dalloc(&b->d_morder, n)) ||
+        (st = dalloc(&b->d_toff0, n)) || (st = dalloc(&b->d_tlen0, n)))
         return bail(st);
     // Myers bins: by blocks per pattern (the lane kernel's register class), then text length.
     std::vector<uint32_t> order(n);
@@ -370,12 +389,21 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
     }
     for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];
     hipError_t e1 = hipMemcpy(b->d_seq, seq.data(), seq.size() * 4, hipMemcpyHostToDevice);
-    hipError_t e2 = hipMemcpy(b->d_poff, poff.data(), n * 8, hipMemcpyHostToDevice);
-    hipError_t e3 = hipMemcpy(b->d_toff, toff.data(), n * 8, hipMemcpyHostToDevice);
+    // The kernels address bases: word offsets * 16.
+    std::vector<uint64_t> pbase(n), tbase(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        pbase[i] = poff[i] * 16;
+        tbase[i] = toff[i] * 16;
+    }
+    hipError_t e2 = hipMemcpy(b->d_poff, pbase.data(), n * 8, hipMemcpyHostToDevice);
+    hipError_t e3 = hipMemcpy(b->d_toff, tbase.data(), n * 8, hipMemcpyHostToDevice);
     hipError_t e4 = hipMemcpy(b->d_plen, b->plen.data(), n * 4, hipMemcpyHostToDevice);
     hipError_t e5 = hipMemcpy(b->d_tlen, b->tlen.data(), n * 4, hipMemcpyHostToDevice);
     hipError_t e6 = hipMemcpy(b->d_morder, order.data(), n * 4, hipMemcpyHostToDevice);
-    if (e1 || e2 || e3 || e4 || e5 || e6) return bail(fail(SF_EDEVICE, "upload failed"));
+    hipError_t e7 = hipMemcpy(b->d_toff0, tbase.data(), n * 8, hipMemcpyHostToDevice);
+    hipError_t e8 = hipMemcpy(b->d_tlen0, b->tlen.data(), n * 4, hipMemcpyHostToDevice);
+    if (e1 || e2 || e3 || e4 || e5 || e6 || e7 || e8) return bail(fail(SF_EDEVICE, "upload failed"));
+    b->tlen0 = b->tlen;
     *out = b;
     return SF_OK;
 }
@@ -404,6 +432,7 @@ int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
     }
     if ((st = timed_end(b, 0, e0))) return st;
     b->myers_cells += b->myers_cells_per_run;
+    b->located_ready = locate == 2;
     if (locate == 2 && mode == SF_INFIX) {
         if ((st = timed_begin(b, 1, &e0))) return st;
         for (auto &bin : b->mbins) {
@@ -586,6 +615,31 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     if ((st = timed_end(b, 3, e0))) return st;
     b->affine_cells += b->affine_cells_cur;
     b->cigar_valid = true;
+    return SF_OK;
+}
+
+int sf_batch_use_located(sf_batch *b, int on) {
+    if (!b) return fail(SF_EINVAL, "batch is null");
+    int st;
+    if ((st = ensure_device())) return st;
+    const uint64_t n = b->n;
+    if (on) {
+        if (!b->located_ready) return fail(SF_EINVAL, "no Myers run with start and end on this batch");
+        if (n) {
+            hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, n,
+                               b->d_dist, b->d_start, b->d_end, b->d_toff0, b->d_tlen0, b->d_toff, b->d_tlen);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(b->tlen.data(), b->d_tlen, n * 4, hipMemcpyDeviceToHost, b->stream));
+            HIP_TRY(hipStreamSynchronize(b->stream));
+        }
+    } else {
+        HIP_TRY(hipMemcpyAsync(b->d_toff, b->d_toff0, n * 8, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->d_tlen, b->d_tlen0, n * 4, hipMemcpyDeviceToDevice, b->stream));
+        b->tlen = b->tlen0;
+    }
+    b->akey_valid = false; // the affine shape classes depend on the text lengths
+    b->myers_cells_per_run = 0;
+    for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];
     return SF_OK;
 }
 

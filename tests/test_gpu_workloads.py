@@ -1,0 +1,55 @@
+"""GPU parity on the uneven and long-segment workloads, end to end through device-resident batches.
+
+Uneven (workload 3): log-normal lengths 100..20,000 bp, 5-15% error, per-pair band
+max(32, 1.2|m - n| + 32) capped at 512; Myers Infix + affine Global with CIGAR.
+Long (workload 4): queries 2-10 kb, targets 1.05x; Myers Infix distance, end and start, then
+affine Global band 256 with CIGAR on the located window text[start..end).
+"""
+import numpy as np
+import pytest
+
+from oracle import sf_oracle as O
+from spearfish_amd import align as A
+from spearfish_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+
+def cig(r, i):
+    return [int(x) for x in r["cigar"][int(r["cigar_off"][i]):int(r["cigar_off"][i + 1])]]
+
+
+def test_uneven_workload_subset():
+    ps, bands = W.generate(3, 160, 0)
+    b = A.Batch(ps)
+    b.myers(A.Mode.Infix, locate=2)
+    dist, end, start = b.myers_results(2)
+    b.affine(A.Mode.Global, A.Affine.ONT, bands=bands, cigar=True)
+    r = b.affine_results(cigar=True)
+    for i in range(len(ps)):
+        p, t = ps.pattern(i), ps.text(i)
+        assert (int(dist[i]), int(end[i]), int(start[i])) == O.locate(p, t, O.INFIX), i
+        s, col, ts, c = O.align(p, t, O.GLOBAL, O.ONT, int(bands[i]))
+        assert (int(r["score"][i]), int(r["score_col"][i]), cig(r, i)) == (s, col, c), (i, len(p), len(t), int(bands[i]))
+    b.close()
+
+
+def test_long_workload_located_windows():
+    ps, bands = W.generate(4, 12, 0)
+    b = A.Batch(ps)
+    b.myers(A.Mode.Infix, locate=2)
+    dist, end, start = b.myers_results(2)
+    b.use_located(True)
+    b.affine(A.Mode.Global, A.Affine.ONT, band=256, cigar=True)
+    r = b.affine_results(cigar=True)
+    for i in range(len(ps)):
+        p, t = ps.pattern(i), ps.text(i)
+        d, e, s = O.locate(p, t, O.INFIX)
+        assert (int(dist[i]), int(end[i]), int(start[i])) == (d, e, s), i
+        sc, col, ts, c = O.align(p, t[s:e], O.GLOBAL, O.ONT, 256)
+        assert (int(r["score"][i]), cig(r, i)) == (sc, c), i
+    # Restoring the texts gives the whole-text results again.
+    b.use_located(False)
+    b.myers(A.Mode.Infix, locate=2)
+    assert np.array_equal(b.myers_results(2)[0], dist)
+    b.close()
