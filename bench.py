@@ -13,8 +13,18 @@ value = DP cells of a step (sum of m*n for Myers plus the affine band cells, cou
 reference's DP_WORK does) / wall time per step, in GCUPS.
 
     python bench.py [--steps K] [--warmup W] [--pairs N] [--no-cpu]
+    python bench.py --workload uneven|long|serve      # the other measured workloads
 
 Prints one JSON line.
+
+Other workloads (same JSON shape, their own config):
+  uneven  5e5 pairs, targets LogNormal(median 800 bp, sigma 0.8) clipped to 100..20,000, 5-15%
+          error, band max(32, 1.2|m - n| + 32) capped at 512; Myers Infix + affine Global scores.
+  long    2e5 pairs, queries 2-10 kb, targets 1.05x, 12% error; Myers Infix distance, end and start,
+          then affine Global band 256 with CIGAR on text[start..end).
+  serve   20,000-transcript synthetic transcriptome, reads of median 1.2 kb at 8% error with 1-16
+          isoform segments, Poisson arrivals at 5k, 20k and 50k reads/s; reads/s and p50/p99
+          latency with continuous batching next to one read per launch.
 """
 from __future__ import annotations
 
@@ -48,6 +58,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pairs", type=int, default=0, help="CPU baseline sample size (0: auto)")
     ap.add_argument("--per-pair", type=int, default=256, help="pairs for the per-pair-launch comparison")
+    ap.add_argument("--workload", default="uniform", choices=["uniform", "uneven", "long", "serve"])
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     return ap.parse_args()
 
 
@@ -101,12 +113,7 @@ def per_pair_launch_gcups(pairs, bands, k):
     return cells / dt / 1e9
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested)",
-              file=sys.stderr)
-        sys.exit(2)
+def bench_uniform(args):
     from spearfish_amd import align as A
     from spearfish_amd import workload as W
 
@@ -201,6 +208,148 @@ def main():
                                "sample_bit_exact_vs_gpu": cb["bit_exact"]}
     batch.close()
     print(json.dumps(out))
+
+
+def common(args, value, unit, ms, dtype, config, higher=True):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic", "config": config}
+
+
+def bench_uneven(args):
+    from spearfish_amd import align as A
+    from spearfish_amd import workload as W
+    n = args.pairs if args.pairs != 1_000_000 else 500_000
+    pairs, bands = W.generate(3, n, args.seed)
+    log(f"generated {n} uneven pairs")
+    b = A.Batch(pairs)
+    for w in range(args.warmup):
+        b.myers(A.Mode.Infix, locate=1)
+        b.affine(A.Mode.Global, A.Affine.ONT, bands=bands)
+        b.sync()
+        log(f"warmup {w + 1} done")
+    b.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.myers(A.Mode.Infix, locate=1)
+        b.affine(A.Mode.Global, A.Affine.ONT, bands=bands)
+    b.sync()
+    wall = time.perf_counter() - t0
+    tm = b.timing()
+    log(f"timed {args.steps} steps in {wall:.3f}s")
+    cells = tm.myers_cells + tm.affine_cells
+    out = common(args, round(cells / wall / 1e9, 2), "GCUPS", round(wall / args.steps * 1e3, 2), "int32",
+                 {"workload": "uneven-5e5-lognormal800-5to15pct-adaptive-band", "pairs": n,
+                  "step": "myers infix distance+end, affine global ONT adaptive band score"})
+    out["myers_gcups"] = round(tm.myers_cells / (tm.myers_ms * 1e-3) / 1e9, 1)
+    out["affine_gcups"] = round(tm.affine_cells / (tm.affine_ms * 1e-3) / 1e9, 1)
+    out["kernel_ms_per_step"] = {"myers": round(tm.myers_ms / args.steps, 2), "affine": round(tm.affine_ms / args.steps, 2)}
+    out["affine_launches_per_step"] = tm.affine_launches // args.steps
+    if not args.no_cpu:
+        dist, _, _ = b.myers_results(0)
+        res = b.affine_results()
+        threads = min(16, os.cpu_count() or 1)
+        k = args.cpu_pairs or 2000
+        from spearfish_amd.align import PairSet
+        sub = PairSet(pairs.pat, pairs.pat_off[:k], pairs.pat_len[:k], pairs.txt, pairs.txt_off[:k], pairs.txt_len[:k])
+        cb = cpu_baseline(sub, bands[:k], {"dist": dist, "score": res["score"]}, threads)
+        out["cpu_baseline"] = {"value": round(cb["cells"] / cb["seconds"] / 1e9, 3), "unit": "GCUPS", "cores": threads,
+                               "kind": "port", "sample": f"first {k} pairs, scalar C restatement, scores only, "
+                                                         f"{cb['seconds']:.1f}s",
+                               "sample_bit_exact_vs_gpu": cb["bit_exact"]}
+    b.close()
+    print(json.dumps(out))
+
+
+def bench_long(args):
+    from spearfish_amd import align as A
+    from spearfish_amd import workload as W
+    n = args.pairs if args.pairs != 1_000_000 else 200_000
+    pairs, _ = W.generate(4, n, args.seed)
+    log(f"generated {n} long pairs")
+    b = A.Batch(pairs)
+
+    def step():
+        b.use_located(False)
+        b.myers(A.Mode.Infix, locate=2)
+        b.use_located(True)
+        b.affine(A.Mode.Global, A.Affine.ONT, band=256, cigar=True)
+
+    for w in range(args.warmup):
+        step()
+        b.sync()
+        log(f"warmup {w + 1} done")
+    b.reset_timing()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step()
+        log(f"step {s + 1} queued")
+    b.sync()
+    wall = time.perf_counter() - t0
+    tm = b.timing()
+    cells = tm.myers_cells + tm.affine_cells
+    out = common(args, round(cells / wall / 1e9, 2), "GCUPS", round(wall / args.steps * 1e3, 2), "int32",
+                 {"workload": "long-2e5x2to10kb-12pct-band256", "pairs": n,
+                  "step": "myers infix distance+end+start, affine global ONT band 256 + CIGAR on located window"})
+    out["kernel_ms_per_step"] = {"myers": round(tm.myers_ms / args.steps, 2), "myers_start": round(tm.locate_ms / args.steps, 2),
+                                 "affine_tb": round(tm.affine_ms / args.steps, 2), "walk": round(tm.walk_ms / args.steps, 2)}
+    out["myers_gcups"] = round(tm.myers_cells / ((tm.myers_ms) * 1e-3) / 1e9, 1)
+    b.close()
+    print(json.dumps(out))
+
+
+def bench_serve(args):
+    from spearfish_amd import serve as S
+    rates = [5000, 20000, 50000]
+    dur = args.serve_seconds
+    n_all = int(max(rates) * dur)
+    store, reads = S.generate_serve_workload(n_all, seed=args.seed)
+    log(f"transcriptome {store.bases} bases, {n_all} reads, {int(reads.nseg.sum())} segments")
+    res = S.Results(reads)
+    reqs = S.requests(reads, res)
+    rows = []
+    for rate in rates:
+        n = int(rate * dur)
+        arr = S.poisson_arrivals(n, rate, seed=rate)
+        with S.Server(store, slots=4, band=64) as s:
+            s.replay(reqs, S.poisson_arrivals(min(n, 2000), rate, seed=1))  # warm-up
+            s.stats(reset=True)
+            wall = s.replay(reqs, arr)
+            st = s.stats()
+            refused = s.refused
+        row = {"rate": rate, "mode": "continuous batching", "reads": n, "refused": refused,
+               "reads_per_s": round(st.reads_done / wall, 1),
+               "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
+               "mean_reads_per_batch": round(st.mean_reads_per_batch, 1), "gcups": round(st.cells / wall / 1e9, 1)}
+        log(str(row))
+        rows.append(row)
+        npr = min(n, 4000)
+        with S.Server(store, slots=1, max_reads_per_batch=1, band=64) as s:
+            s.replay(reqs, S.poisson_arrivals(200, rate, seed=1))
+            s.stats(reset=True)
+            wall = s.replay(reqs, arr[:npr])
+            st = s.stats()
+        row = {"rate": rate, "mode": "one read per launch", "reads": npr, "reads_per_s": round(st.reads_done / wall, 1),
+               "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3), "mean_reads_per_batch": 1.0}
+        log(str(row))
+        rows.append(row)
+    top = [r for r in rows if r["mode"] == "continuous batching"][-1]
+    out = common(args, top["reads_per_s"], "reads/s", None, "int32",
+                 {"workload": "serve-20k-transcripts-poisson-5k-20k-50k", "band": 64,
+                  "per_segment": "myers infix distance+end, affine global ONT band 64 score",
+                  "server": "4 slots, 256 MiB HBM budget"})
+    out["steps"], out["warmup"] = None, None
+    out["serve"] = rows
+    print(json.dumps(out))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested)",
+              file=sys.stderr)
+        sys.exit(2)
+    {"uniform": bench_uniform, "uneven": bench_uneven, "long": bench_long, "serve": bench_serve}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -142,6 +142,65 @@ int sf_batch_affine_results(sf_batch *b, int32_t *score, uint32_t *score_col, ui
 int sf_batch_timing(sf_batch *b, sf_timing *t);
 void sf_batch_reset_timing(sf_batch *b);
 
+/* ---- serving: continuous batching over a device-resident transcript store ----
+ * No reference counterpart as an object: it serves the calls the tie-break makes per read
+ * (crates/map/src/lib.rs:1307-1492, `seg`), many reads at a time.  Each request is a read and
+ * its segments (read slices against windows of the store); per segment the server computes the
+ * Myers distance and end column (`myers_mode`, k unbounded) and the banded affine score and end
+ * column (`affine_mode`, `sc`, `band`).  Up to `slots` batches run at once; a free slot admits the
+ * oldest waiting requests while they fit its share of `hbm_budget_bytes`; a request larger than
+ * one share is refused (SF_ENOMEM) at submit. */
+typedef struct sf_server sf_server;
+
+typedef struct {
+    uint64_t hbm_budget_bytes;    /* device memory for batches in flight, split over the slots */
+    uint32_t slots;               /* batches in flight at once (1..16), one HIP stream each */
+    uint32_t max_reads_per_batch; /* 0: limited by memory only; 1: one read per launch */
+    int32_t myers_mode;
+    int32_t affine_mode;
+    sf_scoring sc;
+    uint64_t band;
+    uint32_t fail_batch; /* test hook: the n-th batch (1-based) fails without a launch; 0 off */
+} sf_server_config;
+
+typedef struct {
+    const uint8_t *read; /* one code per byte */
+    uint32_t read_len;
+    uint32_t nseg;
+    const uint32_t *seg_read_off, *seg_read_len; /* read slice of each segment */
+    const uint64_t *seg_tx_off;                  /* text of each segment: store bases */
+    const uint32_t *seg_tx_len;
+    /* outputs, nseg each (nullable), written before the request completes; caller-owned */
+    uint32_t *dist, *end;
+    int32_t *score;
+    uint32_t *score_col;
+} sf_read_request;
+
+typedef struct {
+    uint64_t reads_done, reads_failed, batches, segments, cells;
+    double mean_reads_per_batch;
+    double p50_ms, p99_ms, max_ms; /* submit-to-completion latency of the completed reads */
+    double span_s;                 /* first submit to last completion */
+    uint64_t cap_segments_per_batch, cap_read_bases_per_batch;
+} sf_server_stats;
+
+/* store_words: the transcripts, 2-bit packed as in sf_pairs, store_bases long; copied to HBM. */
+int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf_server_config *cfg,
+                     sf_server **out);
+/* Queues one read; the request's input arrays may be reused after the call, its outputs must stay
+ * valid until sf_server_wait returns for the ticket. */
+int sf_server_submit(sf_server *s, const sf_read_request *req, uint64_t *ticket);
+/* Waits for a ticket (timeout_s < 0: forever); returns the request's status. */
+int sf_server_wait(sf_server *s, uint64_t ticket, double timeout_s);
+int sf_server_get_stats(sf_server *s, sf_server_stats *out, int reset);
+/* Drains the queue, then frees everything. */
+void sf_server_destroy(sf_server *s);
+/* Load generator for measurements: submits req[i] at t_arrive[i] seconds after the call, waits
+ * for all; *wall_s = first arrival to last completion, *refused = requests refused at admission
+ * (nullable).  Returns the first failure status of an admitted request, else SF_OK. */
+int sf_server_replay(sf_server *s, const sf_read_request *req, const double *t_arrive, uint64_t n, double *wall_s,
+                     uint64_t *refused);
+
 /* Diagnostic (no reference counterpart): runs a one-wave kernel that records the lane moves of
  * the DPP and shuffle encodings the kernels rely on, 5 x 64 values into `out` (320 int32). */
 int sf_selftest_lanes(int32_t *out);

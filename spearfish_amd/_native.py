@@ -54,6 +54,52 @@ class Timing(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_uint64),
+        ("slots", C.c_uint32),
+        ("max_reads_per_batch", C.c_uint32),
+        ("myers_mode", C.c_int32),
+        ("affine_mode", C.c_int32),
+        ("sc", Scoring),
+        ("band", C.c_uint64),
+        ("fail_batch", C.c_uint32),
+    ]
+
+
+class ReadRequest(C.Structure):
+    _fields_ = [
+        ("read", C.c_void_p),
+        ("read_len", C.c_uint32),
+        ("nseg", C.c_uint32),
+        ("seg_read_off", C.c_void_p),
+        ("seg_read_len", C.c_void_p),
+        ("seg_tx_off", C.c_void_p),
+        ("seg_tx_len", C.c_void_p),
+        ("dist", C.c_void_p),
+        ("end", C.c_void_p),
+        ("score", C.c_void_p),
+        ("score_col", C.c_void_p),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("reads_done", C.c_uint64),
+        ("reads_failed", C.c_uint64),
+        ("batches", C.c_uint64),
+        ("segments", C.c_uint64),
+        ("cells", C.c_uint64),
+        ("mean_reads_per_batch", C.c_double),
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("max_ms", C.c_double),
+        ("span_s", C.c_double),
+        ("cap_segments_per_batch", C.c_uint64),
+        ("cap_read_bases_per_batch", C.c_uint64),
+    ]
+
+
 # Every symbol include/sf_align.h declares, with its ctypes signature.
 _vp, _u32, _u64, _i32, _int = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_int
 SIGNATURES = {
@@ -75,6 +121,13 @@ SIGNATURES = {
     "sf_batch_affine_results": ([_vp, _vp, _vp, _vp, _vp, _u64, _vp], _int),
     "sf_batch_timing": ([_vp, C.POINTER(Timing)], _int),
     "sf_batch_reset_timing": ([_vp], None),
+    "sf_server_create": ([_vp, _u64, C.POINTER(ServerConfig), C.POINTER(_vp)], _int),
+    "sf_server_submit": ([_vp, C.POINTER(ReadRequest), C.POINTER(_u64)], _int),
+    "sf_server_wait": ([_vp, _u64, C.c_double], _int),
+    "sf_server_get_stats": ([_vp, C.POINTER(ServerStats), _int], _int),
+    "sf_server_destroy": ([_vp], None),
+    "sf_server_replay": ([_vp, C.POINTER(ReadRequest), C.POINTER(C.c_double), _u64, C.POINTER(C.c_double),
+                          C.POINTER(_u64)], _int),
     "sf_selftest_lanes": ([_vp], _int),
 }
 

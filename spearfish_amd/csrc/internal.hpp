@@ -178,3 +178,39 @@ hipError_t sf_launch_probe(int32_t *out, hipStream_t s);
 // Supported shape classes.
 int sf_myers_maxb_for(uint32_t nblocks);     // 0 if unsupported
 int sf_affine_k_for(uint64_t width);         // 0 if unsupported
+
+namespace sf {
+
+// Exact-equivalent band geometry of a pair (m, n): (below, above) of align.rs:2468-2481, with any
+// band that keeps every row at [0, n] (Infix, usize::MAX, or band >= max(m, n)) as (m, n).
+inline void band_geom(uint32_t m, uint32_t n, int mode, uint64_t band, uint64_t *below, uint64_t *above) {
+    const uint64_t mx = m > n ? m : n;
+    if (mode == kInfix || band == UINT64_MAX || band >= mx) {
+        *below = m;
+        *above = n;
+    } else {
+        *below = band + (m > n ? m - n : 0);
+        *above = band + (n > m ? n - m : 0);
+    }
+}
+
+// Cells of a pass as the reference's DP_WORK counts them (align.rs:2573-2578).
+inline uint64_t band_cells(uint32_t m, uint32_t n, uint64_t below, uint64_t above) {
+    uint64_t cells = 0;
+    for (uint64_t r = 1; r <= m; ++r) {
+        const uint64_t lo = r > below ? r - below : 0;
+        const uint64_t hi = r + above < n ? r + above : n;
+        if (lo > hi) break;
+        cells += hi + 1 - lo;
+    }
+    return cells;
+}
+
+// |score| bound of a band (the host check that keeps values far above NEG).
+inline bool scores_in_range(int32_t mat, int32_t mis, int32_t open, int32_t ext, uint64_t m, uint64_t n,
+                            uint64_t width) {
+    const uint64_t bound = (uint64_t)(mat + mis + ext) * (m + n + width + 2) + 2ull * (uint64_t)open;
+    return bound < (1ull << 28);
+}
+
+} // namespace sf

@@ -264,19 +264,6 @@ int timed_end(sf_batch *b, int cat, hipEvent_t a) {
     return SF_OK;
 }
 
-// Exact-equivalent band geometry of pair (m, n): (below, above) of align.rs:2468-2481, with any
-// band that keeps every row at [0, n] (Infix, usize::MAX, or band >= max(m, n)) as (m, n).
-inline void band_geom(uint32_t m, uint32_t n, int mode, uint64_t band, uint64_t *below, uint64_t *above) {
-    const uint64_t mx = std::max(m, n);
-    if (mode == SF_INFIX || band == SF_BAND_EXACT || band >= mx) {
-        *below = m;
-        *above = n;
-    } else {
-        *below = band + (m > n ? m - n : 0);
-        *above = band + (n > m ? n - m : 0);
-    }
-}
-
 __global__ void restrict_kernel(uint64_t n, const uint32_t *dist, const uint32_t *start, const uint32_t *end,
                                 const uint64_t *toff0, const uint32_t *tlen0, uint64_t *toff, uint32_t *tlen) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -291,6 +278,14 @@ __global__ void restrict_kernel(uint64_t n, const uint32_t *dist, const uint32_t
 }
 
 } // namespace
+
+// Hooks for the server (serve.hip).
+int sf_runtime_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+int sf_runtime_device() {
+    (void)ensure_device();
+    return g_device < 0 ? 0 : g_device;
+}
+int sf_runtime_ensure() { return ensure_device(); }
 
 extern "C" {
 
@@ -470,24 +465,18 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         for (uint64_t i = 0; i < n; ++i) {
             uint64_t below, above;
             const uint32_t m = b->plen[i], t = b->tlen[i];
-            band_geom(m, t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
+            sf::band_geom(m, t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
             const uint64_t width = below + above + 1;
             const int K = sf_affine_k_for(width);
             if (K == 0)
                 return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this build handles up to 2048",
                             (unsigned long long)i, (unsigned long long)width);
             // Values stay far above NEG: |score| <= (mat + mis + ext) * (m + n + width) + 2 open.
-            const uint64_t bound = (uint64_t)(sc->mat + sc->mis + sc->ext) * (m + t + width + 2) + 2ull * sc->open;
-            if (bound >= (1ull << 28)) return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
+            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t, width))
+                return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
             ba[i] = below | (above << 32);
             kcls[i] = (uint8_t)K;
-            // DP_WORK cells (align.rs:2573-2578).
-            for (uint64_t r = 1; r <= m; ++r) {
-                const uint64_t lo = r > below ? r - below : 0;
-                const uint64_t hi = std::min<uint64_t>(r + above, t);
-                if (lo > hi) break;
-                cells += hi + 1 - lo;
-            }
+            cells += sf::band_cells(m, t, below, above); // DP_WORK (align.rs:2573-2578)
         }
         std::vector<uint32_t> order(n);
         std::iota(order.begin(), order.end(), 0u);

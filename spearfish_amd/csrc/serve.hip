@@ -1,0 +1,496 @@
+// serve.hip -- continuous-batching alignment server (sf_server_* in include/sf_align.h).
+//
+// Reads arrive one request at a time (a read plus its segments: read slices against windows of
+// a transcript store that stays resident in HBM).  A scheduler thread keeps up to `slots`
+// batches on the device at once, each on its own HIP stream.  Whenever a slot is free it admits
+// the waiting requests, oldest first, while they fit the slot's share of the HBM budget (read
+// bases and segment count), packs them into the slot's pinned staging memory, copies them over,
+// launches the Myers and affine kernels for every segment and queues the copy of the results
+// back; requests that arrive meanwhile wait only for the next free slot, so their work overlaps
+// the batches already running.  A finished batch's results go straight into the caller's output
+// arrays and the caller's wait returns.  Per request the latency from submit to completion is
+// recorded; sf_server_get_stats reports p50/p99 next to the counts.
+//
+// Per segment: Myers Infix distance and end column (align.rs:92-168 semantics, k unbounded)
+// and the banded affine Global score and end column (align.rs:699-701, the server's band and
+// scoring) -- the same kernels as sf_batch_*, so a segment's results do not depend on which
+// other requests share its batch.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/sf_align.h"
+#include "internal.hpp"
+
+using namespace sf;
+using Clock = std::chrono::steady_clock;
+
+int sf_runtime_fail(int code, const char *msg); // runtime.hip
+int sf_runtime_device();                        // runtime.hip
+int sf_runtime_ensure();                        // runtime.hip
+
+namespace {
+
+struct Req {
+    uint64_t ticket = 0;
+    std::vector<uint32_t> words; // the read, 2-bit packed
+    std::vector<uint32_t> roff, rlen, tlen;
+    std::vector<uint64_t> toff;
+    uint32_t *dist = nullptr, *end = nullptr, *col = nullptr;
+    int32_t *score = nullptr;
+    Clock::time_point t_submit, t_done;
+    int status = SF_OK;
+    bool done = false;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    uint64_t words_base = 0; // word offset of this slot's read area in the device sequence buffer
+    // device
+    uint64_t *d_poff = nullptr, *d_toff = nullptr, *d_ba = nullptr;
+    uint32_t *d_plen = nullptr, *d_tlen = nullptr, *d_morder = nullptr, *d_aorder = nullptr;
+    uint32_t *d_res = nullptr; // dist | end | score | col, 4 x cap_seg
+    // host (pinned)
+    uint32_t *h_words = nullptr;
+    uint64_t *h_poff = nullptr, *h_toff = nullptr, *h_ba = nullptr;
+    uint32_t *h_plen = nullptr, *h_tlen = nullptr, *h_morder = nullptr, *h_aorder = nullptr;
+    uint32_t *h_res = nullptr;
+    std::vector<Req *> reqs;
+    std::vector<uint32_t> seg0;
+    uint32_t nseg = 0;
+    bool busy = false;
+};
+
+template <class T>
+bool dev_alloc(T **p, uint64_t count) {
+    return hipMalloc((void **)p, std::max<uint64_t>(count, 1) * sizeof(T)) == hipSuccess;
+}
+template <class T>
+bool host_alloc(T **p, uint64_t count) {
+    return hipHostMalloc((void **)p, std::max<uint64_t>(count, 1) * sizeof(T), hipHostMallocDefault) == hipSuccess;
+}
+
+double pct(std::vector<double> v, double q) {
+    if (v.empty()) return 0;
+    std::sort(v.begin(), v.end());
+    const double pos = q * (double)(v.size() - 1);
+    const size_t lo = (size_t)pos;
+    const size_t hi = std::min(lo + 1, v.size() - 1);
+    return v[lo] + (v[hi] - v[lo]) * (pos - (double)lo);
+}
+
+} // namespace
+
+struct sf_server {
+    sf_server_config cfg{};
+    uint64_t store_bases = 0, store_words = 0;
+    uint32_t *d_seq = nullptr; // [store words | slot read areas]
+    uint64_t cap_words = 0, cap_seg = 0;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_sched, cv_done;
+    std::deque<Req *> pending;
+    std::unordered_map<uint64_t, Req *> reqs;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::thread th;
+    // stats
+    std::vector<double> lat_ms;
+    uint64_t reads_done = 0, reads_failed = 0, batches = 0, segments = 0, batch_reads = 0;
+    uint64_t cells = 0;
+    Clock::time_point t_first, t_last;
+    bool any = false;
+
+    void loop();
+    void launch(Slot &s, std::vector<Req *> &batch);
+    void finish(Slot &s, int status);
+};
+
+void sf_server::finish(Slot &s, int status) {
+    const auto now = Clock::now();
+    const uint32_t *rd = s.h_res, *re = s.h_res + cap_seg, *rc = s.h_res + 3 * cap_seg;
+    const int32_t *rs = reinterpret_cast<const int32_t *>(s.h_res + 2 * cap_seg);
+    for (size_t q = 0; q < s.reqs.size(); ++q) {
+        Req *r = s.reqs[q];
+        const uint32_t k0 = s.seg0[q];
+        if (status == SF_OK) {
+            for (uint32_t k = 0; k < (uint32_t)r->rlen.size(); ++k) {
+                if (r->dist) r->dist[k] = rd[k0 + k];
+                if (r->end) r->end[k] = re[k0 + k];
+                if (r->score) r->score[k] = rs[k0 + k];
+                if (r->col) r->col[k] = rc[k0 + k];
+            }
+            ++reads_done;
+            lat_ms.push_back(std::chrono::duration<double, std::milli>(now - r->t_submit).count());
+        } else {
+            ++reads_failed;
+        }
+        r->status = status;
+        r->t_done = now;
+        r->done = true;
+    }
+    t_last = now;
+    s.reqs.clear();
+    s.seg0.clear();
+    s.busy = false;
+    cv_done.notify_all();
+}
+
+void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
+    // Stage: reads, then one descriptor per segment.
+    uint64_t words = 0;
+    uint32_t seg = 0;
+    s.reqs = batch;
+    s.seg0.clear();
+    for (Req *r : batch) {
+        std::memcpy(s.h_words + words, r->words.data(), r->words.size() * 4);
+        const uint64_t rbase = (store_words + s.words_base + words) * 16;
+        s.seg0.push_back(seg);
+        for (size_t k = 0; k < r->rlen.size(); ++k, ++seg) {
+            s.h_poff[seg] = rbase + r->roff[k];
+            s.h_plen[seg] = r->rlen[k];
+            s.h_toff[seg] = r->toff[k];
+            s.h_tlen[seg] = r->tlen[k];
+        }
+        words += r->words.size();
+    }
+    s.nseg = seg;
+    // Shape classes (as sf_batch_*): Myers by pattern blocks, affine by band width.
+    std::vector<int> mcls(seg), acls(seg);
+    for (uint32_t i = 0; i < seg; ++i) {
+        mcls[i] = sf_myers_maxb_for((s.h_plen[i] + 63) / 64);
+        uint64_t below, above;
+        band_geom(s.h_plen[i], s.h_tlen[i], cfg.affine_mode, cfg.band, &below, &above);
+        s.h_ba[i] = below | (above << 32);
+        acls[i] = sf_affine_k_for(below + above + 1);
+        cells += (uint64_t)s.h_plen[i] * s.h_tlen[i] + band_cells(s.h_plen[i], s.h_tlen[i], below, above);
+    }
+    std::iota(s.h_morder, s.h_morder + seg, 0u);
+    std::iota(s.h_aorder, s.h_aorder + seg, 0u);
+    std::stable_sort(s.h_morder, s.h_morder + seg, [&](uint32_t x, uint32_t y) { return mcls[x] < mcls[y]; });
+    std::stable_sort(s.h_aorder, s.h_aorder + seg, [&](uint32_t x, uint32_t y) { return acls[x] < acls[y]; });
+    hipStream_t st = s.stream;
+    int status = SF_OK;
+    auto ok = [&](hipError_t e) {
+        if (e != hipSuccess && status == SF_OK) status = SF_EDEVICE;
+    };
+    ok(hipMemcpyAsync(d_seq + store_words + s.words_base, s.h_words, words * 4, hipMemcpyHostToDevice, st));
+    ok(hipMemcpyAsync(s.d_poff, s.h_poff, seg * 8, hipMemcpyHostToDevice, st));
+    ok(hipMemcpyAsync(s.d_toff, s.h_toff, seg * 8, hipMemcpyHostToDevice, st));
+    ok(hipMemcpyAsync(s.d_plen, s.h_plen, seg * 4, hipMemcpyHostToDevice, st));
+    ok(hipMemcpyAsync(s.d_tlen, s.h_tlen, seg * 4, hipMemcpyHostToDevice, st));
+    ok(hipMemcpyAsync(s.d_ba, s.h_ba, seg * 8, hipMemcpyHostToDevice, st));
+    ok(hipMemcpyAsync(s.d_morder, s.h_morder, seg * 4, hipMemcpyHostToDevice, st));
+    ok(hipMemcpyAsync(s.d_aorder, s.h_aorder, seg * 4, hipMemcpyHostToDevice, st));
+    const PairsView pv{d_seq, s.d_poff, s.d_toff, s.d_plen, s.d_tlen};
+    uint32_t *dist = s.d_res, *end = s.d_res + cap_seg, *col = s.d_res + 3 * cap_seg;
+    int32_t *score = reinterpret_cast<int32_t *>(s.d_res + 2 * cap_seg);
+    const bool inject = cfg.fail_batch && batches + 1 == cfg.fail_batch;
+    if (!inject) {
+        for (uint32_t a = 0; a < seg;) {
+            uint32_t e = a;
+            while (e < seg && mcls[s.h_morder[e]] == mcls[s.h_morder[a]]) ++e;
+            MyersArgs ma{};
+            ma.pv = pv;
+            ma.order = s.d_morder + a;
+            ma.count = e - a;
+            ma.mode = cfg.myers_mode;
+            ma.k = UINT32_MAX;
+            ma.dist = dist;
+            ma.end = end;
+            ok(sf_launch_myers(ma, mcls[s.h_morder[a]], st));
+            a = e;
+        }
+        for (uint32_t a = 0; a < seg;) {
+            uint32_t e = a;
+            while (e < seg && acls[s.h_aorder[e]] == acls[s.h_aorder[a]]) ++e;
+            AffineArgs aa{};
+            aa.pv = pv;
+            aa.order = s.d_aorder + a;
+            aa.below_above = s.d_ba;
+            aa.count = e - a;
+            aa.mode = cfg.affine_mode;
+            aa.mat = cfg.sc.mat, aa.mis = cfg.sc.mis, aa.open = cfg.sc.open, aa.ext = cfg.sc.ext;
+            aa.score = score;
+            aa.score_col = col;
+            ok(sf_launch_affine(aa, acls[s.h_aorder[a]], false, st));
+            a = e;
+        }
+    } else {
+        status = SF_EDEVICE; // fault injected from the host: the batch is failed without a launch
+    }
+    for (int q = 0; q < 4; ++q)
+        ok(hipMemcpyAsync(s.h_res + q * cap_seg, s.d_res + q * cap_seg, seg * 4, hipMemcpyDeviceToHost, st));
+    ok(hipEventRecord(s.ev, st));
+    ++batches;
+    segments += seg;
+    batch_reads += batch.size();
+    s.busy = true;
+    if (status != SF_OK) {
+        (void)hipStreamSynchronize(st);
+        finish(s, status);
+    }
+}
+
+void sf_server::loop() {
+    (void)hipSetDevice(sf_runtime_device());
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        bool progress = false;
+        for (Slot &s : slots) {
+            if (s.busy) {
+                const hipError_t q = hipEventQuery(s.ev);
+                if (q == hipSuccess) {
+                    finish(s, SF_OK);
+                    progress = true;
+                } else if (q != hipErrorNotReady) {
+                    (void)hipGetLastError();
+                    finish(s, SF_EDEVICE);
+                    progress = true;
+                }
+            }
+        }
+        if (!pending.empty()) {
+            for (Slot &s : slots) {
+                if (s.busy || pending.empty()) continue;
+                std::vector<Req *> batch;
+                uint64_t words = 0, seg = 0;
+                while (!pending.empty()) {
+                    Req *r = pending.front();
+                    if (!batch.empty() && (words + r->words.size() > cap_words || seg + r->rlen.size() > cap_seg ||
+                                           (cfg.max_reads_per_batch && batch.size() >= cfg.max_reads_per_batch)))
+                        break;
+                    words += r->words.size();
+                    seg += r->rlen.size();
+                    batch.push_back(r);
+                    pending.pop_front();
+                }
+                launch(s, batch);
+                progress = true;
+            }
+        }
+        bool busy = false;
+        for (Slot &s : slots) busy |= s.busy;
+        if (stop && pending.empty() && !busy) break;
+        if (!progress) {
+            if (busy) cv_sched.wait_for(lk, std::chrono::microseconds(20));
+            else cv_sched.wait(lk, [&] { return stop || !pending.empty(); });
+        }
+    }
+}
+
+extern "C" {
+
+int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf_server_config *cfg,
+                     sf_server **out) {
+    if (!out || !cfg) return sf_runtime_fail(SF_EINVAL, "null argument");
+    *out = nullptr;
+    if (const int st = sf_runtime_ensure()) return st;
+    if (store_bases && !store_words) return sf_runtime_fail(SF_EINVAL, "store is null");
+    if (cfg->slots == 0 || cfg->slots > 16) return sf_runtime_fail(SF_EINVAL, "slots must be 1..16");
+    if (cfg->myers_mode < SF_INFIX || cfg->myers_mode > SF_GLOBAL || cfg->affine_mode < SF_INFIX ||
+        cfg->affine_mode > SF_GLOBAL)
+        return sf_runtime_fail(SF_EINVAL, "mode");
+    if (cfg->sc.mat < 0 || cfg->sc.mis < 0 || cfg->sc.open < 0 || cfg->sc.ext < 0 || cfg->sc.mat > 65535 ||
+        cfg->sc.mis > 65535 || cfg->sc.open > 65535 || cfg->sc.ext > 65535)
+        return sf_runtime_fail(SF_EINVAL, "scores must be in 0..65535");
+    // Per slot: half the share for read bases (4 B per 16 bases), half for segments (64 B each).
+    const uint64_t share = cfg->hbm_budget_bytes / cfg->slots;
+    if (share < (1u << 16)) return sf_runtime_fail(SF_EINVAL, "hbm budget too small for the slots");
+    auto *sv = new sf_server();
+    sv->cfg = *cfg;
+    sv->store_bases = store_bases;
+    sv->store_words = (store_bases + 15) / 16;
+    sv->cap_words = share / 2 / 4;
+    sv->cap_seg = share / 2 / 64;
+    auto bail = [&](const char *m) {
+        delete sv;
+        return sf_runtime_fail(SF_ENOMEM, m);
+    };
+    if (!dev_alloc(&sv->d_seq, sv->store_words + cfg->slots * sv->cap_words)) return bail("device sequence buffer");
+    if (sv->store_words && hipMemcpy(sv->d_seq, store_words, sv->store_words * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return bail("store upload");
+    sv->slots.resize(cfg->slots);
+    for (uint32_t q = 0; q < cfg->slots; ++q) {
+        Slot &s = sv->slots[q];
+        const uint64_t c = sv->cap_seg;
+        s.words_base = q * sv->cap_words;
+        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess)
+            return bail("stream");
+        if (!dev_alloc(&s.d_poff, c) || !dev_alloc(&s.d_toff, c) || !dev_alloc(&s.d_ba, c) || !dev_alloc(&s.d_plen, c) ||
+            !dev_alloc(&s.d_tlen, c) || !dev_alloc(&s.d_morder, c) || !dev_alloc(&s.d_aorder, c) ||
+            !dev_alloc(&s.d_res, 4 * c))
+            return bail("slot device buffers");
+        if (!host_alloc(&s.h_words, sv->cap_words) || !host_alloc(&s.h_poff, c) || !host_alloc(&s.h_toff, c) ||
+            !host_alloc(&s.h_ba, c) || !host_alloc(&s.h_plen, c) || !host_alloc(&s.h_tlen, c) ||
+            !host_alloc(&s.h_morder, c) || !host_alloc(&s.h_aorder, c) || !host_alloc(&s.h_res, 4 * c))
+            return bail("slot pinned buffers");
+    }
+    sv->th = std::thread([sv] { sv->loop(); });
+    *out = sv;
+    return SF_OK;
+}
+
+int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket) {
+    if (!sv || !rq || !ticket) return sf_runtime_fail(SF_EINVAL, "null argument");
+    if (rq->nseg && (!rq->seg_read_off || !rq->seg_read_len || !rq->seg_tx_off || !rq->seg_tx_len))
+        return sf_runtime_fail(SF_EINVAL, "null segment arrays");
+    if (rq->read_len && !rq->read) return sf_runtime_fail(SF_EINVAL, "read is null");
+    auto *r = new Req();
+    r->roff.assign(rq->seg_read_off, rq->seg_read_off + rq->nseg);
+    r->rlen.assign(rq->seg_read_len, rq->seg_read_len + rq->nseg);
+    r->toff.assign(rq->seg_tx_off, rq->seg_tx_off + rq->nseg);
+    r->tlen.assign(rq->seg_tx_len, rq->seg_tx_len + rq->nseg);
+    for (uint32_t k = 0; k < rq->nseg; ++k) {
+        const uint32_t m = r->rlen[k], n = r->tlen[k];
+        bool bad = (uint64_t)r->roff[k] + m > rq->read_len || r->toff[k] + n > sv->store_bases;
+        uint64_t below, above;
+        band_geom(m, n, sv->cfg.affine_mode, sv->cfg.band, &below, &above);
+        const bool shape = sf_myers_maxb_for((m + 63) / 64) && sf_affine_k_for(below + above + 1) &&
+                           scores_in_range(sv->cfg.sc.mat, sv->cfg.sc.mis, sv->cfg.sc.open, sv->cfg.sc.ext, m, n,
+                                           below + above + 1);
+        if (bad || !shape) {
+            delete r;
+            return sf_runtime_fail(bad ? SF_EINVAL : SF_ERANGE, bad ? "segment outside the read or the store"
+                                                                    : "segment beyond this build's shape classes");
+        }
+    }
+    r->words.assign((rq->read_len + 15) / 16, 0u);
+    for (uint32_t k = 0; k < rq->read_len; ++k) r->words[k >> 4] |= (uint32_t)(rq->read[k] & 3u) << (2 * (k & 15));
+    if (r->words.size() > sv->cap_words || r->rlen.size() > sv->cap_seg) {
+        delete r;
+        return sf_runtime_fail(SF_ENOMEM, "request exceeds one slot's share of the HBM budget");
+    }
+    r->dist = rq->dist;
+    r->end = rq->end;
+    r->score = rq->score;
+    r->col = rq->score_col;
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (sv->stop) {
+            delete r;
+            return sf_runtime_fail(SF_EINVAL, "server is shutting down");
+        }
+        r->ticket = sv->next_ticket++;
+        sv->reqs[r->ticket] = r;
+        sv->pending.push_back(r);
+        if (!sv->any) {
+            sv->any = true;
+            sv->t_first = r->t_submit;
+        }
+        *ticket = r->ticket;
+    }
+    sv->cv_sched.notify_one();
+    return SF_OK;
+}
+
+int sf_server_wait(sf_server *sv, uint64_t ticket, double timeout_s) {
+    if (!sv) return sf_runtime_fail(SF_EINVAL, "server is null");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return sf_runtime_fail(SF_EINVAL, "unknown ticket");
+    Req *r = it->second;
+    auto pred = [&] { return r->done; };
+    if (timeout_s < 0) sv->cv_done.wait(lk, pred);
+    else if (!sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
+        return sf_runtime_fail(SF_ERANGE, "timed out");
+    const int st = r->status;
+    sv->reqs.erase(it);
+    delete r;
+    if (st != SF_OK) return sf_runtime_fail(st, "the request's batch failed");
+    return SF_OK;
+}
+
+int sf_server_get_stats(sf_server *sv, sf_server_stats *out, int reset) {
+    if (!sv || !out) return sf_runtime_fail(SF_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    std::memset(out, 0, sizeof *out);
+    out->reads_done = sv->reads_done;
+    out->reads_failed = sv->reads_failed;
+    out->batches = sv->batches;
+    out->segments = sv->segments;
+    out->cells = sv->cells;
+    out->mean_reads_per_batch = sv->batches ? (double)sv->batch_reads / (double)sv->batches : 0;
+    out->p50_ms = pct(sv->lat_ms, 0.50);
+    out->p99_ms = pct(sv->lat_ms, 0.99);
+    out->max_ms = sv->lat_ms.empty() ? 0 : *std::max_element(sv->lat_ms.begin(), sv->lat_ms.end());
+    out->span_s = sv->any ? std::chrono::duration<double>(sv->t_last - sv->t_first).count() : 0;
+    out->cap_segments_per_batch = sv->cap_seg;
+    out->cap_read_bases_per_batch = sv->cap_words * 16;
+    if (reset) {
+        sv->lat_ms.clear();
+        sv->reads_done = sv->reads_failed = sv->batches = sv->segments = sv->batch_reads = sv->cells = 0;
+        sv->any = false;
+    }
+    return SF_OK;
+}
+
+void sf_server_destroy(sf_server *sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_sched.notify_all();
+    if (sv->th.joinable()) sv->th.join();
+    for (auto &kv : sv->reqs) delete kv.second;
+    for (Slot &s : sv->slots) {
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
+        (void)hipFree(s.d_poff), (void)hipFree(s.d_toff), (void)hipFree(s.d_ba), (void)hipFree(s.d_plen);
+        (void)hipFree(s.d_tlen), (void)hipFree(s.d_morder), (void)hipFree(s.d_aorder), (void)hipFree(s.d_res);
+        (void)hipHostFree(s.h_words), (void)hipHostFree(s.h_poff), (void)hipHostFree(s.h_toff);
+        (void)hipHostFree(s.h_ba), (void)hipHostFree(s.h_plen), (void)hipHostFree(s.h_tlen);
+        (void)hipHostFree(s.h_morder), (void)hipHostFree(s.h_aorder), (void)hipHostFree(s.h_res);
+        if (s.ev) (void)hipEventDestroy(s.ev);
+        if (s.stream) (void)hipStreamDestroy(s.stream);
+    }
+    (void)hipFree(sv->d_seq);
+    delete sv;
+}
+
+// Replays a request stream: request i is submitted at t_arrive[i] seconds after the call starts
+// (a submitter thread sleeps until each arrival), then every request is waited for.  Returns the
+// wall time from the first arrival to the last completion in *wall_s and the number of requests
+// refused at admission in *refused.  Latency statistics are the server's own (sf_server_get_stats).
+int sf_server_replay(sf_server *sv, const sf_read_request *rq, const double *t_arrive, uint64_t n, double *wall_s,
+                     uint64_t *refused) {
+    if (!sv || (n && (!rq || !t_arrive))) return sf_runtime_fail(SF_EINVAL, "null argument");
+    std::vector<uint64_t> tickets(n, 0);
+    std::vector<int> sub(n, SF_OK);
+    const auto t0 = Clock::now();
+    std::thread submitter([&] {
+        for (uint64_t i = 0; i < n; ++i) {
+            const auto due = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(t_arrive[i]));
+            std::this_thread::sleep_until(due);
+            sub[i] = sf_server_submit(sv, &rq[i], &tickets[i]);
+        }
+    });
+    submitter.join();
+    int status = SF_OK;
+    uint64_t nref = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (sub[i] != SF_OK) {
+            ++nref; // refused at admission: counted, not fatal
+            continue;
+        }
+        const int st = sf_server_wait(sv, tickets[i], -1.0);
+        if (st != SF_OK) status = st;
+    }
+    if (refused) *refused = nref;
+    const auto t1 = Clock::now();
+    if (wall_s) *wall_s = std::chrono::duration<double>(t1 - t0).count() - (n ? t_arrive[0] : 0.0);
+    return status;
+}
+
+} // extern "C"

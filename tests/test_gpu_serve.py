@@ -1,0 +1,83 @@
+"""The serving loop on the device: each read's results are the same alone and in the mix (and
+equal the oracle's), oversized requests are refused by the HBM budget, and a failed batch fails
+only its own reads (the failure is injected from the host; no kernel is made to fault)."""
+import numpy as np
+import pytest
+
+from oracle import sf_oracle as O
+from spearfish_amd import _native as N
+from spearfish_amd import serve as S
+
+pytestmark = pytest.mark.gpu
+
+BAND = 64
+
+
+@pytest.fixture(scope="module")
+def workload():
+    return S.generate_serve_workload(300, seed=7)
+
+
+def store_codes(store):
+    w = store.words
+    k = np.arange(store.bases, dtype=np.int64)
+    return ((w[k >> 4] >> (2 * (k & 15)).astype(np.uint32)) & 3).astype(np.uint8)
+
+
+def test_results_alone_equal_in_mix_and_oracle(workload):
+    store, reads = workload
+    n = len(reads)
+    alone, mix = S.Results(reads), S.Results(reads)
+    reqs_alone = S.requests(reads, alone)
+    with S.Server(store, band=BAND) as s:
+        for i in range(n):
+            t = s.submit(reqs_alone[i])
+            assert s.wait(t) == N.SF_OK
+        st = s.stats(reset=True)
+        assert st.reads_done == n and st.batches == n
+        reqs_mix = S.requests(reads, mix)
+        s.replay(reqs_mix, S.poisson_arrivals(n, 200000.0, seed=1))
+        st = s.stats()
+        assert st.reads_done == n and st.batches < n  # reads shared batches
+    for a in ("dist", "end", "score", "score_col"):
+        assert np.array_equal(getattr(alone, a), getattr(mix, a)), a
+    codes = store_codes(store)
+    for i in range(0, n, 7):
+        s0 = int(reads.seg0[i])
+        r = reads.codes[int(reads.read_off[i]):int(reads.read_off[i]) + int(reads.read_len[i])]
+        for k in range(int(reads.nseg[i])):
+            q = s0 + k
+            p = r[int(reads.seg_read_off[q]):int(reads.seg_read_off[q]) + int(reads.seg_read_len[q])]
+            t = codes[int(reads.seg_tx_off[q]):int(reads.seg_tx_off[q]) + int(reads.seg_tx_len[q])]
+            d, e, _ = O.locate(p, t, O.INFIX)
+            assert (int(mix.dist[q]), int(mix.end[q])) == (d, e)
+            dp = O.affine_dp(p, t, O.GLOBAL, O.ONT, BAND)
+            assert (int(mix.score[q]), int(mix.score_col[q])) == (dp.score, dp.score_col)
+
+
+def test_admission_refuses_requests_over_the_budget(workload):
+    store, reads = workload
+    res = S.Results(reads)
+    reqs = S.requests(reads, res)
+    big = int(np.argmax(reads.read_len))
+    # A share of 64 KiB per slot holds 128 K bases and 512 segments: the longest read still fits,
+    # a read whose bases exceed it is refused.
+    with S.Server(store, hbm_budget_bytes=1 << 17, slots=2, band=BAND) as s:
+        t = s.submit(reqs[big])
+        assert s.wait(t) == N.SF_OK
+        huge = np.zeros(200_000, np.uint8)
+        r = N.ReadRequest(huge.ctypes.data, len(huge), 0, None, None, None, None, None, None, None, None)
+        assert s.submit_status(r) == N.SF_ENOMEM
+
+
+def test_injected_batch_failure_fails_only_its_reads(workload):
+    store, reads = workload
+    res = S.Results(reads)
+    reqs = S.requests(reads, res, idx=range(6))
+    with S.Server(store, slots=1, max_reads_per_batch=1, band=BAND, fail_batch=3) as s:
+        status = []
+        for i in range(6):
+            status.append(s.wait(s.submit(reqs[i])))
+        assert status == [N.SF_OK, N.SF_OK, N.SF_EDEVICE, N.SF_OK, N.SF_OK, N.SF_OK]
+        st = s.stats()
+        assert st.reads_done == 5 and st.reads_failed == 1
