@@ -57,7 +57,8 @@ __device__ __forceinline__ uint32_t push_gt(uint32_t acc, int32_t a, int32_t b) 
 template <int K, bool TB>
 __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
     constexpr int NWL = kTbWords<K>;
-    constexpr int REFRESH = 32 - K + 1; // rows served by one 32-code text window
+    constexpr int BLK = 16;                      // rows per block: one pattern word
+    constexpr int NWIN = (BLK + K - 1 + 15) / 16; // text window words: codes of the block's slots
     const int lane = threadIdx.x & 63;
     const uint32_t w = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (w >= a.count) return; // whole waves only
@@ -87,88 +88,79 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
         Fb[q] = kNeg;
         Ub[q] = Hb[q] + dlt;
     }
-    // The sentinel slot Wd: lane and slot-in-lane (none when the band fills the wave).
-    const int32_t sent_lane = Wd < 64 * K ? Wd / K : -1;
-    const int32_t sent_q = Wd < 64 * K ? Wd % K : -1;
-    uint64_t win = 0;
-    uint32_t pw = 0;
+    // Slots s >= Wd lie right of the band.  Their up-values Ub carry an extra -2^30, so every
+    // up-value above the band stays below NEG (|Hb| < 2^28 over all 64K slots, checked on the
+    // host) and F there is exactly NEG: the last band slot sees the same NEG up-neighbour as
+    // align.rs:2548-2556 gives it, with no per-row reset.
+    int32_t dltv[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) dltv[q] = s0 + q >= Wd ? dlt - (1 << 30) : dlt;
+    // Row i has no band cell once i - below > n (align.rs:2543-2556); only a band laid out for a
+    // longer text than this one gets there.
+    const int32_t last = m < n + below ? m : n + below;
     uint32_t *tbp = nullptr;
     if constexpr (TB) tbp = a.tb + a.tb_off[w] + lane;
-    bool stopped = false;
-    int32_t refresh = 0;
-    for (int32_t i = 1; i <= m; ++i) {
-        const int32_t b = i - below;
-        const int32_t lo = b > 0 ? b : 0;
-        const int32_t hi = (b + Wd - 1) < n ? (b + Wd - 1) : n;
-        if (lo > hi) {
-            stopped = true; // align.rs:2543-2556
-            break;
-        }
-        if (((i - 1) & 15) == 0) pw = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)codes16(P, i - 1));
-        const uint32_t pc = pw & 3u;
-        pw >>= 2;
-        if (refresh == 0) {
-            const int32_t pos = b + s0 - 1; // text base of this lane's first slot
-            win = (uint64_t)codes16(T, pos) | ((uint64_t)codes16(T, pos + 16) << 32);
-            refresh = REFRESH;
-        }
-        --refresh;
-        const int32_t Un = dpp<kWaveShl1>(kNeg, Ub[0]);
-        const int32_t Fn = dpp<kWaveShl1>(kNeg, Fb[0]);
-        int32_t X[K], FQ[K], G[K], D[K], Uu[K];
+    for (int32_t i0 = 1; i0 <= last; i0 += BLK) {
+        uint32_t pw = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)codes16(P, i0 - 1));
+        uint32_t win[NWIN];
+        const int32_t pos = i0 - below + s0 - 1; // text base of this lane's first slot in row i0
 #pragma unroll
-        for (int q = 0; q < K; ++q) {
-            const int32_t uu = q < K - 1 ? Ub[q + 1] : Un;
-            const int32_t fu = q < K - 1 ? Fb[q + 1] : Fn;
-            const uint32_t code = (uint32_t)(win >> (2 * q)) & 3u;
-            const int32_t sc = code == pc ? scm : scx;
-            const int32_t f = imax(uu, fu);
-            const int32_t d = Hb[q] + sc;
-            const int32_t x = imax(d, f);
-            X[q] = x;
-            FQ[q] = f;
-            D[q] = d;
-            Uu[q] = uu;
-            const int32_t g = x + dlt;
-            G[q] = q ? imax(G[q - 1], g) : g;
-        }
-        const int32_t C = wave_excl_max_min(G[K - 1]);
-        int32_t E[K];
+        for (int k = 0; k < NWIN; ++k) win[k] = codes16(T, pos + 16 * k);
+        const int32_t cnt = last - i0 + 1 < BLK ? last - i0 + 1 : BLK;
+        for (int32_t r = 0; r < cnt; ++r) {
+            const uint32_t pc = pw & 3u;
+            pw >>= 2;
+            const int32_t Un = dpp<kWaveShl1>(kNeg, Ub[0]);
+            const int32_t Fn = dpp<kWaveShl1>(kNeg, Fb[0]);
+            int32_t X[K], FQ[K], G[K], D[K], Uu[K];
 #pragma unroll
-        for (int q = 0; q < K; ++q) {
-            E[q] = q ? imax(C, G[q - 1]) : C; // biased horizontal-gap value entering slot q
-            Hb[q] = imax(X[q], E[q]);
-            Fb[q] = FQ[q];
-        }
-        if (sent_q >= 0) {
-#pragma unroll
-            for (int q = 0; q < K; ++q)
-                if (q == sent_q) {
-                    Hb[q] = lane == sent_lane ? kNeg : Hb[q];
-                    Fb[q] = lane == sent_lane ? kNeg : Fb[q];
-                }
-        }
-#pragma unroll
-        for (int q = 0; q < K; ++q) Ub[q] = Hb[q] + dlt;
-        if constexpr (TB) {
-            const int32_t ul0 = dpp<kWaveShr1>(kNeg, Ub[K - 1]);
-            uint32_t *row = tbp + (uint32_t)(i - 1) * (64 * NWL);
-#pragma unroll
-            for (int wd = 0; wd < NWL; ++wd) {
-                uint32_t acc = 0;
-#pragma unroll
-                for (int q = wd * 8; q < K && q < wd * 8 + 8; ++q) {
-                    const int32_t ul = q ? Ub[q - 1] : ul0;
-                    acc = push_ne(acc, Hb[q], D[q]);
-                    acc = push_ne(acc, Hb[q], X[q]);
-                    acc = push_gt(acc, FQ[q], Uu[q]);
-                    acc = push_gt(acc, E[q], ul);
-                }
-                row[wd * 64] = acc;
+            for (int q = 0; q < K; ++q) {
+                const int32_t uu = q < K - 1 ? Ub[q + 1] : Un;
+                const int32_t fu = q < K - 1 ? Fb[q + 1] : Fn;
+                const uint32_t code = (win[q / 16] >> (2 * (q % 16))) & 3u;
+                const int32_t sc = code == pc ? scm : scx;
+                const int32_t f = imax(uu, fu);
+                const int32_t d = Hb[q] + sc;
+                const int32_t x = imax(d, f);
+                X[q] = x;
+                FQ[q] = f;
+                D[q] = d;
+                Uu[q] = uu;
+                const int32_t g = x + dlt;
+                G[q] = q ? imax(G[q - 1], g) : g;
             }
+            const int32_t C = wave_excl_max_min(G[K - 1]);
+            int32_t E[K];
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                E[q] = q ? imax(C, G[q - 1]) : C; // biased horizontal-gap value entering slot q
+                Hb[q] = imax(X[q], E[q]);
+                Fb[q] = FQ[q];
+            }
+#pragma unroll
+            for (int q = 0; q < K; ++q) Ub[q] = Hb[q] + dltv[q];
+            if constexpr (TB) {
+                const int32_t ul0 = dpp<kWaveShr1>(kNeg, Ub[K - 1]);
+                uint32_t *row = tbp + (uint32_t)(i0 + r - 1) * (64 * NWL);
+#pragma unroll
+                for (int wd = 0; wd < NWL; ++wd) {
+                    uint32_t acc = 0;
+#pragma unroll
+                    for (int q = wd * 8; q < K && q < wd * 8 + 8; ++q) {
+                        const int32_t ul = q ? Ub[q - 1] : ul0;
+                        acc = push_ne(acc, Hb[q], D[q]);
+                        acc = push_ne(acc, Hb[q], X[q]);
+                        acc = push_gt(acc, FQ[q], Uu[q]);
+                        acc = push_gt(acc, E[q], ul);
+                    }
+                    row[wd * 64] = acc;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NWIN; ++k) win[k] = k + 1 < NWIN ? __builtin_amdgcn_alignbit(win[k + 1], win[k], 2) : win[k] >> 2;
         }
-        win >>= 2;
     }
+    const bool stopped = last < m;
     int32_t score;
     uint32_t col;
     const int32_t rowbias = 2 * ext * m;
@@ -219,7 +211,16 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
 
 // One lane per pair: walks the traceback bits from the end cell back to the start, emitting
 // BAM ops last to first (rules in the header comment of affine_wave_kernel and the oracle).
+// Each step reads one u32 -- the word of the cell's lane for that row, which holds all K slots of
+// the lane -- so the walk keeps the last 16 rows of its current lane-word in LDS: diagonal,
+// vertical and horizontal moves stay inside it until the path crosses to another lane's slots
+// or 16 rows up, and a refill issues its 16 loads at once.  Pattern and text codes are read a
+// word (16 bases) at a time.
+constexpr int kWalkRows = 16;
+
 __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
+    __shared__ uint32_t cache[256 * (kWalkRows + 1)];
+    uint32_t *my = cache + threadIdx.x * (kWalkRows + 1);
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= a.count) return;
     const uint32_t pi = a.order[t];
@@ -234,6 +235,7 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
     const SeqRef P = seqref(a.pv.seq, a.pv.poff[pi], (uint32_t)m);
     const SeqRef T = seqref(a.pv.seq, a.pv.toff[pi], (uint32_t)n);
     const int K = a.K;
+    const int64_t nwl = (4 * K + 31) / 32;
     const uint32_t *tb = a.tb + a.tb_off[t];
     int64_t i = m, j = a.mode == kGlobal ? n : (int64_t)a.score_col[pi];
     int state = 0; // 0 H, 1 F (vertical), 2 E (horizontal)
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
         }
     };
     bool bad = false;
-    const int64_t nwl = (4 * K + 31) / 32;
+    int64_t cL = -1, cwd = -1, clo = 0; // cached lane-word and its first row
     // The 4 bits (nd, ne, fx, ex) of cell (ii, jj) in bits 3..0 of the result.
     auto bits_at = [&](int64_t ii, int64_t jj) -> uint32_t {
         const int64_t s = jj - (ii - below);
@@ -261,9 +263,23 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
         }
         const int64_t L = s / K, q = s % K;
         const int64_t wd = q / 8, nq = (K - wd * 8) < 8 ? (K - wd * 8) : 8;
-        const uint32_t word = tb[(uint64_t)(ii - 1) * (64 * nwl) + wd * 64 + L];
+        if (L != cL || wd != cwd || ii < clo || ii >= clo + kWalkRows) {
+            clo = ii - (kWalkRows - 1) > 1 ? ii - (kWalkRows - 1) : 1;
+            const uint32_t *col = tb + wd * 64 + L;
+            uint32_t w[kWalkRows];
+#pragma unroll
+            for (int r = 0; r < kWalkRows; ++r)
+                w[r] = clo + r <= ii ? col[(uint64_t)(clo + r - 1) * (64 * nwl)] : 0u;
+#pragma unroll
+            for (int r = 0; r < kWalkRows; ++r) my[r] = w[r];
+            cL = L;
+            cwd = wd;
+        }
+        const uint32_t word = my[ii - clo];
         return (word >> (4 * (nq - 1 - (q % 8)))) & 15u;
     };
+    int64_t pwi = -1, twi = -1; // cached code words (pattern, text)
+    uint32_t pwv = 0, twv = 0;
     for (;;) {
         if (bad) break;
         if (state == 0) {
@@ -278,8 +294,16 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
             }
             const uint32_t w = bits_at(i, j);
             if ((w & 8u) == 0) {
-                const uint32_t pc = code_at(P, i - 1);
-                const uint32_t tc = code_at(T, j - 1);
+                if ((i - 1) >> 4 != pwi) {
+                    pwi = (i - 1) >> 4;
+                    pwv = codes16(P, pwi * 16);
+                }
+                if ((j - 1) >> 4 != twi) {
+                    twi = (j - 1) >> 4;
+                    twv = codes16(T, twi * 16);
+                }
+                const uint32_t pc = (pwv >> (2 * ((i - 1) & 15))) & 3u;
+                const uint32_t tc = (twv >> (2 * ((j - 1) & 15))) & 3u;
                 emit(pc == tc ? 7u : 8u, 1);
                 --i;
                 --j;

@@ -206,10 +206,13 @@ inline uint64_t band_cells(uint32_t m, uint32_t n, uint64_t below, uint64_t abov
     return cells;
 }
 
-// |score| bound of a band (the host check that keeps values far above NEG).
+// Bound on the biased values Hb = H + ext (s + 2 i) over `width` slots: |H| <= (mat + mis + ext)
+// (m + n + 2) + 2 open, and the bias adds at most ext (width + 2 m).  Under 2^28 the band stays
+// far above NEG = -2^29 and slots right of the band, lowered by 2^30, stay below it.
 inline bool scores_in_range(int32_t mat, int32_t mis, int32_t open, int32_t ext, uint64_t m, uint64_t n,
                             uint64_t width) {
-    const uint64_t bound = (uint64_t)(mat + mis + ext) * (m + n + width + 2) + 2ull * (uint64_t)open;
+    const uint64_t bound = (uint64_t)(mat + mis + ext) * (m + n + 2) + 2ull * (uint64_t)open +
+                           (uint64_t)ext * (width + 2 * m);
     return bound < (1ull << 28);
 }
 

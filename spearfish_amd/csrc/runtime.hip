@@ -101,9 +101,16 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
+// Words of one traceback scratch buffer (two are used, alternately): SF_TB_BUDGET_MB, else an
+// eighth of the device's memory, at most 32 GiB.
 uint64_t tb_budget_words() {
-    uint64_t mb = 16384;
+    uint64_t mb = 0;
     if (const char *e = std::getenv("SF_TB_BUDGET_MB")) mb = std::strtoull(e, nullptr, 10);
+    if (mb == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) tot = 16ull << 30;
+        mb = std::min<uint64_t>(32768, tot / 8 / (1ull << 20));
+    }
     return std::max<uint64_t>(mb, 16) * (1ull << 20) / 4;
 }
 
@@ -169,8 +176,10 @@ struct sf_batch {
     int32_t *d_score = nullptr;
     uint32_t *d_col = nullptr, *d_tstart = nullptr, *d_nruns = nullptr, *d_runs = nullptr, *d_cig = nullptr;
     uint64_t *d_runs_off = nullptr, *d_out_off = nullptr, *d_tboff = nullptr;
-    uint32_t *d_tb = nullptr;
+    uint32_t *d_tb[2] = {nullptr, nullptr}; // traceback scratch, alternating between chunks
     uint64_t tb_words = 0;
+    hipStream_t wstream = nullptr;           // traceback walks, overlapping the next chunk's DP
+    hipEvent_t dp_done[2] = {nullptr, nullptr}, walk_done[2] = {nullptr, nullptr}, walks_end = nullptr;
     bool cigar_valid = false;
     uint64_t affine_cells_cur = 0;
     // timing
@@ -204,13 +213,17 @@ struct sf_batch {
         pending.clear();
     }
     ~sf_batch() {
+        if (wstream) (void)hipStreamSynchronize(wstream);
         if (stream) (void)hipStreamSynchronize(stream);
         collect();
         for (auto e : pool) (void)hipEventDestroy(e);
         dfree(d_seq), dfree(d_poff), dfree(d_toff), dfree(d_plen), dfree(d_tlen), dfree(d_morder);
         dfree(d_dist), dfree(d_end), dfree(d_start), dfree(d_ba), dfree(d_aorder), dfree(d_score), dfree(d_col);
         dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
-        dfree(d_tboff), dfree(d_tb), dfree(d_toff0), dfree(d_tlen0);
+        dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0);
+        for (hipEvent_t e : {dp_done[0], dp_done[1], walk_done[0], walk_done[1], walks_end})
+            if (e) (void)hipEventDestroy(e);
+        if (wstream) (void)hipStreamDestroy(wstream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -250,16 +263,16 @@ void pack_into(uint32_t *dst, const void *buf, int packed, uint64_t off, uint32_
     }
 }
 
-int timed_begin(sf_batch *b, int cat, hipEvent_t *a) {
+int timed_begin(sf_batch *b, int cat, hipEvent_t *a, hipStream_t s = nullptr) {
     *a = b->ev();
-    HIP_TRY(hipEventRecord(*a, b->stream));
+    HIP_TRY(hipEventRecord(*a, s ? s : b->stream));
     (void)cat;
     return SF_OK;
 }
 
-int timed_end(sf_batch *b, int cat, hipEvent_t a) {
+int timed_end(sf_batch *b, int cat, hipEvent_t a, hipStream_t s = nullptr) {
     hipEvent_t e = b->ev();
-    HIP_TRY(hipEventRecord(e, b->stream));
+    HIP_TRY(hipEventRecord(e, s ? s : b->stream));
     b->pending.push_back({cat, a, e});
     return SF_OK;
 }
@@ -471,8 +484,9 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
             if (K == 0)
                 return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this build handles up to 2048",
                             (unsigned long long)i, (unsigned long long)width);
-            // Values stay far above NEG: |score| <= (mat + mis + ext) * (m + n + width) + 2 open.
-            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t, width))
+            // Values stay far above NEG over all 64 K slots of the class:
+            // |score| <= (mat + mis + ext) * (m + n + 64 K) + 2 open.
+            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t, 64ull * (uint64_t)K))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
             ba[i] = below | (above << 32);
             kcls[i] = (uint8_t)K;
@@ -550,9 +564,19 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         need_max = std::max(need_max, used);
     }
     if (need_max > b->tb_words) {
-        dfree(b->d_tb);
-        if ((st = dalloc(&b->d_tb, need_max))) return st;
+        dfree(b->d_tb[0]);
+        dfree(b->d_tb[1]);
+        if ((st = dalloc(&b->d_tb[0], need_max))) return st;
+        if (chunks.size() > 1 && (st = dalloc(&b->d_tb[1], need_max))) return st;
         b->tb_words = need_max;
+    }
+    if (chunks.size() > 1 && !b->d_tb[1]) {
+        if ((st = dalloc(&b->d_tb[1], b->tb_words))) return st;
+    }
+    if (!b->wstream) {
+        HIP_TRY(hipStreamCreateWithFlags(&b->wstream, hipStreamNonBlocking));
+        for (hipEvent_t *e : {&b->dp_done[0], &b->dp_done[1], &b->walk_done[0], &b->walk_done[1], &b->walks_end})
+            HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
     if (!b->d_tboff) {
         if ((st = dalloc(&b->d_tboff, n)) || (st = dalloc(&b->d_tstart, n)) || (st = dalloc(&b->d_nruns, n)) ||
@@ -568,34 +592,44 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         HIP_TRY(hipMemcpy(b->d_runs_off, roff.data(), n * 8, hipMemcpyHostToDevice));
     }
     HIP_TRY(hipMemcpy(b->d_tboff, tboff.data(), n * 8, hipMemcpyHostToDevice));
-    a.tb = b->d_tb;
     WalkArgs w{};
     w.pv = view(b);
     w.below_above = b->d_ba;
     w.mode = mode;
     w.score = b->d_score;
     w.score_col = b->d_col;
-    w.tb = b->d_tb;
     w.runs = b->d_runs;
     w.runs_off = b->d_runs_off;
     w.nruns = b->d_nruns;
     w.tstart = b->d_tstart;
-    for (auto &c : chunks) {
-        a.order = b->d_aorder + c.begin;
-        a.count = c.end - c.begin;
-        a.tb_off = b->d_tboff + c.begin;
+    // Chunk c's DP (main stream) writes scratch c % 2; its walk (walk stream) reads it while the
+    // DP of chunk c + 1 runs; the DP of chunk c + 2 waits for that walk before reusing the buffer.
+    for (size_t c = 0; c < chunks.size(); ++c) {
+        const Chunk &ch = chunks[c];
+        const int buf = (int)(c & 1);
+        if (c >= 2) HIP_TRY(hipStreamWaitEvent(b->stream, b->walk_done[buf], 0));
+        a.tb = b->d_tb[buf];
+        a.order = b->d_aorder + ch.begin;
+        a.count = ch.end - ch.begin;
+        a.tb_off = b->d_tboff + ch.begin;
         if ((st = timed_begin(b, 2, &e0))) return st;
-        HIP_TRY(sf_launch_affine(a, c.K, true, b->stream));
+        HIP_TRY(sf_launch_affine(a, ch.K, true, b->stream));
         if ((st = timed_end(b, 2, e0))) return st;
         ++b->affine_launches;
+        HIP_TRY(hipEventRecord(b->dp_done[buf], b->stream));
+        HIP_TRY(hipStreamWaitEvent(b->wstream, b->dp_done[buf], 0));
+        w.tb = b->d_tb[buf];
         w.order = a.order;
         w.count = a.count;
-        w.K = c.K;
+        w.K = ch.K;
         w.tb_off = a.tb_off;
-        if ((st = timed_begin(b, 3, &e0))) return st;
-        HIP_TRY(sf_launch_walk(w, b->stream));
-        if ((st = timed_end(b, 3, e0))) return st;
+        if ((st = timed_begin(b, 3, &e0, b->wstream))) return st;
+        HIP_TRY(sf_launch_walk(w, b->wstream));
+        if ((st = timed_end(b, 3, e0, b->wstream))) return st;
+        HIP_TRY(hipEventRecord(b->walk_done[buf], b->wstream));
     }
+    HIP_TRY(hipEventRecord(b->walks_end, b->wstream));
+    HIP_TRY(hipStreamWaitEvent(b->stream, b->walks_end, 0));
     if ((st = timed_begin(b, 3, &e0))) return st;
     hipLaunchKernelGGL(scan_u32_kernel, dim3(1), dim3(1024), 0, b->stream, b->d_nruns, n, b->d_out_off);
     HIP_TRY(hipGetLastError());

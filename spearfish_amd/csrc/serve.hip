@@ -356,9 +356,10 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
         bool bad = (uint64_t)r->roff[k] + m > rq->read_len || r->toff[k] + n > sv->store_bases;
         uint64_t below, above;
         band_geom(m, n, sv->cfg.affine_mode, sv->cfg.band, &below, &above);
-        const bool shape = sf_myers_maxb_for((m + 63) / 64) && sf_affine_k_for(below + above + 1) &&
+        const int K = sf_affine_k_for(below + above + 1);
+        const bool shape = sf_myers_maxb_for((m + 63) / 64) && K &&
                            scores_in_range(sv->cfg.sc.mat, sv->cfg.sc.mis, sv->cfg.sc.open, sv->cfg.sc.ext, m, n,
-                                           below + above + 1);
+                                           64ull * (uint64_t)K);
         if (bad || !shape) {
             delete r;
             return sf_runtime_fail(bad ? SF_EINVAL : SF_ERANGE, bad ? "segment outside the read or the store"
