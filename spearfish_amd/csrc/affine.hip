@@ -19,15 +19,19 @@
 // DPP max-scan across the 64 lanes.  With open >= 0 this is Gotoh's recurrence (align.rs:
 // 1060-1063); every value is exact i32.  The reference clamps F at NEG (align.rs:1053); that
 // clamp only acts where both vertical terms are below NEG, where H comes from the diagonal, so
-// it changes no H and is dropped.
+// it changes no H and is dropped.  Registers hold each value plus 2^31 (mod 2^32): unsigned max
+// then orders them as signed values, and DPP's zero fill (lanes with no source) is below all.
+// The diagonal term is Hb_diag + (mat + 2 ext) - (mat + mis) * [text code != pattern code], the
+// mismatch bit taken from one xor of the lane's text codes with the row's pattern code.
 //
-// Cells outside the band.  The reference reads a NEG sentinel just right of each row's band
-// (align.rs:1047-1048) and starts the horizontal gap of a row beginning at lo > 0 from NEG.
-// Here slot Wd (just right of the widest row) is set to a sentinel each row -- the next row's
-// rightmost band cell reads it; a row starting at lo > 0 starts at slot 0, whose scan carry is
-// below every value; slots left of column 0 (top rows) and right of column n (bottom rows) hold
-// values that real band values dominate and that no band cell reads as a vertical or diagonal
-// neighbour.  The host rejects scorings for which band values could come within reach of NEG.
+// Cells outside the band.  The reference reads NEG just right of each row's band (align.rs:
+// 1047-1048) and starts the horizontal gap of a row beginning at lo > 0 from NEG.  Here the slots
+// s >= Wd right of the widest row make their up-values 2^30 lower, so the rightmost band cell's
+// vertical terms are NEG-level, as in the reference; a row starting at lo > 0 starts at slot 0,
+// whose scan carry is below every value; slots left of column 0 (top rows) and right of column n
+// (bottom rows) hold values that real band values dominate and that no band cell reads as a
+// vertical or diagonal neighbour.  The host rejects scorings for which band values could come
+// within reach of NEG.
 //
 // Traceback bits (own definition; the reference has no traceback), per cell
 //   nd: H != Hdiag + s,  ne: H != x,  fx: F > Hup - go,  ex: E > Hleft - go,
@@ -44,21 +48,26 @@ namespace {
 template <int K>
 constexpr int kTbWords = (4 * K + 31) / 32;
 
-// acc = 2 * acc + (a != b) and 2 * acc + (a > b): the compare's VCC is the carry-in of one add.
+// acc = 2 * acc + (a != b): the compare's VCC is the carry-in of one add.
 __device__ __forceinline__ uint32_t push_ne(uint32_t acc, int32_t a, int32_t b) {
     asm("v_cmp_ne_u32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc" : "+v"(acc) : "v"(a), "v"(b) : "vcc");
     return acc;
 }
-__device__ __forceinline__ uint32_t push_gt(uint32_t acc, int32_t a, int32_t b) {
-    asm("v_cmp_gt_i32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc" : "+v"(acc) : "v"(a), "v"(b) : "vcc");
+
+// acc = 2 * acc + (a > b) for values in the offset-unsigned domain.
+__device__ __forceinline__ uint32_t push_ugt(uint32_t acc, uint32_t a, uint32_t b) {
+    asm("v_cmp_gt_u32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc" : "+v"(acc) : "v"(a), "v"(b) : "vcc");
     return acc;
 }
+
+constexpr uint32_t kOff = 0x80000000u; // offset-unsigned domain: v = x + 2^31 (mod 2^32)
 
 template <int K, bool TB>
 __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
     constexpr int NWL = kTbWords<K>;
-    constexpr int BLK = 16;                      // rows per block: one pattern word
+    constexpr int BLK = 16;                       // rows per block: one pattern word
     constexpr int NWIN = (BLK + K - 1 + 15) / 16; // text window words: codes of the block's slots
+    constexpr int NY = (K + 15) / 16;             // mismatch words per row
     const int lane = threadIdx.x & 63;
     const uint32_t w = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (w >= a.count) return; // whole waves only
@@ -71,30 +80,33 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
     const int32_t Wd = below + above + 1;
     const SeqRef P = seqref(a.pv.seq, a.pv.poff[pi], (uint32_t)m);
     const SeqRef T = seqref(a.pv.seq, a.pv.toff[pi], (uint32_t)n);
-    const int32_t mat = a.mat, mis = a.mis, ext = a.ext, go = a.open + a.ext;
-    const int32_t dlt = ext - go; // = -open
-    const int32_t scm = mat + 2 * ext, scx = 2 * ext - mis;
+    const int32_t mat = a.mat, mis = a.mis, ext = a.ext;
+    const uint32_t dlt = (uint32_t)-a.open; // ext - go
+    const uint32_t scm = (uint32_t)(mat + 2 * ext);
+    const int32_t ndl = -(mat + mis); // scx - scm
     const bool infix = a.mode == kInfix;
     const int32_t s0 = lane * K;
 
-    int32_t Hb[K], Fb[K], Ub[K];
+    // Values live as v = Hb + 2^31 (mod 2^32), so unsigned max orders them as signed and DPP's
+    // zero fill is below everything.
+    uint32_t V[K], Fb[K], Ub[K];
     const int32_t hi0 = above < n ? above : n;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         const int32_t s = s0 + q;
         const int32_t j = s - below;
         const int32_t h = (j >= 0 && j <= hi0) ? ((j == 0 || infix) ? 0 : -(a.open + ext * j)) : kNeg;
-        Hb[q] = h == kNeg ? kNeg : h + ext * s;
-        Fb[q] = kNeg;
-        Ub[q] = Hb[q] + dlt;
+        V[q] = (uint32_t)(h == kNeg ? kNeg : h + ext * s) + kOff;
+        Fb[q] = (uint32_t)kNeg + kOff;
+        Ub[q] = V[q] + dlt;
     }
     // Slots s >= Wd lie right of the band.  Their up-values Ub carry an extra -2^30, so every
     // up-value above the band stays below NEG (|Hb| < 2^28 over all 64K slots, checked on the
     // host) and F there is exactly NEG: the last band slot sees the same NEG up-neighbour as
     // align.rs:2548-2556 gives it, with no per-row reset.
-    int32_t dltv[K];
+    uint32_t dltv[K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) dltv[q] = s0 + q >= Wd ? dlt - (1 << 30) : dlt;
+    for (int q = 0; q < K; ++q) dltv[q] = s0 + q >= Wd ? dlt - (1u << 30) : dlt;
     // Row i has no band cell once i - below > n (align.rs:2543-2556); only a band laid out for a
     // longer text than this one gets there.
     const int32_t last = m < n + below ? m : n + below;
@@ -108,50 +120,56 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
         for (int k = 0; k < NWIN; ++k) win[k] = codes16(T, pos + 16 * k);
         const int32_t cnt = last - i0 + 1 < BLK ? last - i0 + 1 : BLK;
         for (int32_t r = 0; r < cnt; ++r) {
-            const uint32_t pc = pw & 3u;
+            const uint32_t prep = (pw & 3u) * 0x55555555u; // the row's pattern code in every field
             pw >>= 2;
-            const int32_t Un = dpp<kWaveShl1>(kNeg, Ub[0]);
-            const int32_t Fn = dpp<kWaveShl1>(kNeg, Fb[0]);
-            int32_t X[K], FQ[K], G[K], D[K], Uu[K];
+            // Bit 2q+1 of y: slot q's text code differs from the pattern code.
+            uint32_t y[NY];
+#pragma unroll
+            for (int k = 0; k < NY; ++k) {
+                const uint32_t t = win[k] ^ prep;
+                y[k] = (t << 1) | t;
+            }
+            const uint32_t Un = dpp0<kWaveShl1>(Ub[0]);
+            const uint32_t Fn = dpp0<kWaveShl1>(Fb[0]);
+            uint32_t X[K], FQ[K], G[K], D[K], Uu[K];
 #pragma unroll
             for (int q = 0; q < K; ++q) {
-                const int32_t uu = q < K - 1 ? Ub[q + 1] : Un;
-                const int32_t fu = q < K - 1 ? Fb[q + 1] : Fn;
-                const uint32_t code = (win[q / 16] >> (2 * (q % 16))) & 3u;
-                const int32_t sc = code == pc ? scm : scx;
-                const int32_t f = imax(uu, fu);
-                const int32_t d = Hb[q] + sc;
-                const int32_t x = imax(d, f);
+                const uint32_t uu = q < K - 1 ? Ub[q + 1] : Un;
+                const uint32_t fu = q < K - 1 ? Fb[q + 1] : Fn;
+                const uint32_t mism = __builtin_amdgcn_ubfe(y[q / 16], 2 * (q % 16) + 1, 1);
+                const uint32_t f = umax(uu, fu);
+                const uint32_t d = V[q] + (uint32_t)__mul24((int)mism, ndl) + scm;
+                const uint32_t x = umax(d, f);
                 X[q] = x;
                 FQ[q] = f;
                 D[q] = d;
                 Uu[q] = uu;
-                const int32_t g = x + dlt;
-                G[q] = q ? imax(G[q - 1], g) : g;
+                const uint32_t g = x + dlt;
+                G[q] = q ? umax(G[q - 1], g) : g;
             }
-            const int32_t C = wave_excl_max_min(G[K - 1]);
-            int32_t E[K];
+            const uint32_t C = wave_excl_umax(G[K - 1]);
+            uint32_t E[K];
 #pragma unroll
             for (int q = 0; q < K; ++q) {
-                E[q] = q ? imax(C, G[q - 1]) : C; // biased horizontal-gap value entering slot q
-                Hb[q] = imax(X[q], E[q]);
+                E[q] = q ? umax(C, G[q - 1]) : C; // biased horizontal-gap value entering slot q
+                V[q] = umax(X[q], E[q]);
                 Fb[q] = FQ[q];
             }
 #pragma unroll
-            for (int q = 0; q < K; ++q) Ub[q] = Hb[q] + dltv[q];
+            for (int q = 0; q < K; ++q) Ub[q] = V[q] + dltv[q];
             if constexpr (TB) {
-                const int32_t ul0 = dpp<kWaveShr1>(kNeg, Ub[K - 1]);
+                const uint32_t ul0 = dpp0<kWaveShr1>(Ub[K - 1]);
                 uint32_t *row = tbp + (uint32_t)(i0 + r - 1) * (64 * NWL);
 #pragma unroll
                 for (int wd = 0; wd < NWL; ++wd) {
                     uint32_t acc = 0;
 #pragma unroll
                     for (int q = wd * 8; q < K && q < wd * 8 + 8; ++q) {
-                        const int32_t ul = q ? Ub[q - 1] : ul0;
-                        acc = push_ne(acc, Hb[q], D[q]);
-                        acc = push_ne(acc, Hb[q], X[q]);
-                        acc = push_gt(acc, FQ[q], Uu[q]);
-                        acc = push_gt(acc, E[q], ul);
+                        const uint32_t ul = q ? Ub[q - 1] : ul0;
+                        acc = push_ne(acc, (int32_t)V[q], (int32_t)D[q]);
+                        acc = push_ne(acc, (int32_t)V[q], (int32_t)X[q]);
+                        acc = push_ugt(acc, FQ[q], Uu[q]);
+                        acc = push_ugt(acc, E[q], ul);
                     }
                     row[wd * 64] = acc;
                 }
@@ -170,10 +188,10 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
     } else if (a.mode == kGlobal) {
         const int32_t sN = n - m + below; // slot of (m, n)
         const int owner = sN / K, qq = sN % K;
-        int32_t v = Hb[0];
+        uint32_t v = V[0];
 #pragma unroll
-        for (int q = 1; q < K; ++q) v = q == qq ? Hb[q] : v;
-        score = __builtin_amdgcn_readlane(v, owner) - ext * sN - rowbias;
+        for (int q = 1; q < K; ++q) v = q == qq ? V[q] : v;
+        score = (int32_t)((uint32_t)__builtin_amdgcn_readlane((int)v, owner) - kOff) - ext * sN - rowbias;
         col = (uint32_t)n;
     } else {
         // align.rs:2643-2650: max of the last row, first argmax.
@@ -186,7 +204,7 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             const int32_t s = s0 + q;
-            const int32_t h = Hb[q] - ext * s - rowbias;
+            const int32_t h = (int32_t)(V[q] - kOff) - ext * s - rowbias;
             if (s >= slo && s <= shi && h > bv) {
                 bv = h;
                 bs = s;

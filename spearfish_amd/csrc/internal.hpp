@@ -137,6 +137,27 @@ constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0
 constexpr int kWaveShl1 = 0x130, kWaveShr1 = 0x138;
 constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
 
+// A DPP move whose lanes without a source read 0 (bound_ctrl): the minimum of unsigned values.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp0(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, true);
+}
+
+__device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
+// Exclusive max-scan of unsigned values across the 64 lanes of a wave, lane 0 getting 0.  DPP's
+// zero fill of lanes without a source is the identity, so each step is one v_max_u32 with a DPP
+// operand.
+__device__ __forceinline__ uint32_t wave_excl_umax(uint32_t x) {
+    x = umax(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr1, 0xf, 0xf, true));
+    x = umax(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr2, 0xf, 0xf, true));
+    x = umax(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr4, 0xf, 0xf, true));
+    x = umax(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr8, 0xf, 0xf, true));
+    x = umax(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowBcast15, 0xa, 0xf, true));
+    x = umax(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowBcast31, 0xc, 0xf, true));
+    return dpp0<kWaveShr1>(x);
+}
+
 // Exclusive max-scan across the 64 lanes of a wave, lane 0 getting INT32_MIN.  Signed values are
 // mapped to unsigned order (x ^ 0x80000000) so that DPP's zero fill of lanes without a source is
 // the identity and each step is one v_max_u32 with a DPP operand.
