@@ -293,15 +293,17 @@ __global__ void probe_kernel(int32_t *out) {
 
 } // namespace
 
-// Shape classes: lane kernel MAXB (1..32), then pipelined classes encoded as 1000 * G + R.
+// Shape classes: lane kernel MAXB (1..24; more planes would spill), then pipelined classes
+// encoded as 1000 * G + R: 16 lanes per pair up to 96 blocks, 64 lanes per pair beyond, which
+// keeps the per-lane chain of a long pattern (its critical path) at most 24 blocks.
 int sf_myers_maxb_for(uint32_t nblocks) {
-    static const int lane_cls[] = {1, 2, 4, 8, 12, 16, 20, 24, 32};
+    static const int lane_cls[] = {1, 2, 4, 8, 12, 16, 20, 24};
     for (int c : lane_cls)
         if (nblocks <= (uint32_t)c) return c;
-    static const int pipe16[] = {3, 4, 5, 6, 8, 10, 12, 16, 20, 24};
+    static const int pipe16[] = {2, 3, 4, 5, 6};
     for (int r : pipe16)
         if (nblocks <= (uint32_t)(16 * r)) return 16000 + r;
-    static const int pipe64[] = {8, 12, 16, 20, 24};
+    static const int pipe64[] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24};
     for (int r : pipe64)
         if (nblocks <= (uint32_t)(64 * r)) return 64000 + r;
     return 0;
@@ -317,9 +319,9 @@ hipError_t sf_launch_myers(const MyersArgs &a, int maxb, hipStream_t s) {
         switch (maxb) {
 #define PCASE(GG, RR) \
     case GG * 1000 + RR: hipLaunchKernelGGL((myers_pipe_kernel<GG, RR>), pg, block, 0, s, a); break;
-            PCASE(16, 3) PCASE(16, 4) PCASE(16, 5) PCASE(16, 6) PCASE(16, 8) PCASE(16, 10) PCASE(16, 12)
-            PCASE(16, 16) PCASE(16, 20) PCASE(16, 24) PCASE(64, 8) PCASE(64, 12) PCASE(64, 16) PCASE(64, 20)
-            PCASE(64, 24)
+            PCASE(16, 2) PCASE(16, 3) PCASE(16, 4) PCASE(16, 5) PCASE(16, 6) PCASE(64, 2) PCASE(64, 3)
+            PCASE(64, 4) PCASE(64, 5) PCASE(64, 6) PCASE(64, 8) PCASE(64, 10) PCASE(64, 12) PCASE(64, 16)
+            PCASE(64, 20) PCASE(64, 24)
 #undef PCASE
         default:
             return hipErrorInvalidValue;
@@ -331,7 +333,7 @@ hipError_t sf_launch_myers(const MyersArgs &a, int maxb, hipStream_t s) {
     switch (maxb) {
 #define CASE(B) \
     case B: hipLaunchKernelGGL(myers_lane_kernel<B>, grid, block, 0, s, a); break;
-        CASE(1) CASE(2) CASE(4) CASE(8) CASE(12) CASE(16) CASE(20) CASE(24) CASE(32)
+        CASE(1) CASE(2) CASE(4) CASE(8) CASE(12) CASE(16) CASE(20) CASE(24)
 #undef CASE
     default:
         return hipErrorInvalidValue;

@@ -161,6 +161,10 @@ struct sf_batch {
     struct Bin {
         int cls;
         uint32_t begin, end;
+        // Estimated time in VALU wave-instructions: the longest single wave's chain, and the
+        // whole bin's work spread over the chip's 1024 SIMDs.  Only orders the launches.
+        double chain = 0, spread = 0;
+        double est() const { return chain > spread ? chain : spread; }
     };
     std::vector<Bin> mbins;
     uint32_t *d_morder = nullptr;
@@ -179,6 +183,10 @@ struct sf_batch {
     uint32_t *d_tb[2] = {nullptr, nullptr}; // traceback scratch, alternating between chunks
     uint64_t tb_words = 0;
     hipStream_t wstream = nullptr;           // traceback walks, overlapping the next chunk's DP
+    // Extra streams on which independent bins run beside the main stream's.
+    static constexpr int kAux = 3;
+    hipStream_t aux[kAux] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, joins[kAux] = {nullptr, nullptr, nullptr};
     hipEvent_t dp_done[2] = {nullptr, nullptr}, walk_done[2] = {nullptr, nullptr}, walks_end = nullptr;
     bool cigar_valid = false;
     uint64_t affine_cells_cur = 0;
@@ -221,8 +229,13 @@ struct sf_batch {
         dfree(d_dist), dfree(d_end), dfree(d_start), dfree(d_ba), dfree(d_aorder), dfree(d_score), dfree(d_col);
         dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
         dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0);
-        for (hipEvent_t e : {dp_done[0], dp_done[1], walk_done[0], walk_done[1], walks_end})
+        for (hipStream_t x : aux)
+            if (x) (void)hipStreamSynchronize(x);
+        for (hipEvent_t e : {dp_done[0], dp_done[1], walk_done[0], walk_done[1], walks_end, fork, joins[0], joins[1],
+                             joins[2]})
             if (e) (void)hipEventDestroy(e);
+        for (hipStream_t x : aux)
+            if (x) (void)hipStreamDestroy(x);
         if (wstream) (void)hipStreamDestroy(wstream);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -288,6 +301,50 @@ __global__ void restrict_kernel(uint64_t n, const uint32_t *dist, const uint32_t
         toff[i] = toff0[i] + start[i];
         tlen[i] = end[i] - start[i];
     }
+}
+
+// Runs one launch per bin on the main stream and the extra streams: longest estimate first, each
+// on the stream with the least estimated work so far, so that a few long-running bins (long
+// patterns, few waves) overlap the many short ones instead of leaving the chip idle behind them.
+// The main stream then waits for every stream used.
+template <class F>
+int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch) {
+    constexpr int S = 1 + sf_batch::kAux;
+    if (bins.size() <= 1) {
+        for (auto &bin : bins) HIP_TRY(launch(bin, b->stream));
+        return SF_OK;
+    }
+    if (!b->fork) {
+        HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        for (int i = 0; i < sf_batch::kAux; ++i) {
+            HIP_TRY(hipStreamCreateWithFlags(&b->aux[i], hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming));
+        }
+    }
+    std::vector<size_t> idx(bins.size());
+    std::iota(idx.begin(), idx.end(), (size_t)0);
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return bins[x].est() > bins[y].est(); });
+    hipStream_t ss[S] = {b->stream, b->aux[0], b->aux[1], b->aux[2]};
+    double load[S] = {0, 0, 0, 0};
+    bool used[S] = {true, false, false, false};
+    HIP_TRY(hipEventRecord(b->fork, b->stream));
+    for (size_t i : idx) {
+        int s = 0;
+        for (int t = 1; t < S; ++t)
+            if (load[t] < load[s]) s = t;
+        if (!used[s]) {
+            HIP_TRY(hipStreamWaitEvent(ss[s], b->fork, 0));
+            used[s] = true;
+        }
+        HIP_TRY(launch(bins[i], ss[s]));
+        load[s] += bins[i].est();
+    }
+    for (int t = 1; t < S; ++t)
+        if (used[t]) {
+            HIP_TRY(hipEventRecord(b->joins[t - 1], ss[t]));
+            HIP_TRY(hipStreamWaitEvent(b->stream, b->joins[t - 1], 0));
+        }
+    return SF_OK;
 }
 
 } // namespace
@@ -392,7 +449,16 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
             snprintf(msg, sizeof msg, "pattern of %u bases: Myers supports up to 98304", b->plen[order[q]]);
             b->myers_unsupported = msg;
         }
-        b->mbins.push_back({cls, (uint32_t)q, (uint32_t)e});
+        sf_batch::Bin bin{cls, (uint32_t)q, (uint32_t)e};
+        // Per wave: lane classes take 64 pairs of `cls` blocks a column; pipelined ones take
+        // 64 / G pairs, R blocks a lane a column, G - 1 columns of skew.
+        const double G = cls >= 16000 ? cls / 1000 : 1, R = cls >= 16000 ? cls % 1000 : cls;
+        for (uint64_t x = q; x < e; ++x) {
+            const double cols = (double)b->tlen[order[x]] + (G - 1);
+            bin.chain = std::max(bin.chain, cols * R);
+            bin.spread += cols * R * G / 64.0 / 1024.0;
+        }
+        b->mbins.push_back(bin);
         q = e;
     }
     for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];
@@ -432,23 +498,27 @@ int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
     a.start = locate == 2 ? b->d_start : nullptr;
     hipEvent_t e0;
     if ((st = timed_begin(b, 0, &e0))) return st;
-    for (auto &bin : b->mbins) {
-        a.order = b->d_morder + bin.begin;
-        a.count = bin.end - bin.begin;
-        a.reverse = 0;
-        HIP_TRY(sf_launch_myers(a, bin.cls, b->stream));
-    }
+    a.reverse = 0;
+    if ((st = run_bins(b, b->mbins, [&](const sf_batch::Bin &bin, hipStream_t s) {
+             MyersArgs x = a;
+             x.order = b->d_morder + bin.begin;
+             x.count = bin.end - bin.begin;
+             return sf_launch_myers(x, bin.cls, s);
+         })))
+        return st;
     if ((st = timed_end(b, 0, e0))) return st;
     b->myers_cells += b->myers_cells_per_run;
     b->located_ready = locate == 2;
     if (locate == 2 && mode == SF_INFIX) {
         if ((st = timed_begin(b, 1, &e0))) return st;
-        for (auto &bin : b->mbins) {
-            a.order = b->d_morder + bin.begin;
-            a.count = bin.end - bin.begin;
-            a.reverse = 1;
-            HIP_TRY(sf_launch_myers(a, bin.cls, b->stream));
-        }
+        a.reverse = 1;
+        if ((st = run_bins(b, b->mbins, [&](const sf_batch::Bin &bin, hipStream_t s) {
+                 MyersArgs x = a;
+                 x.order = b->d_morder + bin.begin;
+                 x.count = bin.end - bin.begin;
+                 return sf_launch_myers(x, bin.cls, s);
+             })))
+            return st;
         if ((st = timed_end(b, 1, e0))) return st;
     }
     return SF_OK;
@@ -502,7 +572,13 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         for (uint64_t q = 0; q < n;) {
             uint64_t e = q;
             while (e < n && kcls[order[e]] == kcls[order[q]]) ++e;
-            b->abins.push_back({kcls[order[q]], (uint32_t)q, (uint32_t)e});
+            sf_batch::Bin bin{kcls[order[q]], (uint32_t)q, (uint32_t)e};
+            const double per_row = 9.0 * bin.cls + 13.0; // VALU per row of one wave
+            for (uint64_t x = q; x < e; ++x) {
+                bin.chain = std::max(bin.chain, b->plen[order[x]] * per_row);
+                bin.spread += b->plen[order[x]] * per_row / 1024.0;
+            }
+            b->abins.push_back(bin);
             q = e;
         }
         if (!b->d_ba && ((st = dalloc(&b->d_ba, n)) || (st = dalloc(&b->d_aorder, n)) || (st = dalloc(&b->d_score, n)) ||
@@ -526,12 +602,14 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     hipEvent_t e0;
     if (!cigar) {
         if ((st = timed_begin(b, 2, &e0))) return st;
-        for (auto &bin : b->abins) {
-            a.order = b->d_aorder + bin.begin;
-            a.count = bin.end - bin.begin;
-            HIP_TRY(sf_launch_affine(a, bin.cls, false, b->stream));
-            ++b->affine_launches;
-        }
+        if ((st = run_bins(b, b->abins, [&](const sf_batch::Bin &bin, hipStream_t s) {
+                 AffineArgs x = a;
+                 x.order = b->d_aorder + bin.begin;
+                 x.count = bin.end - bin.begin;
+                 return sf_launch_affine(x, bin.cls, false, s);
+             })))
+            return st;
+        b->affine_launches += b->abins.size();
         if ((st = timed_end(b, 2, e0))) return st;
         b->affine_cells += b->affine_cells_cur;
         b->cigar_valid = false;

@@ -183,11 +183,13 @@ def test_batch_object_reruns_are_identical():
 
 
 def test_myers_long_patterns_class_boundaries():
-    # Patterns past the lane kernel (> 2048 bases) run 16 or 64 lanes per pair; lengths at the
-    # edges of the block classes, texts short enough for the oracle.
+    # Patterns past the lane kernel (> 24 blocks, 1536 bases) run 16 lanes per pair up to 96
+    # blocks and 64 beyond; lengths at the edges of the block classes, texts short enough for the
+    # oracle.
     rng = random.Random(41)
     P, T = [], []
-    for m in (2049, 2112, 3072, 3073, 4096, 5121, 16 * 64 * 24, 16 * 64 * 24 + 1, 30000):
+    for m in (1536, 1537, 2048, 2049, 3072, 3073, 6144, 6145, 8192, 8193, 12289, 16 * 64 * 24,
+              16 * 64 * 24 + 1, 30000, 64 * 64 * 24):
         n = rng.choice([0, 1, 150, 300])
         t = [rng.randrange(4) for _ in range(n)]
         p = [rng.randrange(4) for _ in range(m)]
