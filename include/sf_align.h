@@ -201,6 +201,12 @@ void sf_server_destroy(sf_server *s);
 int sf_server_replay(sf_server *s, const sf_read_request *req, const double *t_arrive, uint64_t n, double *wall_s,
                      uint64_t *refused);
 
+/* Tuning (no reference counterpart; results never depend on it): bands of up to `slots` cells
+ * per row run with 16 lanes per pair, four pairs per wave; wider bands, or every band when
+ * `slots` is 0, run with 64 lanes per pair.  Default 512 (or SF_AFFINE_LP16_MAX from the
+ * environment).  Applies to affine runs started afterwards in this process. */
+int sf_set_affine_lp16_max(uint64_t slots);
+
 /* Diagnostic (no reference counterpart): runs a one-wave kernel that records the lane moves of
  * the DPP and shuffle encodings the kernels rely on, 5 x 64 values into `out` (320 int32). */
 int sf_selftest_lanes(int32_t *out);

@@ -128,6 +128,7 @@ SIGNATURES = {
     "sf_server_destroy": ([_vp], None),
     "sf_server_replay": ([_vp, C.POINTER(ReadRequest), C.POINTER(C.c_double), _u64, C.POINTER(C.c_double),
                           C.POINTER(_u64)], _int),
+    "sf_set_affine_lp16_max": ([C.c_uint64], _int),
     "sf_selftest_lanes": ([_vp], _int),
 }
 

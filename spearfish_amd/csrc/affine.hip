@@ -40,6 +40,9 @@
 // The walk reads them back under the oracle's tie rules.
 #include "internal.hpp"
 
+#include <atomic>
+#include <cstdlib>
+
 using namespace sf;
 
 namespace {
@@ -62,30 +65,55 @@ __device__ __forceinline__ uint32_t push_ugt(uint32_t acc, uint32_t a, uint32_t 
 
 constexpr uint32_t kOff = 0x80000000u; // offset-unsigned domain: v = x + 2^31 (mod 2^32)
 
-template <int K, bool TB>
+// Exclusive max-scan of unsigned values over each group of LP lanes (16: one DPP row; 64: the
+// wave); the group's first lane gets 0.
+template <int LP>
+__device__ __forceinline__ uint32_t group_excl_umax(uint32_t x) {
+    if constexpr (LP == 64) {
+        return wave_excl_umax(x);
+    } else {
+        static_assert(LP == 16, "groups of 16 or 64 lanes");
+        x = umax(x, dpp0<kRowShr1>(x));
+        x = umax(x, dpp0<kRowShr2>(x));
+        x = umax(x, dpp0<kRowShr4>(x));
+        x = umax(x, dpp0<kRowShr8>(x));
+        return dpp0<kRowShr1>(x);
+    }
+}
+
+// LP lanes per pair, K slots per lane: 64 / LP pairs per wave.  With LP = 16 every lane move
+// stays inside a DPP row, so the four pairs of a wave never see each other; each runs its own
+// rows (a pair whose rows are done sits out the wave's remaining ones) and its own geometry.
+template <int LP, int K, bool TB>
 __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
+    constexpr bool ONE = LP == 64; // one pair per wave: pair values are wave-uniform
+    constexpr int SHL = ONE ? kWaveShl1 : kRowShl1;
+    constexpr int SHR = ONE ? kWaveShr1 : kRowShr1;
     constexpr int NWL = kTbWords<K>;
     constexpr int BLK = 16;                       // rows per block: one pattern word
     constexpr int NWIN = (BLK + K - 1 + 15) / 16; // text window words: codes of the block's slots
     constexpr int NY = (K + 15) / 16;             // mismatch words per row
     const int lane = threadIdx.x & 63;
-    const uint32_t w = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (w >= a.count) return; // whole waves only
-    const uint32_t pi = __builtin_amdgcn_readfirstlane(a.order[w]);
-    const int32_t m = __builtin_amdgcn_readfirstlane((int32_t)a.pv.plen[pi]);
-    const int32_t n = __builtin_amdgcn_readfirstlane((int32_t)a.pv.tlen[pi]);
-    const uint64_t ba = a.below_above[pi];
-    const int32_t below = __builtin_amdgcn_readfirstlane((int32_t)(uint32_t)ba);
-    const int32_t above = __builtin_amdgcn_readfirstlane((int32_t)(ba >> 32));
+    const int l = lane % LP;
+    const uint32_t w = (blockIdx.x * 256u + threadIdx.x) / LP; // pair slot in the launch order
+    const bool valid = w < a.count;
+    if (ONE && !valid) return; // whole waves only
+    auto uni = [](int32_t v) { return ONE ? __builtin_amdgcn_readfirstlane(v) : v; };
+    const uint32_t pi = valid ? (uint32_t)uni((int32_t)a.order[w]) : 0u;
+    const int32_t m = valid ? uni((int32_t)a.pv.plen[pi]) : 0;
+    const int32_t n = valid ? uni((int32_t)a.pv.tlen[pi]) : 0;
+    const uint64_t ba = valid ? a.below_above[pi] : 0;
+    const int32_t below = uni((int32_t)(uint32_t)ba);
+    const int32_t above = uni((int32_t)(ba >> 32));
     const int32_t Wd = below + above + 1;
-    const SeqRef P = seqref(a.pv.seq, a.pv.poff[pi], (uint32_t)m);
-    const SeqRef T = seqref(a.pv.seq, a.pv.toff[pi], (uint32_t)n);
+    const SeqRef P = seqref(a.pv.seq, valid ? a.pv.poff[pi] : 0, (uint32_t)m);
+    const SeqRef T = seqref(a.pv.seq, valid ? a.pv.toff[pi] : 0, (uint32_t)n);
     const int32_t mat = a.mat, mis = a.mis, ext = a.ext;
     const uint32_t dlt = (uint32_t)-a.open; // ext - go
     const uint32_t scm = (uint32_t)(mat + 2 * ext);
     const int32_t ndl = -(mat + mis); // scx - scm
     const bool infix = a.mode == kInfix;
-    const int32_t s0 = lane * K;
+    const int32_t s0 = l * K;
 
     // Values live as v = Hb + 2^31 (mod 2^32), so unsigned max orders them as signed and DPP's
     // zero fill is below everything.
@@ -101,27 +129,37 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
         Ub[q] = V[q] + dlt;
     }
     // Slots s >= Wd lie right of the band.  Their up-values Ub carry an extra -2^30, so every
-    // up-value above the band stays below NEG (|Hb| < 2^28 over all 64K slots, checked on the
-    // host) and F there is exactly NEG: the last band slot sees the same NEG up-neighbour as
-    // align.rs:2548-2556 gives it, with no per-row reset.
+    // up-value above the band stays below NEG (|Hb| < 2^28 over all LP * K slots, checked on
+    // the host) and F there is exactly NEG: the last band slot sees the same NEG up-neighbour
+    // as align.rs:2548-2556 gives it, with no per-row reset.
     uint32_t dltv[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) dltv[q] = s0 + q >= Wd ? dlt - (1u << 30) : dlt;
     // Row i has no band cell once i - below > n (align.rs:2543-2556); only a band laid out for a
     // longer text than this one gets there.
-    const int32_t last = m < n + below ? m : n + below;
+    const int32_t last = valid ? (m < n + below ? m : n + below) : 0;
+    int32_t rows = last; // the wave's row count
+    if constexpr (!ONE) {
+#pragma unroll
+        for (int o = 32; o >= LP; o >>= 1) rows = imax(rows, __shfl_xor(rows, o));
+        rows = __builtin_amdgcn_readfirstlane(rows);
+    }
     uint32_t *tbp = nullptr;
-    if constexpr (TB) tbp = a.tb + a.tb_off[w] + lane;
-    for (int32_t i0 = 1; i0 <= last; i0 += BLK) {
-        uint32_t pw = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)codes16(P, i0 - 1));
+    if constexpr (TB) tbp = a.tb + (valid ? a.tb_off[w] : 0) + l;
+    for (int32_t i0 = 1; i0 <= rows; i0 += BLK) {
+        uint32_t pw = (uint32_t)uni((int32_t)codes16(P, i0 - 1));
         uint32_t win[NWIN];
         const int32_t pos = i0 - below + s0 - 1; // text base of this lane's first slot in row i0
 #pragma unroll
         for (int k = 0; k < NWIN; ++k) win[k] = codes16(T, pos + 16 * k);
-        const int32_t cnt = last - i0 + 1 < BLK ? last - i0 + 1 : BLK;
+        const int32_t cnt = rows - i0 + 1 < BLK ? rows - i0 + 1 : BLK;
         for (int32_t r = 0; r < cnt; ++r) {
-            const uint32_t prep = (pw & 3u) * 0x55555555u; // the row's pattern code in every field
+            const uint32_t pc = pw & 3u;
             pw >>= 2;
+            if (!ONE && i0 + r > last) continue; // this pair's rows are done
+            // The row's pattern code in every 2-bit field (the low 12 fields suffice up to K = 12).
+            const uint32_t p5 = pc * 0x5555u;
+            const uint32_t prep = ONE ? pc * 0x55555555u : (K <= 12 ? pc * 0x555555u : p5 | (p5 << 16));
             // Bit 2q+1 of y: slot q's text code differs from the pattern code.
             uint32_t y[NY];
 #pragma unroll
@@ -129,8 +167,8 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
                 const uint32_t t = win[k] ^ prep;
                 y[k] = (t << 1) | t;
             }
-            const uint32_t Un = dpp0<kWaveShl1>(Ub[0]);
-            const uint32_t Fn = dpp0<kWaveShl1>(Fb[0]);
+            const uint32_t Un = dpp0<SHL>(Ub[0]);
+            const uint32_t Fn = dpp0<SHL>(Fb[0]);
             uint32_t X[K], FQ[K], G[K], D[K], Uu[K];
 #pragma unroll
             for (int q = 0; q < K; ++q) {
@@ -147,7 +185,7 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
                 const uint32_t g = x + dlt;
                 G[q] = q ? umax(G[q - 1], g) : g;
             }
-            const uint32_t C = wave_excl_umax(G[K - 1]);
+            const uint32_t C = group_excl_umax<LP>(G[K - 1]);
             uint32_t E[K];
 #pragma unroll
             for (int q = 0; q < K; ++q) {
@@ -158,8 +196,8 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
 #pragma unroll
             for (int q = 0; q < K; ++q) Ub[q] = V[q] + dltv[q];
             if constexpr (TB) {
-                const uint32_t ul0 = dpp0<kWaveShr1>(Ub[K - 1]);
-                uint32_t *row = tbp + (uint32_t)(i0 + r - 1) * (64 * NWL);
+                const uint32_t ul0 = dpp0<SHR>(Ub[K - 1]);
+                uint32_t *row = tbp + (uint32_t)(i0 + r - 1) * (LP * NWL);
 #pragma unroll
                 for (int wd = 0; wd < NWL; ++wd) {
                     uint32_t acc = 0;
@@ -171,7 +209,7 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
                         acc = push_ugt(acc, FQ[q], Uu[q]);
                         acc = push_ugt(acc, E[q], ul);
                     }
-                    row[wd * 64] = acc;
+                    row[wd * LP] = acc;
                 }
             }
 #pragma unroll
@@ -182,16 +220,15 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
     int32_t score;
     uint32_t col;
     const int32_t rowbias = 2 * ext * m;
-    if (stopped) {
-        score = kNeg;
-        col = 0;
-    } else if (a.mode == kGlobal) {
+    if (a.mode == kGlobal) {
         const int32_t sN = n - m + below; // slot of (m, n)
         const int owner = sN / K, qq = sN % K;
         uint32_t v = V[0];
 #pragma unroll
         for (int q = 1; q < K; ++q) v = q == qq ? V[q] : v;
-        score = (int32_t)((uint32_t)__builtin_amdgcn_readlane((int)v, owner) - kOff) - ext * sN - rowbias;
+        const uint32_t vo = ONE ? (uint32_t)__builtin_amdgcn_readlane((int)v, owner)
+                                : (uint32_t)__shfl((int)v, (lane - l) + owner);
+        score = (int32_t)(vo - kOff) - ext * sN - rowbias;
         col = (uint32_t)n;
     } else {
         // align.rs:2643-2650: max of the last row, first argmax.
@@ -211,7 +248,7 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
             }
         }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
+        for (int off = LP / 2; off > 0; off >>= 1) {
             const int32_t ov = __shfl_xor(bv, off), os = __shfl_xor(bs, off);
             if (ov > bv || (ov == bv && os < bs)) {
                 bv = ov;
@@ -221,7 +258,11 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
         score = bv;
         col = (uint32_t)(b + bs);
     }
-    if (lane == 0) {
+    if (stopped) {
+        score = kNeg;
+        col = 0;
+    }
+    if (valid && l == 0) {
         a.score[pi] = score;
         if (a.score_col) a.score_col[pi] = col;
     }
@@ -252,7 +293,7 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
     const int64_t below = (int64_t)(uint32_t)a.below_above[pi];
     const SeqRef P = seqref(a.pv.seq, a.pv.poff[pi], (uint32_t)m);
     const SeqRef T = seqref(a.pv.seq, a.pv.toff[pi], (uint32_t)n);
-    const int K = a.K;
+    const int K = a.K, LP = a.LP;
     const int64_t nwl = (4 * K + 31) / 32;
     const uint32_t *tb = a.tb + a.tb_off[t];
     int64_t i = m, j = a.mode == kGlobal ? n : (int64_t)a.score_col[pi];
@@ -273,7 +314,7 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
     // The 4 bits (nd, ne, fx, ex) of cell (ii, jj) in bits 3..0 of the result.
     auto bits_at = [&](int64_t ii, int64_t jj) -> uint32_t {
         const int64_t s = jj - (ii - below);
-        if (ii < 1 || ii > m || s < 0 || s >= 64 * (int64_t)K) {
+        if (ii < 1 || ii > m || s < 0 || s >= (int64_t)LP * K) {
             // Off the band: cannot happen on a path the DP produced; stop rather than read out of
             // the pair's bits.
             bad = true;
@@ -283,11 +324,11 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
         const int64_t wd = q / 8, nq = (K - wd * 8) < 8 ? (K - wd * 8) : 8;
         if (L != cL || wd != cwd || ii < clo || ii >= clo + kWalkRows) {
             clo = ii - (kWalkRows - 1) > 1 ? ii - (kWalkRows - 1) : 1;
-            const uint32_t *col = tb + wd * 64 + L;
+            const uint32_t *col = tb + wd * LP + L;
             uint32_t w[kWalkRows];
 #pragma unroll
             for (int r = 0; r < kWalkRows; ++r)
-                w[r] = clo + r <= ii ? col[(uint64_t)(clo + r - 1) * (64 * nwl)] : 0u;
+                w[r] = clo + r <= ii ? col[(uint64_t)(clo + r - 1) * (LP * nwl)] : 0u;
 #pragma unroll
             for (int r = 0; r < kWalkRows; ++r) my[r] = w[r];
             cL = L;
@@ -363,24 +404,49 @@ __global__ __launch_bounds__(256) void compact_kernel(const uint32_t *order, uin
 
 } // namespace
 
-int sf_affine_k_for(uint64_t width) {
+// Shape class for a band of `width` cells per row: 16 lanes per pair (four pairs per wave) up to
+// the tuning limit (default 16 * 32 slots), where the per-row lane scan is shared by four pairs,
+// then 64 lanes per pair.
+static std::atomic<uint64_t> g_lp16_max{[] {
+    const char *e = getenv("SF_AFFINE_LP16_MAX");
+    return e ? (uint64_t)strtoull(e, nullptr, 10) : (uint64_t)16 * 32;
+}()};
+
+uint64_t sf_affine_lp16_max() { return g_lp16_max.load(std::memory_order_relaxed); }
+
+int sf_affine_class(uint64_t width, bool lp16) {
     static const int cls[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32};
+    if (lp16 && width <= 16 * 32)
+        for (int c : cls)
+            if (width <= (uint64_t)c * 16) return 16000 + c;
     for (int c : cls)
         if (width <= (uint64_t)c * 64) return c;
     return 0;
 }
 
-hipError_t sf_launch_affine(const AffineArgs &a, int K, bool tb, hipStream_t s) {
+int sf_affine_k_for(uint64_t width) { return sf_affine_class(width, width <= sf_affine_lp16_max()); }
+
+extern "C" int sf_set_affine_lp16_max(uint64_t slots) {
+    g_lp16_max.store(slots, std::memory_order_relaxed);
+    return 0;
+}
+
+hipError_t sf_launch_affine(const AffineArgs &a, int cls, bool tb, hipStream_t s) {
     if (a.count == 0) return hipSuccess;
-    dim3 grid((a.count + 3) / 4), block(256);
-    switch (K) {
-#define CASE(KK)                                                                                 \
-    case KK:                                                                                     \
-        if (tb) hipLaunchKernelGGL((affine_wave_kernel<KK, true>), grid, block, 0, s, a);       \
-        else hipLaunchKernelGGL((affine_wave_kernel<KK, false>), grid, block, 0, s, a);         \
+    const int LP = aff_lp(cls);
+    dim3 grid((unsigned)(((uint64_t)a.count * LP + 255) / 256)), block(256);
+    switch (cls) {
+#define CASE(LL, KK)                                                                             \
+    case (LL == 64 ? 0 : LL * 1000) + KK:                                                       \
+        if (tb) hipLaunchKernelGGL((affine_wave_kernel<LL, KK, true>), grid, block, 0, s, a);   \
+        else hipLaunchKernelGGL((affine_wave_kernel<LL, KK, false>), grid, block, 0, s, a);     \
         break;
-        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(10) CASE(12) CASE(14) CASE(16)
-        CASE(20) CASE(24) CASE(28) CASE(32)
+#define CASES(LL)                                                                                \
+    CASE(LL, 1) CASE(LL, 2) CASE(LL, 3) CASE(LL, 4) CASE(LL, 5) CASE(LL, 6) CASE(LL, 7) CASE(LL, 8) \
+    CASE(LL, 10) CASE(LL, 12) CASE(LL, 14) CASE(LL, 16) CASE(LL, 20) CASE(LL, 24) CASE(LL, 28) CASE(LL, 32)
+        CASES(16)
+        CASES(64)
+#undef CASES
 #undef CASE
     default:
         return hipErrorInvalidValue;

@@ -57,7 +57,7 @@ struct WalkArgs {
     const uint64_t *below_above;
     uint32_t count;
     int mode;
-    int K;
+    int K, LP; // the DP's shape: K slots per lane, LP lanes per pair
     const int32_t *score;
     const uint32_t *score_col;
     const uint32_t *tb;
@@ -133,7 +133,7 @@ template <int CTRL, int ROWMASK = 0xf>
 __device__ __forceinline__ int32_t dpp(int32_t fill, int32_t v) {
     return __builtin_amdgcn_update_dpp(fill, v, CTRL, ROWMASK, 0xf, false);
 }
-constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118, kRowShl1 = 0x101;
 constexpr int kWaveShl1 = 0x130, kWaveShr1 = 0x138;
 constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
 
@@ -188,9 +188,19 @@ __device__ __forceinline__ int32_t wave_excl_max(int32_t v, int32_t ident) {
 
 } // namespace sf
 
+// Affine shape classes: K slots per lane and LP lanes per pair, coded K for LP = 64 and
+// 1000 * LP + K otherwise.
+constexpr int aff_lp(int cls) { return cls >= 1000 ? cls / 1000 : 64; }
+constexpr int aff_k(int cls) { return cls >= 1000 ? cls % 1000 : cls; }
+constexpr uint64_t aff_slots(int cls) { return (uint64_t)aff_lp(cls) * (uint64_t)aff_k(cls); }
+// Traceback words per row of one pair: 4 bits per slot, a lane's slots in ceil(4K / 32) words.
+constexpr uint64_t aff_tb_words_per_row(int cls) { return (uint64_t)aff_lp(cls) * ((4 * aff_k(cls) + 31) / 32); }
+
 // Launchers (defined in the kernel TUs).  Each returns hipSuccess or the launch error.
 hipError_t sf_launch_myers(const sf::MyersArgs &a, int maxb, hipStream_t s);
 hipError_t sf_launch_affine(const sf::AffineArgs &a, int K, bool tb, hipStream_t s);
+uint64_t sf_affine_lp16_max(); // the current 16-lane limit (slots), part of the shape choice
+int sf_affine_class(uint64_t width, bool lp16); // 16 lanes per pair if allowed and it fits, else 64
 hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
 hipError_t sf_launch_compact(const uint32_t *order, uint32_t count, const uint32_t *runs, const uint64_t *runs_off,
                              const uint32_t *nruns, const uint64_t *out_off, uint32_t *out, hipStream_t s);

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -303,10 +304,12 @@ __global__ void restrict_kernel(uint64_t n, const uint32_t *dist, const uint32_t
     }
 }
 
-// Runs one launch per bin on the main stream and the extra streams: longest estimate first, each
-// on the stream with the least estimated work so far, so that a few long-running bins (long
-// patterns, few waves) overlap the many short ones instead of leaving the chip idle behind them.
-// The main stream then waits for every stream used.
+// Runs one launch per bin on the main stream and the extra streams.  A bin whose work fills the
+// chip (spread >= chain) runs alone well, so those go one after another on the main stream; the
+// bins bound by their longest chain (few waves, long sequences) go beside them on the extra
+// streams, longest first, each to the stream with the least estimated time so far, so that none
+// of them holds up a big bin.  Without big bins every stream takes chain-bound ones.  The main
+// stream then waits for every stream used.
 template <class F>
 int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch) {
     constexpr int S = 1 + sf_batch::kAux;
@@ -321,16 +324,21 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch) {
             HIP_TRY(hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming));
         }
     }
-    std::vector<size_t> idx(bins.size());
-    std::iota(idx.begin(), idx.end(), (size_t)0);
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return bins[x].est() > bins[y].est(); });
+    std::vector<size_t> big, small;
+    for (size_t i = 0; i < bins.size(); ++i) (bins[i].spread >= bins[i].chain ? big : small).push_back(i);
+    std::stable_sort(small.begin(), small.end(), [&](size_t x, size_t y) { return bins[x].est() > bins[y].est(); });
     hipStream_t ss[S] = {b->stream, b->aux[0], b->aux[1], b->aux[2]};
     double load[S] = {0, 0, 0, 0};
     bool used[S] = {true, false, false, false};
     HIP_TRY(hipEventRecord(b->fork, b->stream));
-    for (size_t i : idx) {
-        int s = 0;
-        for (int t = 1; t < S; ++t)
+    for (size_t i : big) {
+        HIP_TRY(launch(bins[i], b->stream));
+        load[0] += bins[i].est();
+    }
+    const int first = big.empty() ? 0 : 1;
+    for (size_t i : small) {
+        int s = first;
+        for (int t = first + 1; t < S; ++t)
             if (load[t] < load[s]) s = t;
         if (!used[s]) {
             HIP_TRY(hipStreamWaitEvent(ss[s], b->fork, 0));
@@ -536,31 +544,53 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     if ((st = ensure_device())) return st;
     const uint64_t n = b->n;
     // Shape classes, cached across runs with the same mode and bands.
-    uint64_t key = 1469598103934665603ull ^ (uint64_t)mode;
+    uint64_t key = (1469598103934665603ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8)) * 1099511628211ull;
     if (band)
         for (uint64_t i = 0; i < n; ++i) key = (key ^ band[i]) * 1099511628211ull;
     else
         key ^= 0x9e3779b97f4a7c15ull;
     if (!b->akey_valid || b->akey != key) {
         std::vector<uint64_t> ba(n);
-        std::vector<uint8_t> kcls(n);
+        std::vector<int> kcls(n), k64(n);
         uint64_t cells = 0;
         for (uint64_t i = 0; i < n; ++i) {
             uint64_t below, above;
             const uint32_t m = b->plen[i], t = b->tlen[i];
             sf::band_geom(m, t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
             const uint64_t width = below + above + 1;
-            const int K = sf_affine_k_for(width);
-            if (K == 0)
+            kcls[i] = sf_affine_k_for(width);
+            k64[i] = sf_affine_class(width, false);
+            if (k64[i] == 0)
                 return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this build handles up to 2048",
                             (unsigned long long)i, (unsigned long long)width);
-            // Values stay far above NEG over all 64 K slots of the class:
-            // |score| <= (mat + mis + ext) * (m + n + 64 K) + 2 open.
-            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t, 64ull * (uint64_t)K))
-                return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
             ba[i] = below | (above << 32);
-            kcls[i] = (uint8_t)K;
             cells += sf::band_cells(m, t, below, above); // DP_WORK (align.rs:2573-2578)
+        }
+        // A 16-lane class with too few waves to fill the chip is bound by its longest chain of
+        // rows, which 64 lanes per pair run about four times shorter: per class, the shape with
+        // the smaller max(chain, spread) estimate (VALU per row as in the bins below) wins.
+        struct Cost {
+            double chain16 = 0, spread16 = 0, chain64 = 0, spread64 = 0;
+        };
+        std::map<int, Cost> c16;
+        for (uint64_t i = 0; i < n; ++i) {
+            if (aff_lp(kcls[i]) != 16) continue;
+            Cost &c = c16[kcls[i]];
+            const double m = b->plen[i], r16 = 9.0 * aff_k(kcls[i]) + 15.0, r64 = 9.0 * aff_k(k64[i]) + 15.0;
+            c.chain16 = std::max(c.chain16, m * r16);
+            c.spread16 += m * r16 / 4.0 / 1024.0;
+            c.chain64 = std::max(c.chain64, m * r64);
+            c.spread64 += m * r64 / 1024.0;
+        }
+        for (uint64_t i = 0; i < n; ++i) {
+            auto it = c16.find(kcls[i]);
+            if (it != c16.end()) {
+                const Cost &c = it->second;
+                if (std::max(c.chain64, c.spread64) < std::max(c.chain16, c.spread16)) kcls[i] = k64[i];
+            }
+            // Values stay far above NEG over all slots of the class.
+            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, b->plen[i], b->tlen[i], aff_slots(kcls[i])))
+                return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
         }
         std::vector<uint32_t> order(n);
         std::iota(order.begin(), order.end(), 0u);
@@ -573,10 +603,11 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
             uint64_t e = q;
             while (e < n && kcls[order[e]] == kcls[order[q]]) ++e;
             sf_batch::Bin bin{kcls[order[q]], (uint32_t)q, (uint32_t)e};
-            const double per_row = 9.0 * bin.cls + 13.0; // VALU per row of one wave
+            // VALU per row of one wave, shared by its 64 / LP pairs.
+            const double per_row = 9.0 * aff_k(bin.cls) + 15.0, ppw = 64.0 / aff_lp(bin.cls);
             for (uint64_t x = q; x < e; ++x) {
                 bin.chain = std::max(bin.chain, b->plen[order[x]] * per_row);
-                bin.spread += b->plen[order[x]] * per_row / 1024.0;
+                bin.spread += b->plen[order[x]] * per_row / ppw / 1024.0;
             }
             b->abins.push_back(bin);
             q = e;
@@ -628,7 +659,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         uint64_t used = 0;
         uint32_t cb = bin.begin;
         for (uint32_t q = bin.begin; q < bin.end; ++q) {
-            const uint64_t words = (uint64_t)b->plen[b->aorder[q]] * 64 * ((4 * bin.cls + 31) / 32);
+            const uint64_t words = (uint64_t)b->plen[b->aorder[q]] * aff_tb_words_per_row(bin.cls);
             if (used + words > budget && q > cb) {
                 chunks.push_back({bin.cls, cb, q});
                 need_max = std::max(need_max, used);
@@ -699,7 +730,8 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         w.tb = b->d_tb[buf];
         w.order = a.order;
         w.count = a.count;
-        w.K = ch.K;
+        w.K = aff_k(ch.K);
+        w.LP = aff_lp(ch.K);
         w.tb_off = a.tb_off;
         if ((st = timed_begin(b, 3, &e0, b->wstream))) return st;
         HIP_TRY(sf_launch_walk(w, b->wstream));

@@ -176,8 +176,16 @@ void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
     }
     std::iota(s.h_morder, s.h_morder + seg, 0u);
     std::iota(s.h_aorder, s.h_aorder + seg, 0u);
-    std::stable_sort(s.h_morder, s.h_morder + seg, [&](uint32_t x, uint32_t y) { return mcls[x] < mcls[y]; });
-    std::stable_sort(s.h_aorder, s.h_aorder + seg, [&](uint32_t x, uint32_t y) { return acls[x] < acls[y]; });
+    // Within a class, similar lengths side by side: the lanes (Myers) and the pairs sharing a
+    // wave (affine) then run about as many columns and rows.
+    std::stable_sort(s.h_morder, s.h_morder + seg, [&](uint32_t x, uint32_t y) {
+        if (mcls[x] != mcls[y]) return mcls[x] < mcls[y];
+        return s.h_tlen[x] < s.h_tlen[y];
+    });
+    std::stable_sort(s.h_aorder, s.h_aorder + seg, [&](uint32_t x, uint32_t y) {
+        if (acls[x] != acls[y]) return acls[x] < acls[y];
+        return s.h_plen[x] > s.h_plen[y];
+    });
     hipStream_t st = s.stream;
     int status = SF_OK;
     auto ok = [&](hipError_t e) {
@@ -359,7 +367,7 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
         const int K = sf_affine_k_for(below + above + 1);
         const bool shape = sf_myers_maxb_for((m + 63) / 64) && K &&
                            scores_in_range(sv->cfg.sc.mat, sv->cfg.sc.mis, sv->cfg.sc.open, sv->cfg.sc.ext, m, n,
-                                           64ull * (uint64_t)K);
+                                           aff_slots(K));
         if (bad || !shape) {
             delete r;
             return sf_runtime_fail(bad ? SF_EINVAL : SF_ERANGE, bad ? "segment outside the read or the store"

@@ -142,6 +142,34 @@ def test_affine_small_random(mode, sc):
     check_affine(ps, mode, sc, np.array(B, np.uint64), cigar=mode != A.Mode.Infix or True)
 
 
+@pytest.fixture
+def lanes64():
+    """Every band on 64 lanes per pair for the test, then the default back."""
+    lib = A.N.load()
+    lib.sf_set_affine_lp16_max(0)
+    yield
+    lib.sf_set_affine_lp16_max(512)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_affine_64_lane_kernel(mode, lanes64):
+    # The small bands that normally run 16 lanes per pair, through the 64-lane kernel.
+    rng = random.Random(7 + int(mode))
+    P, T, B = [], [], []
+    for _ in range(200):
+        n = rng.randrange(0, 260)
+        t = [rng.randrange(4) for _ in range(n)]
+        p = [x if rng.random() > 0.08 else rng.randrange(4) for x in t]
+        if rng.random() < 0.3:
+            p = p[rng.randrange(len(p) + 1):]
+        P.append(p)
+        T.append(t)
+        B.append(rng.choice([0, 1, 5, 16, 64, 130, A.USIZE_MAX]))
+    ps = A.PairSet.from_seqs(P, T)
+    for sc in (A.Affine.ONT, A.Affine(3, 2, 5, 1)):
+        check_affine(ps, mode, sc, np.array(B, np.uint64))
+
+
 def test_affine_edges():
     P, T = edge_pairs()
     ps = A.PairSet.from_seqs(P[:60], T[:60])
