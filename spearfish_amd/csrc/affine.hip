@@ -158,8 +158,8 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
             pw >>= 2;
             if (!ONE && i0 + r > last) continue; // this pair's rows are done
             // The row's pattern code in every 2-bit field (the low 12 fields suffice up to K = 12).
-            const uint32_t p5 = pc * 0x5555u;
-            const uint32_t prep = ONE ? pc * 0x55555555u : (K <= 12 ? pc * 0x555555u : p5 | (p5 << 16));
+            const uint32_t p5 = __umul24(pc, 0x5555u);
+            const uint32_t prep = ONE ? pc * 0x55555555u : (K <= 12 ? __umul24(pc, 0x555555u) : p5 | (p5 << 16));
             // Bit 2q+1 of y: slot q's text code differs from the pattern code.
             uint32_t y[NY];
 #pragma unroll
@@ -250,6 +250,194 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
 #pragma unroll
         for (int off = LP / 2; off > 0; off >>= 1) {
             const int32_t ov = __shfl_xor(bv, off), os = __shfl_xor(bs, off);
+            if (ov > bv || (ov == bv && os < bs)) {
+                bv = ov;
+                bs = os;
+            }
+        }
+        score = bv;
+        col = (uint32_t)(b + bs);
+    }
+    if (stopped) {
+        score = kNeg;
+        col = 0;
+    }
+    if (valid && l == 0) {
+        a.score[pi] = score;
+        if (a.score_col) a.score_col[pi] = col;
+    }
+}
+
+// ---- 16-bit packed kernel: scores only, 16 lanes per pair ----
+//
+// For scorings with mis = 2 ext (ONT, HIFI) a second bias, Hb = H + ext*s + mis*i, makes both gap
+// extensions and a mismatch step free: a match adds dm = mat + mis, a gap open subtracts `open`,
+// nothing else changes a value.  Band cells then satisfy -open - ext*below <= Hb <= dm*m + ext*s
+// (a path along the main diagonal and one gap reaches every band cell; H <= mat*i), and cells
+// outside the band -- left of column 0, which only exist in the first `below` rows -- start at
+// the floor and gain at most dm a row.  Stored as Hb + off with off = open + (dm + ext)*below + 2
+// every value is exact in unsigned 16 bits whenever off + dm*m + ext*(slots) <= 65535 (checked on
+// the host), out-of-band values stay below every band value, and the saturating subtractions only
+// ever floor out-of-band values.  This is the reference's narrow i16 pass (align.rs:2381-2426) on
+// the device, with its own bound.
+//
+// Each lane holds 2*K2 slots, slot s0 + r in the low half of register r and s0 + K2 + r in the
+// high half, so the vertical neighbour of register r is register r + 1 in both halves; the text
+// window pairs the codes the same way, so one shift puts both halves' mismatch bits 16 apart.
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ uint32_t as1(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ u16x2 pmax(u16x2 a, u16x2 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ u16x2 psub(u16x2 a, u16x2 b) { return __builtin_elementwise_sub_sat(a, b); }
+
+template <int K2>
+__global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
+    constexpr int S = 2 * K2;                     // slots per lane
+    constexpr int BLK = 16;                       // rows per block: one pattern word
+    constexpr int NWP = (K2 + BLK - 1 + 7) / 8;   // paired text words: fields q + r, q < K2, r < 16
+    const int lane = threadIdx.x & 63;
+    const int l = lane & 15;
+    const uint32_t w = (blockIdx.x * 256u + threadIdx.x) >> 4;
+    const bool valid = w < a.count;
+    const uint32_t pi = valid ? a.order[w] : 0u;
+    const int32_t m = valid ? (int32_t)a.pv.plen[pi] : 0;
+    const int32_t n = valid ? (int32_t)a.pv.tlen[pi] : 0;
+    const uint64_t ba = valid ? a.below_above[pi] : 0;
+    const int32_t below = (int32_t)(uint32_t)ba;
+    const int32_t above = (int32_t)(ba >> 32);
+    const int32_t Wd = below + above + 1;
+    const SeqRef P = seqref(a.pv.seq, valid ? a.pv.poff[pi] : 0, (uint32_t)m);
+    const SeqRef T = seqref(a.pv.seq, valid ? a.pv.toff[pi] : 0, (uint32_t)n);
+    const int32_t ext = a.ext, open = a.open, mis = a.mis;
+    const int32_t dm = a.mat + mis;
+    const bool infix = a.mode == kInfix;
+    const int32_t off = open + (dm + ext) * below + 2;
+    const int32_t s0 = l * S;
+    const u16x2 vopen = (u16x2)((uint16_t)open), vdm = (u16x2)((uint16_t)dm);
+
+    u16x2 V[K2], Fb[K2], Ub[K2], dl[K2];
+    const int32_t hi0 = above < n ? above : n;
+#pragma unroll
+    for (int r = 0; r < K2; ++r) {
+        uint32_t hv[2], dv[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int32_t s = s0 + r + h * K2;
+            const int32_t j = s - below;
+            const bool in = j >= 0 && j <= hi0;
+            hv[h] = in ? (uint32_t)(((j == 0 || infix) ? 0 : -(open + ext * j)) + ext * s + off) : 0u;
+            // Right of the band the up-values are floored, so the last band slot's vertical
+            // terms are the floor, as the reference's NEG sentinel (align.rs:1047-1048).
+            dv[h] = s >= Wd ? 0xFFFFu : (uint32_t)open;
+        }
+        V[r] = as2(hv[0] | (hv[1] << 16));
+        Fb[r] = as2(0u);
+        dl[r] = as2(dv[0] | (dv[1] << 16));
+        Ub[r] = psub(V[r], dl[r]);
+    }
+    const int32_t last = valid ? (m < n + below ? m : n + below) : 0;
+    int32_t rows = imax(last, __shfl_xor(last, 16));
+    rows = imax(rows, __shfl_xor(rows, 32));
+    rows = __builtin_amdgcn_readfirstlane(rows);
+    for (int32_t i0 = 1; i0 <= rows; i0 += BLK) {
+        const uint32_t pw = codes16(P, i0 - 1);
+        const int32_t pos = i0 - below + s0 - 1; // text position of register 0's low slot in row i0
+        uint32_t wp[NWP];
+#pragma unroll
+        for (int k = 0; k < NWP; ++k)
+            wp[k] = (codes16(T, pos + 8 * k) & 0xFFFFu) | (codes16(T, pos + K2 + 8 * k) << 16);
+#pragma unroll
+        for (int r = 0; r < BLK; ++r) {
+            if (i0 + r <= last) { // this pair's rows are not done
+                const uint32_t pc = (pw >> (2 * r)) & 3u;
+                const uint32_t p5 = __umul24(pc, 0x5555u);
+                const uint32_t prep = p5 | (p5 << 16);
+                // Bit 2f+1 (low half) and 2f+17 (high half) of y[k]: field f of word k mismatches.
+                const int k0 = r / 8, k1 = (r + K2 - 1) / 8;
+                uint32_t y[NWP];
+#pragma unroll
+                for (int k = k0; k <= k1; ++k) {
+                    const uint32_t t = wp[k] ^ prep;
+                    y[k] = (t << 1) | t;
+                }
+                const uint32_t un = dpp0<kRowShl1>(as1(Ub[0]));
+                const uint32_t fn = dpp0<kRowShl1>(as1(Fb[0]));
+                // Register K2 - 1: low half reads slot s0 + K2 (register 0, high half), high half
+                // reads slot s0 + 2 K2 (the next lane's register 0, low half).
+                const u16x2 uuK = as2(__builtin_amdgcn_alignbit(un, as1(Ub[0]), 16));
+                const u16x2 fuK = as2(__builtin_amdgcn_alignbit(fn, as1(Fb[0]), 16));
+                u16x2 X[K2], F[K2], g[K2];
+#pragma unroll
+                for (int q = 0; q < K2; ++q) {
+                    const u16x2 uu = q < K2 - 1 ? Ub[q + 1] : uuK;
+                    const u16x2 fu = q < K2 - 1 ? Fb[q + 1] : fuK;
+                    const int fld = q + r;
+                    const uint32_t mt = ~(y[fld / 8] >> (2 * (fld % 8) + 1)) & 0x00010001u; // match bits
+                    const u16x2 d = V[q] + as2(mt) * vdm;
+                    const u16x2 f = pmax(uu, fu);
+                    X[q] = pmax(d, f);
+                    F[q] = f;
+                    g[q] = psub(X[q], vopen);
+                }
+                u16x2 tot = g[0];
+#pragma unroll
+                for (int q = 1; q < K2; ++q) tot = pmax(tot, g[q]);
+                const uint32_t tt = as1(tot);
+                const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
+                const uint32_t C = group_excl_umax<16>(tlo > thi ? tlo : thi);
+                u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16)); // gap values entering register 0
+#pragma unroll
+                for (int q = 0; q < K2; ++q) {
+                    V[q] = pmax(X[q], E);
+                    if (q < K2 - 1) E = pmax(E, g[q]);
+                    Fb[q] = F[q];
+                    Ub[q] = psub(V[q], dl[q]);
+                }
+            }
+        }
+    }
+    const bool stopped = last < m;
+    int32_t score;
+    uint32_t col;
+    const int32_t rowbias = mis * m + off;
+    auto slot_value = [&](int32_t wi) { // this lane's value at its slot s0 + wi
+        uint32_t v = 0;
+#pragma unroll
+        for (int q = 0; q < K2; ++q) {
+            const uint32_t x = as1(V[q]);
+            v = wi == q ? (x & 0xFFFFu) : (wi == q + K2 ? (x >> 16) : v);
+        }
+        return v;
+    };
+    if (a.mode == kGlobal) {
+        const int32_t sN = n - m + below; // slot of (m, n)
+        const int32_t owner = sN / S, wi = sN - owner * S;
+        const uint32_t v = slot_value(wi);
+        const uint32_t vo = (uint32_t)__shfl((int)v, (lane - l) + owner);
+        score = (int32_t)vo - ext * sN - rowbias;
+        col = (uint32_t)n;
+    } else {
+        // align.rs:2643-2650: max of the last row, first argmax.
+        const int32_t b = m - below;
+        const int32_t lo = b > 0 ? b : 0;
+        const int32_t hi = (b + Wd - 1) < n ? (b + Wd - 1) : n;
+        const int32_t slo = lo - b, shi = hi - b;
+        int32_t bv = INT32_MIN;
+        int32_t bs = INT32_MAX;
+#pragma unroll
+        for (int wi = 0; wi < S; ++wi) {
+            const int32_t s = s0 + wi;
+            const uint32_t x = as1(V[wi % K2]);
+            const int32_t v = (int32_t)(wi < K2 ? (x & 0xFFFFu) : (x >> 16));
+            const int32_t h = v - ext * s - rowbias;
+            if (s >= slo && s <= shi && h > bv) {
+                bv = h;
+                bs = s;
+            }
+        }
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) {
+            const int32_t ov = __shfl_xor(bv, o), os = __shfl_xor(bs, o);
             if (ov > bv || (ov == bv && os < bs)) {
                 bv = ov;
                 bs = os;
@@ -426,6 +614,15 @@ int sf_affine_class(uint64_t width, bool lp16) {
 
 int sf_affine_k_for(uint64_t width) { return sf_affine_class(width, width <= sf_affine_lp16_max()); }
 
+int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m, uint32_t below) {
+    if (mis != 2 * ext) return 0;
+    const int kp = K + (K & 1); // two slots a register
+    const int64_t dm = (int64_t)mat + mis;
+    const int64_t off = (int64_t)open + (dm + ext) * below + 2;
+    const int64_t top = off + dm * m + (int64_t)ext * 16 * kp;
+    return top <= 65535 ? 16100 + kp : 0;
+}
+
 extern "C" int sf_set_affine_lp16_max(uint64_t slots) {
     g_lp16_max.store(slots, std::memory_order_relaxed);
     return 0;
@@ -435,6 +632,18 @@ hipError_t sf_launch_affine(const AffineArgs &a, int cls, bool tb, hipStream_t s
     if (a.count == 0) return hipSuccess;
     const int LP = aff_lp(cls);
     dim3 grid((unsigned)(((uint64_t)a.count * LP + 255) / 256)), block(256);
+    if (aff_packed(cls)) {
+        if (tb) return hipErrorInvalidValue;
+        switch (aff_k(cls)) {
+#define PK(KK)                                                                                   \
+    case 2 * KK: hipLaunchKernelGGL((affine_pk16_kernel<KK>), grid, block, 0, s, a); break;
+            PK(1) PK(2) PK(3) PK(4) PK(5) PK(6) PK(7) PK(8) PK(10) PK(12) PK(14) PK(16)
+#undef PK
+        default:
+            return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (cls) {
 #define CASE(LL, KK)                                                                             \
     case (LL == 64 ? 0 : LL * 1000) + KK:                                                       \

@@ -72,40 +72,37 @@ struct WalkArgs {
 
 // One sequence inside the packed words: code k of the sequence is code (base + k) of `seq`.
 // Word loads are clamped to the sequence's own words, so reads just outside it stay in bounds
-// (their codes are unspecified) and nothing needs padding.
+// (their codes are unspecified) and nothing needs padding.  Positions are 32-bit (sequences are
+// under 2^30 bases); only the first word's address is 64-bit.
 struct SeqRef {
-    const uint32_t *seq;
-    int64_t base;
-    int64_t w0, w1; // first and last word holding the sequence (w1 < w0: empty)
+    const uint32_t *w; // first word holding the sequence (word 0 of `seq` for an empty one)
+    int32_t lead;      // position of code 0 inside that word
+    int32_t last;      // index of the last word holding the sequence, relative to `w`
 };
 
 __device__ __forceinline__ SeqRef seqref(const uint32_t *seq, uint64_t base, uint32_t len) {
     SeqRef r;
-    r.seq = seq;
-    r.base = (int64_t)base;
-    r.w0 = (int64_t)(base >> 4);
-    r.w1 = len ? (int64_t)((base + len - 1) >> 4) : r.w0 - 1;
+    r.w = len ? seq + (base >> 4) : seq;
+    r.lead = len ? (int32_t)(base & 15) : 0;
+    r.last = len ? (int32_t)(((base & 15) + len - 1) >> 4) : 0;
     return r;
 }
 
-__device__ __forceinline__ uint32_t ldw(const SeqRef &r, int64_t w) {
-    if (r.w1 < r.w0) return 0;
-    w = w < r.w0 ? r.w0 : (w > r.w1 ? r.w1 : w);
-    return r.seq[w];
+__device__ __forceinline__ uint32_t ldw(const SeqRef &r, int32_t i) {
+    return r.w[i < 0 ? 0 : (i > r.last ? r.last : i)];
 }
 
 // 16 codes starting at sequence position s (any integer): code (s + k) at bits 2k.  Positions
 // outside the sequence give unspecified codes.
-__device__ __forceinline__ uint32_t codes16(const SeqRef &r, int64_t s) {
-    const int64_t abs = r.base + s;
-    const int64_t w = abs >> 4; // floor for negative positions
-    const uint32_t sh = (uint32_t)(abs & 15) * 2;
-    const uint32_t a = ldw(r, w), b = ldw(r, w + 1);
-    return sh ? (a >> sh) | (b << (32 - sh)) : a;
+__device__ __forceinline__ uint32_t codes16(const SeqRef &r, int32_t s) {
+    const int32_t abs = r.lead + s;
+    const int32_t i = abs >> 4; // floor for negative positions
+    const uint32_t a = ldw(r, i), b = ldw(r, i + 1);
+    return __builtin_amdgcn_alignbit(b, a, (uint32_t)(abs & 15) * 2);
 }
 
-__device__ __forceinline__ uint32_t code_at(const SeqRef &r, int64_t k) {
-    const int64_t abs = r.base + k;
+__device__ __forceinline__ uint32_t code_at(const SeqRef &r, int32_t k) {
+    const int32_t abs = r.lead + k;
     return (ldw(r, abs >> 4) >> (2 * (abs & 15))) & 3u;
 }
 
@@ -189,9 +186,10 @@ __device__ __forceinline__ int32_t wave_excl_max(int32_t v, int32_t ident) {
 } // namespace sf
 
 // Affine shape classes: K slots per lane and LP lanes per pair, coded K for LP = 64 and
-// 1000 * LP + K otherwise.
+// 1000 * LP + K otherwise, plus 100 for the 16-bit packed kernel (scores only).
 constexpr int aff_lp(int cls) { return cls >= 1000 ? cls / 1000 : 64; }
-constexpr int aff_k(int cls) { return cls >= 1000 ? cls % 1000 : cls; }
+constexpr int aff_k(int cls) { return cls >= 1000 ? cls % 100 : cls; }
+constexpr bool aff_packed(int cls) { return cls >= 1000 && cls % 1000 >= 100; }
 constexpr uint64_t aff_slots(int cls) { return (uint64_t)aff_lp(cls) * (uint64_t)aff_k(cls); }
 // Traceback words per row of one pair: 4 bits per slot, a lane's slots in ceil(4K / 32) words.
 constexpr uint64_t aff_tb_words_per_row(int cls) { return (uint64_t)aff_lp(cls) * ((4 * aff_k(cls) + 31) / 32); }
@@ -201,6 +199,9 @@ hipError_t sf_launch_myers(const sf::MyersArgs &a, int maxb, hipStream_t s);
 hipError_t sf_launch_affine(const sf::AffineArgs &a, int K, bool tb, hipStream_t s);
 uint64_t sf_affine_lp16_max(); // the current 16-lane limit (slots), part of the shape choice
 int sf_affine_class(uint64_t width, bool lp16); // 16 lanes per pair if allowed and it fits, else 64
+// The 16-bit packed class for a pair of the 16-lane class with K slots per lane, or 0 where its
+// exact range does not hold (the scoring must have mis = 2 ext).
+int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m, uint32_t below);
 hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
 hipError_t sf_launch_compact(const uint32_t *order, uint32_t count, const uint32_t *runs, const uint64_t *runs_off,
                              const uint32_t *nruns, const uint64_t *out_off, uint32_t *out, hipStream_t s);

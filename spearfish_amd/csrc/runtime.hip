@@ -172,12 +172,18 @@ struct sf_batch {
     uint32_t *d_dist = nullptr, *d_end = nullptr, *d_start = nullptr;
     std::string myers_unsupported;
     // affine
-    std::vector<Bin> abins;
-    std::vector<uint32_t> aorder; // host copy of the affine order
-    uint64_t akey = 0;
-    bool akey_valid = false;
-    uint64_t *d_ba = nullptr;
-    uint32_t *d_aorder = nullptr;
+    // Affine launch plans, cached per (mode, bands, shape tuning): one for score-only runs (which
+    // may use the 16-bit kernel), one for runs with traceback.
+    struct AffPlan {
+        uint64_t key = 0;
+        bool valid = false;
+        std::vector<Bin> bins;
+        std::vector<uint32_t> order; // host copy of the launch order
+        uint64_t *d_ba = nullptr;    // per pair: below | (above << 32)
+        uint32_t *d_order = nullptr;
+        uint64_t cells = 0;
+    };
+    AffPlan aplan[2];
     int32_t *d_score = nullptr;
     uint32_t *d_col = nullptr, *d_tstart = nullptr, *d_nruns = nullptr, *d_runs = nullptr, *d_cig = nullptr;
     uint64_t *d_runs_off = nullptr, *d_out_off = nullptr, *d_tboff = nullptr;
@@ -190,7 +196,6 @@ struct sf_batch {
     hipEvent_t fork = nullptr, joins[kAux] = {nullptr, nullptr, nullptr};
     hipEvent_t dp_done[2] = {nullptr, nullptr}, walk_done[2] = {nullptr, nullptr}, walks_end = nullptr;
     bool cigar_valid = false;
-    uint64_t affine_cells_cur = 0;
     // timing
     struct Ev {
         int cat;
@@ -227,7 +232,7 @@ struct sf_batch {
         collect();
         for (auto e : pool) (void)hipEventDestroy(e);
         dfree(d_seq), dfree(d_poff), dfree(d_toff), dfree(d_plen), dfree(d_tlen), dfree(d_morder);
-        dfree(d_dist), dfree(d_end), dfree(d_start), dfree(d_ba), dfree(d_aorder), dfree(d_score), dfree(d_col);
+        dfree(d_dist), dfree(d_end), dfree(d_start), dfree(aplan[0].d_ba), dfree(aplan[0].d_order), dfree(aplan[1].d_ba), dfree(aplan[1].d_order), dfree(d_score), dfree(d_col);
         dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
         dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0);
         for (hipStream_t x : aux)
@@ -543,13 +548,15 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     int st;
     if ((st = ensure_device())) return st;
     const uint64_t n = b->n;
-    // Shape classes, cached across runs with the same mode and bands.
+    // Shape classes, cached across runs with the same mode, bands, scoring and tuning.
+    sf_batch::AffPlan &pl = b->aplan[cigar ? 1 : 0];
     uint64_t key = (1469598103934665603ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8)) * 1099511628211ull;
+    for (int32_t v : {sc->mat, sc->mis, sc->open, sc->ext}) key = (key ^ (uint64_t)(uint32_t)v) * 1099511628211ull;
     if (band)
         for (uint64_t i = 0; i < n; ++i) key = (key ^ band[i]) * 1099511628211ull;
     else
         key ^= 0x9e3779b97f4a7c15ull;
-    if (!b->akey_valid || b->akey != key) {
+    if (!pl.valid || pl.key != key) {
         std::vector<uint64_t> ba(n);
         std::vector<int> kcls(n), k64(n);
         uint64_t cells = 0;
@@ -588,6 +595,12 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
                 const Cost &c = it->second;
                 if (std::max(c.chain64, c.spread64) < std::max(c.chain16, c.spread16)) kcls[i] = k64[i];
             }
+            // Score-only runs take the 16-bit kernel where its exact range holds.
+            if (!cigar && aff_lp(kcls[i]) == 16) {
+                const int kp = sf_affine_pk16_class(aff_k(kcls[i]), sc->mat, sc->mis, sc->open, sc->ext,
+                                                    b->plen[i], (uint32_t)ba[i]);
+                if (kp) kcls[i] = kp;
+            }
             // Values stay far above NEG over all slots of the class.
             if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, b->plen[i], b->tlen[i], aff_slots(kcls[i])))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
@@ -598,34 +611,34 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
             if (kcls[x] != kcls[y]) return kcls[x] < kcls[y];
             return b->plen[x] > b->plen[y];
         });
-        b->abins.clear();
+        pl.bins.clear();
         for (uint64_t q = 0; q < n;) {
             uint64_t e = q;
             while (e < n && kcls[order[e]] == kcls[order[q]]) ++e;
             sf_batch::Bin bin{kcls[order[q]], (uint32_t)q, (uint32_t)e};
-            // VALU per row of one wave, shared by its 64 / LP pairs.
-            const double per_row = 9.0 * aff_k(bin.cls) + 15.0, ppw = 64.0 / aff_lp(bin.cls);
+            // VALU per row of one wave, shared by its 64 / LP pairs (the 16-bit kernel: about
+            // 10 packed ops per two slots).
+            const double per_row = aff_packed(bin.cls) ? 5.0 * aff_k(bin.cls) + 25.0 : 9.0 * aff_k(bin.cls) + 15.0;
+            const double ppw = 64.0 / aff_lp(bin.cls);
             for (uint64_t x = q; x < e; ++x) {
                 bin.chain = std::max(bin.chain, b->plen[order[x]] * per_row);
                 bin.spread += b->plen[order[x]] * per_row / ppw / 1024.0;
             }
-            b->abins.push_back(bin);
+            pl.bins.push_back(bin);
             q = e;
         }
-        if (!b->d_ba && ((st = dalloc(&b->d_ba, n)) || (st = dalloc(&b->d_aorder, n)) || (st = dalloc(&b->d_score, n)) ||
-                         (st = dalloc(&b->d_col, n))))
-            return st;
-        HIP_TRY(hipMemcpy(b->d_ba, ba.data(), n * 8, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(b->d_aorder, order.data(), n * 4, hipMemcpyHostToDevice));
-        b->aorder = std::move(order);
-        b->akey = key;
-        b->akey_valid = true;
-        b->affine_cells_cur = cells;
-        b->tb_words = 0; // traceback offsets depend on the classes
+        if (!pl.d_ba && ((st = dalloc(&pl.d_ba, n)) || (st = dalloc(&pl.d_order, n)))) return st;
+        if (!b->d_score && ((st = dalloc(&b->d_score, n)) || (st = dalloc(&b->d_col, n)))) return st;
+        HIP_TRY(hipMemcpy(pl.d_ba, ba.data(), n * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(pl.d_order, order.data(), n * 4, hipMemcpyHostToDevice));
+        pl.order = std::move(order);
+        pl.key = key;
+        pl.valid = true;
+        pl.cells = cells;
     }
     AffineArgs a{};
     a.pv = view(b);
-    a.below_above = b->d_ba;
+    a.below_above = pl.d_ba;
     a.mode = mode;
     a.mat = sc->mat, a.mis = sc->mis, a.open = sc->open, a.ext = sc->ext;
     a.score = b->d_score;
@@ -633,16 +646,16 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     hipEvent_t e0;
     if (!cigar) {
         if ((st = timed_begin(b, 2, &e0))) return st;
-        if ((st = run_bins(b, b->abins, [&](const sf_batch::Bin &bin, hipStream_t s) {
+        if ((st = run_bins(b, pl.bins, [&](const sf_batch::Bin &bin, hipStream_t s) {
                  AffineArgs x = a;
-                 x.order = b->d_aorder + bin.begin;
+                 x.order = pl.d_order + bin.begin;
                  x.count = bin.end - bin.begin;
                  return sf_launch_affine(x, bin.cls, false, s);
              })))
             return st;
-        b->affine_launches += b->abins.size();
+        b->affine_launches += pl.bins.size();
         if ((st = timed_end(b, 2, e0))) return st;
-        b->affine_cells += b->affine_cells_cur;
+        b->affine_cells += pl.cells;
         b->cigar_valid = false;
         return SF_OK;
     }
@@ -655,11 +668,11 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     std::vector<uint64_t> tboff(n);
     const uint64_t budget = tb_budget_words();
     uint64_t need_max = 0;
-    for (auto &bin : b->abins) {
+    for (auto &bin : pl.bins) {
         uint64_t used = 0;
         uint32_t cb = bin.begin;
         for (uint32_t q = bin.begin; q < bin.end; ++q) {
-            const uint64_t words = (uint64_t)b->plen[b->aorder[q]] * aff_tb_words_per_row(bin.cls);
+            const uint64_t words = (uint64_t)b->plen[pl.order[q]] * aff_tb_words_per_row(bin.cls);
             if (used + words > budget && q > cb) {
                 chunks.push_back({bin.cls, cb, q});
                 need_max = std::max(need_max, used);
@@ -703,7 +716,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     HIP_TRY(hipMemcpy(b->d_tboff, tboff.data(), n * 8, hipMemcpyHostToDevice));
     WalkArgs w{};
     w.pv = view(b);
-    w.below_above = b->d_ba;
+    w.below_above = pl.d_ba;
     w.mode = mode;
     w.score = b->d_score;
     w.score_col = b->d_col;
@@ -718,7 +731,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         const int buf = (int)(c & 1);
         if (c >= 2) HIP_TRY(hipStreamWaitEvent(b->stream, b->walk_done[buf], 0));
         a.tb = b->d_tb[buf];
-        a.order = b->d_aorder + ch.begin;
+        a.order = pl.d_order + ch.begin;
         a.count = ch.end - ch.begin;
         a.tb_off = b->d_tboff + ch.begin;
         if ((st = timed_begin(b, 2, &e0))) return st;
@@ -743,10 +756,10 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     if ((st = timed_begin(b, 3, &e0))) return st;
     hipLaunchKernelGGL(scan_u32_kernel, dim3(1), dim3(1024), 0, b->stream, b->d_nruns, n, b->d_out_off);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(sf_launch_compact(b->d_aorder, (uint32_t)n, b->d_runs, b->d_runs_off, b->d_nruns, b->d_out_off, b->d_cig,
+    HIP_TRY(sf_launch_compact(pl.d_order, (uint32_t)n, b->d_runs, b->d_runs_off, b->d_nruns, b->d_out_off, b->d_cig,
                               b->stream));
     if ((st = timed_end(b, 3, e0))) return st;
-    b->affine_cells += b->affine_cells_cur;
+    b->affine_cells += pl.cells;
     b->cigar_valid = true;
     return SF_OK;
 }
@@ -770,7 +783,7 @@ int sf_batch_use_located(sf_batch *b, int on) {
         HIP_TRY(hipMemcpyAsync(b->d_tlen, b->d_tlen0, n * 4, hipMemcpyDeviceToDevice, b->stream));
         b->tlen = b->tlen0;
     }
-    b->akey_valid = false; // the affine shape classes depend on the text lengths
+    b->aplan[0].valid = b->aplan[1].valid = false; // the affine shape classes depend on the text lengths
     b->myers_cells_per_run = 0;
     for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];
     return SF_OK;

@@ -170,6 +170,39 @@ def test_affine_64_lane_kernel(mode, lanes64):
         check_affine(ps, mode, sc, np.array(B, np.uint64))
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_affine_scores_16bit(mode):
+    # Score-only runs with mis = 2 ext (ONT, HIFI) take the 16-bit kernel where its range holds:
+    # random pairs, all-match and gap-heavy extremes, and lengths around its limit at band 16
+    # (m near 10,700 for ONT), so that both kernels run in one batch.
+    rng = random.Random(53 + int(mode))
+    P, T, B = [], [], []
+    nmax = 450 if mode == A.Mode.Infix else 1500  # Infix is unbanded: m + n + 1 slots a row
+    for _ in range(300):
+        n = rng.randrange(0, nmax)
+        t = [rng.randrange(4) for _ in range(n)]
+        p = [x if rng.random() > 0.1 else rng.randrange(4) for x in t]
+        if rng.random() < 0.2:
+            p = p[:rng.randrange(len(p) + 1)] + [rng.randrange(4) for _ in range(rng.randrange(40))]
+        P.append(p)
+        T.append(t)
+        B.append(rng.choice([0, 1, 7, 16, 64, 128, 200]))
+    for m in ((3000, 500) if mode != A.Mode.Infix else (400, 300)):
+        t = [rng.randrange(4) for _ in range(m)]
+        P += [t, [(x + 1) & 3 for x in t], t[: 2 * m // 3]]
+        T += [t, t, t]
+        B += [128, 128, 64]
+    if mode != A.Mode.Infix:
+        for m in (10600, 10700, 10800, 11000):
+            t = [rng.randrange(4) for _ in range(m)]
+            P.append([x if rng.random() > 0.05 else rng.randrange(4) for x in t])
+            T.append(t)
+            B.append(16)
+    ps = A.PairSet.from_seqs(P, T)
+    for sc in (A.Affine.ONT, A.Affine.HIFI):
+        check_affine(ps, mode, sc, np.array(B, np.uint64), cigar=False)
+
+
 def test_affine_edges():
     P, T = edge_pairs()
     ps = A.PairSet.from_seqs(P[:60], T[:60])
