@@ -41,11 +41,17 @@ sys.path.insert(0, HERE)
 
 METRIC = "GCUPS (bit-exact) batched vs per-pair on 1 GPU; serve reads/s + p50/p99 latency"
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 i32 lanes/clk x 2.4 GHz
+VALU16_PEAK_TOPS = 2 * VALU_PEAK_TOPS      # packed 16-bit: two lane-ops per lane and clock
 HBM_PEAK_GBS = 8000.0
-# Algorithmic i32 ops per affine cell (F: 2 sub + 2 max; substitution: compare + select;
-# x: add + max; E/H: 2 max + 1 sub); the traceback adds 4 compares.
+# Algorithmic ops per affine cell of the reference's recurrence (F: 2 sub + 2 max; substitution:
+# compare + select; x: add + max; E/H: 2 max + 1 sub); the traceback adds 4 compares.
 AFFINE_OPS_PER_CELL = 11
 AFFINE_TB_OPS_PER_CELL = 11 + 4
+# HBM bytes per launch of the dominant kernels on the uniform workload, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the same counters on a 1 GiB copy
+# (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.
+TRAFFIC_BYTES = {"affine_pk16_kernel<10>": int((324116.0 * 2.0 + 58173.9) * 1024),
+                 "affine_wave_kernel<16,20,true>": int((400042.2 * 2.0 + 31494083.6) * 1024)}
 
 
 def parse():
@@ -142,14 +148,16 @@ def bench_uniform(args):
         log(f"{label}: timed {steps} steps in {wall:.3f}s")
         return wall, tm
 
-    def roofline(tm, ops, kernel):
+    def roofline(tm, ops, kernel, peak, unit):
         per_launch_ms = tm.affine_ms / max(tm.affine_launches, 1)
         cells_per_launch = tm.affine_cells / max(tm.affine_launches, 1)
         achieved = cells_per_launch * ops / (per_launch_ms * 1e-3) / 1e12
-        return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 1),
-                "unit": "TOP/s (i32 VALU lane-ops)", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+        traffic = TRAFFIC_BYTES.get(kernel)
+        return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3), "peak": round(peak, 1),
+                "unit": unit, "frac": round(achieved / peak, 4), "traffic": traffic,
                 "ops_per_cell": ops, "cells_per_launch": int(cells_per_launch),
-                "avg_launch_ms": round(per_launch_ms, 3)}
+                "avg_launch_ms": round(per_launch_ms, 3),
+                "hbm_gbs": round(traffic / (per_launch_ms * 1e-3) / 1e9, 1) if traffic else None}
 
     wall, tm = run(False, args.steps, args.warmup, "scores")
     cells = tm.myers_cells + tm.affine_cells
@@ -168,7 +176,7 @@ def bench_uniform(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": "u64 bit-vectors (Myers) + u16x2 packed, exact (affine)",
         "data": "synthetic",
         "config": {"workload": "uniform-1e6x1000bp-6pct-band128", "pairs": len(pairs), "target_bp": 1000,
                    "error": "6% (3% sub, 1.5% ins, 1.5% del, geometric indels p=0.7)", "band": 128,
@@ -177,7 +185,8 @@ def bench_uniform(args):
         "affine_gcups": round(tm.affine_cells / (tm.affine_ms * 1e-3) / 1e9, 1) if tm.affine_ms else None,
         "kernel_ms_per_step": {"myers": round(tm.myers_ms / args.steps, 2),
                                "affine": round(tm.affine_ms / args.steps, 2)},
-        "roofline": roofline(tm, AFFINE_OPS_PER_CELL, "affine_wave_kernel<5,false> (banded affine DP, scores)"),
+        "roofline": roofline(tm, AFFINE_OPS_PER_CELL, "affine_pk16_kernel<10>", VALU16_PEAK_TOPS,
+                             "TOP/s (packed i16 VALU lane-ops; 16 lanes x 20 slots a pair, 4 pairs a wave)"),
     }
     wall_c, tm_c = run(True, args.steps, 1, "with cigar")
     out["with_cigar"] = {
@@ -187,7 +196,8 @@ def bench_uniform(args):
         "kernel_ms_per_step": {"myers": round(tm_c.myers_ms / args.steps, 2),
                                "affine_tb": round(tm_c.affine_ms / args.steps, 2),
                                "walk_compact": round(tm_c.walk_ms / args.steps, 2)},
-        "roofline": roofline(tm_c, AFFINE_TB_OPS_PER_CELL, "affine_wave_kernel<5,true> (DP + traceback bits)"),
+        "roofline": roofline(tm_c, AFFINE_TB_OPS_PER_CELL, "affine_wave_kernel<16,20,true>", VALU_PEAK_TOPS,
+                             "TOP/s (i32 VALU lane-ops; DP + traceback bits)"),
     }
     if args.per_pair:
         try:
@@ -196,7 +206,8 @@ def bench_uniform(args):
             out["per_pair_launch_gcups"] = f"error: {e}"
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        k = args.cpu_pairs or max(200, min(len(pairs), 1000 * threads))
+        # About 10 s of host work at 16 threads.
+        k = args.cpu_pairs or max(200, min(len(pairs), 16000 * threads))
         from spearfish_amd.align import PairSet
         sub = PairSet(pairs.pat, pairs.pat_off[:k], pairs.pat_len[:k], pairs.txt, pairs.txt_off[:k],
                       pairs.txt_len[:k])

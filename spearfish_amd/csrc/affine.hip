@@ -36,7 +36,8 @@
 // Traceback bits (own definition; the reference has no traceback), per cell
 //   nd: H != Hdiag + s,  ne: H != x,  fx: F > Hup - go,  ex: E > Hleft - go,
 // accumulated per lane, 8 slots per u32 (slot q's nibble nd,ne,fx,ex from high to low, the first
-// slot highest): row i, word wd of lane L at (i-1)*64*NWL + wd*64 + L, NWL = ceil(4K / 32).
+// slot highest), NWL = ceil(4K / 32) words per lane and row, stored in 16-row blocks: row i, word
+// wd of lane L at ((b * NWL + wd) * LP + L) * 16 + (i - 1) % 16 with b = (i - 1) / 16.
 // The walk reads them back under the oracle's tie rules.
 #include "internal.hpp"
 
@@ -100,14 +101,14 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
     if (ONE && !valid) return; // whole waves only
     auto uni = [](int32_t v) { return ONE ? __builtin_amdgcn_readfirstlane(v) : v; };
     const uint32_t pi = valid ? (uint32_t)uni((int32_t)a.order[w]) : 0u;
-    const int32_t m = valid ? uni((int32_t)a.pv.plen[pi]) : 0;
-    const int32_t n = valid ? uni((int32_t)a.pv.tlen[pi]) : 0;
-    const uint64_t ba = valid ? a.below_above[pi] : 0;
-    const int32_t below = uni((int32_t)(uint32_t)ba);
-    const int32_t above = uni((int32_t)(ba >> 32));
+    const PairDesc dsc = valid ? a.desc[w] : PairDesc{0, 0, 0, 0, 0, 0};
+    const int32_t m = uni((int32_t)dsc.m);
+    const int32_t n = uni((int32_t)dsc.n);
+    const int32_t below = uni((int32_t)dsc.below);
+    const int32_t above = uni((int32_t)dsc.above);
     const int32_t Wd = below + above + 1;
-    const SeqRef P = seqref(a.pv.seq, valid ? a.pv.poff[pi] : 0, (uint32_t)m);
-    const SeqRef T = seqref(a.pv.seq, valid ? a.pv.toff[pi] : 0, (uint32_t)n);
+    const SeqRef P = seqref(a.pv.seq, dsc.poff, (uint32_t)m);
+    const SeqRef T = seqref(a.pv.seq, dsc.toff, (uint32_t)n);
     const int32_t mat = a.mat, mis = a.mis, ext = a.ext;
     const uint32_t dlt = (uint32_t)-a.open; // ext - go
     const uint32_t scm = (uint32_t)(mat + 2 * ext);
@@ -144,8 +145,14 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
         for (int o = 32; o >= LP; o >>= 1) rows = imax(rows, __shfl_xor(rows, o));
         rows = __builtin_amdgcn_readfirstlane(rows);
     }
+    // Traceback: a lane stages its words of the current 16-row block in LDS (17-word stride:
+    // conflict-free) and then writes each lane-word's 16 rows as one 64-byte line, the unit the
+    // walk reads back.  Layout per pair: word ((b * NWL + wd) * LP + l) * 16 + (i - 1) % 16 for
+    // row i in block b = (i - 1) / 16.
+    __shared__ uint32_t tbs[TB ? 4 * NWL * 64 * 17 : 1];
+    uint32_t *stage = tbs + ((threadIdx.x >> 6) * NWL * 64 + lane) * 17;
     uint32_t *tbp = nullptr;
-    if constexpr (TB) tbp = a.tb + (valid ? a.tb_off[w] : 0) + l;
+    if constexpr (TB) tbp = a.tb + (valid ? a.tb_off[w] : 0) + l * 16;
     for (int32_t i0 = 1; i0 <= rows; i0 += BLK) {
         uint32_t pw = (uint32_t)uni((int32_t)codes16(P, i0 - 1));
         uint32_t win[NWIN];
@@ -197,7 +204,6 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
             for (int q = 0; q < K; ++q) Ub[q] = V[q] + dltv[q];
             if constexpr (TB) {
                 const uint32_t ul0 = dpp0<SHR>(Ub[K - 1]);
-                uint32_t *row = tbp + (uint32_t)(i0 + r - 1) * (LP * NWL);
 #pragma unroll
                 for (int wd = 0; wd < NWL; ++wd) {
                     uint32_t acc = 0;
@@ -209,11 +215,24 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
                         acc = push_ugt(acc, FQ[q], Uu[q]);
                         acc = push_ugt(acc, E[q], ul);
                     }
-                    row[wd * LP] = acc;
+                    stage[wd * 64 * 17 + r] = acc;
                 }
             }
 #pragma unroll
             for (int k = 0; k < NWIN; ++k) win[k] = k + 1 < NWIN ? __builtin_amdgcn_alignbit(win[k + 1], win[k], 2) : win[k] >> 2;
+        }
+        if constexpr (TB) {
+            if (valid && i0 <= last) {
+                uint32_t *blk = tbp + (uint32_t)((i0 - 1) >> 4) * (NWL * LP * 16);
+#pragma unroll
+                for (int wd = 0; wd < NWL; ++wd) {
+                    const uint32_t *src = stage + wd * 64 * 17;
+                    uint4 *dst = reinterpret_cast<uint4 *>(blk + wd * LP * 16);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        dst[k] = make_uint4(src[4 * k], src[4 * k + 1], src[4 * k + 2], src[4 * k + 3]);
+                }
+            }
         }
     }
     const bool stopped = last < m;
@@ -300,14 +319,14 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
     const uint32_t w = (blockIdx.x * 256u + threadIdx.x) >> 4;
     const bool valid = w < a.count;
     const uint32_t pi = valid ? a.order[w] : 0u;
-    const int32_t m = valid ? (int32_t)a.pv.plen[pi] : 0;
-    const int32_t n = valid ? (int32_t)a.pv.tlen[pi] : 0;
-    const uint64_t ba = valid ? a.below_above[pi] : 0;
-    const int32_t below = (int32_t)(uint32_t)ba;
-    const int32_t above = (int32_t)(ba >> 32);
+    const PairDesc dsc = valid ? a.desc[w] : PairDesc{0, 0, 0, 0, 0, 0};
+    const int32_t m = (int32_t)dsc.m;
+    const int32_t n = (int32_t)dsc.n;
+    const int32_t below = (int32_t)dsc.below;
+    const int32_t above = (int32_t)dsc.above;
     const int32_t Wd = below + above + 1;
-    const SeqRef P = seqref(a.pv.seq, valid ? a.pv.poff[pi] : 0, (uint32_t)m);
-    const SeqRef T = seqref(a.pv.seq, valid ? a.pv.toff[pi] : 0, (uint32_t)n);
+    const SeqRef P = seqref(a.pv.seq, dsc.poff, (uint32_t)m);
+    const SeqRef T = seqref(a.pv.seq, dsc.toff, (uint32_t)n);
     const int32_t ext = a.ext, open = a.open, mis = a.mis;
     const int32_t dm = a.mat + mis;
     const bool infix = a.mode == kInfix;
@@ -458,11 +477,11 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
 
 // One lane per pair: walks the traceback bits from the end cell back to the start, emitting
 // BAM ops last to first (rules in the header comment of affine_wave_kernel and the oracle).
-// Each step reads one u32 -- the word of the cell's lane for that row, which holds all K slots of
-// the lane -- so the walk keeps the last 16 rows of its current lane-word in LDS: diagonal,
+// Each step reads one u32 -- the word of the cell's lane for that row, which holds 8 slots of
+// the lane -- so the walk keeps the 16-row block of its current lane-word in LDS: diagonal,
 // vertical and horizontal moves stay inside it until the path crosses to another lane's slots
-// or 16 rows up, and a refill issues its 16 loads at once.  Pattern and text codes are read a
-// word (16 bases) at a time.
+// or another block, and a refill is one 64-byte line.  Pattern and text codes are read a word
+// (16 bases) at a time.
 constexpr int kWalkRows = 16;
 
 __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
@@ -477,14 +496,18 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
         a.tstart[pi] = 0;
         return;
     }
-    const int64_t m = a.pv.plen[pi], n = a.pv.tlen[pi];
-    const int64_t below = (int64_t)(uint32_t)a.below_above[pi];
-    const SeqRef P = seqref(a.pv.seq, a.pv.poff[pi], (uint32_t)m);
-    const SeqRef T = seqref(a.pv.seq, a.pv.toff[pi], (uint32_t)n);
-    const int K = a.K, LP = a.LP;
-    const int64_t nwl = (4 * K + 31) / 32;
+    const PairDesc dsc = a.desc[t];
+    const int32_t m = (int32_t)dsc.m, n = (int32_t)dsc.n, below = (int32_t)dsc.below;
+    const SeqRef P = seqref(a.pv.seq, dsc.poff, (uint32_t)m);
+    const SeqRef T = seqref(a.pv.seq, dsc.toff, (uint32_t)n);
+    const int32_t K = a.K, LP = a.LP;
+    const int32_t nwl = (4 * K + 31) / 32, slots = LP * K;
     const uint32_t *tb = a.tb + a.tb_off[t];
-    int64_t i = m, j = a.mode == kGlobal ? n : (int64_t)a.score_col[pi];
+    int32_t i = m, j = a.mode == kGlobal ? n : (int32_t)a.score_col[pi];
+    // The cell's slot s = j - i + below as lane L and slot-in-lane q: a diagonal move keeps it, a
+    // vertical one adds 1, a horizontal one subtracts 1.
+    int32_t s = j - i + below;
+    int32_t L = s >= 0 ? s / K : -1, q = s - L * K;
     int state = 0; // 0 H, 1 F (vertical), 2 E (horizontal)
     uint32_t nr = 0, cur_op = 0, cur_len = 0, tstart = 0;
     auto emit = [&](uint32_t op, uint32_t len) {
@@ -498,34 +521,37 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
         }
     };
     bool bad = false;
-    int64_t cL = -1, cwd = -1, clo = 0; // cached lane-word and its first row
-    // The 4 bits (nd, ne, fx, ex) of cell (ii, jj) in bits 3..0 of the result.
-    auto bits_at = [&](int64_t ii, int64_t jj) -> uint32_t {
-        const int64_t s = jj - (ii - below);
-        if (ii < 1 || ii > m || s < 0 || s >= (int64_t)LP * K) {
+    int32_t cL = -1, cwd = -1, cb = -1; // cached lane-word and its 16-row block
+    // The 4 bits (nd, ne, fx, ex) of the current cell (i, j) in bits 3..0 of the result.
+    auto bits_here = [&]() -> uint32_t {
+        if (i < 1 || i > m || s < 0 || s >= slots) {
             // Off the band: cannot happen on a path the DP produced; stop rather than read out of
             // the pair's bits.
             bad = true;
             return 0;
         }
-        const int64_t L = s / K, q = s % K;
-        const int64_t wd = q / 8, nq = (K - wd * 8) < 8 ? (K - wd * 8) : 8;
-        if (L != cL || wd != cwd || ii < clo || ii >= clo + kWalkRows) {
-            clo = ii - (kWalkRows - 1) > 1 ? ii - (kWalkRows - 1) : 1;
-            const uint32_t *col = tb + wd * LP + L;
-            uint32_t w[kWalkRows];
+        const int32_t wd = q >> 3, b = (i - 1) >> 4;
+        if (L != cL || wd != cwd || b != cb) {
+            // One 64-byte line: this lane-word's 16 rows of block b.
+            const uint4 *line = reinterpret_cast<const uint4 *>(tb + ((uint64_t)(b * nwl + wd) * LP + L) * 16);
+            uint4 v[4];
 #pragma unroll
-            for (int r = 0; r < kWalkRows; ++r)
-                w[r] = clo + r <= ii ? col[(uint64_t)(clo + r - 1) * (LP * nwl)] : 0u;
+            for (int k = 0; k < 4; ++k) v[k] = line[k];
 #pragma unroll
-            for (int r = 0; r < kWalkRows; ++r) my[r] = w[r];
+            for (int k = 0; k < 4; ++k) {
+                my[4 * k] = v[k].x;
+                my[4 * k + 1] = v[k].y;
+                my[4 * k + 2] = v[k].z;
+                my[4 * k + 3] = v[k].w;
+            }
             cL = L;
             cwd = wd;
+            cb = b;
         }
-        const uint32_t word = my[ii - clo];
-        return (word >> (4 * (nq - 1 - (q % 8)))) & 15u;
+        const int32_t nq = (K - (wd << 3)) < 8 ? (K - (wd << 3)) : 8;
+        return (my[(i - 1) & 15] >> (4 * (nq - 1 - (q & 7)))) & 15u;
     };
-    int64_t pwi = -1, twi = -1; // cached code words (pattern, text)
+    int32_t pwi = -1, twi = -1; // cached code words (pattern, text)
     uint32_t pwv = 0, twv = 0;
     for (;;) {
         if (bad) break;
@@ -539,7 +565,7 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
                 emit(1, (uint32_t)i);
                 break;
             }
-            const uint32_t w = bits_at(i, j);
+            const uint32_t w = bits_here();
             if ((w & 8u) == 0) {
                 if ((i - 1) >> 4 != pwi) {
                     pwi = (i - 1) >> 4;
@@ -561,12 +587,22 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
             }
         } else if (state == 1) {
             emit(1, 1);
-            state = (bits_at(i, j) & 2u) ? 1 : 0;
+            state = (bits_here() & 2u) ? 1 : 0;
             --i;
+            ++s;
+            if (++q == K) {
+                q = 0;
+                ++L;
+            }
         } else {
             emit(2, 1);
-            state = (bits_at(i, j) & 1u) ? 2 : 0;
+            state = (bits_here() & 1u) ? 2 : 0;
             --j;
+            --s;
+            if (--q < 0) {
+                q = K - 1;
+                --L;
+            }
         }
     }
     if (cur_len) out[nr++] = (cur_len << 4) | cur_op;
@@ -660,6 +696,24 @@ hipError_t sf_launch_affine(const AffineArgs &a, int cls, bool tb, hipStream_t s
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+namespace {
+__global__ __launch_bounds__(256) void desc_kernel(uint64_t n, const uint32_t *order, PairsView pv, const uint64_t *ba,
+                                                   PairDesc *desc) {
+    const uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (w >= n) return;
+    const uint32_t pi = order[w];
+    const uint64_t g = ba ? ba[pi] : 0;
+    desc[w] = PairDesc{pv.poff[pi], pv.toff[pi], pv.plen[pi], pv.tlen[pi], (uint32_t)g, (uint32_t)(g >> 32)};
+}
+} // namespace
+
+hipError_t sf_launch_desc(uint64_t n, const uint32_t *order, const PairsView &pv, const uint64_t *ba, PairDesc *desc,
+                          hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(desc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, order, pv, ba, desc);
     return hipGetLastError();
 }
 

@@ -26,8 +26,17 @@ struct PairsView {
     const uint32_t *tlen;
 };
 
+// One pair as a launch reads it, stored in launch order so that a launch reads its pairs' sizes
+// and offsets sequentially (32 bytes a pair) rather than gathering them by pair index.
+struct PairDesc {
+    uint64_t poff, toff; // base offsets of the pattern and the text
+    uint32_t m, n;       // their lengths
+    uint32_t below, above; // affine band geometry (0 for Myers launches)
+};
+
 struct MyersArgs {
     PairsView pv;
+    const PairDesc *desc; // per order slot of the launch
     const uint32_t *order;
     uint32_t count;
     int mode;
@@ -40,8 +49,8 @@ struct MyersArgs {
 
 struct AffineArgs {
     PairsView pv;
+    const PairDesc *desc; // per order slot of the launch
     const uint32_t *order;
-    const uint64_t *below_above; // per pair: below | (above << 32); full mode is (m, n)
     uint32_t count;
     int mode;
     int32_t mat, mis, open, ext;
@@ -53,8 +62,8 @@ struct AffineArgs {
 
 struct WalkArgs {
     PairsView pv;
+    const PairDesc *desc; // per order slot of the launch
     const uint32_t *order;
-    const uint64_t *below_above;
     uint32_t count;
     int mode;
     int K, LP; // the DP's shape: K slots per lane, LP lanes per pair
@@ -203,6 +212,9 @@ int sf_affine_class(uint64_t width, bool lp16); // 16 lanes per pair if allowed 
 // exact range does not hold (the scoring must have mis = 2 ext).
 int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m, uint32_t below);
 hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
+// desc[w] of pair order[w], w < n (ba may be null: no band).
+hipError_t sf_launch_desc(uint64_t n, const uint32_t *order, const sf::PairsView &pv, const uint64_t *ba,
+                          sf::PairDesc *desc, hipStream_t s);
 hipError_t sf_launch_compact(const uint32_t *order, uint32_t count, const uint32_t *runs, const uint64_t *runs_off,
                              const uint32_t *nruns, const uint64_t *out_off, uint32_t *out, hipStream_t s);
 hipError_t sf_launch_probe(int32_t *out, hipStream_t s);
@@ -226,16 +238,13 @@ inline void band_geom(uint32_t m, uint32_t n, int mode, uint64_t band, uint64_t 
     }
 }
 
-// Cells of a pass as the reference's DP_WORK counts them (align.rs:2573-2578).
+// Cells of a pass as the reference's DP_WORK counts them (align.rs:2573-2578): the sum over rows
+// r = 1..R, R = min(m, n + below), of min(n, r + above) - max(0, r - below) + 1, in closed form.
 inline uint64_t band_cells(uint32_t m, uint32_t n, uint64_t below, uint64_t above) {
-    uint64_t cells = 0;
-    for (uint64_t r = 1; r <= m; ++r) {
-        const uint64_t lo = r > below ? r - below : 0;
-        const uint64_t hi = r + above < n ? r + above : n;
-        if (lo > hi) break;
-        cells += hi + 1 - lo;
-    }
-    return cells;
+    const uint64_t R = m < (uint64_t)n + below ? m : (uint64_t)n + below;
+    const uint64_t c = n > above ? ((uint64_t)n - above < R ? (uint64_t)n - above : R) : 0; // rows with r + above < n
+    const uint64_t d = R > below ? R - below : 0;                                           // rows with r > below
+    return R + c * (c + 1) / 2 + c * above + (R - c) * n - d * (d + 1) / 2;
 }
 
 // Bound on the biased values Hb = H + ext (s + 2 i) over `width` slots: |H| <= (mat + mis + ext)

@@ -43,10 +43,11 @@ __global__ __launch_bounds__(256) void myers_lane_kernel(MyersArgs a) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     const bool valid = t < a.count;
     const uint32_t pi = valid ? a.order[t] : 0;
-    uint32_t m = valid ? a.pv.plen[pi] : 0;
-    const uint32_t tlen = valid ? a.pv.tlen[pi] : 0;
-    const SeqRef P = seqref(a.pv.seq, valid ? a.pv.poff[pi] : 0, m);
-    const SeqRef T = seqref(a.pv.seq, valid ? a.pv.toff[pi] : 0, tlen);
+    const PairDesc dsc = valid ? a.desc[t] : PairDesc{0, 0, 0, 0, 0, 0};
+    uint32_t m = dsc.m;
+    const uint32_t tlen = dsc.n;
+    const SeqRef P = seqref(a.pv.seq, dsc.poff, m);
+    const SeqRef T = seqref(a.pv.seq, dsc.toff, tlen);
     const bool rev = a.reverse != 0;
     int mode = a.mode;
     uint32_t n = tlen;
@@ -172,10 +173,11 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
     const uint32_t grp = (blockIdx.x * 256u + threadIdx.x) / G;
     const bool valid = grp < a.count;
     const uint32_t pi = valid ? a.order[grp] : 0;
-    const uint32_t m = valid ? a.pv.plen[pi] : 0;
-    const uint32_t tlen = valid ? a.pv.tlen[pi] : 0;
-    const SeqRef P = seqref(a.pv.seq, valid ? a.pv.poff[pi] : 0, m);
-    const SeqRef T = seqref(a.pv.seq, valid ? a.pv.toff[pi] : 0, tlen);
+    const PairDesc dsc = valid ? a.desc[grp] : PairDesc{0, 0, 0, 0, 0, 0};
+    const uint32_t m = dsc.m;
+    const uint32_t tlen = dsc.n;
+    const SeqRef P = seqref(a.pv.seq, dsc.poff, m);
+    const SeqRef T = seqref(a.pv.seq, dsc.toff, tlen);
     const bool rev = a.reverse != 0;
     int mode = a.mode;
     uint32_t n = tlen;

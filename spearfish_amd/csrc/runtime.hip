@@ -169,6 +169,7 @@ struct sf_batch {
     };
     std::vector<Bin> mbins;
     uint32_t *d_morder = nullptr;
+    PairDesc *d_mdesc = nullptr; // pairs in Myers launch order
     uint32_t *d_dist = nullptr, *d_end = nullptr, *d_start = nullptr;
     std::string myers_unsupported;
     // affine
@@ -181,6 +182,7 @@ struct sf_batch {
         std::vector<uint32_t> order; // host copy of the launch order
         uint64_t *d_ba = nullptr;    // per pair: below | (above << 32)
         uint32_t *d_order = nullptr;
+        PairDesc *d_desc = nullptr;  // pairs in launch order
         uint64_t cells = 0;
     };
     AffPlan aplan[2];
@@ -232,7 +234,8 @@ struct sf_batch {
         collect();
         for (auto e : pool) (void)hipEventDestroy(e);
         dfree(d_seq), dfree(d_poff), dfree(d_toff), dfree(d_plen), dfree(d_tlen), dfree(d_morder);
-        dfree(d_dist), dfree(d_end), dfree(d_start), dfree(aplan[0].d_ba), dfree(aplan[0].d_order), dfree(aplan[1].d_ba), dfree(aplan[1].d_order), dfree(d_score), dfree(d_col);
+        dfree(d_dist), dfree(d_end), dfree(d_start), dfree(aplan[0].d_ba), dfree(aplan[0].d_order), dfree(aplan[1].d_ba), dfree(aplan[1].d_order), dfree(d_score);
+        dfree(aplan[0].d_desc), dfree(aplan[1].d_desc), dfree(d_mdesc), dfree(d_col);
         dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
         dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0);
         for (hipStream_t x : aux)
@@ -443,6 +446,7 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
     if ((st = dalloc(&b->d_seq, seq.size())) || (st = dalloc(&b->d_poff, n)) || (st = dalloc(&b->d_toff, n)) ||
         (st = dalloc(&b->d_plen, n)) || (st = dalloc(&b->d_tlen, n)) || (st = dalloc(&b->d_dist, n)) ||
         (st = dalloc(&b->d_end, n)) || (st = dalloc(&b->d_start, n)) || (st = dalloc(&b->d_morder, n)) ||
+        (st = dalloc(&b->d_mdesc, n)) ||
         (st = dalloc(&b->d_toff0, n)) || (st = dalloc(&b->d_tlen0, n)))
         return bail(st);
     // Myers bins: by blocks per pattern (the lane kernel's register class), then text length.
@@ -490,6 +494,9 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
     hipError_t e7 = hipMemcpy(b->d_toff0, tbase.data(), n * 8, hipMemcpyHostToDevice);
     hipError_t e8 = hipMemcpy(b->d_tlen0, b->tlen.data(), n * 4, hipMemcpyHostToDevice);
     if (e1 || e2 || e3 || e4 || e5 || e6 || e7 || e8) return bail(fail(SF_EDEVICE, "upload failed"));
+    if (sf_launch_desc(n, b->d_morder, view(b), nullptr, b->d_mdesc, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return bail(fail(SF_EDEVICE, "launch records failed"));
     b->tlen0 = b->tlen;
     *out = b;
     return SF_OK;
@@ -515,6 +522,7 @@ int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
     if ((st = run_bins(b, b->mbins, [&](const sf_batch::Bin &bin, hipStream_t s) {
              MyersArgs x = a;
              x.order = b->d_morder + bin.begin;
+             x.desc = b->d_mdesc + bin.begin;
              x.count = bin.end - bin.begin;
              return sf_launch_myers(x, bin.cls, s);
          })))
@@ -528,6 +536,7 @@ int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
         if ((st = run_bins(b, b->mbins, [&](const sf_batch::Bin &bin, hipStream_t s) {
                  MyersArgs x = a;
                  x.order = b->d_morder + bin.begin;
+                 x.desc = b->d_mdesc + bin.begin;
                  x.count = bin.end - bin.begin;
                  return sf_launch_myers(x, bin.cls, s);
              })))
@@ -627,10 +636,12 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
             pl.bins.push_back(bin);
             q = e;
         }
-        if (!pl.d_ba && ((st = dalloc(&pl.d_ba, n)) || (st = dalloc(&pl.d_order, n)))) return st;
+        if (!pl.d_ba && ((st = dalloc(&pl.d_ba, n)) || (st = dalloc(&pl.d_order, n)) || (st = dalloc(&pl.d_desc, n))))
+            return st;
         if (!b->d_score && ((st = dalloc(&b->d_score, n)) || (st = dalloc(&b->d_col, n)))) return st;
         HIP_TRY(hipMemcpy(pl.d_ba, ba.data(), n * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(pl.d_order, order.data(), n * 4, hipMemcpyHostToDevice));
+        HIP_TRY(sf_launch_desc(n, pl.d_order, view(b), pl.d_ba, pl.d_desc, b->stream));
         pl.order = std::move(order);
         pl.key = key;
         pl.valid = true;
@@ -638,7 +649,6 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     }
     AffineArgs a{};
     a.pv = view(b);
-    a.below_above = pl.d_ba;
     a.mode = mode;
     a.mat = sc->mat, a.mis = sc->mis, a.open = sc->open, a.ext = sc->ext;
     a.score = b->d_score;
@@ -649,6 +659,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         if ((st = run_bins(b, pl.bins, [&](const sf_batch::Bin &bin, hipStream_t s) {
                  AffineArgs x = a;
                  x.order = pl.d_order + bin.begin;
+                 x.desc = pl.d_desc + bin.begin;
                  x.count = bin.end - bin.begin;
                  return sf_launch_affine(x, bin.cls, false, s);
              })))
@@ -672,7 +683,8 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         uint64_t used = 0;
         uint32_t cb = bin.begin;
         for (uint32_t q = bin.begin; q < bin.end; ++q) {
-            const uint64_t words = (uint64_t)b->plen[pl.order[q]] * aff_tb_words_per_row(bin.cls);
+            // Whole 16-row blocks (the kernel writes and the walk reads a block's line at once).
+            const uint64_t words = (((uint64_t)b->plen[pl.order[q]] + 15) & ~15ull) * aff_tb_words_per_row(bin.cls);
             if (used + words > budget && q > cb) {
                 chunks.push_back({bin.cls, cb, q});
                 need_max = std::max(need_max, used);
@@ -716,7 +728,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     HIP_TRY(hipMemcpy(b->d_tboff, tboff.data(), n * 8, hipMemcpyHostToDevice));
     WalkArgs w{};
     w.pv = view(b);
-    w.below_above = pl.d_ba;
+
     w.mode = mode;
     w.score = b->d_score;
     w.score_col = b->d_col;
@@ -732,6 +744,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         if (c >= 2) HIP_TRY(hipStreamWaitEvent(b->stream, b->walk_done[buf], 0));
         a.tb = b->d_tb[buf];
         a.order = pl.d_order + ch.begin;
+        a.desc = pl.d_desc + ch.begin;
         a.count = ch.end - ch.begin;
         a.tb_off = b->d_tboff + ch.begin;
         if ((st = timed_begin(b, 2, &e0))) return st;
@@ -742,6 +755,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         HIP_TRY(hipStreamWaitEvent(b->wstream, b->dp_done[buf], 0));
         w.tb = b->d_tb[buf];
         w.order = a.order;
+        w.desc = a.desc;
         w.count = a.count;
         w.K = aff_k(ch.K);
         w.LP = aff_lp(ch.K);
@@ -783,6 +797,7 @@ int sf_batch_use_located(sf_batch *b, int on) {
         HIP_TRY(hipMemcpyAsync(b->d_tlen, b->d_tlen0, n * 4, hipMemcpyDeviceToDevice, b->stream));
         b->tlen = b->tlen0;
     }
+    HIP_TRY(sf_launch_desc(n, b->d_morder, view(b), nullptr, b->d_mdesc, b->stream)); // the texts moved
     b->aplan[0].valid = b->aplan[1].valid = false; // the affine shape classes depend on the text lengths
     b->myers_cells_per_run = 0;
     for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];

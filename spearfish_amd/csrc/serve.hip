@@ -58,6 +58,7 @@ struct Slot {
     // device
     uint64_t *d_poff = nullptr, *d_toff = nullptr, *d_ba = nullptr;
     uint32_t *d_plen = nullptr, *d_tlen = nullptr, *d_morder = nullptr, *d_aorder = nullptr;
+    PairDesc *d_mdesc = nullptr, *d_adesc = nullptr; // segments in Myers and affine launch order
     uint32_t *d_res = nullptr; // dist | end | score | col, 4 x cap_seg
     // host (pinned)
     uint32_t *h_words = nullptr;
@@ -172,6 +173,11 @@ void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
         band_geom(s.h_plen[i], s.h_tlen[i], cfg.affine_mode, cfg.band, &below, &above);
         s.h_ba[i] = below | (above << 32);
         acls[i] = sf_affine_k_for(below + above + 1);
+        if (aff_lp(acls[i]) == 16) { // scores only: the 16-bit kernel where its range holds
+            const int kp = sf_affine_pk16_class(aff_k(acls[i]), cfg.sc.mat, cfg.sc.mis, cfg.sc.open, cfg.sc.ext,
+                                                s.h_plen[i], (uint32_t)below);
+            if (kp) acls[i] = kp;
+        }
         cells += (uint64_t)s.h_plen[i] * s.h_tlen[i] + band_cells(s.h_plen[i], s.h_tlen[i], below, above);
     }
     std::iota(s.h_morder, s.h_morder + seg, 0u);
@@ -200,6 +206,8 @@ void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
     ok(hipMemcpyAsync(s.d_morder, s.h_morder, seg * 4, hipMemcpyHostToDevice, st));
     ok(hipMemcpyAsync(s.d_aorder, s.h_aorder, seg * 4, hipMemcpyHostToDevice, st));
     const PairsView pv{d_seq, s.d_poff, s.d_toff, s.d_plen, s.d_tlen};
+    ok(sf_launch_desc(seg, s.d_morder, pv, nullptr, s.d_mdesc, st));
+    ok(sf_launch_desc(seg, s.d_aorder, pv, s.d_ba, s.d_adesc, st));
     uint32_t *dist = s.d_res, *end = s.d_res + cap_seg, *col = s.d_res + 3 * cap_seg;
     int32_t *score = reinterpret_cast<int32_t *>(s.d_res + 2 * cap_seg);
     const bool inject = cfg.fail_batch && batches + 1 == cfg.fail_batch;
@@ -210,6 +218,7 @@ void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
             MyersArgs ma{};
             ma.pv = pv;
             ma.order = s.d_morder + a;
+            ma.desc = s.d_mdesc + a;
             ma.count = e - a;
             ma.mode = cfg.myers_mode;
             ma.k = UINT32_MAX;
@@ -224,7 +233,7 @@ void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
             AffineArgs aa{};
             aa.pv = pv;
             aa.order = s.d_aorder + a;
-            aa.below_above = s.d_ba;
+            aa.desc = s.d_adesc + a;
             aa.count = e - a;
             aa.mode = cfg.affine_mode;
             aa.mat = cfg.sc.mat, aa.mis = cfg.sc.mis, aa.open = cfg.sc.open, aa.ext = cfg.sc.ext;
@@ -311,7 +320,8 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
     if (cfg->sc.mat < 0 || cfg->sc.mis < 0 || cfg->sc.open < 0 || cfg->sc.ext < 0 || cfg->sc.mat > 65535 ||
         cfg->sc.mis > 65535 || cfg->sc.open > 65535 || cfg->sc.ext > 65535)
         return sf_runtime_fail(SF_EINVAL, "scores must be in 0..65535");
-    // Per slot: half the share for read bases (4 B per 16 bases), half for segments (64 B each).
+    // Per slot: half the share for read bases (4 B per 16 bases), half for segments (128 B each:
+    // offsets, lengths, band, orders, two launch records and results).
     const uint64_t share = cfg->hbm_budget_bytes / cfg->slots;
     if (share < (1u << 16)) return sf_runtime_fail(SF_EINVAL, "hbm budget too small for the slots");
     auto *sv = new sf_server();
@@ -319,7 +329,7 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
     sv->store_bases = store_bases;
     sv->store_words = (store_bases + 15) / 16;
     sv->cap_words = share / 2 / 4;
-    sv->cap_seg = share / 2 / 64;
+    sv->cap_seg = share / 2 / 128;
     auto bail = [&](const char *m) {
         delete sv;
         return sf_runtime_fail(SF_ENOMEM, m);
@@ -337,7 +347,7 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
             return bail("stream");
         if (!dev_alloc(&s.d_poff, c) || !dev_alloc(&s.d_toff, c) || !dev_alloc(&s.d_ba, c) || !dev_alloc(&s.d_plen, c) ||
             !dev_alloc(&s.d_tlen, c) || !dev_alloc(&s.d_morder, c) || !dev_alloc(&s.d_aorder, c) ||
-            !dev_alloc(&s.d_res, 4 * c))
+            !dev_alloc(&s.d_mdesc, c) || !dev_alloc(&s.d_adesc, c) || !dev_alloc(&s.d_res, 4 * c))
             return bail("slot device buffers");
         if (!host_alloc(&s.h_words, sv->cap_words) || !host_alloc(&s.h_poff, c) || !host_alloc(&s.h_toff, c) ||
             !host_alloc(&s.h_ba, c) || !host_alloc(&s.h_plen, c) || !host_alloc(&s.h_tlen, c) ||
@@ -458,6 +468,7 @@ void sf_server_destroy(sf_server *sv) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         (void)hipFree(s.d_poff), (void)hipFree(s.d_toff), (void)hipFree(s.d_ba), (void)hipFree(s.d_plen);
         (void)hipFree(s.d_tlen), (void)hipFree(s.d_morder), (void)hipFree(s.d_aorder), (void)hipFree(s.d_res);
+        (void)hipFree(s.d_mdesc), (void)hipFree(s.d_adesc);
         (void)hipHostFree(s.h_words), (void)hipHostFree(s.h_poff), (void)hipHostFree(s.h_toff);
         (void)hipHostFree(s.h_ba), (void)hipHostFree(s.h_plen), (void)hipHostFree(s.h_tlen);
         (void)hipHostFree(s.h_morder), (void)hipHostFree(s.h_aorder), (void)hipHostFree(s.h_res);

@@ -243,6 +243,28 @@ def test_batch_object_reruns_are_identical():
     b.close()
 
 
+def test_affine_cell_count_is_dp_work():
+    # The GCUPS numerator: cells per affine run equal the reference's DP_WORK count (align.rs:
+    # 2573-2578) as the oracle restates it, over modes, bands and shapes.
+    rng = random.Random(61)
+    P, T, B = [], [], []
+    for _ in range(200):
+        n = rng.randrange(0, 900)
+        T.append([rng.randrange(4) for _ in range(n)])
+        P.append([rng.randrange(4) for _ in range(max(0, n + rng.randrange(-200, 200)))])
+        B.append(rng.choice([0, 3, 64, 128, 700, A.USIZE_MAX]))
+    ps = A.PairSet.from_seqs(P, T)
+    bands = np.array(B, np.uint64)
+    for mode in MODES:
+        b = A.Batch(ps)
+        b.affine(mode, A.Affine.ONT, bands=bands)
+        b.sync()
+        want = sum(O.affine_cells(len(P[i]), len(T[i]), int(mode), None if B[i] == A.USIZE_MAX else B[i])
+                   for i in range(len(P)))
+        assert b.timing().affine_cells == want, mode
+        b.close()
+
+
 def test_myers_long_patterns_class_boundaries():
     # Patterns past the lane kernel (> 24 blocks, 1536 bases) run 16 lanes per pair up to 96
     # blocks and 64 beyond; lengths at the edges of the block classes, texts short enough for the

@@ -60,7 +60,7 @@ def test_admission_refuses_requests_over_the_budget(workload):
     res = S.Results(reads)
     reqs = S.requests(reads, res)
     big = int(np.argmax(reads.read_len))
-    # A share of 64 KiB per slot holds 128 K bases and 512 segments: the longest read still fits,
+    # A share of 64 KiB per slot holds 128 K bases and 256 segments: the longest read still fits,
     # a read whose bases exceed it is refused.
     with S.Server(store, hbm_budget_bytes=1 << 17, slots=2, band=BAND) as s:
         t = s.submit(reqs[big])
