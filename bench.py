@@ -260,7 +260,7 @@ def bench_uneven(args):
         dist, _, _ = b.myers_results(0)
         res = b.affine_results()
         threads = min(16, os.cpu_count() or 1)
-        k = args.cpu_pairs or 2000
+        k = args.cpu_pairs or 200_000  # about 10 s of host work at 16 threads
         from spearfish_amd.align import PairSet
         sub = PairSet(pairs.pat, pairs.pat_off[:k], pairs.pat_len[:k], pairs.txt, pairs.txt_off[:k], pairs.txt_len[:k])
         cb = cpu_baseline(sub, bands[:k], {"dist": dist, "score": res["score"]}, threads)
