@@ -142,6 +142,60 @@ int sf_batch_affine_results(sf_batch *b, int32_t *score, uint32_t *score_col, ui
 int sf_batch_timing(sf_batch *b, sf_timing *t);
 void sf_batch_reset_timing(sf_batch *b);
 
+/* ---- passes with per-row bookkeeping: X-drop extension, cut, band layout, local bands ---- */
+
+/* align.rs:2111-2129 `Dp`: the usize fields as u32 (sequences are under 2^30 bases). */
+typedef struct {
+    int32_t score;
+    uint32_t score_col;
+    int32_t clip_best;
+    uint32_t clip_i, clip_j; /* clip_cell (pattern bases, text bases); (0, 0) when not tracked */
+    uint32_t text_read, last_row;
+    int32_t cut_best, cut_score;
+} sf_dp;
+
+/* align.rs:2183-2208 `affine_dp` per pair: `geom` (nullable: the text length) lays the band out as
+ * for a text of that many bases; has_clip with `xdrop` and `clip_cell` is `clip: Some((xdrop,
+ * clip_cell))`; `cut` (nullable: usize::MAX for every pair) tracks scores over the first cut text
+ * bases.  Results are read with sf_batch_dp_results. */
+int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t *band, const uint64_t *geom,
+                       int has_clip, int32_t xdrop, int clip_cell, const uint64_t *cut);
+int sf_batch_dp_results(sf_batch *b, sf_dp *out);
+/* align.rs:741-760 `local_band_ends` per pair: local affine over the diagonals dlo[i]..=dhi[i]
+ * (text position minus read position), bonus_start / bonus_end for reaching the read's ends.
+ * Pattern codes of 4 or more (unpacked input) match nothing.  Results: sf_batch_local_results. */
+int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, const int64_t *dhi, int32_t bonus_start,
+                        int32_t bonus_end);
+int sf_batch_local_results(sf_batch *b, int32_t *best);
+
+/* Host-buffer forms (one device pass over all pairs; the same results as per-pair calls). */
+int sf_affine_dp(const sf_pairs *pairs, int mode, const sf_scoring *sc, const uint64_t *band, const uint64_t *geom,
+                 int has_clip, int32_t xdrop, int clip_cell, const uint64_t *cut, sf_dp *out);
+/* align.rs:710-729 `affine_extend`: max(clip_best, score + end_bonus) of a Prefix pass with X-drop. */
+int sf_affine_extend(const sf_pairs *pairs, const sf_scoring *sc, const uint64_t *band, int32_t end_bonus,
+                     int32_t xdrop, int32_t *score);
+/* align.rs:957-982 `affine_extend_window`: (score, pattern bases, text bases, text bases read)
+ * with the band laid out for max(window[i], text length). */
+typedef struct {
+    int32_t score;
+    uint32_t pat_bases, txt_bases, txt_read;
+} sf_extend_window;
+int sf_affine_extend_window(const sf_pairs *pairs, const uint64_t *window, const sf_scoring *sc, const uint64_t *band,
+                            int32_t end_bonus, int32_t xdrop, sf_extend_window *out);
+/* align.rs:903-949 `affine_extend_sorted_cut_bonus` (and `affine_extend_sorted_cut` =
+ * track_cut 1, bonus_within_cut 0), one text per pair: (whole, cut, text bases covered).  Pairs
+ * sharing a pattern give the same results as the reference's sorted call over their texts
+ * (align.rs:2210-2213). */
+typedef struct {
+    int32_t whole, cut;
+    uint32_t cols;
+} sf_extend_cut;
+int sf_affine_extend_cut_bonus(const sf_pairs *pairs, const sf_scoring *sc, const uint64_t *band, int32_t end_bonus,
+                               int32_t xdrop, int diagonal, const uint64_t *window, const uint64_t *cut, int track_cut,
+                               int bonus_within_cut, sf_extend_cut *out);
+int sf_local_band_ends(const sf_pairs *pairs, const sf_scoring *sc, const int64_t *dlo, const int64_t *dhi,
+                       int32_t bonus_start, int32_t bonus_end, int32_t *best);
+
 /* ---- serving: continuous batching over a device-resident transcript store ----
  * No reference counterpart as an object: it serves the calls the tie-break makes per read
  * (crates/map/src/lib.rs:1307-1492, `seg`), many reads at a time.  Each request is a read and

@@ -42,6 +42,30 @@ class Pairs(C.Structure):
     ]
 
 
+class Dp(C.Structure):
+    """sf_dp: align.rs:2111-2129 `Dp`."""
+
+    _fields_ = [
+        ("score", C.c_int32),
+        ("score_col", C.c_uint32),
+        ("clip_best", C.c_int32),
+        ("clip_i", C.c_uint32),
+        ("clip_j", C.c_uint32),
+        ("text_read", C.c_uint32),
+        ("last_row", C.c_uint32),
+        ("cut_best", C.c_int32),
+        ("cut_score", C.c_int32),
+    ]
+
+
+class ExtendWindow(C.Structure):
+    _fields_ = [("score", C.c_int32), ("pat_bases", C.c_uint32), ("txt_bases", C.c_uint32), ("txt_read", C.c_uint32)]
+
+
+class ExtendCut(C.Structure):
+    _fields_ = [("whole", C.c_int32), ("cut", C.c_int32), ("cols", C.c_uint32)]
+
+
 class Timing(C.Structure):
     _fields_ = [
         ("myers_ms", C.c_double),
@@ -128,6 +152,15 @@ SIGNATURES = {
     "sf_server_destroy": ([_vp], None),
     "sf_server_replay": ([_vp, C.POINTER(ReadRequest), C.POINTER(C.c_double), _u64, C.POINTER(C.c_double),
                           C.POINTER(_u64)], _int),
+    "sf_batch_affine_dp": ([_vp, _int, _vp, _vp, _vp, _int, C.c_int32, _int, _vp], _int),
+    "sf_batch_dp_results": ([_vp, _vp], _int),
+    "sf_batch_local_band": ([_vp, _vp, _vp, _vp, C.c_int32, C.c_int32], _int),
+    "sf_batch_local_results": ([_vp, _vp], _int),
+    "sf_affine_dp": ([_vp, _int, _vp, _vp, _vp, _int, C.c_int32, _int, _vp, _vp], _int),
+    "sf_affine_extend": ([_vp, _vp, _vp, C.c_int32, C.c_int32, _vp], _int),
+    "sf_affine_extend_window": ([_vp, _vp, _vp, _vp, C.c_int32, C.c_int32, _vp], _int),
+    "sf_affine_extend_cut_bonus": ([_vp, _vp, _vp, C.c_int32, C.c_int32, _int, _vp, _vp, _int, _int, _vp], _int),
+    "sf_local_band_ends": ([_vp, _vp, _vp, _vp, C.c_int32, C.c_int32, _vp], _int),
     "sf_set_affine_lp16_max": ([C.c_uint64], _int),
     "sf_selftest_lanes": ([_vp], _int),
 }

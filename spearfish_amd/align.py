@@ -13,7 +13,7 @@ from __future__ import annotations
 import ctypes as C
 import enum
 from dataclasses import dataclass
-from typing import Optional, Sequence
+from typing import NamedTuple, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -222,6 +222,121 @@ class Batch:
 
     def reset_timing(self):
         self.lib.sf_batch_reset_timing(self.h)
+
+
+# ---------------------------------------------------------------------------------------------
+# passes with per-row bookkeeping: X-drop extension, cut, band layout, local bands
+# ---------------------------------------------------------------------------------------------
+
+class Dp(NamedTuple):
+    """align.rs:2111-2129 `Dp` (clip_cell as (clip_i, clip_j))."""
+
+    score: int
+    score_col: int
+    clip_best: int
+    clip_i: int
+    clip_j: int
+    text_read: int
+    last_row: int
+    cut_best: int
+    cut_score: int
+
+
+def _u64s(v, n):
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(USIZE_MAX if v is None else v, dtype=np.uint64), (n,)))
+
+
+def affine_dp_batch(pairs: PairSet, mode: Mode, sc: Affine = Affine.ONT, band=None, geom=None,
+                    clip: Optional[Tuple[int, bool]] = None, cut=None) -> list:
+    """align.rs:2183-2208 `affine_dp` for every pair: band, geom and cut per pair or shared
+    (geom None: each text's length; cut None: usize::MAX); clip = (xdrop, clip_cell) or None."""
+    lib = N.load()
+    n = len(pairs)
+    out = (N.Dp * max(n, 1))()
+    bands = _bands(band, n)
+    g = None if geom is None else _u64s(geom, n)
+    c = None if cut is None else _u64s(cut, n)
+    has, xd, cell = (0, 0, 0) if clip is None else (1, int(clip[0]), int(bool(clip[1])))
+    cp, scc = pairs.c(), sc.c()
+    N.check(lib.sf_affine_dp(C.byref(cp), int(mode), C.byref(scc), bands.ctypes.data, _p(g), has, xd, cell, _p(c),
+                             C.cast(out, C.c_void_p)))
+    return [Dp(o.score, o.score_col, o.clip_best, o.clip_i, o.clip_j, o.text_read, o.last_row, o.cut_best,
+               o.cut_score) for o in out[:n]]
+
+
+def affine_dp(pattern, text, mode: Mode, sc: Affine = Affine.ONT, band: int = USIZE_MAX, clip=None,
+              geom: Optional[int] = None, cut: int = USIZE_MAX) -> Dp:
+    """align.rs:2183-2208 `affine_dp` (clip = (xdrop, clip_cell) or None; geom None: len(text))."""
+    return affine_dp_batch(PairSet.from_seqs([pattern], [text]), mode, sc, band, geom, clip, cut)[0]
+
+
+def affine_extend(pattern, text, sc: Affine, band: int, end_bonus: int, xdrop: int) -> int:
+    """align.rs:710-729."""
+    lib = N.load()
+    ps = PairSet.from_seqs([pattern], [text])
+    out = np.empty(1, np.int32)
+    cp, scc = ps.c(), sc.c()
+    N.check(lib.sf_affine_extend(C.byref(cp), C.byref(scc), _bands(band, 1).ctypes.data, end_bonus, xdrop,
+                                 out.ctypes.data))
+    return int(out[0])
+
+
+def affine_extend_window(pattern, text, window: int, sc: Affine, band: int, end_bonus: int, xdrop: int):
+    """align.rs:957-982: (score, pattern bases, text bases, text bases read)."""
+    lib = N.load()
+    ps = PairSet.from_seqs([pattern], [text])
+    out = N.ExtendWindow()
+    cp, scc = ps.c(), sc.c()
+    N.check(lib.sf_affine_extend_window(C.byref(cp), _u64s(window, 1).ctypes.data, C.byref(scc),
+                                        _bands(band, 1).ctypes.data, end_bonus, xdrop, C.byref(out)))
+    return (out.score, out.pat_bases, out.txt_bases, out.txt_read)
+
+
+def affine_extend_sorted_cut_bonus(pattern, texts: Sequence, sc: Affine, band: int, end_bonus: int, xdrop: int,
+                                   diagonal: bool, window: int, cut: int, track_cut: bool,
+                                   bonus_within_cut: bool) -> list:
+    """align.rs:903-949: (whole, cut, text bases covered) per text; equal to the sorted call."""
+    if not len(texts):
+        return []
+    lib = N.load()
+    n = len(texts)
+    ps = PairSet.from_seqs([pattern] * n, texts)
+    out = (N.ExtendCut * n)()
+    cp, scc = ps.c(), sc.c()
+    N.check(lib.sf_affine_extend_cut_bonus(C.byref(cp), C.byref(scc), _bands(band, n).ctypes.data, end_bonus, xdrop,
+                                           int(diagonal), _u64s(window, n).ctypes.data, _u64s(cut, n).ctypes.data,
+                                           int(track_cut), int(bonus_within_cut), C.cast(out, C.c_void_p)))
+    return [(o.whole, o.cut, o.cols) for o in out]
+
+
+def affine_extend_sorted_cut(pattern, texts: Sequence, sc: Affine, band: int, end_bonus: int, xdrop: int,
+                             diagonal: bool, window: int, cut: int) -> list:
+    """align.rs:880-901."""
+    return affine_extend_sorted_cut_bonus(pattern, texts, sc, band, end_bonus, xdrop, diagonal, window, cut, True,
+                                          False)
+
+
+def local_band_ends_batch(pairs: PairSet, sc: Affine, dlo, dhi, end_bonus: Tuple[int, int]) -> np.ndarray:
+    """align.rs:741-760 for every pair (dlo, dhi per pair or shared)."""
+    lib = N.load()
+    n = len(pairs)
+    lo = np.ascontiguousarray(np.broadcast_to(np.asarray(dlo, np.int64), (n,)))
+    hi = np.ascontiguousarray(np.broadcast_to(np.asarray(dhi, np.int64), (n,)))
+    out = np.empty(max(n, 1), np.int32)
+    cp, scc = pairs.c(), sc.c()
+    N.check(lib.sf_local_band_ends(C.byref(cp), C.byref(scc), lo.ctypes.data, hi.ctypes.data, int(end_bonus[0]),
+                                   int(end_bonus[1]), out.ctypes.data))
+    return out[:n]
+
+
+def local_band_ends(read, text, sc: Affine, dlo: int, dhi: int, end_bonus: Tuple[int, int]) -> int:
+    """align.rs:741-760: best local score over diagonals dlo..=dhi; read codes >= 4 match nothing."""
+    return int(local_band_ends_batch(PairSet.from_seqs([read], [text]), sc, dlo, dhi, end_bonus)[0])
+
+
+def local_band(read, text, sc: Affine, dlo: int, dhi: int, end_bonus: int) -> int:
+    """align.rs:735-737."""
+    return local_band_ends(read, text, sc, dlo, dhi, (end_bonus, end_bonus))
 
 
 # ---------------------------------------------------------------------------------------------

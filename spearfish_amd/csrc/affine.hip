@@ -475,6 +475,297 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
     }
 }
 
+// ---- passes with per-row bookkeeping ----
+//
+// `affine_dp` with a clip (X-drop stop, best cell) or a cut, and `local_band_ends`, need each
+// row's maximum over its band cells (and for the best cell its first column), so these passes add
+// a group reduction per row to the 32-bit recurrence of affine_wave_kernel (same layout, bias and
+// out-of-band handling).  The reductions are DPP row rotations, so every lane of a group ends with
+// the group's value.
+
+constexpr int kRowRor1 = 0x121, kRowRor2 = 0x122, kRowRor4 = 0x124, kRowRor8 = 0x128;
+
+template <int LP>
+__device__ __forceinline__ int32_t group_max(int32_t x) {
+    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor8, 0xf, 0xf, false));
+    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor4, 0xf, 0xf, false));
+    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor2, 0xf, 0xf, false));
+    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor1, 0xf, 0xf, false));
+    if constexpr (LP == 64) {
+        x = imax(x, __shfl_xor(x, 16));
+        x = imax(x, __shfl_xor(x, 32));
+    }
+    return x;
+}
+
+template <int LP>
+__device__ __forceinline__ int32_t group_min(int32_t x) {
+    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor8, 0xf, 0xf, false));
+    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor4, 0xf, 0xf, false));
+    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor2, 0xf, 0xf, false));
+    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor1, 0xf, 0xf, false));
+    if constexpr (LP == 64) {
+        x = imin(x, __shfl_xor(x, 16));
+        x = imin(x, __shfl_xor(x, 32));
+    }
+    return x;
+}
+
+__device__ __forceinline__ int32_t sat_add32(int32_t a, int32_t b) {
+    const int64_t r = (int64_t)a + b;
+    return r > INT32_MAX ? INT32_MAX : (r < INT32_MIN ? INT32_MIN : (int32_t)r);
+}
+
+template <int LP, int K, bool LOCAL>
+__global__ __launch_bounds__(256) void affine_pass_kernel(PassArgs a) {
+    constexpr bool ONE = LP == 64;
+    constexpr int SHL = ONE ? kWaveShl1 : kRowShl1;
+    constexpr int BLK = 16;
+    constexpr int NWIN = (BLK + K - 1 + 15) / 16;
+    constexpr int NY = (K + 15) / 16;
+    const int lane = threadIdx.x & 63;
+    const int l = lane % LP;
+    const uint32_t w = (blockIdx.x * 256u + threadIdx.x) / LP;
+    const bool valid = w < a.count;
+    if (ONE && !valid) return; // whole waves only
+    const uint32_t pi = valid ? a.order[w] : 0u;
+    const PairDesc dsc = valid ? a.desc[w] : PairDesc{0, 0, 0, 0, 0, 0};
+    const int32_t m = (int32_t)dsc.m, n = (int32_t)dsc.n;
+    const int32_t below = (int32_t)dsc.below, above = (int32_t)dsc.above; // local: -dlo, dhi
+    const int32_t Wd = below + above + 1;
+    const SeqRef P = seqref(a.pv.seq, dsc.poff, (uint32_t)m);
+    const SeqRef T = seqref(a.pv.seq, dsc.toff, (uint32_t)n);
+    const int32_t mat = a.mat, mis = a.mis, ext = a.ext;
+    const uint32_t dlt = (uint32_t)-a.open;
+    const uint32_t scm = (uint32_t)(mat + 2 * ext);
+    const int32_t ndl = -(mat + mis);
+    const bool infix = a.mode == kInfix;
+    const int32_t s0 = l * K;
+
+    uint32_t V[K], Fb[K], Ub[K], dltv[K];
+    {
+        const int32_t lo0 = below < 0 ? -below : 0; // row 0 covers [max(0, dlo), min(n, dhi)] (local)
+        const int32_t hi0 = above < n ? above : n;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const int32_t s = s0 + q;
+            const int32_t j = s - below;
+            int32_t h = kNeg;
+            if (j >= lo0 && j <= hi0) h = LOCAL ? a.bonus0 : ((j == 0 || infix) ? 0 : -(a.open + ext * j));
+            V[q] = (uint32_t)(h == kNeg ? kNeg : h + ext * s) + kOff;
+            Fb[q] = (uint32_t)kNeg + kOff;
+            Ub[q] = V[q] + dlt;
+            dltv[q] = s >= Wd ? dlt - (1u << 30) : dlt;
+        }
+    }
+    // Rows past n + below have no band cell (the band ran off the text).
+    const int32_t last = valid ? (m < n + below ? m : n + below) : 0;
+    int32_t rows = last;
+    if constexpr (!ONE) {
+#pragma unroll
+        for (int o = 32; o >= LP; o >>= 1) rows = imax(rows, __shfl_xor(rows, o));
+        rows = __builtin_amdgcn_readfirstlane(rows);
+    }
+    const int32_t xdrop = a.xdrop;
+    // cut = usize::MAX is "no cut"; any other cut past the text covers the whole row.
+    const uint64_t cut_in = (!LOCAL && valid && a.cut) ? a.cut[pi] : UINT64_MAX;
+    const int64_t cut = cut_in == UINT64_MAX ? -1 : (int64_t)(cut_in < (uint64_t)INT32_MAX ? cut_in : INT32_MAX);
+    int32_t clip_best = 0, cut_best = 0, best = 0;
+    int32_t ci = 0, cj = 0;
+    bool xstop = false;
+    int32_t stop_row = 0, stop_read = 0;
+    for (int32_t i0 = 1; i0 <= rows; i0 += BLK) {
+        uint32_t pw = codes16(P, i0 - 1);
+        uint32_t inv = 0; // local: rows whose read code matches nothing
+        if (LOCAL && a.pinv) {
+            const uint64_t b0 = dsc.poff + (uint64_t)(i0 - 1);
+            const uint32_t lo32 = a.pinv[b0 >> 5], hi32 = a.pinv[(b0 >> 5) + 1];
+            inv = __builtin_amdgcn_alignbit(hi32, lo32, (uint32_t)(b0 & 31));
+        }
+        uint32_t win[NWIN];
+        const int32_t pos = i0 - below + s0 - 1;
+#pragma unroll
+        for (int k = 0; k < NWIN; ++k) win[k] = codes16(T, pos + 16 * k);
+        const int32_t cnt = rows - i0 + 1 < BLK ? rows - i0 + 1 : BLK;
+        for (int32_t r = 0; r < cnt; ++r) {
+            const int32_t i = i0 + r;
+            const uint32_t pc = pw & 3u;
+            pw >>= 2;
+            const bool inv_row = (inv >> r) & 1u;
+            if (i > last || xstop) continue; // this pair's rows are done
+            const uint32_t p5 = __umul24(pc, 0x5555u);
+            const uint32_t prep = p5 | (p5 << 16);
+            uint32_t y[NY];
+#pragma unroll
+            for (int k = 0; k < NY; ++k) {
+                const uint32_t t = win[k] ^ prep;
+                y[k] = inv_row ? 0xFFFFFFFFu : ((t << 1) | t);
+            }
+            // Band cells of row i: slots [slo, shi] (columns 0..n).
+            const int32_t slo = below - i > 0 ? below - i : 0;
+            const int32_t shi = n - i + below < Wd - 1 ? n - i + below : Wd - 1;
+            const uint32_t rowb = (uint32_t)(2 * ext * i) + kOff; // bias of value 0 at slot 0 of row i
+            const uint32_t Un = dpp0<SHL>(Ub[0]);
+            const uint32_t Fn = dpp0<SHL>(Fb[0]);
+            uint32_t X[K], G[K], FQ[K];
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const uint32_t uu = q < K - 1 ? Ub[q + 1] : Un;
+                const uint32_t fu = q < K - 1 ? Fb[q + 1] : Fn;
+                const uint32_t mism = __builtin_amdgcn_ubfe(y[q / 16], 2 * (q % 16) + 1, 1);
+                const uint32_t f = umax(uu, fu);
+                const uint32_t d = V[q] + (uint32_t)__mul24((int)mism, ndl) + scm;
+                uint32_t x = umax(d, f);
+                if constexpr (LOCAL) {
+                    // Local: a band cell never drops below 0 (align.rs:800-806), except in the last row.
+                    const int32_t s = s0 + q;
+                    if (i < m && s >= slo && s <= shi) x = umax(x, rowb + (uint32_t)(ext * s));
+                }
+                X[q] = x;
+                FQ[q] = f;
+                const uint32_t g = x + dlt;
+                G[q] = q ? umax(G[q - 1], g) : g;
+            }
+            const uint32_t C = group_excl_umax<LP>(G[K - 1]);
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                V[q] = umax(X[q], q ? umax(C, G[q - 1]) : C);
+                Fb[q] = FQ[q];
+                Ub[q] = V[q] + dltv[q];
+            }
+            // The row's maximum over its band cells (align.rs:2599-2628 `row_max`).
+            int32_t hq[K];
+            int32_t lm = INT32_MIN;
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const int32_t s = s0 + q;
+                hq[q] = (int32_t)(V[q] - rowb) - ext * s;
+                if (s >= slo && s <= shi) lm = imax(lm, hq[q]);
+            }
+            const int32_t rmax = group_max<LP>(lm);
+            if constexpr (LOCAL) {
+                best = imax(best, rmax);
+                if (i == m) best = imax(best, sat_add32(rmax, a.bonus1));
+            } else {
+                const int32_t lo = i - below > 0 ? i - below : 0;
+                const int32_t hi = i + above < n ? i + above : n;
+                if (cut >= 0 && lo <= cut) {
+                    int32_t rc = rmax;
+                    if (hi > cut) {
+                        const int32_t sc_hi = (int32_t)cut - i + below;
+                        int32_t lc = INT32_MIN;
+#pragma unroll
+                        for (int q = 0; q < K; ++q) {
+                            const int32_t s = s0 + q;
+                            if (s >= slo && s <= sc_hi) lc = imax(lc, hq[q]);
+                        }
+                        rc = group_max<LP>(lc);
+                    }
+                    cut_best = imax(cut_best, rc);
+                }
+                if (a.has_clip) {
+                    if (rmax > clip_best) {
+                        clip_best = rmax;
+                        if (a.clip_cell) {
+                            int32_t ls = INT32_MAX;
+#pragma unroll
+                            for (int q = 0; q < K; ++q) {
+                                const int32_t s = s0 + q;
+                                if (s >= slo && s <= shi && hq[q] == rmax) ls = imin(ls, s);
+                            }
+                            const int32_t sm = group_min<LP>(ls);
+                            ci = i;
+                            cj = sm + i - below;
+                        }
+                    }
+                    // X-drop: the whole row is more than xdrop below the best so far.
+                    const int64_t thr = (int64_t)clip_best - xdrop;
+                    const int64_t sat = thr < INT32_MIN ? INT32_MIN : (thr > INT32_MAX ? INT32_MAX : thr);
+                    if ((int64_t)rmax < sat) {
+                        xstop = true;
+                        stop_row = i;
+                        stop_read = hi;
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NWIN; ++k) win[k] = k + 1 < NWIN ? __builtin_amdgcn_alignbit(win[k + 1], win[k], 2) : win[k] >> 2;
+        }
+    }
+    if constexpr (LOCAL) {
+        if (valid && l == 0) a.best[pi] = (m == 0 || Wd <= 0) ? 0 : best;
+        return;
+    } else {
+        // End of pass (align.rs:2643-2665): score and column, the cut score, bases read.
+        const int32_t rowb = 2 * ext * m;
+        const int32_t plo = m - below > 0 ? m - below : 0;
+        const int32_t phi = m + above < n ? m + above : n;
+        const int32_t slo = plo - (m - below), shi = phi - (m - below);
+        int32_t score;
+        uint32_t col;
+        if (a.mode == kGlobal) {
+            // H[m][n]; NEG when (m, n) lies outside the last row's band (a band laid out for
+            // another text length).
+            const int32_t sN = n - m + below;
+            const bool inb = sN >= 0 && sN <= Wd - 1;
+            const int owner = inb ? sN / K : 0, qq = inb ? sN % K : 0;
+            uint32_t v = V[0];
+#pragma unroll
+            for (int q = 1; q < K; ++q) v = q == qq ? V[q] : v;
+            const uint32_t vo = ONE ? (uint32_t)__builtin_amdgcn_readlane((int)v, owner)
+                                    : (uint32_t)__shfl((int)v, (lane - l) + owner);
+            score = inb ? (int32_t)(vo - kOff) - ext * sN - rowb : kNeg;
+            col = (uint32_t)n;
+        } else {
+            int32_t bv = INT32_MIN, bs = INT32_MAX;
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const int32_t s = s0 + q;
+                const int32_t h = (int32_t)(V[q] - kOff) - ext * s - rowb;
+                if (s >= slo && s <= shi && h > bv) {
+                    bv = h;
+                    bs = s;
+                }
+            }
+            const int32_t mx = group_max<LP>(bv);
+            const int32_t sm = group_min<LP>(bv == mx ? bs : INT32_MAX);
+            score = mx;
+            col = (uint32_t)(sm + (m - below));
+        }
+        int32_t cut_score = kNeg;
+        if (cut >= 0 && plo <= cut) {
+            const int32_t top = phi < cut ? phi : (int32_t)cut;
+            const int32_t stop_s = top - (m - below);
+            int32_t lc = INT32_MIN;
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const int32_t s = s0 + q;
+                const int32_t h = (int32_t)(V[q] - kOff) - ext * s - rowb;
+                if (s >= slo && s <= stop_s) lc = imax(lc, h);
+            }
+            cut_score = imax(kNeg, group_max<LP>(lc));
+        }
+        if (valid && l == 0) {
+            DpOut o;
+            o.clip_best = clip_best;
+            o.clip_i = (uint32_t)ci;
+            o.clip_j = (uint32_t)cj;
+            o.cut_best = cut_best;
+            if (xstop) {
+                o.score = kNeg, o.score_col = 0, o.text_read = (uint32_t)stop_read, o.last_row = (uint32_t)stop_row;
+                o.cut_score = kNeg;
+            } else if (last < m) {
+                o.score = kNeg, o.score_col = 0, o.text_read = (uint32_t)n, o.last_row = (uint32_t)(last + 1);
+                o.cut_score = kNeg;
+            } else {
+                o.score = score, o.score_col = col, o.text_read = (uint32_t)phi, o.last_row = (uint32_t)m;
+                o.cut_score = cut_score;
+            }
+            a.out[pi] = o;
+        }
+    }
+}
+
 // One lane per pair: walks the traceback bits from the end cell back to the start, emitting
 // BAM ops last to first (rules in the header comment of affine_wave_kernel and the oracle).
 // Each step reads one u32 -- the word of the cell's lane for that row, which holds 8 slots of
@@ -714,6 +1005,30 @@ hipError_t sf_launch_desc(uint64_t n, const uint32_t *order, const PairsView &pv
                           hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(desc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, order, pv, ba, desc);
+    return hipGetLastError();
+}
+
+hipError_t sf_launch_pass(const PassArgs &a, int cls, bool local, hipStream_t s) {
+    if (a.count == 0) return hipSuccess;
+    if (aff_packed(cls)) return hipErrorInvalidValue;
+    const int LP = aff_lp(cls);
+    dim3 grid((unsigned)(((uint64_t)a.count * LP + 255) / 256)), block(256);
+    switch (cls) {
+#define CASE(LL, KK)                                                                             \
+    case (LL == 64 ? 0 : LL * 1000) + KK:                                                       \
+        if (local) hipLaunchKernelGGL((affine_pass_kernel<LL, KK, true>), grid, block, 0, s, a); \
+        else hipLaunchKernelGGL((affine_pass_kernel<LL, KK, false>), grid, block, 0, s, a);      \
+        break;
+#define CASES(LL)                                                                                \
+    CASE(LL, 1) CASE(LL, 2) CASE(LL, 3) CASE(LL, 4) CASE(LL, 5) CASE(LL, 6) CASE(LL, 7) CASE(LL, 8) \
+    CASE(LL, 10) CASE(LL, 12) CASE(LL, 14) CASE(LL, 16) CASE(LL, 20) CASE(LL, 24) CASE(LL, 28) CASE(LL, 32)
+        CASES(16)
+        CASES(64)
+#undef CASES
+#undef CASE
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

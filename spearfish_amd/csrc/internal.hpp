@@ -60,6 +60,34 @@ struct AffineArgs {
     const uint64_t *tb_off; // per order slot of the launch: word offset into tb
 };
 
+// A pass with per-row bookkeeping: align.rs:2446-2666 run_core with clip (X-drop, best cell) and
+// cut tracking, or (local) align.rs:783-852 local_band_core.  Band geometry comes in the pair
+// records (local: below = -dlo, above = dhi, as two's complement).
+struct DpOut { // align.rs:2111-2129 `Dp` (the usize fields fit 32 bits: sequences are < 2^30)
+    int32_t score;
+    uint32_t score_col;
+    int32_t clip_best;
+    uint32_t clip_i, clip_j;
+    uint32_t text_read, last_row;
+    int32_t cut_best, cut_score;
+};
+
+struct PassArgs {
+    PairsView pv;
+    const PairDesc *desc;
+    const uint32_t *order;
+    uint32_t count;
+    int mode;
+    int32_t mat, mis, open, ext;
+    int has_clip, clip_cell;
+    int32_t xdrop;
+    const uint64_t *cut;     // per pair (null: no cut)
+    DpOut *out;              // per pair (run_core passes)
+    int32_t bonus0, bonus1;  // local: end bonuses for the read's start and end
+    const uint32_t *pinv;    // local: bit (base offset) set where a pattern code is >= 4 (null: none)
+    int32_t *best;           // per pair (local passes)
+};
+
 struct WalkArgs {
     PairsView pv;
     const PairDesc *desc; // per order slot of the launch
@@ -212,6 +240,8 @@ int sf_affine_class(uint64_t width, bool lp16); // 16 lanes per pair if allowed 
 // exact range does not hold (the scoring must have mis = 2 ext).
 int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m, uint32_t below);
 hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
+// cls: a 32-bit affine class (16 or 64 lanes); local selects local_band_core.
+hipError_t sf_launch_pass(const sf::PassArgs &a, int cls, bool local, hipStream_t s);
 // desc[w] of pair order[w], w < n (ba may be null: no band).
 hipError_t sf_launch_desc(uint64_t n, const uint32_t *order, const sf::PairsView &pv, const uint64_t *ba,
                           sf::PairDesc *desc, hipStream_t s);
@@ -235,6 +265,26 @@ inline void band_geom(uint32_t m, uint32_t n, int mode, uint64_t band, uint64_t 
     } else {
         *below = band + (m > n ? m - n : 0);
         *above = band + (n > m ? n - m : 0);
+    }
+}
+
+// The band of a pass laid out for a text of `geom` bases (align.rs:2464-2474): below = band +
+// (m - geom)+, above = band + (geom - m)+; the whole matrix for Infix, band = usize::MAX, or a band
+// that covers every row anyway.
+inline void band_geom_for(uint32_t m, uint32_t n, uint64_t geom, int mode, uint64_t band, uint64_t *below,
+                          uint64_t *above) {
+    if (mode == kInfix || band >= (uint64_t)m + n) { // (band = usize::MAX included)
+        *below = m;
+        *above = n;
+        return;
+    }
+    const uint64_t bl = band + (m > geom ? m - geom : 0), ab = band + (geom > m ? geom - m : 0);
+    if (bl >= m && ab >= n) {
+        *below = m;
+        *above = n;
+    } else {
+        *below = bl;
+        *above = ab;
     }
 }
 

@@ -185,7 +185,11 @@ struct sf_batch {
         PairDesc *d_desc = nullptr;  // pairs in launch order
         uint64_t cells = 0;
     };
-    AffPlan aplan[2];
+    AffPlan aplan[4]; // scores only, with traceback, passes with bookkeeping, local bands
+    DpOut *d_dp = nullptr;      // per pair: the last sf_batch_affine_dp run
+    int32_t *d_best = nullptr;  // per pair: the last sf_batch_local_band run
+    uint32_t *d_pinv = nullptr; // bit (base offset) set where a pattern code is >= 4; null if none
+    uint64_t pinv_words = 0;
     int32_t *d_score = nullptr;
     uint32_t *d_col = nullptr, *d_tstart = nullptr, *d_nruns = nullptr, *d_runs = nullptr, *d_cig = nullptr;
     uint64_t *d_runs_off = nullptr, *d_out_off = nullptr, *d_tboff = nullptr;
@@ -235,7 +239,9 @@ struct sf_batch {
         for (auto e : pool) (void)hipEventDestroy(e);
         dfree(d_seq), dfree(d_poff), dfree(d_toff), dfree(d_plen), dfree(d_tlen), dfree(d_morder);
         dfree(d_dist), dfree(d_end), dfree(d_start), dfree(aplan[0].d_ba), dfree(aplan[0].d_order), dfree(aplan[1].d_ba), dfree(aplan[1].d_order), dfree(d_score);
-        dfree(aplan[0].d_desc), dfree(aplan[1].d_desc), dfree(d_mdesc), dfree(d_col);
+        dfree(aplan[0].d_desc), dfree(aplan[1].d_desc), dfree(d_mdesc);
+        for (int q = 2; q < 4; ++q) dfree(aplan[q].d_ba), dfree(aplan[q].d_order), dfree(aplan[q].d_desc);
+        dfree(d_dp), dfree(d_best), dfree(d_pinv), dfree(d_col);
         dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
         dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0);
         for (hipStream_t x : aux)
@@ -360,6 +366,47 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch) {
             HIP_TRY(hipEventRecord(b->joins[t - 1], ss[t]));
             HIP_TRY(hipStreamWaitEvent(b->stream, b->joins[t - 1], 0));
         }
+    return SF_OK;
+}
+
+// Groups the pairs of a batch into bins of one shape class each (longer patterns first inside a
+// class), estimates each bin's time for the stream schedule, and uploads the launch order, the
+// band geometry and the launch-order pair records.
+int build_plan(sf_batch *b, sf_batch::AffPlan &pl, const std::vector<uint64_t> &ba, const std::vector<int> &kcls,
+               uint64_t key, uint64_t cells) {
+    const uint64_t n = b->n;
+    int st;
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        if (kcls[x] != kcls[y]) return kcls[x] < kcls[y];
+        return b->plen[x] > b->plen[y];
+    });
+    pl.bins.clear();
+    for (uint64_t q = 0; q < n;) {
+        uint64_t e = q;
+        while (e < n && kcls[order[e]] == kcls[order[q]]) ++e;
+        sf_batch::Bin bin{kcls[order[q]], (uint32_t)q, (uint32_t)e};
+        // VALU per row of one wave, shared by its 64 / LP pairs (the 16-bit kernel: about 10
+        // packed ops per two slots).
+        const double per_row = aff_packed(bin.cls) ? 5.0 * aff_k(bin.cls) + 25.0 : 9.0 * aff_k(bin.cls) + 15.0;
+        const double ppw = 64.0 / aff_lp(bin.cls);
+        for (uint64_t x = q; x < e; ++x) {
+            bin.chain = std::max(bin.chain, b->plen[order[x]] * per_row);
+            bin.spread += b->plen[order[x]] * per_row / ppw / 1024.0;
+        }
+        pl.bins.push_back(bin);
+        q = e;
+    }
+    if (!pl.d_ba && ((st = dalloc(&pl.d_ba, n)) || (st = dalloc(&pl.d_order, n)) || (st = dalloc(&pl.d_desc, n))))
+        return st;
+    HIP_TRY(hipMemcpy(pl.d_ba, ba.data(), n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(pl.d_order, order.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(sf_launch_desc(n, pl.d_order, view(b), pl.d_ba, pl.d_desc, b->stream));
+    pl.order = std::move(order);
+    pl.key = key;
+    pl.valid = true;
+    pl.cells = cells;
     return SF_OK;
 }
 
@@ -498,6 +545,25 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
         hipStreamSynchronize(b->stream) != hipSuccess)
         return bail(fail(SF_EDEVICE, "launch records failed"));
     b->tlen0 = b->tlen;
+    // Pattern codes of 4 or more (unpacked input only) match nothing in local passes
+    // (align.rs:731-734); the other functions mask codes with & 3 as the reference does.
+    if (!pairs->packed) {
+        const uint8_t *pb = static_cast<const uint8_t *>(pairs->pat);
+        std::vector<uint32_t> inv;
+        for (uint64_t i = 0; i < n; ++i)
+            for (uint32_t k = 0; k < b->plen[i]; ++k)
+                if (pb[pairs->pat_off[i] + k] >= 4) {
+                    if (inv.empty()) inv.assign((seq.size() * 16 + 64) / 32 + 2, 0u);
+                    const uint64_t bit = pbase[i] + k;
+                    inv[bit >> 5] |= 1u << (bit & 31);
+                }
+        if (!inv.empty()) {
+            if ((st = dalloc(&b->d_pinv, inv.size()))) return bail(st);
+            if (hipMemcpy(b->d_pinv, inv.data(), inv.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+                return bail(fail(SF_EDEVICE, "upload failed"));
+            b->pinv_words = inv.size();
+        }
+    }
     *out = b;
     return SF_OK;
 }
@@ -614,39 +680,9 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
             if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, b->plen[i], b->tlen[i], aff_slots(kcls[i])))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
         }
-        std::vector<uint32_t> order(n);
-        std::iota(order.begin(), order.end(), 0u);
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-            if (kcls[x] != kcls[y]) return kcls[x] < kcls[y];
-            return b->plen[x] > b->plen[y];
-        });
-        pl.bins.clear();
-        for (uint64_t q = 0; q < n;) {
-            uint64_t e = q;
-            while (e < n && kcls[order[e]] == kcls[order[q]]) ++e;
-            sf_batch::Bin bin{kcls[order[q]], (uint32_t)q, (uint32_t)e};
-            // VALU per row of one wave, shared by its 64 / LP pairs (the 16-bit kernel: about
-            // 10 packed ops per two slots).
-            const double per_row = aff_packed(bin.cls) ? 5.0 * aff_k(bin.cls) + 25.0 : 9.0 * aff_k(bin.cls) + 15.0;
-            const double ppw = 64.0 / aff_lp(bin.cls);
-            for (uint64_t x = q; x < e; ++x) {
-                bin.chain = std::max(bin.chain, b->plen[order[x]] * per_row);
-                bin.spread += b->plen[order[x]] * per_row / ppw / 1024.0;
-            }
-            pl.bins.push_back(bin);
-            q = e;
-        }
-        if (!pl.d_ba && ((st = dalloc(&pl.d_ba, n)) || (st = dalloc(&pl.d_order, n)) || (st = dalloc(&pl.d_desc, n))))
-            return st;
-        if (!b->d_score && ((st = dalloc(&b->d_score, n)) || (st = dalloc(&b->d_col, n)))) return st;
-        HIP_TRY(hipMemcpy(pl.d_ba, ba.data(), n * 8, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(pl.d_order, order.data(), n * 4, hipMemcpyHostToDevice));
-        HIP_TRY(sf_launch_desc(n, pl.d_order, view(b), pl.d_ba, pl.d_desc, b->stream));
-        pl.order = std::move(order);
-        pl.key = key;
-        pl.valid = true;
-        pl.cells = cells;
+        if ((st = build_plan(b, pl, ba, kcls, key, cells))) return st;
     }
+    if (!b->d_score && ((st = dalloc(&b->d_score, n)) || (st = dalloc(&b->d_col, n)))) return st;
     AffineArgs a{};
     a.pv = view(b);
     a.mode = mode;
@@ -798,7 +834,7 @@ int sf_batch_use_located(sf_batch *b, int on) {
         b->tlen = b->tlen0;
     }
     HIP_TRY(sf_launch_desc(n, b->d_morder, view(b), nullptr, b->d_mdesc, b->stream)); // the texts moved
-    b->aplan[0].valid = b->aplan[1].valid = false; // the affine shape classes depend on the text lengths
+    for (auto &pl : b->aplan) pl.valid = false; // the affine shape classes depend on the text lengths
     b->myers_cells_per_run = 0;
     for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];
     return SF_OK;
@@ -872,6 +908,277 @@ void sf_batch_reset_timing(sf_batch *b) {
 }
 
 // ---- one-shot calls ----
+
+// ---- passes with per-row bookkeeping (affine_dp with clip/cut/geom; local bands) ----
+
+int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t *band, const uint64_t *geom,
+                       int has_clip, int32_t xdrop, int clip_cell, const uint64_t *cut) {
+    if (!b) return fail(SF_EINVAL, "batch is null");
+    if (!sc) return fail(SF_EINVAL, "scoring is null");
+    if (mode < SF_INFIX || mode > SF_GLOBAL) return fail(SF_EINVAL, "mode %d", mode);
+    if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0)
+        return fail(SF_EINVAL, "scores must be non-negative (align.rs:2511 requires open, ext >= 0)");
+    if (sc->mat > 65535 || sc->mis > 65535 || sc->open > 65535 || sc->ext > 65535)
+        return fail(SF_ERANGE, "scores above 65535 are outside this build's bound");
+    int st;
+    if ((st = ensure_device())) return st;
+    const uint64_t n = b->n;
+    sf_batch::AffPlan &pl = b->aplan[2];
+    uint64_t key = (0x51ed270b27f1a3c5ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8)) * 1099511628211ull;
+    for (uint64_t i = 0; i < n; ++i) {
+        key = (key ^ (band ? band[i] : SF_BAND_EXACT)) * 1099511628211ull;
+        key = (key ^ (geom ? geom[i] : 0x9e3779b97f4a7c15ull)) * 1099511628211ull;
+    }
+    if (!pl.valid || pl.key != key) {
+        std::vector<uint64_t> ba(n);
+        std::vector<int> kcls(n);
+        uint64_t cells = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            uint64_t below, above;
+            const uint32_t m = b->plen[i], t = b->tlen[i];
+            // align.rs:2464-2474: the band laid out for a text of geom bases.
+            sf::band_geom_for(m, t, geom ? geom[i] : t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
+            const uint64_t width = below + above + 1;
+            kcls[i] = sf_affine_class(width, width <= sf_affine_lp16_max());
+            if (kcls[i] == 0)
+                return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this build handles up to 2048",
+                            (unsigned long long)i, (unsigned long long)width);
+            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t, aff_slots(kcls[i])))
+                return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
+            ba[i] = below | (above << 32);
+            cells += sf::band_cells(m, t, below, above);
+        }
+        if ((st = build_plan(b, pl, ba, kcls, key, cells))) return st;
+    }
+    if (!b->d_dp && (st = dalloc(&b->d_dp, n))) return st;
+    uint64_t *d_cut = nullptr;
+    if (cut) {
+        if ((st = dalloc(&d_cut, n))) return st;
+        if (hipMemcpy(d_cut, cut, n * 8, hipMemcpyHostToDevice) != hipSuccess) {
+            dfree(d_cut);
+            return fail(SF_EDEVICE, "upload failed");
+        }
+    }
+    PassArgs a{};
+    a.pv = view(b);
+    a.mode = mode;
+    a.mat = sc->mat, a.mis = sc->mis, a.open = sc->open, a.ext = sc->ext;
+    a.has_clip = has_clip != 0;
+    a.clip_cell = clip_cell != 0;
+    a.xdrop = xdrop;
+    a.cut = d_cut;
+    a.out = b->d_dp;
+    hipEvent_t e0;
+    if ((st = timed_begin(b, 2, &e0))) return st;
+    st = run_bins(b, pl.bins, [&](const sf_batch::Bin &bin, hipStream_t s) {
+        PassArgs x = a;
+        x.order = pl.d_order + bin.begin;
+        x.desc = pl.d_desc + bin.begin;
+        x.count = bin.end - bin.begin;
+        return sf_launch_pass(x, bin.cls, false, s);
+    });
+    if (!st) st = timed_end(b, 2, e0);
+    if (d_cut) {
+        (void)hipStreamSynchronize(b->stream);
+        dfree(d_cut);
+    }
+    if (st) return st;
+    b->affine_cells += pl.cells;
+    b->affine_launches += pl.bins.size();
+    return SF_OK;
+}
+
+int sf_batch_dp_results(sf_batch *b, sf_dp *out) {
+    int st;
+    if ((st = sf_batch_sync(b))) return st;
+    if (!out) return fail(SF_EINVAL, "out is null");
+    if (!b->d_dp) return fail(SF_EINVAL, "no sf_batch_affine_dp run on this batch");
+    static_assert(sizeof(sf_dp) == sizeof(DpOut), "sf_dp and the device record agree");
+    HIP_TRY(hipMemcpy(out, b->d_dp, b->n * sizeof(sf_dp), hipMemcpyDeviceToHost));
+    return SF_OK;
+}
+
+int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, const int64_t *dhi, int32_t bonus_start,
+                        int32_t bonus_end) {
+    if (!b) return fail(SF_EINVAL, "batch is null");
+    if (!sc || !dlo || !dhi) return fail(SF_EINVAL, "scoring and diagonals are required");
+    if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0)
+        return fail(SF_EINVAL, "scores must be non-negative (align.rs:2511 requires open, ext >= 0)");
+    if (sc->mat > 65535 || sc->mis > 65535 || sc->open > 65535 || sc->ext > 65535)
+        return fail(SF_ERANGE, "scores above 65535 are outside this build's bound");
+    if (bonus_start < -(1 << 24) || bonus_start > (1 << 24) || bonus_end < -(1 << 24) || bonus_end > (1 << 24))
+        return fail(SF_ERANGE, "end bonuses beyond +-2^24 are outside this build's bound");
+    int st;
+    if ((st = ensure_device())) return st;
+    const uint64_t n = b->n;
+    sf_batch::AffPlan &pl = b->aplan[3];
+    uint64_t key = (0x2545f4914f6cdd1dull ^ (sf_affine_lp16_max() << 8)) * 1099511628211ull;
+    for (uint64_t i = 0; i < n; ++i) {
+        key = (key ^ (uint64_t)dlo[i]) * 1099511628211ull;
+        key = (key ^ (uint64_t)dhi[i]) * 1099511628211ull;
+    }
+    if (!pl.valid || pl.key != key) {
+        std::vector<uint64_t> ba(n);
+        std::vector<int> kcls(n);
+        uint64_t cells = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t m = b->plen[i], t = b->tlen[i];
+            // Diagonals dlo..=dhi (j - i): row i covers [i + dlo, i + dhi] within [0, n]
+            // (align.rs:790-795) -- the band below = -dlo, above = dhi.  An empty range scores 0.
+            const int64_t lo = dlo[i], hi = dhi[i] < lo ? lo : dhi[i];
+            if (lo < -(int64_t)(1u << 30) || hi > (int64_t)(1u << 30))
+                return fail(SF_ERANGE, "pair %llu: diagonals beyond +-2^30", (unsigned long long)i);
+            const uint64_t width = (uint64_t)(hi - lo + 1);
+            kcls[i] = sf_affine_class(width, width <= sf_affine_lp16_max());
+            if (kcls[i] == 0)
+                return fail(SF_ERANGE, "pair %llu: band of %llu diagonals; this build handles up to 2048",
+                            (unsigned long long)i, (unsigned long long)width);
+            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t, aff_slots(kcls[i])))
+                return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
+            const uint32_t below = (uint32_t)(int32_t)(-lo), above = (uint32_t)(int32_t)(dhi[i] < lo ? lo - 1 : dhi[i]);
+            ba[i] = (uint64_t)below | ((uint64_t)above << 32);
+            for (uint64_t r = 1; r <= m && dhi[i] >= lo; ++r) {
+                const int64_t a0 = (int64_t)r + lo < 0 ? 0 : (int64_t)r + lo;
+                const int64_t a1 = (int64_t)r + dhi[i] > (int64_t)t ? (int64_t)t : (int64_t)r + dhi[i];
+                if (a1 >= a0) cells += (uint64_t)(a1 - a0 + 1);
+            }
+        }
+        if ((st = build_plan(b, pl, ba, kcls, key, cells))) return st;
+    }
+    if (!b->d_best && (st = dalloc(&b->d_best, n))) return st;
+    PassArgs a{};
+    a.pv = view(b);
+    a.mode = SF_PREFIX;
+    a.mat = sc->mat, a.mis = sc->mis, a.open = sc->open, a.ext = sc->ext;
+    a.bonus0 = bonus_start;
+    a.bonus1 = bonus_end;
+    a.pinv = b->d_pinv;
+    a.best = b->d_best;
+    hipEvent_t e0;
+    if ((st = timed_begin(b, 2, &e0))) return st;
+    if ((st = run_bins(b, pl.bins, [&](const sf_batch::Bin &bin, hipStream_t s) {
+             PassArgs x = a;
+             x.order = pl.d_order + bin.begin;
+             x.desc = pl.d_desc + bin.begin;
+             x.count = bin.end - bin.begin;
+             return sf_launch_pass(x, bin.cls, true, s);
+         })))
+        return st;
+    if ((st = timed_end(b, 2, e0))) return st;
+    b->affine_cells += pl.cells;
+    b->affine_launches += pl.bins.size();
+    return SF_OK;
+}
+
+int sf_batch_local_results(sf_batch *b, int32_t *best) {
+    int st;
+    if ((st = sf_batch_sync(b))) return st;
+    if (!best) return fail(SF_EINVAL, "best is null");
+    if (!b->d_best) return fail(SF_EINVAL, "no sf_batch_local_band run on this batch");
+    HIP_TRY(hipMemcpy(best, b->d_best, b->n * 4, hipMemcpyDeviceToHost));
+    return SF_OK;
+}
+
+namespace {
+int32_t sat_add(int32_t a, int32_t b) {
+    const int64_t r = (int64_t)a + b;
+    return r > INT32_MAX ? INT32_MAX : (r < INT32_MIN ? INT32_MIN : (int32_t)r);
+}
+
+// One pass per pair on a temporary batch: the batch forms of the reference's per-call wrappers.
+int run_dp(const sf_pairs *pairs, int mode, const sf_scoring *sc, const uint64_t *band, const uint64_t *geom,
+           int has_clip, int32_t xdrop, int clip_cell, const uint64_t *cut, std::vector<sf_dp> &out) {
+    out.assign(pairs ? pairs->n : 0, sf_dp{});
+    if (pairs && pairs->n == 0) return check_pairs(pairs);
+    sf_batch *b;
+    int st = sf_batch_create(pairs, &b);
+    if (st) return st;
+    st = sf_batch_affine_dp(b, mode, sc, band, geom, has_clip, xdrop, clip_cell, cut);
+    if (!st) st = sf_batch_dp_results(b, out.data());
+    sf_batch_destroy(b);
+    return st;
+}
+} // namespace
+
+int sf_affine_dp(const sf_pairs *pairs, int mode, const sf_scoring *sc, const uint64_t *band, const uint64_t *geom,
+                 int has_clip, int32_t xdrop, int clip_cell, const uint64_t *cut, sf_dp *out) {
+    if (!out) return fail(SF_EINVAL, "out is null");
+    std::vector<sf_dp> r;
+    const int st = run_dp(pairs, mode, sc, band, geom, has_clip, xdrop, clip_cell, cut, r);
+    if (!st) std::copy(r.begin(), r.end(), out);
+    return st;
+}
+
+int sf_affine_extend(const sf_pairs *pairs, const sf_scoring *sc, const uint64_t *band, int32_t end_bonus,
+                     int32_t xdrop, int32_t *score) {
+    if (!score) return fail(SF_EINVAL, "score is null");
+    std::vector<sf_dp> r;
+    const int st = run_dp(pairs, SF_PREFIX, sc, band, nullptr, 1, xdrop, 0, nullptr, r);
+    if (st) return st;
+    for (size_t i = 0; i < r.size(); ++i) score[i] = std::max(r[i].clip_best, sat_add(r[i].score, end_bonus));
+    return SF_OK;
+}
+
+int sf_affine_extend_window(const sf_pairs *pairs, const uint64_t *window, const sf_scoring *sc, const uint64_t *band,
+                            int32_t end_bonus, int32_t xdrop, sf_extend_window *out) {
+    if (!out || !window) return fail(SF_EINVAL, "window and out are required");
+    const uint64_t n = pairs ? pairs->n : 0;
+    std::vector<uint64_t> geom(n);
+    for (uint64_t i = 0; i < n; ++i) geom[i] = std::max<uint64_t>(window[i], pairs->txt_len[i]);
+    std::vector<sf_dp> r;
+    const int st = run_dp(pairs, SF_PREFIX, sc, band, geom.data(), 1, xdrop, 1, nullptr, r);
+    if (st) return st;
+    for (uint64_t i = 0; i < n; ++i) {
+        const int32_t whole = sat_add(r[i].score, end_bonus);
+        if (r[i].clip_best >= whole)
+            out[i] = sf_extend_window{r[i].clip_best, r[i].clip_i, r[i].clip_j, r[i].text_read};
+        else
+            out[i] = sf_extend_window{whole, pairs->pat_len[i], r[i].score_col, r[i].text_read};
+    }
+    return SF_OK;
+}
+
+int sf_affine_extend_cut_bonus(const sf_pairs *pairs, const sf_scoring *sc, const uint64_t *band, int32_t end_bonus,
+                               int32_t xdrop, int diagonal, const uint64_t *window, const uint64_t *cut, int track_cut,
+                               int bonus_within_cut, sf_extend_cut *out) {
+    if (!out || !window || !cut) return fail(SF_EINVAL, "window, cut and out are required");
+    const uint64_t n = pairs ? pairs->n : 0;
+    std::vector<uint64_t> geom(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t m = pairs->pat_len[i], t = pairs->txt_len[i];
+        geom[i] = diagonal ? std::min(m, t) : std::max(t, window[i]);
+    }
+    // align.rs:924-933: the best cell is tracked only with track_cut and a cut.
+    // align.rs:924-933: the best cell is tracked only with track_cut and a cut; untracked, it stays
+    // (0, 0).  Tracking changes no score, so one pass tracking it for all pairs serves both.
+    std::vector<sf_dp> r(n);
+    const int st = n ? run_dp(pairs, SF_PREFIX, sc, band, geom.data(), 1, xdrop, track_cut ? 1 : 0, cut, r) : SF_OK;
+    if (st) return st;
+    for (uint64_t i = 0; i < n; ++i) {
+        sf_dp &d = r[i];
+        if (!(track_cut && cut[i] != SF_BAND_EXACT)) d.clip_i = d.clip_j = 0;
+        const int32_t end = bonus_within_cut && cut[i] != SF_BAND_EXACT ? std::max(d.score, sat_add(d.cut_score, end_bonus))
+                                                                       : sat_add(d.score, end_bonus);
+        const int32_t whole = std::max(d.clip_best, end);
+        const uint32_t cols = d.clip_best >= end ? d.clip_j : d.score_col;
+        const int32_t cutv = cut[i] == SF_BAND_EXACT ? whole : std::max(d.cut_best, sat_add(d.cut_score, end_bonus));
+        out[i] = sf_extend_cut{whole, cutv, cols};
+    }
+    return SF_OK;
+}
+
+int sf_local_band_ends(const sf_pairs *pairs, const sf_scoring *sc, const int64_t *dlo, const int64_t *dhi,
+                       int32_t bonus_start, int32_t bonus_end, int32_t *best) {
+    if (!best) return fail(SF_EINVAL, "best is null");
+    if (pairs && pairs->n == 0) return check_pairs(pairs);
+    sf_batch *b;
+    int st = sf_batch_create(pairs, &b);
+    if (st) return st;
+    st = sf_batch_local_band(b, sc, dlo, dhi, bonus_start, bonus_end);
+    if (!st) st = sf_batch_local_results(b, best);
+    sf_batch_destroy(b);
+    return st;
+}
 
 int sf_myers(const sf_pairs *pairs, int mode, uint32_t k, uint32_t *dist, uint32_t *end, uint32_t *start) {
     if (!dist) return fail(SF_EINVAL, "dist is null");
