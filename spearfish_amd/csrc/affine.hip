@@ -66,6 +66,22 @@ __device__ __forceinline__ uint32_t push_ugt(uint32_t acc, uint32_t a, uint32_t 
 
 constexpr uint32_t kOff = 0x80000000u; // offset-unsigned domain: v = x + 2^31 (mod 2^32)
 
+// The 2-bit code at bits sh, sh+1 of w, replicated into all 16 fields of a word: a byte of four
+// copies, broadcast to the word's four bytes by one byte permute (no 32-bit multiply).
+__device__ __forceinline__ uint32_t rep16(uint32_t w, uint32_t sh) {
+    const uint32_t c4 = __umul24(__builtin_amdgcn_ubfe(w, sh, 2), 0x55u); // the code in 4 fields: one byte
+    return __builtin_amdgcn_perm(0u, c4, 0u);                            // that byte in all 4 bytes
+}
+
+// Bit 2f+1 of the result is set where field f of `win` differs from field f of `rep`:
+// t = win ^ rep, then (t << 1) | t in one v_lshl_or.
+__device__ __forceinline__ uint32_t mismatch_bits(uint32_t win, uint32_t rep) {
+    const uint32_t t = win ^ rep;
+    uint32_t y;
+    asm("v_lshl_or_b32 %0, %1, 1, %1" : "=v"(y) : "v"(t));
+    return y;
+}
+
 // Exclusive max-scan of unsigned values over each group of LP lanes (16: one DPP row; 64: the
 // wave); the group's first lane gets 0.
 template <int LP>
@@ -161,19 +177,15 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
         for (int k = 0; k < NWIN; ++k) win[k] = codes16(T, pos + 16 * k);
         const int32_t cnt = rows - i0 + 1 < BLK ? rows - i0 + 1 : BLK;
         for (int32_t r = 0; r < cnt; ++r) {
-            const uint32_t pc = pw & 3u;
+            const uint32_t pcw = pw;
             pw >>= 2;
             if (!ONE && i0 + r > last) continue; // this pair's rows are done
-            // The row's pattern code in every 2-bit field (the low 12 fields suffice up to K = 12).
-            const uint32_t p5 = __umul24(pc, 0x5555u);
-            const uint32_t prep = ONE ? pc * 0x55555555u : (K <= 12 ? __umul24(pc, 0x555555u) : p5 | (p5 << 16));
+            // The row's pattern code in every 2-bit field.
+            const uint32_t prep = rep16(pcw, 0);
             // Bit 2q+1 of y: slot q's text code differs from the pattern code.
             uint32_t y[NY];
 #pragma unroll
-            for (int k = 0; k < NY; ++k) {
-                const uint32_t t = win[k] ^ prep;
-                y[k] = (t << 1) | t;
-            }
+            for (int k = 0; k < NY; ++k) y[k] = mismatch_bits(win[k], prep);
             const uint32_t Un = dpp0<SHL>(Ub[0]);
             const uint32_t Fn = dpp0<SHL>(Fb[0]);
             uint32_t X[K], FQ[K], G[K], D[K], Uu[K];
@@ -183,6 +195,7 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
                 const uint32_t fu = q < K - 1 ? Fb[q + 1] : Fn;
                 const uint32_t mism = __builtin_amdgcn_ubfe(y[q / 16], 2 * (q % 16) + 1, 1);
                 const uint32_t f = umax(uu, fu);
+                Fb[q] = f; // in place: slot q - 1 has read Fb[q] already
                 const uint32_t d = V[q] + (uint32_t)__mul24((int)mism, ndl) + scm;
                 const uint32_t x = umax(d, f);
                 X[q] = x;
@@ -198,7 +211,6 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
             for (int q = 0; q < K; ++q) {
                 E[q] = q ? umax(C, G[q - 1]) : C; // biased horizontal-gap value entering slot q
                 V[q] = umax(X[q], E[q]);
-                Fb[q] = FQ[q];
             }
 #pragma unroll
             for (int q = 0; q < K; ++q) Ub[q] = V[q] + dltv[q];
@@ -358,6 +370,9 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
     int32_t rows = imax(last, __shfl_xor(last, 16));
     rows = imax(rows, __shfl_xor(rows, 32));
     rows = __builtin_amdgcn_readfirstlane(rows);
+    int32_t rows_all = imin(last, __shfl_xor(last, 16)); // rows every pair of the wave has
+    rows_all = imin(rows_all, __shfl_xor(rows_all, 32));
+    rows_all = __builtin_amdgcn_readfirstlane(rows_all);
     for (int32_t i0 = 1; i0 <= rows; i0 += BLK) {
         const uint32_t pw = codes16(P, i0 - 1);
         const int32_t pos = i0 - below + s0 - 1; // text position of register 0's low slot in row i0
@@ -365,54 +380,56 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
 #pragma unroll
         for (int k = 0; k < NWP; ++k)
             wp[k] = (codes16(T, pos + 8 * k) & 0xFFFFu) | (codes16(T, pos + K2 + 8 * k) << 16);
+        // One row, r static after unrolling (so every window field position is a constant).
+        auto row = [&](const int r) {
+            const uint32_t prep = rep16(pw, 2 * r);
+            // Bit 2f+1 (low half) and 2f+17 (high half) of y[k]: field f of word k mismatches.
+            const int k0 = r / 8, k1 = (r + K2 - 1) / 8;
+            uint32_t y[NWP];
 #pragma unroll
-        for (int r = 0; r < BLK; ++r) {
-            if (i0 + r <= last) { // this pair's rows are not done
-                const uint32_t pc = (pw >> (2 * r)) & 3u;
-                const uint32_t p5 = __umul24(pc, 0x5555u);
-                const uint32_t prep = p5 | (p5 << 16);
-                // Bit 2f+1 (low half) and 2f+17 (high half) of y[k]: field f of word k mismatches.
-                const int k0 = r / 8, k1 = (r + K2 - 1) / 8;
-                uint32_t y[NWP];
+            for (int k = k0; k <= k1; ++k) y[k] = mismatch_bits(wp[k], prep);
+            const uint32_t un = dpp0<kRowShl1>(as1(Ub[0]));
+            const uint32_t fn = dpp0<kRowShl1>(as1(Fb[0]));
+            // Register K2 - 1: low half reads slot s0 + K2 (register 0, high half), high half
+            // reads slot s0 + 2 K2 (the next lane's register 0, low half).
+            const u16x2 uuK = as2(__builtin_amdgcn_alignbit(un, as1(Ub[0]), 16));
+            const u16x2 fuK = as2(__builtin_amdgcn_alignbit(fn, as1(Fb[0]), 16));
+            u16x2 X[K2], g[K2];
 #pragma unroll
-                for (int k = k0; k <= k1; ++k) {
-                    const uint32_t t = wp[k] ^ prep;
-                    y[k] = (t << 1) | t;
-                }
-                const uint32_t un = dpp0<kRowShl1>(as1(Ub[0]));
-                const uint32_t fn = dpp0<kRowShl1>(as1(Fb[0]));
-                // Register K2 - 1: low half reads slot s0 + K2 (register 0, high half), high half
-                // reads slot s0 + 2 K2 (the next lane's register 0, low half).
-                const u16x2 uuK = as2(__builtin_amdgcn_alignbit(un, as1(Ub[0]), 16));
-                const u16x2 fuK = as2(__builtin_amdgcn_alignbit(fn, as1(Fb[0]), 16));
-                u16x2 X[K2], F[K2], g[K2];
-#pragma unroll
-                for (int q = 0; q < K2; ++q) {
-                    const u16x2 uu = q < K2 - 1 ? Ub[q + 1] : uuK;
-                    const u16x2 fu = q < K2 - 1 ? Fb[q + 1] : fuK;
-                    const int fld = q + r;
-                    const uint32_t mt = ~(y[fld / 8] >> (2 * (fld % 8) + 1)) & 0x00010001u; // match bits
-                    const u16x2 d = V[q] + as2(mt) * vdm;
-                    const u16x2 f = pmax(uu, fu);
-                    X[q] = pmax(d, f);
-                    F[q] = f;
-                    g[q] = psub(X[q], vopen);
-                }
-                u16x2 tot = g[0];
-#pragma unroll
-                for (int q = 1; q < K2; ++q) tot = pmax(tot, g[q]);
-                const uint32_t tt = as1(tot);
-                const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
-                const uint32_t C = group_excl_umax<16>(tlo > thi ? tlo : thi);
-                u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16)); // gap values entering register 0
-#pragma unroll
-                for (int q = 0; q < K2; ++q) {
-                    V[q] = pmax(X[q], E);
-                    if (q < K2 - 1) E = pmax(E, g[q]);
-                    Fb[q] = F[q];
-                    Ub[q] = psub(V[q], dl[q]);
-                }
+            for (int q = 0; q < K2; ++q) {
+                const u16x2 uu = q < K2 - 1 ? Ub[q + 1] : uuK;
+                const u16x2 fu = q < K2 - 1 ? Fb[q + 1] : fuK;
+                const int fld = q + r;
+                const uint32_t mt = ~(y[fld / 8] >> (2 * (fld % 8) + 1)) & 0x00010001u; // match bits
+                const u16x2 d = V[q] + as2(mt) * vdm;
+                const u16x2 f = pmax(uu, fu);
+                Fb[q] = f; // in place: register q - 1 has read Fb[q] already
+                X[q] = pmax(d, f);
+                g[q] = psub(X[q], vopen);
             }
+            u16x2 tot = g[0];
+#pragma unroll
+            for (int q = 1; q < K2; ++q) tot = pmax(tot, g[q]);
+            const uint32_t tt = as1(tot);
+            const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
+            const uint32_t C = group_excl_umax<16>(tlo > thi ? tlo : thi);
+            u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16)); // gap values entering register 0
+#pragma unroll
+            for (int q = 0; q < K2; ++q) {
+                V[q] = pmax(X[q], E);
+                if (q < K2 - 1) E = pmax(E, g[q]);
+                Ub[q] = psub(V[q], dl[q]);
+            }
+        };
+        if (i0 + BLK - 1 <= rows_all) {
+            // Every pair of the wave has all 16 rows of this block: no per-row guard, so the
+            // rows' values stay in fixed registers.
+#pragma unroll
+            for (int r = 0; r < BLK; ++r) row(r);
+        } else {
+#pragma unroll
+            for (int r = 0; r < BLK; ++r)
+                if (i0 + r <= last) row(r); // this pair's rows are not done
         }
     }
     const bool stopped = last < m;
@@ -589,18 +606,14 @@ __global__ __launch_bounds__(256) void affine_pass_kernel(PassArgs a) {
         const int32_t cnt = rows - i0 + 1 < BLK ? rows - i0 + 1 : BLK;
         for (int32_t r = 0; r < cnt; ++r) {
             const int32_t i = i0 + r;
-            const uint32_t pc = pw & 3u;
+            const uint32_t pcw = pw;
             pw >>= 2;
             const bool inv_row = (inv >> r) & 1u;
             if (i > last || xstop) continue; // this pair's rows are done
-            const uint32_t p5 = __umul24(pc, 0x5555u);
-            const uint32_t prep = p5 | (p5 << 16);
+            const uint32_t prep = rep16(pcw, 0);
             uint32_t y[NY];
 #pragma unroll
-            for (int k = 0; k < NY; ++k) {
-                const uint32_t t = win[k] ^ prep;
-                y[k] = inv_row ? 0xFFFFFFFFu : ((t << 1) | t);
-            }
+            for (int k = 0; k < NY; ++k) y[k] = inv_row ? 0xFFFFFFFFu : mismatch_bits(win[k], prep);
             // Band cells of row i: slots [slo, shi] (columns 0..n).
             const int32_t slo = below - i > 0 ? below - i : 0;
             const int32_t shi = n - i + below < Wd - 1 ? n - i + below : Wd - 1;
