@@ -942,12 +942,28 @@ static std::atomic<uint64_t> g_lp16_max{[] {
 
 uint64_t sf_affine_lp16_max() { return g_lp16_max.load(std::memory_order_relaxed); }
 
+// Slots per lane of the DP kernels' classes: every count up to 16, then steps that keep the unused
+// slots of a class under about 1/9 (a pair uses the smallest class that holds its band).
+static const int kClasses16[] = {1,  2,  3,  4,  5,  6,  7,  8,  9,  10, 11, 12, 13, 14, 15, 16,
+                                 18, 20, 22, 24, 26, 28, 30, 32};
+static const int kClasses64[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 24, 28, 32};
+// The passes with bookkeeping (small calls) use fewer classes.
+static const int kClassesPass[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32};
+
 int sf_affine_class(uint64_t width, bool lp16) {
-    static const int cls[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32};
     if (lp16 && width <= 16 * 32)
-        for (int c : cls)
+        for (int c : kClasses16)
             if (width <= (uint64_t)c * 16) return 16000 + c;
-    for (int c : cls)
+    for (int c : kClasses64)
+        if (width <= (uint64_t)c * 64) return c;
+    return 0;
+}
+
+int sf_affine_pass_class(uint64_t width, bool lp16) {
+    if (lp16 && width <= 16 * 32)
+        for (int c : kClassesPass)
+            if (width <= (uint64_t)c * 16) return 16000 + c;
+    for (int c : kClassesPass)
         if (width <= (uint64_t)c * 64) return c;
     return 0;
 }
@@ -977,7 +993,8 @@ hipError_t sf_launch_affine(const AffineArgs &a, int cls, bool tb, hipStream_t s
         switch (aff_k(cls)) {
 #define PK(KK)                                                                                   \
     case 2 * KK: hipLaunchKernelGGL((affine_pk16_kernel<KK>), grid, block, 0, s, a); break;
-            PK(1) PK(2) PK(3) PK(4) PK(5) PK(6) PK(7) PK(8) PK(10) PK(12) PK(14) PK(16)
+            PK(1) PK(2) PK(3) PK(4) PK(5) PK(6) PK(7) PK(8) PK(9) PK(10) PK(11) PK(12) PK(13) PK(14) PK(15)
+            PK(16)
 #undef PK
         default:
             return hipErrorInvalidValue;
@@ -992,9 +1009,11 @@ hipError_t sf_launch_affine(const AffineArgs &a, int cls, bool tb, hipStream_t s
         break;
 #define CASES(LL)                                                                                \
     CASE(LL, 1) CASE(LL, 2) CASE(LL, 3) CASE(LL, 4) CASE(LL, 5) CASE(LL, 6) CASE(LL, 7) CASE(LL, 8) \
-    CASE(LL, 10) CASE(LL, 12) CASE(LL, 14) CASE(LL, 16) CASE(LL, 20) CASE(LL, 24) CASE(LL, 28) CASE(LL, 32)
+    CASE(LL, 9) CASE(LL, 10) CASE(LL, 11) CASE(LL, 12) CASE(LL, 13) CASE(LL, 14) CASE(LL, 15)        \
+    CASE(LL, 16) CASE(LL, 18) CASE(LL, 20) CASE(LL, 24) CASE(LL, 28) CASE(LL, 32)
         CASES(16)
         CASES(64)
+        CASE(16, 22) CASE(16, 26) CASE(16, 30)
 #undef CASES
 #undef CASE
     default:

@@ -235,7 +235,8 @@ constexpr uint64_t aff_tb_words_per_row(int cls) { return (uint64_t)aff_lp(cls) 
 hipError_t sf_launch_myers(const sf::MyersArgs &a, int maxb, hipStream_t s);
 hipError_t sf_launch_affine(const sf::AffineArgs &a, int K, bool tb, hipStream_t s);
 uint64_t sf_affine_lp16_max(); // the current 16-lane limit (slots), part of the shape choice
-int sf_affine_class(uint64_t width, bool lp16); // 16 lanes per pair if allowed and it fits, else 64
+int sf_affine_class(uint64_t width, bool lp16);      // 16 lanes per pair if allowed and it fits, else 64
+int sf_affine_pass_class(uint64_t width, bool lp16); // the same for the passes with bookkeeping
 // The 16-bit packed class for a pair of the 16-lane class with K slots per lane, or 0 where its
 // exact range does not hold (the scoring must have mis = 2 ext).
 int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m, uint32_t below);

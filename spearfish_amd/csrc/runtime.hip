@@ -939,7 +939,7 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
             // align.rs:2464-2474: the band laid out for a text of geom bases.
             sf::band_geom_for(m, t, geom ? geom[i] : t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
             const uint64_t width = below + above + 1;
-            kcls[i] = sf_affine_class(width, width <= sf_affine_lp16_max());
+            kcls[i] = sf_affine_pass_class(width, width <= sf_affine_lp16_max());
             if (kcls[i] == 0)
                 return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this build handles up to 2048",
                             (unsigned long long)i, (unsigned long long)width);
@@ -1029,7 +1029,7 @@ int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, c
             if (lo < -(int64_t)(1u << 30) || hi > (int64_t)(1u << 30))
                 return fail(SF_ERANGE, "pair %llu: diagonals beyond +-2^30", (unsigned long long)i);
             const uint64_t width = (uint64_t)(hi - lo + 1);
-            kcls[i] = sf_affine_class(width, width <= sf_affine_lp16_max());
+            kcls[i] = sf_affine_pass_class(width, width <= sf_affine_lp16_max());
             if (kcls[i] == 0)
                 return fail(SF_ERANGE, "pair %llu: band of %llu diagonals; this build handles up to 2048",
                             (unsigned long long)i, (unsigned long long)width);
