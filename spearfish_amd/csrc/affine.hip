@@ -314,7 +314,7 @@ __global__ __launch_bounds__(256) void affine_wave_kernel(AffineArgs a) {
 //
 // Each lane holds 2*K2 slots, slot s0 + r in the low half of register r and s0 + K2 + r in the
 // high half, so the vertical neighbour of register r is register r + 1 in both halves; the text
-// window pairs the codes the same way, so one shift puts both halves' mismatch bits 16 apart.
+// window pairs the codes the same way.
 typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ u16x2 as2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
 __device__ __forceinline__ uint32_t as1(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
@@ -325,7 +325,8 @@ template <int K2>
 __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
     constexpr int S = 2 * K2;                     // slots per lane
     constexpr int BLK = 16;                       // rows per block: one pattern word
-    constexpr int NWP = (K2 + BLK - 1 + 7) / 8;   // paired text words: fields q + r, q < K2, r < 16
+    constexpr int NF = K2 + BLK - 1;              // text fields of a block: q + r, q < K2, r < 16
+    constexpr int NWP = (NF + 7) / 8;             // paired text words holding them
     const int lane = threadIdx.x & 63;
     const int l = lane & 15;
     const uint32_t w = (blockIdx.x * 256u + threadIdx.x) >> 4;
@@ -346,7 +347,14 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
     const int32_t s0 = l * S;
     const u16x2 vopen = (u16x2)((uint16_t)open), vdm = (u16x2)((uint16_t)dm);
 
-    u16x2 V[K2], Fb[K2], Ub[K2], dl[K2];
+    // G = X - open (X = max(diagonal, F): H before the row's horizontal gaps) is both the row's
+    // gap-open term for E and, as U = max(G, F), the next row's vertical term: F never opens from
+    // E, a horizontal gap turning straight into a vertical one.  With mis = 2 ext that turn never scores
+    // more than cutting its corner: a horizontal gap of a and a vertical gap of b cost
+    // 2 open + ext (a + b); a - 1, a diagonal step and b - 1 cost at most 2 open + ext (a + b - 2)
+    // + mis, the same, over cells the first path used.  So every H (and the score) is unchanged.
+    // Right of the band G is floored, as the up-values there (align.rs:1047-1048).
+    u16x2 V[K2], U[K2], dl[K2];
     const int32_t hi0 = above < n ? above : n;
 #pragma unroll
     for (int r = 0; r < K2; ++r) {
@@ -357,14 +365,11 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
             const int32_t j = s - below;
             const bool in = j >= 0 && j <= hi0;
             hv[h] = in ? (uint32_t)(((j == 0 || infix) ? 0 : -(open + ext * j)) + ext * s + off) : 0u;
-            // Right of the band the up-values are floored, so the last band slot's vertical
-            // terms are the floor, as the reference's NEG sentinel (align.rs:1047-1048).
             dv[h] = s >= Wd ? 0xFFFFu : (uint32_t)open;
         }
         V[r] = as2(hv[0] | (hv[1] << 16));
-        Fb[r] = as2(0u);
         dl[r] = as2(dv[0] | (dv[1] << 16));
-        Ub[r] = psub(V[r], dl[r]);
+        U[r] = psub(V[r], dl[r]); // row 0: the boundary's H opens the gaps, as the reference's
     }
     const int32_t last = valid ? (m < n + below ? m : n + below) : 0;
     int32_t rows = imax(last, __shfl_xor(last, 16));
@@ -380,46 +385,45 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
 #pragma unroll
         for (int k = 0; k < NWP; ++k)
             wp[k] = (codes16(T, pos + 8 * k) & 0xFFFFu) | (codes16(T, pos + K2 + 8 * k) << 16);
-        // One row, r static after unrolling (so every window field position is a constant).
-        auto row = [&](const int r) {
-            const uint32_t prep = rep16(pw, 2 * r);
-            // Bit 2f+1 (low half) and 2f+17 (high half) of y[k]: field f of word k mismatches.
-            const int k0 = r / 8, k1 = (r + K2 - 1) / 8;
-            uint32_t y[NWP];
+        // Field f of the block: the text codes of both halves' slots, as byte selectors (bytes 0
+        // and 2; 0x0c selects a zero byte), so one byte permute against the row's one-hot pattern
+        // code gives a register's two match bits.
+        uint32_t W[NF];
 #pragma unroll
-            for (int k = k0; k <= k1; ++k) y[k] = mismatch_bits(wp[k], prep);
-            const uint32_t un = dpp0<kRowShl1>(as1(Ub[0]));
-            const uint32_t fn = dpp0<kRowShl1>(as1(Fb[0]));
+        for (int f = 0; f < NF; ++f)
+            W[f] = ((wp[f / 8] >> (2 * (f % 8))) & 0x00030003u) | 0x0c000c00u;
+        // One row, r static after unrolling (so every field index is a constant).
+        auto row = [&](const int r) {
+            const uint32_t onehot = 1u << (__builtin_amdgcn_ubfe(pw, 2 * r, 2) << 3);
             // Register K2 - 1: low half reads slot s0 + K2 (register 0, high half), high half
             // reads slot s0 + 2 K2 (the next lane's register 0, low half).
-            const u16x2 uuK = as2(__builtin_amdgcn_alignbit(un, as1(Ub[0]), 16));
-            const u16x2 fuK = as2(__builtin_amdgcn_alignbit(fn, as1(Fb[0]), 16));
-            u16x2 X[K2], g[K2];
+            const uint32_t u0 = as1(U[0]);
+            const u16x2 fK = as2(__builtin_amdgcn_alignbit(dpp0<kRowShl1>(u0), u0, 16));
+            u16x2 X[K2], G[K2];
 #pragma unroll
             for (int q = 0; q < K2; ++q) {
-                const u16x2 uu = q < K2 - 1 ? Ub[q + 1] : uuK;
-                const u16x2 fu = q < K2 - 1 ? Fb[q + 1] : fuK;
-                const int fld = q + r;
-                const uint32_t mt = ~(y[fld / 8] >> (2 * (fld % 8) + 1)) & 0x00010001u; // match bits
+                const uint32_t mt = __builtin_amdgcn_perm(0u, onehot, W[q + r]); // match bits
                 const u16x2 d = V[q] + as2(mt) * vdm;
-                const u16x2 f = pmax(uu, fu);
-                Fb[q] = f; // in place: register q - 1 has read Fb[q] already
+                const u16x2 f = q < K2 - 1 ? U[q + 1] : fK; // F of this row
                 X[q] = pmax(d, f);
-                g[q] = psub(X[q], vopen);
+                G[q] = psub(X[q], dl[q]);
+                U[q] = pmax(G[q], f); // in place: register q - 1 has read U[q] already
             }
-            u16x2 tot = g[0];
+            // Horizontal gaps: Pre[q] = max(G[0 .. q-1]), the lane's own prefix, is built while
+            // the group scan runs; then every register takes max(X, Pre, gaps entering the lane)
+            // with no chain after the scan.
+            u16x2 Pre[K2];
+            Pre[1] = G[0];
 #pragma unroll
-            for (int q = 1; q < K2; ++q) tot = pmax(tot, g[q]);
+            for (int q = 2; q < K2; ++q) Pre[q] = pmax(Pre[q - 1], G[q - 1]);
+            const u16x2 tot = K2 > 1 ? pmax(Pre[K2 - 1], G[K2 - 1]) : G[0];
             const uint32_t tt = as1(tot);
             const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
             const uint32_t C = group_excl_umax<16>(tlo > thi ? tlo : thi);
-            u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16)); // gap values entering register 0
+            const u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16)); // gap values entering register 0
+            V[0] = pmax(X[0], E);
 #pragma unroll
-            for (int q = 0; q < K2; ++q) {
-                V[q] = pmax(X[q], E);
-                if (q < K2 - 1) E = pmax(E, g[q]);
-                Ub[q] = psub(V[q], dl[q]);
-            }
+            for (int q = 1; q < K2; ++q) V[q] = pmax(X[q], pmax(Pre[q], E));
         };
         if (i0 + BLK - 1 <= rows_all) {
             // Every pair of the wave has all 16 rows of this block: no per-row guard, so the

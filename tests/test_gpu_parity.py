@@ -203,6 +203,32 @@ def test_affine_scores_16bit(mode):
         check_affine(ps, mode, sc, np.array(B, np.uint64), cigar=False)
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_affine_scores_16bit_gap_corners(mode):
+    # The 16-bit kernel lets a vertical gap open only from H before the row's horizontal gaps (a
+    # horizontal gap never turns straight into a vertical one), which leaves every H unchanged when
+    # mis = 2 ext: the turn ties with cutting its corner.  Pairs built from adjacent deletion and
+    # insertion runs, under scorings where those ties are exact (open = 0, mat = 0), against the
+    # oracle.
+    rng = random.Random(61 + int(mode))
+    P, T, B = [], [], []
+    nmax = 300 if mode == A.Mode.Infix else 700
+    for _ in range(400):
+        n = rng.randrange(0, nmax)
+        t = [rng.randrange(4) for _ in range(n)]
+        p = list(t)
+        for _ in range(rng.randrange(0, 6)):
+            k = rng.randrange(len(p) + 1)
+            a, b = rng.randrange(1, 9), rng.randrange(1, 9)
+            p = p[:k] + [rng.randrange(4) for _ in range(b)] + p[k + a:]
+        P.append(p)
+        T.append(t)
+        B.append(rng.choice([0, 2, 5, 16, 40, 120]))
+    ps = A.PairSet.from_seqs(P, T)
+    for sc in (A.Affine.ONT, A.Affine(0, 2, 0, 1), A.Affine(1, 4, 0, 2), A.Affine(3, 2, 7, 1)):
+        check_affine(ps, mode, sc, np.array(B, np.uint64), cigar=False)
+
+
 def test_affine_edges():
     P, T = edge_pairs()
     ps = A.PairSet.from_seqs(P[:60], T[:60])
