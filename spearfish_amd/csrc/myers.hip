@@ -38,6 +38,58 @@ __device__ __forceinline__ void load_planes(const SeqRef &P, uint32_t m, int b, 
     p2 = (uint64_t)lo2 | ((uint64_t)hi2 << 32);
 }
 
+// Three-input bitwise ops (v_bitop3_b32): the immediate is the truth table, bit 4a + 2b + c, so a
+// function's table is the function applied to the three columns below.
+template <uint32_t TT>
+__device__ __forceinline__ uint32_t lop3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, TT);
+}
+constexpr uint32_t kColA = 0xF0u, kColB = 0xCCu, kColC = 0xAAu;
+constexpr uint32_t kXorAnd = ((kColA ^ kColB) & kColC) & 0xFFu; // (a ^ b) & c
+constexpr uint32_t kXorOr = ((kColA ^ kColB) | kColC) & 0xFFu;  // (a ^ b) | c
+constexpr uint32_t kOrNor = (kColA | ~(kColB | kColC)) & 0xFFu; // a | ~(b | c)
+
+// One text base through one 64-row block (Hyyro's step, align.rs:51-66) on 32-bit halves (lo,
+// hi), so no value needs an aligned register pair.  M1/M2 are all-ones or zero from the base's
+// code bits; hp/hn carry the horizontal delta in (0 or 1 each) and leave with the delta out of
+// row `bit` of the block.
+struct Blk {
+    uint32_t p1l, p1h, p2l, p2h; // pattern bit-planes
+    uint32_t vpl, vph, vml, vmh; // vertical deltas +1 / -1
+};
+
+template <bool LAST63>
+__device__ __forceinline__ void block_step(Blk &k, uint32_t M1, uint32_t M2, uint32_t &hp, uint32_t &hn, unsigned bit) {
+    const uint32_t eql = lop3<kXorAnd>(k.p1l, M1, k.p2l ^ M2);
+    const uint32_t eqh = lop3<kXorAnd>(k.p1h, M1, k.p2h ^ M2);
+    const uint32_t xvl = eql | k.vml, xvh = eqh | k.vmh;
+    const uint32_t enl = eql | hn, enh = eqh;
+    // (en & pv) + pv over 64 bits: one add with carry out, one with carry in.
+    const uint32_t al = enl & k.vpl, ah = enh & k.vph;
+    const uint32_t suml = al + k.vpl;
+    const uint32_t sumh = ah + k.vph + (suml < al ? 1u : 0u);
+    const uint32_t xhl = lop3<kXorOr>(suml, k.vpl, enl), xhh = lop3<kXorOr>(sumh, k.vph, enh);
+    const uint32_t hpl = lop3<kOrNor>(k.vml, xhl, k.vpl), hph = lop3<kOrNor>(k.vmh, xhh, k.vph); // +1 out
+    const uint32_t hnl = k.vpl & xhl, hnh = k.vph & xhh;                                         // -1 out
+    uint32_t op, om;
+    if (LAST63) {
+        op = hph >> 31;
+        om = hnh >> 31;
+    } else {
+        const bool lo = bit < 32;
+        op = __builtin_amdgcn_ubfe(lo ? hpl : hph, bit & 31u, 1);
+        om = __builtin_amdgcn_ubfe(lo ? hnl : hnh, bit & 31u, 1);
+    }
+    const uint32_t spl = (hpl << 1) | hp, sph = __builtin_amdgcn_alignbit(hph, hpl, 31);
+    const uint32_t sml = (hnl << 1) | hn, smh = __builtin_amdgcn_alignbit(hnh, hnl, 31);
+    k.vpl = lop3<kOrNor>(sml, xvl, spl);
+    k.vph = lop3<kOrNor>(smh, xvh, sph);
+    k.vml = spl & xvl;
+    k.vmh = sph & xvh;
+    hp = op;
+    hn = om;
+}
+
 template <int MAXB>
 __global__ __launch_bounds__(256) void myers_lane_kernel(MyersArgs a) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
@@ -80,15 +132,15 @@ __global__ __launch_bounds__(256) void myers_lane_kernel(MyersArgs a) {
     const int lastb = nb - 1;
     const unsigned last_bit = work ? (m - 1) & 63u : 63u;
 
-    uint64_t P1[MAXB], P2[MAXB], PV[MAXB], MV[MAXB];
+    Blk K[MAXB];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
-        P1[b] = P2[b] = 0;
-        if (b < nb && work) load_planes(P, m, b, rev, P1[b], P2[b]);
-        PV[b] = ~0ull;
-        MV[b] = 0;
+        uint64_t p1 = 0, p2 = 0;
+        if (b < nb && work) load_planes(P, m, b, rev, p1, p2);
+        K[b] = Blk{(uint32_t)p1, (uint32_t)(p1 >> 32), (uint32_t)p2, (uint32_t)(p2 >> 32), ~0u, ~0u, 0u, 0u};
     }
-    const uint64_t top = mode == kInfix ? 0 : 1;
+    const uint32_t top = mode == kInfix ? 0 : 1;
+    const bool full = nb_lo == MAXB; // every lane of the wave has all MAXB blocks
     int32_t score = (int32_t)m; // D[m][0]
     int32_t best = (int32_t)m;
     uint32_t bestj = 0;
@@ -99,42 +151,20 @@ __global__ __launch_bounds__(256) void myers_lane_kernel(MyersArgs a) {
         }
         const uint32_t c = tw & 3;
         tw >>= 2;
-        const uint64_t M1 = (c & 1) ? 0ull : ~0ull;
-        const uint64_t M2 = (c & 2) ? 0ull : ~0ull;
-        uint64_t hp = top, hn = 0;
+        const uint32_t M1 = (c & 1) ? 0u : ~0u;
+        const uint32_t M2 = (c & 2) ? 0u : ~0u;
+        uint32_t hp = top, hn = 0;
+        if (full) {
 #pragma unroll
-        for (int b = 0; b < MAXB; ++b) {
-            if (b < nb_lo - 1) {
-                const uint64_t eq = (P1[b] ^ M1) & (P2[b] ^ M2);
-                const uint64_t p = PV[b], mm = MV[b];
-                const uint64_t xv = eq | mm;
-                const uint64_t eqn = eq | hn;
-                const uint64_t xh = (((eqn & p) + p) ^ p) | eqn;
-                uint64_t ph = mm | ~(xh | p);
-                uint64_t mh = p & xh;
-                const uint64_t op = ph >> 63, om = mh >> 63;
-                ph = (ph << 1) | hp;
-                mh = (mh << 1) | hn;
-                PV[b] = mh | ~(xv | ph);
-                MV[b] = ph & xv;
-                hp = op;
-                hn = om;
-            } else if (b < nb) {
-                const uint64_t eq = (P1[b] ^ M1) & (P2[b] ^ M2);
-                const uint64_t p = PV[b], mm = MV[b];
-                const uint64_t xv = eq | mm;
-                const uint64_t eqn = eq | hn;
-                const uint64_t xh = (((eqn & p) + p) ^ p) | eqn;
-                uint64_t ph = mm | ~(xh | p);
-                uint64_t mh = p & xh;
-                const unsigned bit = b == lastb ? last_bit : 63u;
-                const uint64_t op = (ph >> bit) & 1, om = (mh >> bit) & 1;
-                ph = (ph << 1) | hp;
-                mh = (mh << 1) | hn;
-                PV[b] = mh | ~(xv | ph);
-                MV[b] = ph & xv;
-                hp = op;
-                hn = om;
+            for (int b = 0; b < MAXB - 1; ++b) block_step<true>(K[b], M1, M2, hp, hn, 63u);
+            block_step<false>(K[MAXB - 1], M1, M2, hp, hn, last_bit);
+        } else {
+#pragma unroll
+            for (int b = 0; b < MAXB; ++b) {
+                if (b < nb_lo - 1)
+                    block_step<true>(K[b], M1, M2, hp, hn, 63u);
+                else if (b < nb)
+                    block_step<false>(K[b], M1, M2, hp, hn, b == lastb ? last_bit : 63u);
             }
         }
         score += (int32_t)hp - (int32_t)hn;
@@ -191,14 +221,13 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
     const int nb = work ? (int)((m + 63) / 64) : 0;
     const int off = G * R - nb; // >= 0: the launcher's class holds every block
     const unsigned last_bit = work ? (m - 1) & 63u : 63u;
-    uint64_t P1[R], P2[R], PV[R], MV[R];
+    Blk K[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        P1[r] = P2[r] = 0;
+        uint64_t p1 = 0, p2 = 0;
         const int b = l * R + r - off;
-        if (work && b >= 0) load_planes(P, m, b, rev, P1[r], P2[r]);
-        PV[r] = ~0ull;
-        MV[r] = 0;
+        if (work && b >= 0) load_planes(P, m, b, rev, p1, p2);
+        K[r] = Blk{(uint32_t)p1, (uint32_t)(p1 >> 32), (uint32_t)p2, (uint32_t)(p2 >> 32), ~0u, ~0u, 0u, 0u};
     }
     // Steps: the wave runs until its longest text has passed every lane.
     int32_t steps = work ? (int32_t)n + G - 1 : 0;
@@ -218,34 +247,16 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
         const int32_t hin = __builtin_amdgcn_update_dpp(topv, hv, SHR, 0xf, 0xf, false);
         const int32_t j = t - l;
         if (work && j >= 0 && j < (int32_t)n) {
-            const uint64_t M1 = (c & 1) ? 0ull : ~0ull;
-            const uint64_t M2 = (c & 2) ? 0ull : ~0ull;
-            uint64_t hp = (uint64_t)(hin & 1), hn = (uint64_t)((hin >> 1) & 1);
+            const uint32_t M1 = (c & 1) ? 0u : ~0u;
+            const uint32_t M2 = (c & 2) ? 0u : ~0u;
+            uint32_t hp = (uint32_t)(hin & 1), hn = (uint32_t)((hin >> 1) & 1);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if (l * R + r >= off) {
-                    const uint64_t eq = (P1[r] ^ M1) & (P2[r] ^ M2);
-                    const uint64_t p = PV[r], mm = MV[r];
-                    const uint64_t xv = eq | mm;
-                    const uint64_t eqn = eq | hn;
-                    const uint64_t xh = (((eqn & p) + p) ^ p) | eqn;
-                    uint64_t ph = mm | ~(xh | p);
-                    uint64_t mh = p & xh;
-                    uint64_t op, om;
-                    if (r < R - 1) {
-                        op = ph >> 63;
-                        om = mh >> 63;
-                    } else {
-                        const unsigned bit = l == G - 1 ? last_bit : 63u;
-                        op = (ph >> bit) & 1;
-                        om = (mh >> bit) & 1;
-                    }
-                    ph = (ph << 1) | hp;
-                    mh = (mh << 1) | hn;
-                    PV[r] = mh | ~(xv | ph);
-                    MV[r] = ph & xv;
-                    hp = op;
-                    hn = om;
+                    if (r < R - 1)
+                        block_step<true>(K[r], M1, M2, hp, hn, 63u);
+                    else
+                        block_step<false>(K[r], M1, M2, hp, hn, l == G - 1 ? last_bit : 63u);
                 }
             }
             hv = (int32_t)(hp | (hn << 1));
