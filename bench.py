@@ -50,7 +50,7 @@ AFFINE_TB_OPS_PER_CELL = 11 + 4
 # HBM bytes per launch of the dominant kernels on the uniform workload, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the same counters on a 1 GiB copy
 # (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.
-TRAFFIC_BYTES = {"affine_pk16_kernel<9>": 735_104_922, "affine_wave_kernel<16,18,true>": 33_032_402_432}
+TRAFFIC_BYTES = {"affine_pk16_kernel<9>": 735_104_922, "affine_pk16_tb_kernel<9>": None}
 
 
 def parse():
@@ -198,8 +198,9 @@ def bench_uniform(args):
         "kernel_ms_per_step": {"myers": round(tm_c.myers_ms / args.steps, 2),
                                "affine_tb": round(tm_c.affine_ms / args.steps, 2),
                                "walk_compact": round(tm_c.walk_ms / args.steps, 2)},
-        "roofline": roofline(tm_c, AFFINE_TB_OPS_PER_CELL, "affine_wave_kernel<16,18,true>", VALU_PEAK_TOPS,
-                             "TOP/s (i32 VALU lane-ops; DP + traceback bits; average over the step's chunk launches)",
+        "roofline": roofline(tm_c, AFFINE_TB_OPS_PER_CELL, "affine_pk16_tb_kernel<9>", VALU16_PEAK_TOPS,
+                             "TOP/s (packed i16 VALU lane-ops; DP + traceback bits; average over the step's chunk "
+                             "launches)",
                              # every chunk is one launch of the dominant class plus, for this
                              # workload, one small LP64 launch a step for the widest pairs
                              max(tm_c.affine_launches // max(args.steps, 1) - 1, 1)),

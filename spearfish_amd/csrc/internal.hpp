@@ -238,8 +238,9 @@ uint64_t sf_affine_lp16_max(); // the current 16-lane limit (slots), part of the
 int sf_affine_class(uint64_t width, bool lp16);      // 16 lanes per pair if allowed and it fits, else 64
 int sf_affine_pass_class(uint64_t width, bool lp16); // the same for the passes with bookkeeping
 // The 16-bit packed class for a pair of the 16-lane class with K slots per lane, or 0 where its
-// exact range does not hold (the scoring must have mis = 2 ext).
-int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m, uint32_t below);
+// exact range does not hold (the scoring must have mis = 2 ext); tb: for the traceback kernel.
+int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m, uint32_t below,
+                         bool tb);
 hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
 // cls: a 32-bit affine class (16 or 64 lanes); local selects local_band_core.
 hipError_t sf_launch_pass(const sf::PassArgs &a, int cls, bool local, hipStream_t s);

@@ -670,10 +670,10 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
                 const Cost &c = it->second;
                 if (std::max(c.chain64, c.spread64) < std::max(c.chain16, c.spread16)) kcls[i] = k64[i];
             }
-            // Score-only runs take the 16-bit kernel where its exact range holds.
-            if (!cigar && aff_lp(kcls[i]) == 16) {
+            // 16-lane pairs take the 16-bit kernels where their exact range holds.
+            if (aff_lp(kcls[i]) == 16) {
                 const int kp = sf_affine_pk16_class(aff_k(kcls[i]), sc->mat, sc->mis, sc->open, sc->ext,
-                                                    b->plen[i], (uint32_t)ba[i]);
+                                                    b->plen[i], (uint32_t)ba[i], cigar);
                 if (kp) kcls[i] = kp;
             }
             // Values stay far above NEG over all slots of the class.
