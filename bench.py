@@ -50,7 +50,7 @@ AFFINE_TB_OPS_PER_CELL = 11 + 4
 # HBM bytes per launch of the dominant kernels on the uniform workload, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the same counters on a 1 GiB copy
 # (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.
-TRAFFIC_BYTES = {"affine_pk16_kernel<9>": 735_104_922, "affine_pk16_tb_kernel<9>": None}
+TRAFFIC_BYTES = {"affine_pk16_kernel<9>": 735_071_436, "affine_pk16_tb_kernel<9>": 32_791_216_537}
 
 
 def parse():

@@ -1120,16 +1120,18 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
     a.tstart[pi] = tstart;
 }
 
+// 16 lanes per pair: each pass copies 16 consecutive runs (one 64-byte line each way), reversed.
 __global__ __launch_bounds__(256) void compact_kernel(const uint32_t *order, uint32_t count, const uint32_t *runs,
                                                       const uint64_t *runs_off, const uint32_t *nruns,
                                                       const uint64_t *out_off, uint32_t *out) {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= count) return;
-    const uint32_t pi = order[t];
+    const uint64_t g = ((uint64_t)blockIdx.x * 256u + threadIdx.x) >> 4;
+    const uint32_t l = threadIdx.x & 15;
+    if (g >= count) return;
+    const uint32_t pi = order[g];
     const uint32_t nr = nruns[pi];
     const uint32_t *src = runs + runs_off[pi];
     uint32_t *dst = out + out_off[pi];
-    for (uint32_t r = 0; r < nr; ++r) dst[r] = src[nr - 1 - r];
+    for (uint32_t r = l; r < nr; r += 16) dst[r] = src[nr - 1 - r];
 }
 
 } // namespace
@@ -1279,7 +1281,7 @@ hipError_t sf_launch_walk(const WalkArgs &a, hipStream_t s) {
 hipError_t sf_launch_compact(const uint32_t *order, uint32_t count, const uint32_t *runs, const uint64_t *runs_off,
                              const uint32_t *nruns, const uint64_t *out_off, uint32_t *out, hipStream_t s) {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(compact_kernel, dim3((count + 255) / 256), dim3(256), 0, s, order, count, runs, runs_off, nruns,
-                       out_off, out);
+    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)(((uint64_t)count * 16 + 255) / 256)), dim3(256), 0, s, order,
+                       count, runs, runs_off, nruns, out_off, out);
     return hipGetLastError();
 }

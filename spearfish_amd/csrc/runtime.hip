@@ -115,30 +115,85 @@ uint64_t tb_budget_words() {
     return std::max<uint64_t>(mb, 16) * (1ull << 20) / 4;
 }
 
-__global__ void scan_u32_kernel(const uint32_t *in, uint64_t n, uint64_t *out) {
-    // Exclusive prefix sum of n u32 into n + 1 u64, one block of 1024 threads.
-    __shared__ uint64_t part[1024];
-    const uint64_t per = (n + 1023) / 1024;
-    const uint64_t a = threadIdx.x * per, b = a + per < n ? a + per : n;
+// Exclusive prefix sum of n u32 into n + 1 u64 (out[n] = the total), in tiles of 4096 values (1024
+// threads x 4): tile sums, their scan in one block, then each tile's own scan plus its offset.
+constexpr int kScanTile = 4096;
+
+// Exclusive scan of one u64 per thread over a block of 1024 threads; `total` gets the block's sum.
+__device__ uint64_t block_excl_scan(uint64_t v, uint64_t &total) {
+    __shared__ uint64_t wsum[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    uint64_t pre = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint64_t t = wsum[k];
+        pre += k < wv ? t : 0;
+        all += t;
+    }
+    __syncthreads(); // wsum is reused by the next call
+    total = all;
+    return pre + x - v;
+}
+
+__device__ uint64_t tile_values(const uint32_t *in, uint64_t n, uint64_t base, uint32_t (&v)[4]) {
     uint64_t s = 0;
-    for (uint64_t i = a; i < b; ++i) s += in[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t acc = 0;
-        for (int q = 0; q < 1024; ++q) {
-            uint64_t v = part[q];
-            part[q] = acc;
-            acc += v;
-        }
-        out[n] = acc;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t i = base + 4 * (uint64_t)threadIdx.x + k;
+        v[k] = i < n ? in[i] : 0u;
+        s += v[k];
     }
-    __syncthreads();
-    uint64_t acc = part[threadIdx.x];
-    for (uint64_t i = a; i < b; ++i) {
-        out[i] = acc;
-        acc += in[i];
+    return s;
+}
+
+__global__ __launch_bounds__(1024) void scan_sums_kernel(const uint32_t *in, uint64_t n, uint64_t *sums) {
+    uint32_t v[4];
+    uint64_t total;
+    block_excl_scan(tile_values(in, n, (uint64_t)blockIdx.x * kScanTile, v), total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void scan_offsets_kernel(uint64_t *sums, uint64_t tiles, uint64_t *out_total) {
+    uint64_t carry = 0;
+    for (uint64_t c = 0; c < tiles; c += 1024) {
+        const uint64_t i = c + threadIdx.x;
+        const uint64_t v = i < tiles ? sums[i] : 0;
+        uint64_t total;
+        const uint64_t e = block_excl_scan(v, total);
+        if (i < tiles) sums[i] = carry + e;
+        carry += total;
     }
+    if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ __launch_bounds__(1024) void scan_apply_kernel(const uint32_t *in, uint64_t n, const uint64_t *offs,
+                                                          uint64_t *out) {
+    uint32_t v[4];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint64_t total;
+    uint64_t acc = offs[blockIdx.x] + block_excl_scan(tile_values(in, n, base, v), total);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t i = base + 4 * (uint64_t)threadIdx.x + k;
+        if (i < n) out[i] = acc;
+        acc += v[k];
+    }
+}
+
+hipError_t launch_scan(const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *tmp, hipStream_t s) {
+    const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles) hipLaunchKernelGGL(scan_sums_kernel, dim3((unsigned)tiles), dim3(1024), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(scan_offsets_kernel, dim3(1), dim3(1024), 0, s, tmp, tiles, out + n);
+    if (tiles) hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)tiles), dim3(1024), 0, s, in, n, tmp, out);
+    return hipGetLastError();
 }
 
 } // namespace
@@ -193,6 +248,7 @@ struct sf_batch {
     int32_t *d_score = nullptr;
     uint32_t *d_col = nullptr, *d_tstart = nullptr, *d_nruns = nullptr, *d_runs = nullptr, *d_cig = nullptr;
     uint64_t *d_runs_off = nullptr, *d_out_off = nullptr, *d_tboff = nullptr;
+    uint64_t *d_scan = nullptr; // tile sums of the CIGAR offset scan
     uint32_t *d_tb[2] = {nullptr, nullptr}; // traceback scratch, alternating between chunks
     uint64_t tb_words = 0;
     hipStream_t wstream = nullptr;           // traceback walks, overlapping the next chunk's DP
@@ -243,6 +299,7 @@ struct sf_batch {
         for (int q = 2; q < 4; ++q) dfree(aplan[q].d_ba), dfree(aplan[q].d_order), dfree(aplan[q].d_desc);
         dfree(d_dp), dfree(d_best), dfree(d_pinv), dfree(d_col);
         dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
+        dfree(d_scan);
         dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0);
         for (hipStream_t x : aux)
             if (x) (void)hipStreamSynchronize(x);
@@ -750,7 +807,8 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     }
     if (!b->d_tboff) {
         if ((st = dalloc(&b->d_tboff, n)) || (st = dalloc(&b->d_tstart, n)) || (st = dalloc(&b->d_nruns, n)) ||
-            (st = dalloc(&b->d_runs_off, n)) || (st = dalloc(&b->d_out_off, n + 1)))
+            (st = dalloc(&b->d_runs_off, n)) || (st = dalloc(&b->d_out_off, n + 1)) ||
+            (st = dalloc(&b->d_scan, (n + kScanTile - 1) / kScanTile + 1)))
             return st;
         std::vector<uint64_t> roff(n);
         uint64_t tot = 0;
@@ -804,8 +862,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     HIP_TRY(hipEventRecord(b->walks_end, b->wstream));
     HIP_TRY(hipStreamWaitEvent(b->stream, b->walks_end, 0));
     if ((st = timed_begin(b, 3, &e0))) return st;
-    hipLaunchKernelGGL(scan_u32_kernel, dim3(1), dim3(1024), 0, b->stream, b->d_nruns, n, b->d_out_off);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_scan(b->d_nruns, n, b->d_out_off, b->d_scan, b->stream));
     HIP_TRY(sf_launch_compact(pl.d_order, (uint32_t)n, b->d_runs, b->d_runs_off, b->d_nruns, b->d_out_off, b->d_cig,
                               b->stream));
     if ((st = timed_end(b, 3, e0))) return st;
