@@ -320,6 +320,14 @@ __device__ __forceinline__ u16x2 as2(uint32_t v) { return __builtin_bit_cast(u16
 __device__ __forceinline__ uint32_t as1(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ u16x2 pmax(u16x2 a, u16x2 b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ u16x2 psub(u16x2 a, u16x2 b) { return __builtin_elementwise_sub_sat(a, b); }
+// Max of three values below 0x7C00 in one op: read as half floats they are positive and finite,
+// and positive halves order as their bit patterns (denormals are kept in this mode), so the packed
+// half-float maximum3 is the integer maximum.
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 pmax3(u16x2 a, u16x2 b, u16x2 c) {
+    const h16x2 x = __builtin_bit_cast(h16x2, a), y = __builtin_bit_cast(h16x2, b), z = __builtin_bit_cast(h16x2, c);
+    return __builtin_bit_cast(u16x2, __builtin_elementwise_maximum(x, __builtin_elementwise_maximum(y, z)));
+}
 
 // The packed kernels' result: H(m, n) (Global) or the last row's maximum and its first column,
 // unbiased, or NEG for a pair whose band ended before the last row.
@@ -390,7 +398,8 @@ __device__ __forceinline__ void pk16_result(const AffineArgs &a, const u16x2 (&V
     }
 }
 
-template <int K2>
+// M3: every value is below 0x7C00 (checked on the host), so H takes its three terms in one max.
+template <int K2, bool M3>
 __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
     constexpr int S = 2 * K2;                     // slots per lane
     constexpr int BLK = 16;                       // rows per block: one pattern word
@@ -492,7 +501,7 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
             const u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16)); // gap values entering register 0
             V[0] = pmax(X[0], E);
 #pragma unroll
-            for (int q = 1; q < K2; ++q) V[q] = pmax(X[q], pmax(Pre[q], E));
+            for (int q = 1; q < K2; ++q) V[q] = M3 ? pmax3(X[q], Pre[q], E) : pmax(X[q], pmax(Pre[q], E));
         };
         if (i0 + BLK - 1 <= rows_all) {
             // Every pair of the wave has all 16 rows of this block: no per-row guard, so the
@@ -1182,6 +1191,7 @@ int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t 
     // The traceback kernel's offset is `open` higher (no band term saturates).
     const int64_t off = (int64_t)open * (tb ? 2 : 1) + (dm + ext) * below + 2;
     const int64_t top = off + dm * m + (int64_t)ext * 16 * kp;
+    if (!tb && top < 0x7C00) return 16200 + kp; // the score kernel's three-term max
     return top <= 65535 ? 16100 + kp : 0;
 }
 
@@ -1195,11 +1205,14 @@ hipError_t sf_launch_affine(const AffineArgs &a, int cls, bool tb, hipStream_t s
     const int LP = aff_lp(cls);
     dim3 grid((unsigned)(((uint64_t)a.count * LP + 255) / 256)), block(256);
     if (aff_packed(cls)) {
+        const bool m3 = cls % 1000 >= 200;
+        if (tb && m3) return hipErrorInvalidValue;
         switch (aff_k(cls)) {
 #define PK(KK)                                                                                   \
     case 2 * KK:                                                                                 \
         if (tb) hipLaunchKernelGGL((affine_pk16_tb_kernel<KK>), grid, block, 0, s, a);          \
-        else hipLaunchKernelGGL((affine_pk16_kernel<KK>), grid, block, 0, s, a);                \
+        else if (m3) hipLaunchKernelGGL((affine_pk16_kernel<KK, true>), grid, block, 0, s, a);  \
+        else hipLaunchKernelGGL((affine_pk16_kernel<KK, false>), grid, block, 0, s, a);         \
         break;
             PK(1) PK(2) PK(3) PK(4) PK(5) PK(6) PK(7) PK(8) PK(9) PK(10) PK(11) PK(12) PK(13) PK(14) PK(15)
             PK(16)

@@ -102,15 +102,16 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
-// Words of one traceback scratch buffer (two are used, alternately): SF_TB_BUDGET_MB, else an
-// eighth of the device's memory, at most 32 GiB.
+// Words of one traceback scratch buffer (two are used, alternately): SF_TB_BUDGET_MB, else a
+// quarter of the device's memory, at most 64 GiB.  Larger chunks run more walks at once, and the
+// walk (one dependent chain per pair) needs that concurrency.
 uint64_t tb_budget_words() {
     uint64_t mb = 0;
     if (const char *e = std::getenv("SF_TB_BUDGET_MB")) mb = std::strtoull(e, nullptr, 10);
     if (mb == 0) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) tot = 16ull << 30;
-        mb = std::min<uint64_t>(32768, tot / 8 / (1ull << 20));
+        mb = std::min<uint64_t>(65536, tot / 4 / (1ull << 20));
     }
     return std::max<uint64_t>(mb, 16) * (1ull << 20) / 4;
 }

@@ -172,9 +172,9 @@ def test_affine_64_lane_kernel(mode, lanes64):
 
 @pytest.mark.parametrize("mode", MODES)
 def test_affine_scores_16bit(mode):
-    # Score-only runs with mis = 2 ext (ONT, HIFI) take the 16-bit kernel where its range holds:
-    # random pairs, all-match and gap-heavy extremes, and lengths around its limit at band 16
-    # (m near 10,700 for ONT), so that both kernels run in one batch.
+    # Score-only runs with mis = 2 ext (ONT, HIFI) take the 16-bit kernels where their ranges hold:
+    # random pairs, all-match and gap-heavy extremes, and lengths around their limits at band 16,
+    # so that the 32-bit and both 16-bit kernels run in one batch.
     rng = random.Random(53 + int(mode))
     P, T, B = [], [], []
     nmax = 450 if mode == A.Mode.Infix else 1500  # Infix is unbanded: m + n + 1 slots a row
@@ -193,7 +193,9 @@ def test_affine_scores_16bit(mode):
         T += [t, t, t]
         B += [128, 128, 64]
     if mode != A.Mode.Infix:
-        for m in (10600, 10700, 10800, 11000):
+        # Around the three-term-max kernel's limit (values below 0x7C00, m near 5,200 for ONT at
+        # band 16) and the 16-bit kernel's (m near 10,700).
+        for m in (5150, 5250, 5350, 10600, 10700, 10800, 11000):
             t = [rng.randrange(4) for _ in range(m)]
             P.append([x if rng.random() > 0.05 else rng.randrange(4) for x in t])
             T.append(t)
