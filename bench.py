@@ -50,7 +50,7 @@ AFFINE_TB_OPS_PER_CELL = 11 + 4
 # HBM bytes per launch of the dominant kernels on the uniform workload, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the same counters on a 1 GiB copy
 # (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.
-TRAFFIC_BYTES = {"affine_pk16_kernel<9>": 735_071_436, "affine_pk16_tb_kernel<9>": 32_791_216_537}
+TRAFFIC_BYTES = {"affine_pk16_kernel<9,true>": 735_031_808, "affine_pk16_tb_kernel<9>": 65_583_623_475}
 
 
 def parse():
@@ -187,7 +187,7 @@ def bench_uniform(args):
         "affine_gcups": round(tm.affine_cells / (tm.affine_ms * 1e-3) / 1e9, 1) if tm.affine_ms else None,
         "kernel_ms_per_step": {"myers": round(tm.myers_ms / args.steps, 2),
                                "affine": round(tm.affine_ms / args.steps, 2)},
-        "roofline": roofline(tm, AFFINE_OPS_PER_CELL, "affine_pk16_kernel<9>", VALU16_PEAK_TOPS,
+        "roofline": roofline(tm, AFFINE_OPS_PER_CELL, "affine_pk16_kernel<9,true>", VALU16_PEAK_TOPS,
                              "TOP/s (packed i16 VALU lane-ops; 16 lanes x 18 slots a pair, 4 pairs a wave)", 1),
     }
     wall_c, tm_c = run(True, args.steps, 1, "with cigar")
