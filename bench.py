@@ -40,17 +40,23 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 METRIC = "GCUPS (bit-exact) batched vs per-pair on 1 GPU; serve reads/s + p50/p99 latency"
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 i32 lanes/clk x 2.4 GHz
-VALU16_PEAK_TOPS = 2 * VALU_PEAK_TOPS      # packed 16-bit: two lane-ops per lane and clock
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4 cycles at 2.4 GHz,
+# in lane-instructions (x 64): 39.3 T/s.  Packed 16-bit instructions do two lane-ops each.
+VALU_ISSUE_PEAK_T = 256 * 4 * 64 / 4 * 2.4e9 / 1e12
+VALU16_PEAK_TOPS = 2 * VALU_ISSUE_PEAK_T
 HBM_PEAK_GBS = 8000.0
-# Algorithmic ops per affine cell of the reference's recurrence (F: 2 sub + 2 max; substitution:
-# compare + select; x: add + max; E/H: 2 max + 1 sub); the traceback adds 4 compares.
+# The reference's i32 recurrence, per cell (F: 2 sub + 2 max; substitution: compare + select;
+# x: add + max; E/H: 2 max + 1 sub); the traceback adds 4 compares.
 AFFINE_OPS_PER_CELL = 11
 AFFINE_TB_OPS_PER_CELL = 11 + 4
+# Issued VALU wave-instructions per DP cell on the uniform workload, from rocprofv3 --pmc
+# SQ_INSTS_VALU runs of this script (profiles/valu_issue_uniform_1e6x1000_band128.md): the
+# score step's affine phase (its 8-lane packed bins and the LP64 bin) and the traceback DP.
+VALU_INSTS_PER_CELL = {"score": 1.9089e10 / 2.46731e11, "traceback": 5.5985e10 / 2.46731e11}
 # HBM bytes per launch of the dominant kernels on the uniform workload, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the same counters on a 1 GiB copy
 # (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.
-TRAFFIC_BYTES = {"affine_pk16_kernel<9,true>": 735_031_808, "affine_pk16_tb_kernel<9>": 65_583_623_475}
+TRAFFIC_BYTES = {"score": 728_408_679, "traceback": 65_583_623_475}
 
 
 def parse():
@@ -147,19 +153,29 @@ def bench_uniform(args):
         log(f"{label}: timed {steps} steps in {wall:.3f}s")
         return wall, tm
 
-    def roofline(tm, ops, kernel, peak, unit, launches_per_step):
-        # The dominant kernel's launches per step; the small bins of other shape classes run beside
-        # them on the other streams, so the phase time per launch is the dominant launch's time.
+    def roofline(tm, phase, kernel, ops, launches_per_step):
+        # The unit is one launch of the phase's dominant shape (the score phase's bins run as one
+        # launch each on the batch's streams, so the phase is the unit there).  `achieved` is the
+        # issued VALU lane-instructions a second: measured instructions per cell x cells / time;
+        # `reference_ops` restates it at the reference's 11 i32 ops a cell (the packed kernels
+        # issue fewer, so that figure can pass the packed-i16 peak).
         launches = max(launches_per_step * args.steps, 1)
         per_launch_ms = tm.affine_ms / launches
         cells_per_launch = tm.affine_cells / launches
-        achieved = cells_per_launch * ops / (per_launch_ms * 1e-3) / 1e12
-        traffic = TRAFFIC_BYTES.get(kernel)
-        return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3), "peak": round(peak, 1),
-                "unit": unit, "frac": round(achieved / peak, 4), "traffic": traffic,
-                "ops_per_cell": ops, "cells_per_launch": int(cells_per_launch),
+        secs = per_launch_ms * 1e-3
+        ipc = VALU_INSTS_PER_CELL[phase]
+        achieved = cells_per_launch * ipc * 64 / secs / 1e12
+        traffic = TRAFFIC_BYTES.get(phase)
+        return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3),
+                "peak": round(VALU_ISSUE_PEAK_T, 1),
+                "unit": "T VALU lane-instructions/s issued (SQ_INSTS_VALU per cell x cells / launch time)",
+                "frac": round(achieved / VALU_ISSUE_PEAK_T, 4), "traffic": traffic,
+                "valu_insts_per_cell": round(ipc, 5), "cells_per_launch": int(cells_per_launch),
                 "avg_launch_ms": round(per_launch_ms, 3),
-                "hbm_gbs": round(traffic / (per_launch_ms * 1e-3) / 1e9, 1) if traffic else None}
+                "hbm_gbs": round(traffic / secs / 1e9, 1) if traffic else None,
+                "reference_ops": {"ops_per_cell": ops,
+                                  "tops": round(cells_per_launch * ops / secs / 1e12, 3),
+                                  "packed_i16_peak_tops": round(VALU16_PEAK_TOPS, 1)}}
 
     wall, tm = run(False, args.steps, args.warmup, "scores")
     cells = tm.myers_cells + tm.affine_cells
@@ -187,8 +203,8 @@ def bench_uniform(args):
         "affine_gcups": round(tm.affine_cells / (tm.affine_ms * 1e-3) / 1e9, 1) if tm.affine_ms else None,
         "kernel_ms_per_step": {"myers": round(tm.myers_ms / args.steps, 2),
                                "affine": round(tm.affine_ms / args.steps, 2)},
-        "roofline": roofline(tm, AFFINE_OPS_PER_CELL, "affine_pk16_kernel<9,true>", VALU16_PEAK_TOPS,
-                             "TOP/s (packed i16 VALU lane-ops; 16 lanes x 18 slots a pair, 4 pairs a wave)", 1),
+        "roofline": roofline(tm, "score", "affine_pk16_kernel<8,17|18,true> (8 lanes x 34 or 36 slots a "
+                             "pair, 8 pairs a wave; the score step's affine phase)", AFFINE_OPS_PER_CELL, 1),
     }
     wall_c, tm_c = run(True, args.steps, 1, "with cigar")
     out["with_cigar"] = {
@@ -198,9 +214,8 @@ def bench_uniform(args):
         "kernel_ms_per_step": {"myers": round(tm_c.myers_ms / args.steps, 2),
                                "affine_tb": round(tm_c.affine_ms / args.steps, 2),
                                "walk_compact": round(tm_c.walk_ms / args.steps, 2)},
-        "roofline": roofline(tm_c, AFFINE_TB_OPS_PER_CELL, "affine_pk16_tb_kernel<9>", VALU16_PEAK_TOPS,
-                             "TOP/s (packed i16 VALU lane-ops; DP + traceback bits; average over the step's chunk "
-                             "launches)",
+        "roofline": roofline(tm_c, "traceback", "affine_pk16_tb_kernel<9> (one traceback chunk)",
+                             AFFINE_TB_OPS_PER_CELL,
                              # every chunk is one launch of the dominant class plus, for this
                              # workload, one small LP64 launch a step for the widest pairs
                              max(tm_c.affine_launches // max(args.steps, 1) - 1, 1)),

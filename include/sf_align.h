@@ -261,6 +261,12 @@ int sf_server_replay(sf_server *s, const sf_read_request *req, const double *t_a
  * environment).  Applies to affine runs started afterwards in this process. */
 int sf_set_affine_lp16_max(uint64_t slots);
 
+/* Tuning (no reference counterpart; results never depend on it): score-only affine runs with a
+ * 16-lane packed shape may run 8 lanes per pair, eight pairs per wave.  policy 0: where the
+ * launch planner's estimate favours it (default); 1: wherever the 8-lane kernel's range holds;
+ * -1: never.  Applies to affine runs started afterwards in this process. */
+int sf_set_affine_lp8(int policy);
+
 /* Diagnostic (no reference counterpart): runs a one-wave kernel that records the lane moves of
  * the DPP and shuffle encodings the kernels rely on, 5 x 64 values into `out` (320 int32). */
 int sf_selftest_lanes(int32_t *out);

@@ -162,6 +162,7 @@ SIGNATURES = {
     "sf_affine_extend_cut_bonus": ([_vp, _vp, _vp, C.c_int32, C.c_int32, _int, _vp, _vp, _int, _int, _vp], _int),
     "sf_local_band_ends": ([_vp, _vp, _vp, _vp, C.c_int32, C.c_int32, _vp], _int),
     "sf_set_affine_lp16_max": ([C.c_uint64], _int),
+    "sf_set_affine_lp8": ([C.c_int], _int),
     "sf_selftest_lanes": ([_vp], _int),
 }
 

@@ -88,8 +88,19 @@ template <int LP>
 __device__ __forceinline__ uint32_t group_excl_umax(uint32_t x) {
     if constexpr (LP == 64) {
         return wave_excl_umax(x);
+    } else if constexpr (LP == 8) {
+        // Two groups share a DPP row.  The input moves up a lane inside its group (the group's
+        // first lane takes 0), is scanned inside quads (quad permutes [0,0,1,2] and [0,1,0,1]: a
+        // lane reading itself is harmless under max), and the upper quad of each group takes the
+        // lower quad's total by a row shift of 4 written to banks 1 and 3 only.
+        uint32_t y = dpp0<kRowShr1>(x);
+        y = (threadIdx.x & 7) ? y : 0u;
+        y = umax(y, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)y, 0x90, 0xf, 0xf, false));
+        y = umax(y, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)y, 0x44, 0xf, 0xf, false));
+        const uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)y, 0xff, 0xf, 0xf, false);
+        return umax(y, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, kRowShr4, 0xf, 0xa, false));
     } else {
-        static_assert(LP == 16, "groups of 16 or 64 lanes");
+        static_assert(LP == 16, "groups of 8, 16 or 64 lanes");
         x = umax(x, dpp0<kRowShr1>(x));
         x = umax(x, dpp0<kRowShr2>(x));
         x = umax(x, dpp0<kRowShr4>(x));
@@ -331,7 +342,7 @@ __device__ __forceinline__ u16x2 pmax3(u16x2 a, u16x2 b, u16x2 c) {
 
 // The packed kernels' result: H(m, n) (Global) or the last row's maximum and its first column,
 // unbiased, or NEG for a pair whose band ended before the last row.
-template <int K2>
+template <int LP, int K2>
 __device__ __forceinline__ void pk16_result(const AffineArgs &a, const u16x2 (&V)[K2], bool valid, uint32_t pi,
                                             int lane, int l, int32_t m, int32_t n, int32_t below, int32_t Wd,
                                             int32_t last, int32_t off) {
@@ -378,7 +389,7 @@ __device__ __forceinline__ void pk16_result(const AffineArgs &a, const u16x2 (&V
             }
         }
 #pragma unroll
-        for (int o = 8; o > 0; o >>= 1) {
+        for (int o = LP / 2; o > 0; o >>= 1) {
             const int32_t ov = __shfl_xor(bv, o), os = __shfl_xor(bs, o);
             if (ov > bv || (ov == bv && os < bs)) {
                 bv = ov;
@@ -398,16 +409,19 @@ __device__ __forceinline__ void pk16_result(const AffineArgs &a, const u16x2 (&V
     }
 }
 
-// M3: every value is below 0x7C00 (checked on the host), so H takes its three terms in one max.
-template <int K2, bool M3>
+// LP lanes per pair: 16 (four pairs a wave) or 8 (eight pairs a wave: the per-row work that every
+// lane repeats -- scan, neighbour move, pattern code -- is shared by twice the slots).  M3: every
+// value is below 0x7C00 (checked on the host), so H takes its three terms in one max.
+template <int LP, int K2, bool M3>
 __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
+    static_assert(LP == 8 || LP == 16, "8 or 16 lanes a pair");
     constexpr int S = 2 * K2;                     // slots per lane
     constexpr int BLK = 16;                       // rows per block: one pattern word
     constexpr int NF = K2 + BLK - 1;              // text fields of a block: q + r, q < K2, r < 16
     constexpr int NWP = (NF + 7) / 8;             // paired text words holding them
     const int lane = threadIdx.x & 63;
-    const int l = lane & 15;
-    const uint32_t w = (blockIdx.x * 256u + threadIdx.x) >> 4;
+    const int l = lane & (LP - 1);
+    const uint32_t w = (blockIdx.x * 256u + threadIdx.x) / LP;
     const bool valid = w < a.count;
     const uint32_t pi = valid ? a.order[w] : 0u;
     const PairDesc dsc = valid ? a.desc[w] : PairDesc{0, 0, 0, 0, 0, 0};
@@ -450,11 +464,13 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
         U[r] = psub(V[r], dl[r]); // row 0: the boundary's H opens the gaps, as the reference's
     }
     const int32_t last = valid ? (m < n + below ? m : n + below) : 0;
-    int32_t rows = imax(last, __shfl_xor(last, 16));
-    rows = imax(rows, __shfl_xor(rows, 32));
+    int32_t rows = last, rows_all = last; // the wave's longest pair's rows, and rows every pair has
+#pragma unroll
+    for (int o = LP; o < 64; o <<= 1) {
+        rows = imax(rows, __shfl_xor(rows, o));
+        rows_all = imin(rows_all, __shfl_xor(rows_all, o));
+    }
     rows = __builtin_amdgcn_readfirstlane(rows);
-    int32_t rows_all = imin(last, __shfl_xor(last, 16)); // rows every pair of the wave has
-    rows_all = imin(rows_all, __shfl_xor(rows_all, 32));
     rows_all = __builtin_amdgcn_readfirstlane(rows_all);
     for (int32_t i0 = 1; i0 <= rows; i0 += BLK) {
         const uint32_t pw = codes16(P, i0 - 1);
@@ -475,8 +491,12 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
             const uint32_t onehot = 1u << (__builtin_amdgcn_ubfe(pw, 2 * r, 2) << 3);
             // Register K2 - 1: low half reads slot s0 + K2 (register 0, high half), high half
             // reads slot s0 + 2 K2 (the next lane's register 0, low half).
+            // (With 8 lanes a pair the row shift hands a group's top lane the next group's
+            // value: that lane takes the floor.)
             const uint32_t u0 = as1(U[0]);
-            const u16x2 fK = as2(__builtin_amdgcn_alignbit(dpp0<kRowShl1>(u0), u0, 16));
+            uint32_t un = dpp0<kRowShl1>(u0);
+            if (LP == 8) un = l == LP - 1 ? 0u : un;
+            const u16x2 fK = as2(__builtin_amdgcn_alignbit(un, u0, 16));
             u16x2 X[K2], G[K2];
 #pragma unroll
             for (int q = 0; q < K2; ++q) {
@@ -497,7 +517,7 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
             const u16x2 tot = K2 > 1 ? pmax(Pre[K2 - 1], G[K2 - 1]) : G[0];
             const uint32_t tt = as1(tot);
             const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
-            const uint32_t C = group_excl_umax<16>(tlo > thi ? tlo : thi);
+            const uint32_t C = group_excl_umax<LP>(tlo > thi ? tlo : thi);
             const u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16)); // gap values entering register 0
             V[0] = pmax(X[0], E);
 #pragma unroll
@@ -514,7 +534,7 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
                 if (i0 + r <= last) row(r); // this pair's rows are not done
         }
     }
-    pk16_result<K2>(a, V, valid, pi, lane, l, m, n, below, Wd, last, off);
+    pk16_result<LP, K2>(a, V, valid, pi, lane, l, m, n, below, Wd, last, off);
 }
 
 // Traceback variant of the packed kernel.  Same layout and bias; the offset is raised by `open`
@@ -700,7 +720,7 @@ __global__ __launch_bounds__(256) void affine_pk16_tb_kernel(AffineArgs a) {
             }
         }
     }
-    pk16_result<K2>(a, V, valid, pi, lane, l, m, n, below, Wd, last, off);
+    pk16_result<16, K2>(a, V, valid, pi, lane, l, m, n, below, Wd, last, off);
 }
 
 // ---- passes with per-row bookkeeping ----
@@ -1195,6 +1215,21 @@ int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t 
     return top <= 65535 ? 16100 + kp : 0;
 }
 
+int sf_affine_pk8_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m,
+                        uint32_t below) {
+    if (mis != 2 * ext || width > 8 * 36) return 0;
+    const int k2 = (int)((width + 15) / 16); // registers a lane: 8 lanes x 2 slots each
+    const int64_t dm = (int64_t)mat + mis;
+    const int64_t off = (int64_t)open + (dm + ext) * below + 2;
+    const int64_t top = off + dm * m + (int64_t)ext * 16 * k2;
+    return top < 0x7C00 ? 8200 + 2 * k2 : 0;
+}
+
+static std::atomic<int> g_lp8_policy{0};
+
+int sf_affine_lp8_policy() { return g_lp8_policy.load(std::memory_order_relaxed); }
+void sf_affine_set_lp8_policy(int policy) { g_lp8_policy.store(policy, std::memory_order_relaxed); }
+
 extern "C" int sf_set_affine_lp16_max(uint64_t slots) {
     g_lp16_max.store(slots, std::memory_order_relaxed);
     return 0;
@@ -1204,6 +1239,19 @@ hipError_t sf_launch_affine(const AffineArgs &a, int cls, bool tb, hipStream_t s
     if (a.count == 0) return hipSuccess;
     const int LP = aff_lp(cls);
     dim3 grid((unsigned)(((uint64_t)a.count * LP + 255) / 256)), block(256);
+    if (aff_packed(cls) && LP == 8) {
+        if (tb || cls % 1000 < 200) return hipErrorInvalidValue; // 8 lanes: scores, three-term max
+        switch (aff_k(cls)) {
+#define PK8(KK)                                                                                  \
+    case 2 * KK: hipLaunchKernelGGL((affine_pk16_kernel<8, KK, true>), grid, block, 0, s, a); break;
+            PK8(1) PK8(2) PK8(3) PK8(4) PK8(5) PK8(6) PK8(7) PK8(8) PK8(9) PK8(10) PK8(11) PK8(12) PK8(13)
+            PK8(14) PK8(15) PK8(16) PK8(17) PK8(18)
+#undef PK8
+        default:
+            return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (aff_packed(cls)) {
         const bool m3 = cls % 1000 >= 200;
         if (tb && m3) return hipErrorInvalidValue;
@@ -1211,8 +1259,8 @@ hipError_t sf_launch_affine(const AffineArgs &a, int cls, bool tb, hipStream_t s
 #define PK(KK)                                                                                   \
     case 2 * KK:                                                                                 \
         if (tb) hipLaunchKernelGGL((affine_pk16_tb_kernel<KK>), grid, block, 0, s, a);          \
-        else if (m3) hipLaunchKernelGGL((affine_pk16_kernel<KK, true>), grid, block, 0, s, a);  \
-        else hipLaunchKernelGGL((affine_pk16_kernel<KK, false>), grid, block, 0, s, a);         \
+        else if (m3) hipLaunchKernelGGL((affine_pk16_kernel<16, KK, true>), grid, block, 0, s, a); \
+        else hipLaunchKernelGGL((affine_pk16_kernel<16, KK, false>), grid, block, 0, s, a);     \
         break;
             PK(1) PK(2) PK(3) PK(4) PK(5) PK(6) PK(7) PK(8) PK(9) PK(10) PK(11) PK(12) PK(13) PK(14) PK(15)
             PK(16)

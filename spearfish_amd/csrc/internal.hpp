@@ -241,6 +241,12 @@ int sf_affine_pass_class(uint64_t width, bool lp16); // the same for the passes 
 // exact range does not hold (the scoring must have mis = 2 ext); tb: for the traceback kernel.
 int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m, uint32_t below,
                          bool tb);
+// When score-only runs use the 8-lane packed kernel: 0 by estimate, 1 always, -1 never.
+int sf_affine_lp8_policy();
+void sf_affine_set_lp8_policy(int policy);
+// The 8-lane packed class (scores, three-term max) for a band of `width` slots, or 0.
+int sf_affine_pk8_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m,
+                        uint32_t below);
 hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
 // cls: a 32-bit affine class (16 or 64 lanes); local selects local_band_core.
 hipError_t sf_launch_pass(const sf::PassArgs &a, int cls, bool local, hipStream_t s);

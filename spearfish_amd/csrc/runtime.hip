@@ -683,7 +683,8 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     const uint64_t n = b->n;
     // Shape classes, cached across runs with the same mode, bands, scoring and tuning.
     sf_batch::AffPlan &pl = b->aplan[cigar ? 1 : 0];
-    uint64_t key = (1469598103934665603ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8)) * 1099511628211ull;
+    uint64_t key = (1469598103934665603ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8) ^
+                    ((uint64_t)(sf_affine_lp8_policy() + 1) << 4)) * 1099511628211ull;
     for (int32_t v : {sc->mat, sc->mis, sc->open, sc->ext}) key = (key ^ (uint64_t)(uint32_t)v) * 1099511628211ull;
     if (band)
         for (uint64_t i = 0; i < n; ++i) key = (key ^ band[i]) * 1099511628211ull;
@@ -737,6 +738,36 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
             // Values stay far above NEG over all slots of the class.
             if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, b->plen[i], b->tlen[i], aff_slots(kcls[i])))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
+        }
+        // Scores only: a 16-lane packed class with the three-term max moves to 8 lanes a pair
+        // (twice the registers a lane, half the waves) where that shortens its estimate, as
+        // above: VALU per wave-row about 7 per register and 24 besides.
+        const int lp8 = sf_affine_lp8_policy();
+        if (!cigar && lp8 >= 0) {
+            struct Cost8 {
+                double chain16 = 0, spread16 = 0, chain8 = 0, spread8 = 0;
+            };
+            std::map<int, Cost8> c8;
+            std::vector<int> k8(n, 0);
+            for (uint64_t i = 0; i < n; ++i) {
+                if (!aff_packed(kcls[i]) || aff_lp(kcls[i]) != 16 || kcls[i] % 1000 < 200) continue;
+                const uint64_t width = (ba[i] & 0xFFFFFFFFull) + (ba[i] >> 32) + 1;
+                k8[i] = sf_affine_pk8_class(width, sc->mat, sc->mis, sc->open, sc->ext, b->plen[i],
+                                            (uint32_t)ba[i]);
+                if (!k8[i]) continue;
+                Cost8 &c = c8[kcls[i]];
+                const double m = b->plen[i];
+                const double r16 = 3.5 * aff_k(kcls[i]) + 24.0, r8 = 3.5 * aff_k(k8[i]) + 24.0;
+                c.chain16 = std::max(c.chain16, m * r16);
+                c.spread16 += m * r16 / 4.0 / 1024.0;
+                c.chain8 = std::max(c.chain8, m * r8);
+                c.spread8 += m * r8 / 8.0 / 1024.0;
+            }
+            for (uint64_t i = 0; i < n; ++i) {
+                if (!k8[i]) continue;
+                const Cost8 &c = c8[kcls[i]];
+                if (lp8 > 0 || std::max(c.chain8, c.spread8) < std::max(c.chain16, c.spread16)) kcls[i] = k8[i];
+            }
         }
         if ((st = build_plan(b, pl, ba, kcls, key, cells))) return st;
     }
@@ -895,6 +926,12 @@ int sf_batch_use_located(sf_batch *b, int on) {
     for (auto &pl : b->aplan) pl.valid = false; // the affine shape classes depend on the text lengths
     b->myers_cells_per_run = 0;
     for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];
+    return SF_OK;
+}
+
+int sf_set_affine_lp8(int policy) {
+    if (policy < -1 || policy > 1) return fail(SF_EINVAL, "policy must be -1, 0 or 1");
+    sf_affine_set_lp8_policy(policy);
     return SF_OK;
 }
 

@@ -205,6 +205,44 @@ def test_affine_scores_16bit(mode):
         check_affine(ps, mode, sc, np.array(B, np.uint64), cigar=False)
 
 
+@pytest.fixture
+def lanes8():
+    """Score-only packed runs on 8 lanes per pair wherever that kernel's range holds, then the
+    planner's choice back."""
+    lib = A.N.load()
+    A.N.check(lib.sf_set_affine_lp8(1))
+    yield
+    A.N.check(lib.sf_set_affine_lp8(0))
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_affine_scores_8_lane_kernel(mode, lanes8):
+    # Every band width up to the 8-lane kernel's 288 slots (1 to 18 registers a lane), random and
+    # gap-heavy pairs, lengths around its value limit (m near 5,200 for ONT at band 16).
+    rng = random.Random(67 + int(mode))
+    P, T, B = [], [], []
+    nmax = 140 if mode == A.Mode.Infix else 900
+    for _ in range(500):
+        n = rng.randrange(0, nmax)
+        t = [rng.randrange(4) for _ in range(n)]
+        p = [x if rng.random() > 0.1 else rng.randrange(4) for x in t]
+        for _ in range(rng.randrange(0, 4)):
+            k = rng.randrange(len(p) + 1)
+            p = p[:k] + [rng.randrange(4) for _ in range(rng.randrange(0, 9))] + p[k + rng.randrange(0, 9):]
+        P.append(p)
+        T.append(t)
+        B.append(rng.choice([0, 1, 3, 7, 8, 15, 16, 40, 64, 100, 127, 128, 143]))
+    if mode != A.Mode.Infix:
+        for m in (5150, 5250, 5350):
+            t = [rng.randrange(4) for _ in range(m)]
+            P.append([x if rng.random() > 0.05 else rng.randrange(4) for x in t])
+            T.append(t)
+            B.append(16)
+    ps = A.PairSet.from_seqs(P, T)
+    for sc in (A.Affine.ONT, A.Affine.HIFI, A.Affine(0, 2, 0, 1), A.Affine(3, 2, 7, 1)):
+        check_affine(ps, mode, sc, np.array(B, np.uint64), cigar=False)
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_affine_scores_16bit_gap_corners(mode):
     # The 16-bit kernel lets a vertical gap open only from H before the row's horizontal gaps (a
