@@ -362,6 +362,73 @@ static int32_t row_scalar(const int32_t *hprev, const int32_t *fu, const uint8_t
     return mx;
 }
 
+/* One DP row: the signature of row_scalar, as the reference's `Rows::row` (align.rs:997-1016)
+ * lets run_core take the scalar or the AVX-512 rows. */
+typedef int32_t (*row_fn)(const int32_t *hprev, const int32_t *fu, const uint8_t *tx, int32_t *nh, int32_t *nf,
+                          unsigned c, int32_t mat, int32_t mis, int32_t go, int32_t ext, int32_t e, int32_t floor,
+                          uint64_t w, int32_t *hp, int32_t *gs, int32_t *eout);
+
+#if defined(__x86_64__)
+#include <immintrin.h>
+
+/* The same row on 16 i32 lanes (the reference's AVX-512 rows, align.rs:1150-1267, restated; used
+ * by the CPU baseline only).  Per 16 cells: the vertical and diagonal terms as in row_scalar, the
+ * gap-open values gs = x - go + ext*(k + 1), their exclusive running max (an in-register prefix
+ * max over shifts of 1, 2, 4 and 8 lanes, then one lane over with the carry from the cells
+ * before), and H = max(x, that - ext*k).  Lanes past the row end are masked.  Same integer
+ * operations, so the same results. */
+__attribute__((target("avx512f,avx512bw,avx512vl"))) static int32_t
+row_avx512(const int32_t *hprev, const int32_t *fu, const uint8_t *tx, int32_t *nh, int32_t *nf, unsigned c,
+           int32_t mat, int32_t mis, int32_t go, int32_t ext, int32_t e, int32_t floor, uint64_t w, int32_t *hp,
+           int32_t *gs, int32_t *eout) {
+    (void)hp, (void)gs, (void)eout;
+    nh[w] = NEG;
+    nf[w] = NEG;
+    const __m512i vneg = _mm512_set1_epi32(NEG), vfloor = _mm512_set1_epi32(floor);
+    const __m512i vgo = _mm512_set1_epi32(go), vext = _mm512_set1_epi32(ext);
+    const __m512i vmat = _mm512_set1_epi32(mat), vmis = _mm512_set1_epi32(-mis);
+    const __m512i vc = _mm512_set1_epi32((int32_t)c), v3 = _mm512_set1_epi32(3);
+    const __m512i lanes = _mm512_setr_epi32(0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15);
+    __m512i vmx = vneg;
+    int32_t carry = e; /* max of e and gs of every cell before this vector */
+    for (uint64_t k = 0; k < w; k += 16) {
+        const uint64_t rem = w - k;
+        const __mmask16 mk = rem >= 16 ? (__mmask16)0xFFFF : (__mmask16)((1u << rem) - 1);
+        const __m512i hd = _mm512_maskz_loadu_epi32(mk, hprev + k);
+        const __m512i hu = _mm512_maskz_loadu_epi32(mk, hprev + k + 1);
+        const __m512i fv = _mm512_maskz_loadu_epi32(mk, fu + k);
+        const __m512i t = _mm512_and_si512(_mm512_cvtepu8_epi32(_mm_maskz_loadu_epi8(mk, tx + k)), v3);
+        const __m512i fj = _mm512_max_epi32(_mm512_max_epi32(_mm512_sub_epi32(hu, vgo), _mm512_sub_epi32(fv, vext)),
+                                            vneg);
+        const __m512i sv = _mm512_mask_blend_epi32(_mm512_cmpeq_epi32_mask(t, vc), vmis, vmat);
+        const __m512i x = _mm512_max_epi32(_mm512_max_epi32(_mm512_add_epi32(hd, sv), fj), vfloor);
+        const __m512i ek = _mm512_mullo_epi32(_mm512_add_epi32(lanes, _mm512_set1_epi32((int32_t)k)), vext);
+        const __m512i g = _mm512_mask_mov_epi32(vneg, mk, _mm512_add_epi32(_mm512_sub_epi32(x, vgo),
+                                                                           _mm512_add_epi32(ek, vext)));
+        __m512i p = _mm512_max_epi32(g, _mm512_alignr_epi32(g, vneg, 15));
+        p = _mm512_max_epi32(p, _mm512_alignr_epi32(p, vneg, 14));
+        p = _mm512_max_epi32(p, _mm512_alignr_epi32(p, vneg, 12));
+        p = _mm512_max_epi32(p, _mm512_alignr_epi32(p, vneg, 8));
+        const __m512i vcarry = _mm512_set1_epi32(carry);
+        const __m512i excl = _mm512_max_epi32(_mm512_alignr_epi32(p, vcarry, 15), vcarry);
+        const __m512i hv = _mm512_max_epi32(x, _mm512_sub_epi32(excl, ek));
+        _mm512_mask_storeu_epi32(nh + k, mk, hv);
+        _mm512_mask_storeu_epi32(nf + k, mk, fj);
+        vmx = _mm512_mask_max_epi32(vmx, mk, vmx, hv);
+        carry = max32(carry, _mm512_reduce_max_epi32(p));
+    }
+    return _mm512_reduce_max_epi32(vmx);
+}
+
+int sfo_avx512_available(void) {
+    __builtin_cpu_init();
+    return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw") &&
+           __builtin_cpu_supports("avx512vl");
+}
+#else
+int sfo_avx512_available(void) { return 0; }
+#endif
+
 typedef struct {
     uint64_t below, above;
     int full;
@@ -389,6 +456,138 @@ static inline void range_of(const band_t *b, uint64_t i, uint64_t *lo, uint64_t 
     }
 }
 
+#if defined(__x86_64__)
+/* The 16-bit form of the same row on 32 lanes, for passes whose scores provably fit (see narrow
+ * below; the reference's run16_avx512, align.rs:1348-1877, restated as rows of i16 cells, not
+ * its cache-line layout).  Saturating adds and subtractions keep every unreachable value at the
+ * i16 floor, which orders below every real score as NEG does. */
+#define NEG16 ((int16_t)-32768)
+__attribute__((target("avx512f,avx512bw,avx512vl"))) static void
+row16_avx512(const int16_t *hprev, const int16_t *fu, const uint8_t *tx, int16_t *nh, int16_t *nf, unsigned c,
+             int16_t mat, int16_t mis, int16_t go, int16_t ext, int16_t e, uint64_t w) {
+    nh[w] = NEG16;
+    nf[w] = NEG16;
+    const __m512i vneg = _mm512_set1_epi16(NEG16), vgo = _mm512_set1_epi16(go), vext = _mm512_set1_epi16(ext);
+    const __m512i vmat = _mm512_set1_epi16(mat), vmis = _mm512_set1_epi16((int16_t)-mis);
+    const __m512i vc = _mm512_set1_epi16((int16_t)c), v3 = _mm512_set1_epi16(3);
+    int16_t lane_ix[32];
+    for (int q = 0; q < 32; q++) lane_ix[q] = (int16_t)q;
+    const __m512i lanes = _mm512_loadu_si512(lane_ix);
+    /* Lane q reads lane q - d (d = 1, 2, 4, 8, 16) where q >= d. */
+    __m512i back[5];
+    __mmask32 from[5];
+    for (int t = 0; t < 5; t++) {
+        const int d = 1 << t;
+        back[t] = _mm512_sub_epi16(lanes, _mm512_set1_epi16((int16_t)d));
+        from[t] = (__mmask32)(0xFFFFFFFFu << d);
+    }
+    int16_t carry = e;
+    for (uint64_t k = 0; k < w; k += 32) {
+        const uint64_t rem = w - k;
+        const __mmask32 mk = rem >= 32 ? (__mmask32)0xFFFFFFFFu : (__mmask32)((1u << rem) - 1);
+        const __m512i hd = _mm512_maskz_loadu_epi16(mk, hprev + k);
+        const __m512i hu = _mm512_maskz_loadu_epi16(mk, hprev + k + 1);
+        const __m512i fv = _mm512_maskz_loadu_epi16(mk, fu + k);
+        const __m512i t = _mm512_and_si512(_mm512_cvtepu8_epi16(_mm256_maskz_loadu_epi8(mk, tx + k)), v3);
+        const __m512i fj = _mm512_max_epi16(_mm512_subs_epi16(hu, vgo), _mm512_subs_epi16(fv, vext));
+        const __m512i sv = _mm512_mask_blend_epi16(_mm512_cmpeq_epi16_mask(t, vc), vmis, vmat);
+        const __m512i x = _mm512_max_epi16(_mm512_adds_epi16(hd, sv), fj);
+        const __m512i ek = _mm512_mullo_epi16(_mm512_add_epi16(lanes, _mm512_set1_epi16((int16_t)k)), vext);
+        const __m512i g = _mm512_mask_mov_epi16(
+            vneg, mk, _mm512_adds_epi16(_mm512_subs_epi16(x, vgo), _mm512_adds_epi16(ek, vext)));
+        __m512i p = g;
+        for (int q = 0; q < 5; q++)
+            p = _mm512_max_epi16(p, _mm512_mask_permutexvar_epi16(vneg, from[q], back[q], p));
+        const __m512i vcarry = _mm512_set1_epi16(carry);
+        const __m512i excl = _mm512_max_epi16(_mm512_mask_permutexvar_epi16(vcarry, from[0], back[0], p), vcarry);
+        const __m512i hv = _mm512_max_epi16(x, _mm512_subs_epi16(excl, ek));
+        _mm512_mask_storeu_epi16(nh + k, mk, hv);
+        _mm512_mask_storeu_epi16(nf + k, mk, fj);
+        /* Lane 31 of the inclusive prefix max: the max over the vector (masked lanes are the floor). */
+        const int16_t top = (int16_t)_mm_extract_epi16(_mm512_extracti32x4_epi32(p, 3), 7);
+        carry = top > carry ? top : carry;
+    }
+}
+
+/* align.rs:2381-2426 `narrow`: whether a banded Global/Prefix pass (no clip) fits 16-bit cells. */
+static int narrow16(uint64_t m, int mode, const sfo_scoring *sc, uint64_t band, uint64_t geom) {
+    if (mode == SFO_INFIX || band == SFO_UMAX) return 0;
+    if (sc->mat < 0 || sc->mis < 0 || sc->open < 0 || sc->ext < 0) return 0;
+    const uint64_t mat = (uint64_t)sc->mat, mis = (uint64_t)sc->mis, open = (uint64_t)sc->open,
+                   ext = (uint64_t)sc->ext;
+    const uint64_t below = sat_add_u(band, sat_sub_u(m, geom)), above = sat_add_u(band, sat_sub_u(geom, m));
+    const uint64_t width = sat_add_u(sat_add_u(below, above), 66);
+    const uint64_t gaps = sat_add_u(sat_add_u(below > above ? below : above, width), 1);
+    const uint64_t low = sat_add_u(sat_add_u(sat_add_u(mis * m, open * 2), ext * gaps), mat);
+    const uint64_t high = sat_add_u(mat * m, ext * width);
+    return (low > high ? low : high) <= 32000 && width <= 30000;
+}
+
+/* run_core's score pass (Global or Prefix, no clip, cut or traceback) on 16-bit rows. */
+__attribute__((target("avx512f,avx512bw,avx512vl"))) static void
+run16_core(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_t n, int mode, const sfo_scoring *sc,
+           uint64_t band, int32_t *score_out, uint64_t *col_out) {
+    const int16_t mat = (int16_t)sc->mat, mis = (int16_t)sc->mis, ext = (int16_t)sc->ext;
+    const int16_t go = (int16_t)(sc->open + sc->ext);
+    band_t bd = band_of(m, n, mode, band, n);
+    int16_t *h = (int16_t *)malloc((n + 2) * sizeof(int16_t));
+    int16_t *f = (int16_t *)malloc((n + 2) * sizeof(int16_t));
+    int16_t *nh = (int16_t *)malloc((n + 2) * sizeof(int16_t));
+    int16_t *nf = (int16_t *)malloc((n + 2) * sizeof(int16_t));
+    for (uint64_t j = 0; j < n + 2; j++) h[j] = f[j] = nh[j] = nf[j] = NEG16;
+    uint64_t lo0, hi0;
+    range_of(&bd, 0, &lo0, &hi0);
+    for (uint64_t j = lo0; j <= hi0; j++) h[j] = j == 0 ? 0 : (int16_t)-(sc->open + sc->ext * (int32_t)j);
+    uint64_t plo = lo0, phi = hi0;
+    for (uint64_t i = 1; i <= m; i++) {
+        uint64_t lo, hi;
+        range_of(&bd, i, &lo, &hi);
+        if (lo > hi) {
+            *score_out = NEG;
+            *col_out = 0;
+            goto done;
+        }
+        uint64_t start = lo;
+        int16_t e = NEG16;
+        if (lo == 0) {
+            const int16_t a = (int16_t)(h[0] - go < NEG16 ? NEG16 : h[0] - go);
+            const int16_t b = (int16_t)(f[0] - ext < NEG16 ? NEG16 : f[0] - ext);
+            const int16_t fj = a > b ? a : b;
+            nh[0] = nf[0] = fj;
+            e = (int16_t)(fj - go < NEG16 ? NEG16 : fj - go);
+            start = 1;
+        }
+        if (start <= hi)
+            row16_avx512(&h[start - 1], &f[start], &txt[start - 1], &nh[start], &nf[start], pat[i - 1] & 3u, mat, mis,
+                         go, ext, e, hi + 1 - start);
+        else
+            nh[hi + 1] = nf[hi + 1] = NEG16;
+        int16_t *t;
+        t = h, h = nh, nh = t;
+        t = f, f = nf, nf = t;
+        plo = lo;
+        phi = hi;
+    }
+    if (mode == SFO_GLOBAL) {
+        *score_out = h[n];
+        *col_out = n;
+    } else {
+        int32_t best = NEG;
+        uint64_t col = 0;
+        int first = 1;
+        for (uint64_t j = plo; j <= phi; j++)
+            if (first || h[j] > best) best = h[j], col = j, first = 0;
+        *score_out = best;
+        *col_out = col;
+    }
+done:
+    free(h);
+    free(f);
+    free(nh);
+    free(nf);
+}
+#endif
+
 /* Optional per-row capture of the pass for the traceback. */
 typedef struct {
     uint64_t *lo, *hi; /* per row 0..m */
@@ -399,7 +598,7 @@ typedef struct {
 /* align.rs:2446-2666 `run_core::<ScalarRows>` (no resume, no snapshots). */
 static void run_core(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_t n, int mode, const sfo_scoring *sc,
                      uint64_t band, uint64_t geom, int has_clip, int32_t xdrop, int clip_cell, uint64_t cut,
-                     sfo_dp *out, rows_t *keep) {
+                     sfo_dp *out, rows_t *keep, row_fn row) {
     int32_t clip_best = 0;
     uint64_t ci = 0, cj = 0;
     int32_t cut_best = 0;
@@ -462,8 +661,8 @@ static void run_core(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_
             row_max = fj;
         }
         if (start <= hi) {
-            int32_t mx = row_scalar(&h[start - 1], &f[start], &txt[start - 1], &nh[start], &nf[start], c, mat, mis, go,
-                                    ext, e, NEG, hi + 1 - start, hp, gs, erow ? erow + (start - lo) : NULL);
+            int32_t mx = row(&h[start - 1], &f[start], &txt[start - 1], &nh[start], &nf[start], c, mat, mis, go, ext, e,
+                             NEG, hi + 1 - start, hp, gs, erow ? erow + (start - lo) : NULL);
             row_max = max32(row_max, mx);
         } else {
             nh[hi + 1] = NEG;
@@ -543,13 +742,13 @@ done:
 void sfo_affine_dp(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_t n, int mode, const sfo_scoring *sc,
                    uint64_t band, uint64_t geom, int has_clip, int32_t xdrop, int clip_cell, uint64_t cut,
                    sfo_dp *out) {
-    run_core(pat, m, txt, n, mode, sc, band, geom, has_clip, xdrop, clip_cell, cut, out, NULL);
+    run_core(pat, m, txt, n, mode, sc, band, geom, has_clip, xdrop, clip_cell, cut, out, NULL, row_scalar);
 }
 
 int32_t sfo_affine(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_t n, int mode, const sfo_scoring *sc,
                    uint64_t band) {
     sfo_dp d;
-    run_core(pat, m, txt, n, mode, sc, band, n, 0, 0, 0, SFO_UMAX, &d, NULL);
+    run_core(pat, m, txt, n, mode, sc, band, n, 0, 0, 0, SFO_UMAX, &d, NULL, row_scalar);
     return d.score;
 }
 
@@ -557,7 +756,7 @@ int32_t sfo_affine(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_t 
 int32_t sfo_affine_extend(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_t n, const sfo_scoring *sc,
                           uint64_t band, int32_t end_bonus, int32_t xdrop) {
     sfo_dp d;
-    run_core(pat, m, txt, n, SFO_PREFIX, sc, band, n, 1, xdrop, 0, SFO_UMAX, &d, NULL);
+    run_core(pat, m, txt, n, SFO_PREFIX, sc, band, n, 1, xdrop, 0, SFO_UMAX, &d, NULL, row_scalar);
     return max32(d.clip_best, sat_add_i(d.score, end_bonus));
 }
 
@@ -565,7 +764,8 @@ int32_t sfo_affine_extend(const uint8_t *pat, uint64_t m, const uint8_t *txt, ui
 void sfo_affine_extend_window(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_t n, uint64_t window,
                               const sfo_scoring *sc, uint64_t band, int32_t end_bonus, int32_t xdrop, int64_t out4[4]) {
     sfo_dp d;
-    run_core(pat, m, txt, n, SFO_PREFIX, sc, band, window > n ? window : n, 1, xdrop, 1, SFO_UMAX, &d, NULL);
+    run_core(pat, m, txt, n, SFO_PREFIX, sc, band, window > n ? window : n, 1, xdrop, 1, SFO_UMAX, &d, NULL,
+             row_scalar);
     int32_t whole = sat_add_i(d.score, end_bonus);
     if (d.clip_best >= whole) {
         out4[0] = d.clip_best;
@@ -585,7 +785,8 @@ void sfo_affine_extend_cut_bonus(const uint8_t *pat, uint64_t m, const uint8_t *
                                  uint64_t cut, int track_cut, int bonus_within_cut, int64_t out3[3]) {
     uint64_t geom = diagonal ? (m < n ? m : n) : (n > window ? n : window);
     sfo_dp d;
-    run_core(pat, m, txt, n, SFO_PREFIX, sc, band, geom, 1, xdrop, track_cut && cut != SFO_UMAX, cut, &d, NULL);
+    run_core(pat, m, txt, n, SFO_PREFIX, sc, band, geom, 1, xdrop, track_cut && cut != SFO_UMAX, cut, &d, NULL,
+             row_scalar);
     int32_t end;
     if (bonus_within_cut && cut != SFO_UMAX) end = max32(d.score, sat_add_i(d.cut_score, end_bonus));
     else end = sat_add_i(d.score, end_bonus);
@@ -743,7 +944,7 @@ int64_t sfo_align(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_t n
     keep.f = (int32_t *)malloc((total + 1) * sizeof(int32_t));
     keep.e = (int32_t *)malloc((total + 1) * sizeof(int32_t));
     sfo_dp d;
-    run_core(pat, m, txt, n, mode, sc, band, n, 0, 0, 0, SFO_UMAX, &d, &keep);
+    run_core(pat, m, txt, n, mode, sc, band, n, 0, 0, 0, SFO_UMAX, &d, &keep, row_scalar);
     *score = d.score;
     *score_col = d.score_col;
     *tstart = 0;
@@ -981,6 +1182,8 @@ typedef struct {
     uint32_t *dist;
     int32_t *score;
     uint64_t *score_col;
+    row_fn row;
+    int narrow; /* take the 16-bit rows where narrow16 holds (AVX-512 baseline) */
     atomic_uint_fast64_t next;
 } job_t;
 
@@ -1006,9 +1209,18 @@ static void *affine_worker(void *arg) {
         if (i0 >= jb->npairs) break;
         uint64_t i1 = i0 + CHUNK < jb->npairs ? i0 + CHUNK : jb->npairs;
         for (uint64_t i = i0; i < i1; i++) {
+#if defined(__x86_64__)
+            if (jb->narrow && narrow16(jb->plen[i], jb->mode, jb->sc, jb->band[i], jb->tlen[i])) {
+                uint64_t col;
+                run16_core(jb->pat + jb->poff[i], jb->plen[i], jb->txt + jb->toff[i], jb->tlen[i], jb->mode, jb->sc,
+                           jb->band[i], &jb->score[i], &col);
+                if (jb->score_col) jb->score_col[i] = col;
+                continue;
+            }
+#endif
             sfo_dp d;
             run_core(jb->pat + jb->poff[i], jb->plen[i], jb->txt + jb->toff[i], jb->tlen[i], jb->mode, jb->sc,
-                     jb->band[i], jb->tlen[i], 0, 0, 0, SFO_UMAX, &d, NULL);
+                     jb->band[i], jb->tlen[i], 0, 0, 0, SFO_UMAX, &d, NULL, jb->row);
             jb->score[i] = d.score;
             if (jb->score_col) jb->score_col[i] = d.score_col;
         }
@@ -1042,5 +1254,28 @@ void sfo_batch_affine(const uint8_t *pat, const uint64_t *poff, const uint32_t *
     memset(&jb, 0, sizeof jb);
     jb.pat = pat, jb.poff = poff, jb.plen = plen, jb.txt = txt, jb.toff = toff, jb.tlen = tlen;
     jb.npairs = npairs, jb.mode = mode, jb.sc = sc, jb.band = band, jb.score = score, jb.score_col = score_col;
+    jb.row = row_scalar;
     run_threads(&jb, affine_worker, threads);
+}
+
+/* The same as the reference's x86 dispatch runs it (align.rs:2320-2355): 16-bit AVX-512 rows where
+ * `narrow` holds, 32-bit AVX-512 rows (run_core::<Avx512Rows>) otherwise; -1 where the host has no
+ * AVX-512. */
+int sfo_batch_affine_avx512(const uint8_t *pat, const uint64_t *poff, const uint32_t *plen, const uint8_t *txt,
+                            const uint64_t *toff, const uint32_t *tlen, uint64_t npairs, int mode,
+                            const sfo_scoring *sc, const uint64_t *band, int32_t *score, uint64_t *score_col,
+                            int threads) {
+#if defined(__x86_64__)
+    if (!sfo_avx512_available()) return -1;
+    job_t jb;
+    memset(&jb, 0, sizeof jb);
+    jb.pat = pat, jb.poff = poff, jb.plen = plen, jb.txt = txt, jb.toff = toff, jb.tlen = tlen;
+    jb.npairs = npairs, jb.mode = mode, jb.sc = sc, jb.band = band, jb.score = score, jb.score_col = score_col;
+    jb.row = row_avx512;
+    jb.narrow = 1;
+    run_threads(&jb, affine_worker, threads);
+    return 0;
+#else
+    return -1;
+#endif
 }

@@ -142,6 +142,13 @@ void sfo_batch_affine(const uint8_t *pat, const uint64_t *poff, const uint32_t *
                       const uint8_t *txt, const uint64_t *toff, const uint32_t *tlen,
                       uint64_t npairs, int mode, const sfo_scoring *sc, const uint64_t *band,
                       int32_t *score, uint64_t *score_col, int threads);
+/* The same with AVX-512 rows (the reference's run_core::<Avx512Rows>, align.rs:1150-1267):
+ * equal results; returns -1 (and computes nothing) on a host without AVX-512 F/BW/VL. */
+int sfo_avx512_available(void);
+int sfo_batch_affine_avx512(const uint8_t *pat, const uint64_t *poff, const uint32_t *plen,
+                            const uint8_t *txt, const uint64_t *toff, const uint32_t *tlen,
+                            uint64_t npairs, int mode, const sfo_scoring *sc, const uint64_t *band,
+                            int32_t *score, uint64_t *score_col, int threads);
 
 #ifdef __cplusplus
 }

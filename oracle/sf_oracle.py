@@ -95,6 +95,10 @@ def lib():
         vp = C.c_void_p
         L.sfo_batch_myers.argtypes = [vp, vp, vp, vp, vp, vp, u64, C.c_int, u32, vp, C.c_int]
         L.sfo_batch_affine.argtypes = [vp, vp, vp, vp, vp, vp, u64, C.c_int, sc, vp, vp, vp, C.c_int]
+        L.sfo_batch_affine_avx512.argtypes = [vp, vp, vp, vp, vp, vp, u64, C.c_int, sc, vp, vp, vp, C.c_int]
+        L.sfo_batch_affine_avx512.restype = C.c_int
+        L.sfo_avx512_available.argtypes = []
+        L.sfo_avx512_available.restype = C.c_int
         _lib = L
     return _lib
 
@@ -261,14 +265,25 @@ def batch_myers(pat, poff, plen, txt, toff, tlen, mode, k=NONE, threads=1):
     return out
 
 
-def batch_affine(pat, poff, plen, txt, toff, tlen, mode, sc, band, threads=1):
+def avx512_available() -> bool:
+    return bool(lib().sfo_avx512_available())
+
+
+def batch_affine(pat, poff, plen, txt, toff, tlen, mode, sc, band, threads=1, avx512=False):
+    """run_core over many pairs on `threads` host threads; avx512: with the AVX-512 rows (the
+    host must have AVX-512 F/BW/VL, see avx512_available)."""
     n = len(plen)
     score = np.empty(n, np.int32)
     col = np.empty(n, np.uint64)
     band = np.ascontiguousarray(np.broadcast_to(np.asarray(band, np.uint64), (n,)))
     a = [np.ascontiguousarray(x) for x in (pat, poff, plen, txt, toff, tlen)]
-    lib().sfo_batch_affine(*[x.ctypes.data for x in a], n, mode, C.byref(_sc(sc)), band.ctypes.data,
-                           score.ctypes.data, col.ctypes.data, threads)
+    args = [*[x.ctypes.data for x in a], n, mode, C.byref(_sc(sc)), band.ctypes.data, score.ctypes.data,
+            col.ctypes.data, threads]
+    if avx512:
+        if lib().sfo_batch_affine_avx512(*args) != 0:
+            raise RuntimeError("this host has no AVX-512 F/BW/VL")
+    else:
+        lib().sfo_batch_affine(*args)
     return score, col
 
 

@@ -325,3 +325,34 @@ def test_affine_cells_counts_rows():
     want = sum(min(i + above, n) + 1 - max(i - below, 0) for i in range(1, m + 1))
     assert O.affine_cells(m, n, GLOBAL, band) == want
     assert O.affine_cells(m, n, INFIX, band) == m * (n + 1)
+
+
+@pytest.mark.skipif(not O.avx512_available(), reason="host without AVX-512 F/BW/VL")
+def test_avx512_rows_equal_scalar_rows():
+    # The CPU baseline's AVX-512 rows (run_core::<Avx512Rows>) against the scalar rows: rows
+    # shorter than a vector, whole vectors and ragged tails, every mode, bands from 0 to exact.
+    import numpy as np
+    rng = random.Random(29)
+    P, T, B = [], [], []
+    for _ in range(600):
+        n = rng.choice([0, 1, 5, 15, 16, 17, 31, 33, 64, rng.randrange(0, 400)])
+        t = [rng.randrange(4) for _ in range(n)]
+        p = [x if rng.random() > 0.12 else rng.randrange(4) for x in t]
+        for _ in range(rng.randrange(0, 4)):
+            k = rng.randrange(len(p) + 1)
+            p = p[:k] + [rng.randrange(4) for _ in range(rng.randrange(0, 6))] + p[k + rng.randrange(0, 6):]
+        P.append(bytes(p))
+        T.append(bytes(t))
+        B.append(rng.choice([0, 1, 3, 15, 16, 17, 40, 100, O.UMAX]))
+    poff = np.cumsum([0] + [len(p) for p in P[:-1]]).astype(np.uint64)
+    toff = np.cumsum([0] + [len(t) for t in T[:-1]]).astype(np.uint64)
+    pat = np.frombuffer(b"".join(P) + b"\0", np.uint8)
+    txt = np.frombuffer(b"".join(T) + b"\0", np.uint8)
+    plen = np.array([len(p) for p in P], np.uint32)
+    tlen = np.array([len(t) for t in T], np.uint32)
+    band = np.array(B, np.uint64)
+    for mode in (INFIX, PREFIX, GLOBAL):
+        for sc in (ONT, HIFI, (1, 1, 0, 1), (3, 2, 5, 1)):
+            a = O.batch_affine(pat, poff, plen, txt, toff, tlen, mode, sc, band, threads=4)
+            b = O.batch_affine(pat, poff, plen, txt, toff, tlen, mode, sc, band, threads=4, avx512=True)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (mode, sc)
