@@ -264,7 +264,8 @@ int sf_set_affine_lp16_max(uint64_t slots);
 /* Tuning (no reference counterpart; results never depend on it): score-only affine runs with a
  * 16-lane packed shape may run 8 lanes per pair, eight pairs per wave.  policy 0: where the
  * launch planner's estimate favours it (default); 1: wherever the 8-lane kernel's range holds;
- * -1: never.  Applies to affine runs started afterwards in this process. */
+ * -1: never.  Default 0 (or SF_AFFINE_LP8 from the environment).  Applies to affine runs started
+ * afterwards in this process. */
 int sf_set_affine_lp8(int policy);
 
 /* Diagnostic (no reference counterpart): runs a one-wave kernel that records the lane moves of

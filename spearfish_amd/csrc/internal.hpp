@@ -192,6 +192,38 @@ __device__ __forceinline__ uint32_t wave_excl_umax(uint32_t x) {
     return dpp0<kWaveShr1>(x);
 }
 
+// Traceback words per lane per row: 4 bits per slot.
+template <int K>
+constexpr int kTbWords = (4 * K + 31) / 32;
+
+// Exclusive max-scan of unsigned values over each group of LP lanes (16: one DPP row; 64: the
+// wave); the group's first lane gets 0.
+template <int LP>
+__device__ __forceinline__ uint32_t group_excl_umax(uint32_t x) {
+    if constexpr (LP == 64) {
+        return wave_excl_umax(x);
+    } else if constexpr (LP == 8) {
+        // Two groups share a DPP row.  The input moves up a lane inside its group (the group's
+        // first lane takes 0), is scanned inside quads (quad permutes [0,0,1,2] and [0,1,0,1]: a
+        // lane reading itself is harmless under max), and the upper quad of each group takes the
+        // lower quad's total by a row shift of 4 written to banks 1 and 3 only.
+        uint32_t y = dpp0<kRowShr1>(x);
+        y = (threadIdx.x & 7) ? y : 0u;
+        y = umax(y, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)y, 0x90, 0xf, 0xf, false));
+        y = umax(y, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)y, 0x44, 0xf, 0xf, false));
+        const uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)y, 0xff, 0xf, 0xf, false);
+        return umax(y, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, kRowShr4, 0xf, 0xa, false));
+    } else {
+        static_assert(LP == 16, "groups of 8, 16 or 64 lanes");
+        x = umax(x, dpp0<kRowShr1>(x));
+        x = umax(x, dpp0<kRowShr2>(x));
+        x = umax(x, dpp0<kRowShr4>(x));
+        x = umax(x, dpp0<kRowShr8>(x));
+        return dpp0<kRowShr1>(x);
+    }
+}
+
+
 // Exclusive max-scan across the 64 lanes of a wave, lane 0 getting INT32_MIN.  Signed values are
 // mapped to unsigned order (x ^ 0x80000000) so that DPP's zero fill of lanes without a source is
 // the identity and each step is one v_max_u32 with a DPP operand.
@@ -237,16 +269,20 @@ hipError_t sf_launch_affine(const sf::AffineArgs &a, int K, bool tb, hipStream_t
 uint64_t sf_affine_lp16_max(); // the current 16-lane limit (slots), part of the shape choice
 int sf_affine_class(uint64_t width, bool lp16);      // 16 lanes per pair if allowed and it fits, else 64
 int sf_affine_pass_class(uint64_t width, bool lp16); // the same for the passes with bookkeeping
-// The 16-bit packed class for a pair of the 16-lane class with K slots per lane, or 0 where its
-// exact range does not hold (the scoring must have mis = 2 ext); tb: for the traceback kernel.
+// The 16-bit packed class for a pair of the 32-bit class with `lanes` (16 or 64) lanes and K slots
+// per lane, or 0 where its exact range does not hold (the scoring must have mis = 2 ext); tb: for
+// the traceback kernel.
 int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m, uint32_t below,
-                         bool tb);
+                         bool tb, int lanes);
 // When score-only runs use the 8-lane packed kernel: 0 by estimate, 1 always, -1 never.
 int sf_affine_lp8_policy();
 void sf_affine_set_lp8_policy(int policy);
 // The 8-lane packed class (scores, three-term max) for a band of `width` slots, or 0.
 int sf_affine_pk8_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m,
                         uint32_t below);
+// The packed classes (aff_packed): the 16-bit score and traceback kernels.
+hipError_t sf_launch_affine_packed(const sf::AffineArgs &a, int cls, bool tb, hipStream_t s);
+hipError_t sf_launch_affine_packed64(const sf::AffineArgs &a, int cls, bool tb, hipStream_t s);
 hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
 // cls: a 32-bit affine class (16 or 64 lanes); local selects local_band_core.
 hipError_t sf_launch_pass(const sf::PassArgs &a, int cls, bool local, hipStream_t s);

@@ -729,12 +729,10 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
                 const Cost &c = it->second;
                 if (std::max(c.chain64, c.spread64) < std::max(c.chain16, c.spread16)) kcls[i] = k64[i];
             }
-            // 16-lane pairs take the 16-bit kernels where their exact range holds.
-            if (aff_lp(kcls[i]) == 16) {
-                const int kp = sf_affine_pk16_class(aff_k(kcls[i]), sc->mat, sc->mis, sc->open, sc->ext,
-                                                    b->plen[i], (uint32_t)ba[i], cigar);
-                if (kp) kcls[i] = kp;
-            }
+            // The 16-bit kernels where their exact range holds.
+            const int kp = sf_affine_pk16_class(aff_k(kcls[i]), sc->mat, sc->mis, sc->open, sc->ext, b->plen[i],
+                                                (uint32_t)ba[i], cigar, aff_lp(kcls[i]));
+            if (kp) kcls[i] = kp;
             // Values stay far above NEG over all slots of the class.
             if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, b->plen[i], b->tlen[i], aff_slots(kcls[i])))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);

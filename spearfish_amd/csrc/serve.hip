@@ -175,7 +175,7 @@ void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
         acls[i] = sf_affine_k_for(below + above + 1);
         if (aff_lp(acls[i]) == 16) { // scores only: the 16-bit kernel where its range holds
             const int kp = sf_affine_pk16_class(aff_k(acls[i]), cfg.sc.mat, cfg.sc.mis, cfg.sc.open, cfg.sc.ext,
-                                                s.h_plen[i], (uint32_t)below, false);
+                                                s.h_plen[i], (uint32_t)below, false, 16);
             if (kp) acls[i] = kp;
         }
         cells += (uint64_t)s.h_plen[i] * s.h_tlen[i] + band_cells(s.h_plen[i], s.h_tlen[i], below, above);

@@ -166,7 +166,8 @@ def test_affine_64_lane_kernel(mode, lanes64):
         T.append(t)
         B.append(rng.choice([0, 1, 5, 16, 64, 130, A.USIZE_MAX]))
     ps = A.PairSet.from_seqs(P, T)
-    for sc in (A.Affine.ONT, A.Affine(3, 2, 5, 1)):
+    # mis = 2 ext: the 16-bit kernels on 64 lanes; (1, 1, 0, 1): the 32-bit ones.
+    for sc in (A.Affine.ONT, A.Affine(3, 2, 5, 1), A.Affine(1, 1, 0, 1)):
         check_affine(ps, mode, sc, np.array(B, np.uint64))
 
 
