@@ -569,10 +569,12 @@ __global__ __launch_bounds__(256) void affine_pass_kernel(PassArgs a) {
 // One lane per pair: walks the traceback bits from the end cell back to the start, emitting
 // BAM ops last to first (rules in the header comment of affine_wave_kernel and the oracle).
 // Each step reads one u32 -- the word of the cell's lane for that row, which holds 8 slots of
-// the lane -- so the walk keeps the 16-row block of its current lane-word in LDS: diagonal,
-// vertical and horizontal moves stay inside it until the path crosses to another lane's slots
-// or another block, and a refill is one 64-byte line.  Pattern and text codes are read a word
-// (16 bases) at a time.
+// the lane -- so the walk keeps the 16-row block of its current lane-word in LDS (one 64-byte
+// line), the pattern word of that block and the two text words around the cell's column in
+// registers.  A lane steps on while its next cell is inside that data (diagonal, vertical and
+// horizontal moves stay inside it until the path crosses to another lane-word or block); once
+// every lane is done or needs new data, the lanes that need it load it together, so the wave
+// waits on memory once per refill round rather than once per step of any lane.
 constexpr int kWalkRows = 16;
 
 __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
@@ -611,90 +613,94 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
             cur_len = len;
         }
     };
-    bool bad = false;
-    int32_t cL = -1, cwd = -1, cb = -1; // cached lane-word and its 16-row block
-    // The 4 bits (nd, ne, fx, ex) of the current cell (i, j) in bits 3..0 of the result.
-    auto bits_here = [&]() -> uint32_t {
-        if (i < 1 || i > m || s < 0 || s >= slots) {
-            // Off the band: cannot happen on a path the DP produced; stop rather than read out of
-            // the pair's bits.
-            bad = true;
-            return 0;
-        }
-        const int32_t wd = q >> 3, b = (i - 1) >> 4;
-        if (L != cL || wd != cwd || b != cb) {
-            // One 64-byte line: this lane-word's 16 rows of block b.
-            const uint4 *line = reinterpret_cast<const uint4 *>(tb + ((uint64_t)(b * nwl + wd) * LP + L) * 16);
-            uint4 v[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) v[k] = line[k];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                my[4 * k] = v[k].x;
-                my[4 * k + 1] = v[k].y;
-                my[4 * k + 2] = v[k].z;
-                my[4 * k + 3] = v[k].w;
-            }
-            cL = L;
-            cwd = wd;
-            cb = b;
-        }
-        const int32_t nq = (K - (wd << 3)) < 8 ? (K - (wd << 3)) : 8;
-        return (my[(i - 1) & 15] >> (4 * (nq - 1 - (q & 7)))) & 15u;
-    };
-    int32_t pwi = -1, twi = -1; // cached code words (pattern, text)
-    uint32_t pwv = 0, twv = 0;
+    bool bad = false, done = false;
+    int32_t cL = -1, cwd = -1, cb = -1;         // the lane-word and 16-row block held in LDS
+    int32_t pwi = -1, twi = -2;                 // the pattern word (of block cb) and text words twi, twi - 1
+    uint32_t pwv = 0, tw0 = 0, tw1 = 0;
     for (;;) {
-        if (bad) break;
-        if (state == 0) {
-            if (i == 0) {
-                if (a.mode == kInfix) tstart = (uint32_t)j;
-                else emit(2, (uint32_t)j);
+        // Refill round: the line of the current cell, and for the H state its codes.
+        if (state != 0 || (i > 0 && j > 0)) {
+            if (i < 1 || i > m || s < 0 || s >= slots) {
+                bad = true;
                 break;
             }
-            if (j == 0) {
-                emit(1, (uint32_t)i);
-                break;
+            const int32_t wd = q >> 3, b = (i - 1) >> 4;
+            if (L != cL || wd != cwd || b != cb) {
+                const uint4 *line = reinterpret_cast<const uint4 *>(tb + ((uint64_t)(b * nwl + wd) * LP + L) * 16);
+                uint4 v[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v[k] = line[k];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    my[4 * k] = v[k].x;
+                    my[4 * k + 1] = v[k].y;
+                    my[4 * k + 2] = v[k].z;
+                    my[4 * k + 3] = v[k].w;
+                }
+                cL = L;
+                cwd = wd;
+                cb = b;
             }
-            const uint32_t w = bits_here();
-            if ((w & 8u) == 0) {
-                if ((i - 1) >> 4 != pwi) {
+            if (state == 0) {
+                if (((i - 1) >> 4) != pwi) {
                     pwi = (i - 1) >> 4;
                     pwv = codes16(P, pwi * 16);
                 }
-                if ((j - 1) >> 4 != twi) {
-                    twi = (j - 1) >> 4;
-                    twv = codes16(T, twi * 16);
+                const int32_t tw = (j - 1) >> 4;
+                if (tw != twi && tw != twi - 1) {
+                    twi = tw;
+                    tw0 = codes16(T, tw * 16);
+                    tw1 = tw > 0 ? codes16(T, (tw - 1) * 16) : 0u;
                 }
-                const uint32_t pc = (pwv >> (2 * ((i - 1) & 15))) & 3u;
-                const uint32_t tc = (twv >> (2 * ((j - 1) & 15))) & 3u;
-                emit(pc == tc ? 7u : 8u, 1);
-                --i;
-                --j;
-            } else if ((w & 4u) == 0) {
-                state = 1;
-            } else {
-                state = 2;
-            }
-        } else if (state == 1) {
-            emit(1, 1);
-            state = (bits_here() & 2u) ? 1 : 0;
-            --i;
-            ++s;
-            if (++q == K) {
-                q = 0;
-                ++L;
-            }
-        } else {
-            emit(2, 1);
-            state = (bits_here() & 1u) ? 2 : 0;
-            --j;
-            --s;
-            if (--q < 0) {
-                q = K - 1;
-                --L;
             }
         }
+        // Steps while this lane's data holds, as arithmetic on the state rather than branches
+        // (the lanes of a wave sit in different states), with one exit: per step one nibble, at
+        // most one run written.
+        bool fin, oob;
+        for (;;) {
+            fin = state == 0 && (i == 0 || j == 0);
+            oob = !fin && (i < 1 || i > m || s < 0 || s >= slots); // off the band: see the refill
+            const int32_t wd = q >> 3, b = (i - 1) >> 4, tw = (j - 1) >> 4;
+            const bool have = !fin && !oob && L == cL && wd == cwd && b == cb;
+            const int32_t nq = min(K - (wd << 3), 8);
+            const uint32_t w = (my[(i - 1) & 15] >> (4 * (nq - 1 - (q & 7)))) & 15u;
+            const bool h = state == 0, diag = h && (w & 8u) == 0;
+            const bool codes = b == pwi && (tw == twi || tw == twi - 1);
+            if (!have || (diag && !codes)) break; // done, or a refill first
+            const uint32_t pc = (pwv >> (2 * ((i - 1) & 15))) & 3u;
+            const uint32_t tc = ((tw == twi ? tw0 : tw1) >> (2 * ((j - 1) & 15))) & 3u;
+            // The step: H diagonal (= or X, i - 1, j - 1), H into F or E (no move), F (I, i - 1,
+            // slot + 1, stays in F on fx), E (D, j - 1, slot - 1, stays in E on ex).
+            const uint32_t op = diag ? (pc == tc ? 7u : 8u) : (h ? 0u : (uint32_t)state);
+            const int32_t di = (diag || state == 1) ? 1 : 0, dj = (diag || state == 2) ? 1 : 0;
+            const int32_t ds = state == 1 ? 1 : (state == 2 ? -1 : 0);
+            const int hs = (w & 8u) == 0 ? 0 : ((w & 4u) ? 2 : 1);
+            state = h ? hs : (state == 1 ? ((w & 2u) ? 1 : 0) : ((w & 1u) ? 2 : 0));
+            const bool same = op == cur_op && cur_len != 0;
+            if (op != 0 && !same && cur_len != 0) out[nr++] = (cur_len << 4) | cur_op;
+            cur_len = op != 0 ? (same ? cur_len + 1 : 1u) : cur_len;
+            cur_op = op != 0 ? op : cur_op;
+            i -= di;
+            j -= dj;
+            s += ds;
+            q += ds;
+            L += q == K ? 1 : (q < 0 ? -1 : 0);
+            q = q == K ? 0 : (q < 0 ? K - 1 : q);
+        }
+        if (fin) {
+            if (i == 0) {
+                if (a.mode == kInfix) tstart = (uint32_t)j;
+                else emit(2, (uint32_t)j);
+            } else {
+                emit(1, (uint32_t)i);
+            }
+            done = true;
+        } else if (oob) {
+            bad = true;
+            done = true;
+        }
+        if (done) break;
     }
     if (cur_len) out[nr++] = (cur_len << 4) | cur_op;
     if (bad) {
