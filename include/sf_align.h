@@ -256,16 +256,18 @@ int sf_server_replay(sf_server *s, const sf_read_request *req, const double *t_a
                      uint64_t *refused);
 
 /* Tuning (no reference counterpart; results never depend on it): bands of up to `slots` cells
- * per row run with 16 lanes per pair, four pairs per wave; wider bands, or every band when
+ * per row may run with 16 lanes per pair, four pairs per wave (the launch planner moves a pair
+ * whose rows would outlast the rest of the run to 64 lanes); wider bands, or every band when
  * `slots` is 0, run with 64 lanes per pair.  Default 512 (or SF_AFFINE_LP16_MAX from the
  * environment).  Applies to affine runs started afterwards in this process. */
 int sf_set_affine_lp16_max(uint64_t slots);
 
-/* Tuning (no reference counterpart; results never depend on it): score-only affine runs with a
- * 16-lane packed shape may run 8 lanes per pair, eight pairs per wave.  policy 0: where the
- * launch planner's estimate favours it (default); 1: wherever the 8-lane kernel's range holds;
- * -1: never.  Default 0 (or SF_AFFINE_LP8 from the environment).  Applies to affine runs started
- * afterwards in this process. */
+/* Tuning (no reference counterpart; results never depend on it): lanes per pair of affine runs.
+ * policy 0 (default): per pair, by the launch planner's estimate (score-only runs may use 8 lanes
+ * per pair, eight pairs per wave, where the 16-bit range holds); 1: every pair in its fewest-lanes
+ * shape (8 wherever the 8-lane kernel's range holds, else 16 up to the 16-lane width limit, else
+ * 64); -1: never 8, else as 0.  Default 0 (or SF_AFFINE_LP8 from the environment).  Applies to
+ * affine runs started afterwards in this process. */
 int sf_set_affine_lp8(int policy);
 
 /* Diagnostic (no reference counterpart): runs a one-wave kernel that records the lane moves of

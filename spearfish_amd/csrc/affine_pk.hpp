@@ -110,8 +110,9 @@ __device__ __forceinline__ void pk16_result(const AffineArgs &a, const u16x2 (&V
 // LP lanes per pair: 16 (four pairs a wave) or 8 (eight pairs a wave: the per-row work that every
 // lane repeats -- scan, neighbour move, pattern code -- is shared by twice the slots).  M3: every
 // value is below 0x7C00 (checked on the host), so H takes its three terms in one max.
+// The kernel's body for workgroup `blk` of a launch (a mixed launch runs several classes).
 template <int LP, int K2, bool M3>
-__global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
+__device__ __forceinline__ void affine_pk16_body(const AffineArgs &a, uint32_t blk) {
     static_assert(LP == 8 || LP == 16 || LP == 64, "8, 16 or 64 lanes a pair");
     constexpr int SHL = LP == 64 ? kWaveShl1 : kRowShl1; // next lane up
     constexpr int S = 2 * K2;                     // slots per lane
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
     constexpr int NWP = (NF + 7) / 8;             // paired text words holding them
     const int lane = threadIdx.x & 63;
     const int l = lane & (LP - 1);
-    const uint32_t w = (blockIdx.x * 256u + threadIdx.x) / LP;
+    const uint32_t w = (blk * 256u + threadIdx.x) / LP;
     const bool valid = w < a.count;
     const uint32_t pi = valid ? a.order[w] : 0u;
     const PairDesc dsc = valid ? a.desc[w] : PairDesc{0, 0, 0, 0, 0, 0};
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
     const bool infix = a.mode == kInfix;
     const int32_t off = open + (dm + ext) * below + 2;
     const int32_t s0 = l * S;
-    const u16x2 vopen = (u16x2)((uint16_t)open), vdm = (u16x2)((uint16_t)dm);
+    const u16x2 vdm = (u16x2)((uint16_t)dm);
 
     // G = X - open (X = max(diagonal, F): H before the row's horizontal gaps) is both the row's
     // gap-open term for E and, as U = max(G, F), the next row's vertical term: F never opens from
@@ -234,6 +235,11 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
         }
     }
     pk16_result<LP, K2>(a, V, valid, pi, lane, l, m, n, below, Wd, last, off);
+}
+
+template <int LP, int K2, bool M3>
+__global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
+    affine_pk16_body<LP, K2, M3>(a, blockIdx.x);
 }
 
 // Traceback variant of the packed kernel.  Same layout and bias; the offset is raised by `open`

@@ -60,6 +60,16 @@ struct AffineArgs {
     const uint64_t *tb_off; // per order slot of the launch: word offset into tb
 };
 
+// One launch over several 64-lane score classes: segment i holds count[i] pairs of class cls[i]
+// at launch-order offset off[i] and starts at workgroup blk0[i] (4 pairs a workgroup); blocks is
+// the launch's total.
+struct MixArgs {
+    static constexpr uint32_t kMax = 48;
+    uint32_t nseg, blocks;
+    int32_t cls[kMax];
+    uint32_t off[kMax], count[kMax], blk0[kMax];
+};
+
 // A pass with per-row bookkeeping: align.rs:2446-2666 run_core with clip (X-drop, best cell) and
 // cut tracking, or (local) align.rs:783-852 local_band_core.  Band geometry comes in the pair
 // records (local: below = -dlo, above = dhi, as two's complement).
@@ -283,6 +293,9 @@ int sf_affine_pk8_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, 
 // The packed classes (aff_packed): the 16-bit score and traceback kernels.
 hipError_t sf_launch_affine_packed(const sf::AffineArgs &a, int cls, bool tb, hipStream_t s);
 hipError_t sf_launch_affine_packed64(const sf::AffineArgs &a, int cls, bool tb, hipStream_t s);
+// Score-only 64-lane classes that can share one launch, and that launch.
+bool sf_affine_mixable(int cls);
+hipError_t sf_launch_affine_mix(const sf::AffineArgs &a, const sf::MixArgs &mx, hipStream_t s);
 hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
 // cls: a 32-bit affine class (16 or 64 lanes); local selects local_band_core.
 hipError_t sf_launch_pass(const sf::PassArgs &a, int cls, bool local, hipStream_t s);

@@ -1,11 +1,13 @@
 // runtime.hip -- the C ABI of libsf_align.so (include/sf_align.h): device selection, batches
 // resident in HBM, shape binning, traceback scratch, per-kernel event timing, errors.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -376,17 +378,22 @@ __global__ void restrict_kernel(uint64_t n, const uint32_t *dist, const uint32_t
     }
 }
 
-// Runs one launch per bin on the main stream and the extra streams.  A bin whose work fills the
-// chip (spread >= chain) runs alone well, so those go one after another on the main stream; the
-// bins bound by their longest chain (few waves, long sequences) go beside them on the extra
-// streams, longest first, each to the stream with the least estimated time so far, so that none
-// of them holds up a big bin.  Without big bins every stream takes chain-bound ones.  The main
-// stream then waits for every stream used.
+// Runs the bins on the main stream and the extra streams.  A bin whose work fills the chip
+// (spread >= chain) runs alone well, so those go one after another on the main stream; the bins
+// bound by their longest chain (few waves, long sequences) go beside them on the extra streams,
+// longest first, each to the stream with the least estimated time so far, so that none of them
+// holds up a big bin.  With `mix`, the chain-bound bins whose classes `mixable` accepts share one
+// launch (they then run side by side instead of one after another).  Without big bins every
+// stream takes chain-bound ones.  The main stream then waits for every stream used.  Adds the
+// number of launches to *launches.
+using MixFn = std::function<hipError_t(const std::vector<size_t> &, hipStream_t)>;
 template <class F>
-int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch) {
+int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch, uint64_t *launches = nullptr,
+             const MixFn &mix = nullptr, bool (*mixable)(int) = nullptr) {
     constexpr int S = 1 + sf_batch::kAux;
     if (bins.size() <= 1) {
         for (auto &bin : bins) HIP_TRY(launch(bin, b->stream));
+        if (launches) *launches += bins.size();
         return SF_OK;
     }
     if (!b->fork) {
@@ -396,9 +403,25 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch) {
             HIP_TRY(hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming));
         }
     }
-    std::vector<size_t> big, small;
-    for (size_t i = 0; i < bins.size(); ++i) (bins[i].spread >= bins[i].chain ? big : small).push_back(i);
-    std::stable_sort(small.begin(), small.end(), [&](size_t x, size_t y) { return bins[x].est() > bins[y].est(); });
+    std::vector<size_t> big;
+    std::vector<std::vector<size_t>> small; // launch units: one bin, or the mixed bins together
+    std::vector<size_t> mixed;
+    for (size_t i = 0; i < bins.size(); ++i) {
+        if (bins[i].spread >= bins[i].chain) big.push_back(i);
+        else if (mix && mixable && mixable(bins[i].cls) && mixed.size() < MixArgs::kMax) mixed.push_back(i);
+        else small.push_back({i});
+    }
+    if (!mixed.empty()) small.push_back(mixed);
+    auto est = [&](const std::vector<size_t> &u) {
+        double c = 0, sp = 0;
+        for (size_t i : u) {
+            c = std::max(c, bins[i].chain);
+            sp += bins[i].spread;
+        }
+        return std::max(c, sp);
+    };
+    std::stable_sort(small.begin(), small.end(),
+                     [&](const std::vector<size_t> &x, const std::vector<size_t> &y) { return est(x) > est(y); });
     hipStream_t ss[S] = {b->stream, b->aux[0], b->aux[1], b->aux[2]};
     double load[S] = {0, 0, 0, 0};
     bool used[S] = {true, false, false, false};
@@ -408,7 +431,7 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch) {
         load[0] += bins[i].est();
     }
     const int first = big.empty() ? 0 : 1;
-    for (size_t i : small) {
+    for (const auto &u : small) {
         int s = first;
         for (int t = first + 1; t < S; ++t)
             if (load[t] < load[s]) s = t;
@@ -416,15 +439,93 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch) {
             HIP_TRY(hipStreamWaitEvent(ss[s], b->fork, 0));
             used[s] = true;
         }
-        HIP_TRY(launch(bins[i], ss[s]));
-        load[s] += bins[i].est();
+        if (u.size() == 1) HIP_TRY(launch(bins[u[0]], ss[s]));
+        else HIP_TRY(mix(u, ss[s]));
+        load[s] += est(u);
     }
+    if (launches) *launches += big.size() + small.size();
     for (int t = 1; t < S; ++t)
         if (used[t]) {
             HIP_TRY(hipEventRecord(b->joins[t - 1], ss[t]));
             HIP_TRY(hipStreamWaitEvent(b->stream, b->joins[t - 1], 0));
         }
     return SF_OK;
+}
+
+// Estimated VALU wave-instructions a row of one wave costs in shape class `cls`, shared by its
+// 64 / LP pairs: about 9 a slot and 15 besides for the 32-bit kernels; 3.5 a slot (two slots a
+// register) and 24 besides for the 16-bit packed ones, 30 with the 64-lane scan.
+double aff_row_cost(int cls) {
+    const double k = aff_k(cls);
+    if (!aff_packed(cls)) return 9.0 * k + 15.0;
+    return 3.5 * k + (aff_lp(cls) == 64 ? 30.0 : 24.0);
+}
+
+// Per pair, the shape an affine run uses.  A shape's cost per pair-row is its wave-row cost over
+// its pairs a wave; its chain, the time one pair's wave needs alone, is m wave-rows at about 1.5
+// times the issue interval (dependent VALU, DPP and waits); the run's spread is every pair's rows
+// at its cost per pair-row over the chip's 1024 SIMDs.  Every pair starts in its cheapest shape per
+// pair-row (8 lanes for scores where the 16-bit range holds and the policy allows; 16 up to the
+// 16-lane width limit; 16-bit packed where its range holds), and a pair whose chain would outlast
+// the spread moves to the cheapest shape whose chain does not (else 64 lanes, the shortest).  A
+// class left with too few waves to fill the chip (chain above its own spread) moves to 64 lanes
+// as a whole where that adds little to the run (its 64-lane spread within 5% of the run's): its
+// pairs then run at a quarter to an eighth of the chain, and score-only runs launch all such
+// 64-lane classes together (run_bins).  Policy 1 keeps every pair in its fewest-lanes shape.
+void choose_shapes(const sf_batch *b, const sf_scoring *sc, const std::vector<uint64_t> &ba, bool cigar,
+                   std::vector<int> &kcls) {
+    const uint64_t n = b->n;
+    const int lp8 = sf_affine_lp8_policy();
+    const uint64_t lp16max = sf_affine_lp16_max();
+    std::vector<std::array<int, 3>> cand(n);
+    auto pair_row = [](int c) { return aff_row_cost(c) * aff_lp(c) / 64.0; };
+    auto chain = [&](uint64_t i, int c) { return 1.5 * b->plen[i] * aff_row_cost(c); };
+    double spread = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t m = b->plen[i], below = (uint32_t)ba[i];
+        const uint64_t width = below + (ba[i] >> 32) + 1;
+        int c64 = sf_affine_class(width, false);
+        if (int p = sf_affine_pk16_class(aff_k(c64), sc->mat, sc->mis, sc->open, sc->ext, m, below, cigar, 64)) c64 = p;
+        int c16 = 0;
+        if (width <= lp16max && width <= 16 * 32) {
+            c16 = sf_affine_class(width, true);
+            if (int p = sf_affine_pk16_class(aff_k(c16), sc->mat, sc->mis, sc->open, sc->ext, m, below, cigar, 16))
+                c16 = p;
+        }
+        const int c8 = (!cigar && lp8 >= 0) ? sf_affine_pk8_class(width, sc->mat, sc->mis, sc->open, sc->ext, m, below) : 0;
+        cand[i] = {c8, c16, c64};
+        int pick = 0;
+        for (int c : cand[i])
+            if (c && (!pick || (lp8 <= 0 && pair_row(c) < pair_row(pick)))) pick = c;
+        kcls[i] = pick;
+        spread += m * pair_row(pick) / 1024.0;
+    }
+    if (lp8 > 0) return;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (chain(i, kcls[i]) <= spread) continue;
+        int pick = 0;
+        for (int c : cand[i])
+            if (c && chain(i, c) <= spread && (!pick || pair_row(c) < pair_row(pick))) pick = c;
+        kcls[i] = pick ? pick : cand[i][2];
+    }
+    struct Cls {
+        double chain = 0, spread = 0;
+    };
+    std::map<int, Cls> cls;
+    for (uint64_t i = 0; i < n; ++i) {
+        Cls &c = cls[kcls[i]];
+        c.chain = std::max(c.chain, b->plen[i] * aff_row_cost(kcls[i]));
+        c.spread += b->plen[i] * pair_row(kcls[i]) / 1024.0;
+    }
+    // (The move costs a class 4 to 8 times its spread: only where that stays small.)
+    std::map<int, double> spread64;
+    for (uint64_t i = 0; i < n; ++i)
+        if (aff_lp(kcls[i]) != 64) spread64[kcls[i]] += b->plen[i] * pair_row(cand[i][2]) / 1024.0;
+    for (uint64_t i = 0; i < n; ++i)
+        if (aff_lp(kcls[i]) != 64) {
+            const Cls &c = cls[kcls[i]];
+            if (c.chain > c.spread && spread64[kcls[i]] <= 0.05 * spread) kcls[i] = cand[i][2];
+        }
 }
 
 // Groups the pairs of a batch into bins of one shape class each (longer patterns first inside a
@@ -445,10 +546,7 @@ int build_plan(sf_batch *b, sf_batch::AffPlan &pl, const std::vector<uint64_t> &
         uint64_t e = q;
         while (e < n && kcls[order[e]] == kcls[order[q]]) ++e;
         sf_batch::Bin bin{kcls[order[q]], (uint32_t)q, (uint32_t)e};
-        // VALU per row of one wave, shared by its 64 / LP pairs (the 16-bit kernel: about 10
-        // packed ops per two slots).
-        const double per_row = aff_packed(bin.cls) ? 5.0 * aff_k(bin.cls) + 25.0 : 9.0 * aff_k(bin.cls) + 15.0;
-        const double ppw = 64.0 / aff_lp(bin.cls);
+        const double per_row = aff_row_cost(bin.cls), ppw = 64.0 / aff_lp(bin.cls);
         for (uint64_t x = q; x < e; ++x) {
             bin.chain = std::max(bin.chain, b->plen[order[x]] * per_row);
             bin.spread += b->plen[order[x]] * per_row / ppw / 1024.0;
@@ -692,81 +790,24 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         key ^= 0x9e3779b97f4a7c15ull;
     if (!pl.valid || pl.key != key) {
         std::vector<uint64_t> ba(n);
-        std::vector<int> kcls(n), k64(n);
+        std::vector<int> kcls(n);
         uint64_t cells = 0;
         for (uint64_t i = 0; i < n; ++i) {
             uint64_t below, above;
             const uint32_t m = b->plen[i], t = b->tlen[i];
             sf::band_geom(m, t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
             const uint64_t width = below + above + 1;
-            kcls[i] = sf_affine_k_for(width);
-            k64[i] = sf_affine_class(width, false);
-            if (k64[i] == 0)
+            if (sf_affine_class(width, false) == 0)
                 return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this build handles up to 2048",
                             (unsigned long long)i, (unsigned long long)width);
             ba[i] = below | (above << 32);
             cells += sf::band_cells(m, t, below, above); // DP_WORK (align.rs:2573-2578)
         }
-        // A 16-lane class with too few waves to fill the chip is bound by its longest chain of
-        // rows, which 64 lanes per pair run about four times shorter: per class, the shape with
-        // the smaller max(chain, spread) estimate (VALU per row as in the bins below) wins.
-        struct Cost {
-            double chain16 = 0, spread16 = 0, chain64 = 0, spread64 = 0;
-        };
-        std::map<int, Cost> c16;
-        for (uint64_t i = 0; i < n; ++i) {
-            if (aff_lp(kcls[i]) != 16) continue;
-            Cost &c = c16[kcls[i]];
-            const double m = b->plen[i], r16 = 9.0 * aff_k(kcls[i]) + 15.0, r64 = 9.0 * aff_k(k64[i]) + 15.0;
-            c.chain16 = std::max(c.chain16, m * r16);
-            c.spread16 += m * r16 / 4.0 / 1024.0;
-            c.chain64 = std::max(c.chain64, m * r64);
-            c.spread64 += m * r64 / 1024.0;
-        }
-        for (uint64_t i = 0; i < n; ++i) {
-            auto it = c16.find(kcls[i]);
-            if (it != c16.end()) {
-                const Cost &c = it->second;
-                if (std::max(c.chain64, c.spread64) < std::max(c.chain16, c.spread16)) kcls[i] = k64[i];
-            }
-            // The 16-bit kernels where their exact range holds.
-            const int kp = sf_affine_pk16_class(aff_k(kcls[i]), sc->mat, sc->mis, sc->open, sc->ext, b->plen[i],
-                                                (uint32_t)ba[i], cigar, aff_lp(kcls[i]));
-            if (kp) kcls[i] = kp;
+        choose_shapes(b, sc, ba, cigar, kcls);
+        for (uint64_t i = 0; i < n; ++i)
             // Values stay far above NEG over all slots of the class.
             if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, b->plen[i], b->tlen[i], aff_slots(kcls[i])))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
-        }
-        // Scores only: a 16-lane packed class with the three-term max moves to 8 lanes a pair
-        // (twice the registers a lane, half the waves) where that shortens its estimate, as
-        // above: VALU per wave-row about 7 per register and 24 besides.
-        const int lp8 = sf_affine_lp8_policy();
-        if (!cigar && lp8 >= 0) {
-            struct Cost8 {
-                double chain16 = 0, spread16 = 0, chain8 = 0, spread8 = 0;
-            };
-            std::map<int, Cost8> c8;
-            std::vector<int> k8(n, 0);
-            for (uint64_t i = 0; i < n; ++i) {
-                if (!aff_packed(kcls[i]) || aff_lp(kcls[i]) != 16 || kcls[i] % 1000 < 200) continue;
-                const uint64_t width = (ba[i] & 0xFFFFFFFFull) + (ba[i] >> 32) + 1;
-                k8[i] = sf_affine_pk8_class(width, sc->mat, sc->mis, sc->open, sc->ext, b->plen[i],
-                                            (uint32_t)ba[i]);
-                if (!k8[i]) continue;
-                Cost8 &c = c8[kcls[i]];
-                const double m = b->plen[i];
-                const double r16 = 3.5 * aff_k(kcls[i]) + 24.0, r8 = 3.5 * aff_k(k8[i]) + 24.0;
-                c.chain16 = std::max(c.chain16, m * r16);
-                c.spread16 += m * r16 / 4.0 / 1024.0;
-                c.chain8 = std::max(c.chain8, m * r8);
-                c.spread8 += m * r8 / 8.0 / 1024.0;
-            }
-            for (uint64_t i = 0; i < n; ++i) {
-                if (!k8[i]) continue;
-                const Cost8 &c = c8[kcls[i]];
-                if (lp8 > 0 || std::max(c.chain8, c.spread8) < std::max(c.chain16, c.spread16)) kcls[i] = k8[i];
-            }
-        }
         if ((st = build_plan(b, pl, ba, kcls, key, cells))) return st;
     }
     if (!b->d_score && ((st = dalloc(&b->d_score, n)) || (st = dalloc(&b->d_col, n)))) return st;
@@ -779,15 +820,30 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     hipEvent_t e0;
     if (!cigar) {
         if ((st = timed_begin(b, 2, &e0))) return st;
-        if ((st = run_bins(b, pl.bins, [&](const sf_batch::Bin &bin, hipStream_t s) {
-                 AffineArgs x = a;
-                 x.order = pl.d_order + bin.begin;
-                 x.desc = pl.d_desc + bin.begin;
-                 x.count = bin.end - bin.begin;
-                 return sf_launch_affine(x, bin.cls, false, s);
-             })))
-            return st;
-        b->affine_launches += pl.bins.size();
+        auto one = [&](const sf_batch::Bin &bin, hipStream_t s) {
+            AffineArgs x = a;
+            x.order = pl.d_order + bin.begin;
+            x.desc = pl.d_desc + bin.begin;
+            x.count = bin.end - bin.begin;
+            return sf_launch_affine(x, bin.cls, false, s);
+        };
+        auto mixed = [&](const std::vector<size_t> &ids, hipStream_t s) {
+            MixArgs mx{};
+            for (size_t t = 0; t < ids.size(); ++t) {
+                const sf_batch::Bin &bin = pl.bins[ids[t]];
+                mx.cls[t] = bin.cls;
+                mx.off[t] = bin.begin;
+                mx.count[t] = bin.end - bin.begin;
+                mx.blk0[t] = mx.blocks;
+                mx.blocks += (uint32_t)(((uint64_t)mx.count[t] * 64 + 255) / 256);
+            }
+            mx.nseg = (uint32_t)ids.size();
+            AffineArgs x = a;
+            x.order = pl.d_order;
+            x.desc = pl.d_desc;
+            return sf_launch_affine_mix(x, mx, s);
+        };
+        if ((st = run_bins(b, pl.bins, one, &b->affine_launches, mixed, sf_affine_mixable))) return st;
         if ((st = timed_end(b, 2, e0))) return st;
         b->affine_cells += pl.cells;
         b->cigar_valid = false;

@@ -114,9 +114,19 @@ def test_affine_parity_set_global_band64():
     check_affine(ps, A.Mode.Global, A.Affine.ONT, band)
 
 
+@pytest.fixture(params=[0, 1], ids=["planned", "fewest-lanes"])
+def lane_policy(request):
+    """Lanes per pair as the launch planner picks them (small batches: mostly 64, every pair's
+    rows outlasting the rest), or every pair in its fewest-lanes shape (8 or 16), then back."""
+    lib = A.N.load()
+    A.N.check(lib.sf_set_affine_lp8(request.param))
+    yield request.param
+    A.N.check(lib.sf_set_affine_lp8(0))
+
+
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("sc", [A.Affine.ONT, A.Affine.HIFI, A.Affine(1, 1, 0, 1), A.Affine(3, 2, 5, 1)])
-def test_affine_small_random(mode, sc):
+def test_affine_small_random(mode, sc, lane_policy):
     rng = random.Random(31 + int(mode))
     P, T, B = [], [], []
     for _ in range(300):
@@ -172,7 +182,7 @@ def test_affine_64_lane_kernel(mode, lanes64):
 
 
 @pytest.mark.parametrize("mode", MODES)
-def test_affine_scores_16bit(mode):
+def test_affine_scores_16bit(mode, lane_policy):
     # Score-only runs with mis = 2 ext (ONT, HIFI) take the 16-bit kernels where their ranges hold:
     # random pairs, all-match and gap-heavy extremes, and lengths around their limits at band 16,
     # so that the 32-bit and both 16-bit kernels run in one batch.
