@@ -15,6 +15,9 @@
 // shifts move upward), so the last block needs no mask.
 #include "internal.hpp"
 
+#include <cstdint>
+#include <type_traits>
+
 using namespace sf;
 
 namespace {
@@ -190,14 +193,20 @@ __global__ __launch_bounds__(256) void myers_lane_kernel(MyersArgs a) {
 
 // Long patterns: G lanes per pair (G = 16: four pairs per wave, one DPP row each; G = 64: one
 // pair per wave), lane l holding R consecutive 64-row blocks.  The lanes form a pipeline over the
-// text: at step t lane l works column t - l, taking the horizontal delta out of lane l - 1's last
-// block (computed one step earlier) and lane l - 1's text base, both moved by DPP row_shr:1 /
-// wave_shr:1; lane 0 takes `top` and the next text base.  Blocks are placed so that the pattern's
+// text in steps of CB columns: at step t lane l works columns CB (t - l) .. CB (t - l) + CB - 1,
+// taking each column's horizontal delta out of lane l - 1's last block (computed one step
+// earlier) and the columns' text codes, both moved by DPP row_shr:1 / wave_shr:1; lane 0 takes
+// `top` and the next codes.  A step runs its CB columns block by block (block r's column c needs
+// block r's column c - 1 and block r - 1's column c), so the fetch, the lane moves, the guards
+// and the loop control are paid once per CB columns.  Blocks are placed so that the pattern's
 // last block is the last slot of lane G - 1 (positions below off = G*R - nb hold no rows and pass
-// the delta through), so one slot per lane needs the per-pair last-row bit.
+// the deltas through), so one slot per lane needs the per-pair last-row bit.  Columns past a
+// text's end (in its last step) run on unspecified codes and are left out of the score.
 template <int G, int R>
 __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
     constexpr int SHR = G == 64 ? kWaveShr1 : kRowShr1;
+    constexpr int CB = 4;       // text columns a lane takes per step
+    constexpr int SW = 16 / CB; // steps per fetched text word
     const int lane = threadIdx.x & 63;
     const int l = lane % G;
     const uint32_t grp = (blockIdx.x * 256u + threadIdx.x) / G;
@@ -221,6 +230,7 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
     const int nb = work ? (int)((m + 63) / 64) : 0;
     const int off = G * R - nb; // >= 0: the launcher's class holds every block
     const unsigned last_bit = work ? (m - 1) & 63u : 63u;
+    const unsigned bitR = l == G - 1 ? last_bit : 63u; // row of the delta out of slot R - 1
     Blk K[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -229,46 +239,95 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
         if (work && b >= 0) load_planes(P, m, b, rev, p1, p2);
         K[r] = Blk{(uint32_t)p1, (uint32_t)(p1 >> 32), (uint32_t)p2, (uint32_t)(p2 >> 32), ~0u, ~0u, 0u, 0u};
     }
-    // Steps: the wave runs until its longest text has passed every lane.
-    int32_t steps = work ? (int32_t)n + G - 1 : 0;
+    // Steps: the wave runs until its longest text has passed every lane.  From step G on, up to
+    // the shortest text's last whole step, every lane of the wave has CB valid columns, so those
+    // steps run without guards; the first G steps (lanes starting) and the rest (texts ending)
+    // check each lane and column.
+    int32_t steps = work ? (int32_t)((n + CB - 1) / CB) + G - 1 : 0;
+    int32_t full = work ? (int32_t)(n / CB) : INT32_MAX;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) steps = imax(steps, __shfl_xor(steps, o));
-    const int32_t topv = mode == kInfix ? 0 : 1; // packed delta: bit 0 +1, bit 1 -1
+    for (int o = 32; o > 0; o >>= 1) {
+        steps = imax(steps, __shfl_xor(steps, o));
+        full = imin(full, __shfl_xor(full, o));
+    }
+    steps = __builtin_amdgcn_readfirstlane(steps); // wave-uniform: scalar loops
+    full = __builtin_amdgcn_readfirstlane(full < steps ? full : steps); // (no work in the wave: steps = 0)
+    const int32_t fill = steps < G ? steps : G;
+    const int32_t steady = full > fill ? fill + (full - fill) / SW * SW : fill;
     int32_t score = (int32_t)m, best = (int32_t)m;
-    uint32_t bestj = 0;
-    uint32_t tw = 0;
-    int32_t c = 0, hv = 0;
-    for (int32_t t = 0; t < steps; ++t) {
-        if ((t & 15) == 0 && l == 0 && work)
-            tw = rev ? reverse2(codes16(T, (int64_t)n - 16 - t)) : codes16(T, t);
-        const int32_t newc = (int32_t)(tw & 3u);
-        tw >>= 2;
-        c = __builtin_amdgcn_update_dpp(newc, c, SHR, 0xf, 0xf, false);
-        const int32_t hin = __builtin_amdgcn_update_dpp(topv, hv, SHR, 0xf, 0xf, false);
-        const int32_t j = t - l;
-        if (work && j >= 0 && j < (int32_t)n) {
-            const uint32_t M1 = (c & 1) ? 0u : ~0u;
-            const uint32_t M2 = (c & 2) ? 0u : ~0u;
-            uint32_t hp = (uint32_t)(hin & 1), hn = (uint32_t)((hin >> 1) & 1);
+    uint32_t bestj = 0; // as CB t + c + 1 (wave-uniform per column), the column j + 1 plus CB l
+    uint32_t codes = 0;
+    // hin/nin: the deltas into this lane's first block, per column of the step, moved in from the
+    // lane above by DPP into the same registers, so lane 0 of a pair keeps its `top` (+1 for
+    // Prefix and Global, 0 for Infix) and 0.  hp/hn: the deltas out of this lane's last block.
+    uint32_t hin[CB], nin[CB], hp[CB], hn[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+        hin[c] = mode == kInfix ? 0u : 1u;
+        nin[c] = 0;
+        hp[c] = hn[c] = 0;
+    }
+    // The 4 codes of the step's columns, for lane 0.
+    auto fetch4 = [&](int32_t t) -> uint32_t {
+        if (!work) return 0u;
+        return rev ? reverse2(codes16(T, (int32_t)n - 4 - CB * t)) >> 24 : codes16(T, CB * t) & 0xFFu;
+    };
+    auto step = [&](uint32_t fresh, int32_t t, auto guarded) {
+        constexpr bool GD = decltype(guarded)::value;
+        codes = (uint32_t)__builtin_amdgcn_update_dpp((int32_t)codes, (int32_t)codes, SHR, 0xf, 0xf, false);
+        codes = l == 0 ? fresh : codes;
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+            hin[c] = (uint32_t)__builtin_amdgcn_update_dpp((int32_t)hin[c], (int32_t)hp[c], SHR, 0xf, 0xf, false);
+            nin[c] = (uint32_t)__builtin_amdgcn_update_dpp((int32_t)nin[c], (int32_t)hn[c], SHR, 0xf, 0xf, false);
+        }
+        const int32_t j0 = CB * (t - l);
+        if (!GD || (work && j0 >= 0 && j0 < (int32_t)n)) {
+            // Match masks: all-ones where the column's code bit is 0 (block_step's M1/M2).
+            const uint32_t nc = ~codes;
+            uint32_t M1[CB], M2[CB], xp[CB], xn[CB];
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                M1[c] = (uint32_t)__builtin_amdgcn_sbfe((int32_t)nc, 2 * c, 1);
+                M2[c] = (uint32_t)__builtin_amdgcn_sbfe((int32_t)nc, 2 * c + 1, 1);
+                xp[c] = hin[c];
+                xn[c] = nin[c];
+            }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if (l * R + r >= off) {
-                    if (r < R - 1)
-                        block_step<true>(K[r], M1, M2, hp, hn, 63u);
-                    else
-                        block_step<false>(K[r], M1, M2, hp, hn, l == G - 1 ? last_bit : 63u);
+#pragma unroll
+                    for (int c = 0; c < CB; ++c) {
+                        if (r < R - 1) block_step<true>(K[r], M1[c], M2[c], xp[c], xn[c], 63u);
+                        else block_step<false>(K[r], M1[c], M2[c], xp[c], xn[c], bitR);
+                    }
                 }
             }
-            hv = (int32_t)(hp | (hn << 1));
-            if (l == G - 1) {
-                score += (int32_t)hp - (int32_t)hn;
-                if (score < best) {
-                    best = score;
-                    bestj = (uint32_t)j + 1;
-                }
+            // The score row (lane G - 1's last slot): the pattern's last row.
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                hp[c] = xp[c];
+                hn[c] = xn[c];
+                const int32_t nsc = score + (int32_t)xp[c] - (int32_t)xn[c];
+                const bool in = !GD || j0 + c < (int32_t)n;
+                score = in ? nsc : score;
+                const bool lower = nsc < best;
+                bestj = (in && lower) ? (uint32_t)(CB * t + c + 1) : bestj;
+                best = in ? imin(best, nsc) : best;
             }
         }
+    };
+    std::integral_constant<bool, true> guarded;
+    std::integral_constant<bool, false> bare;
+    int32_t t = 0;
+    for (; t < fill; ++t) step(fetch4(t), t, guarded);
+    for (; t < steady; t += SW) {
+        // 16 codes: the next SW steps' columns (lane 0's; the other lanes load the same word).
+        const uint32_t tw = !work ? 0u : (rev ? reverse2(codes16(T, (int32_t)n - 16 - CB * t)) : codes16(T, CB * t));
+#pragma unroll
+        for (int u = 0; u < SW; ++u) step((tw >> (2 * CB * u)) & 0xFFu, t + u, bare);
     }
+    for (; t < steps; ++t) step(fetch4(t), t, guarded);
     if (valid && !rev && m == 0) {
         const uint32_t d = mode == kGlobal ? tlen : 0;
         const bool ok = d <= a.k;
@@ -280,6 +339,7 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
         return;
     }
     if (!work || l != G - 1) return;
+    if (bestj) bestj -= CB * l;
     if (!rev) {
         if (mode == kGlobal) {
             best = score;
