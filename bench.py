@@ -59,6 +59,25 @@ VALU_INSTS_PER_CELL = {"score": 1.9089e10 / 2.46731e11, "traceback": 5.5985e10 /
 TRAFFIC_BYTES = {"score": 728_408_679, "traceback": 65_583_623_475}
 
 
+# Issue-rate probe (sf_probe_issue): waves of 32 independent VALU instructions of one encoding.
+PROBE_KINDS = {"vop2_i32": 0, "vop3p_pk16": 1, "dpp": 2, "vop3_i32": 3}
+
+
+def issue_probe(blocks=8192, iters=20000):
+    """Measured VALU issue rates on this GPU, in T lane-instructions/s, per encoding: two-operand
+    32-bit ops (VOP2), packed 16-bit ops (VOP3P, the affine kernels'), DPP moves, three-operand
+    32-bit ops (VOP3).  About 35 ms of GPU time per kind."""
+    import ctypes as C
+    from spearfish_amd import _native as N
+    lib = N.load()
+    ms = C.c_float()
+    out = {}
+    for name, kind in PROBE_KINDS.items():
+        N.check(lib.sf_probe_issue(kind, iters, blocks, C.byref(ms)))
+        out[name] = round(blocks * 4 * iters * 32 * 64 / (ms.value * 1e-3) / 1e12, 2)
+    return out
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +88,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pairs", type=int, default=0, help="CPU baseline sample size (0: auto)")
     ap.add_argument("--per-pair", type=int, default=256, help="pairs for the per-pair-launch comparison")
-    ap.add_argument("--workload", default="uniform", choices=["uniform", "uneven", "long", "serve"])
+    ap.add_argument("--workload", default="uniform", choices=["uniform", "uneven", "long", "serve", "probe"])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     return ap.parse_args()
 
@@ -170,10 +189,14 @@ def bench_uniform(args):
         ipc = VALU_INSTS_PER_CELL[phase]
         achieved = cells_per_launch * ipc * 64 / secs / 1e12
         traffic = TRAFFIC_BYTES.get(phase)
+        # The packed kernels issue VOP3P instructions: the probe's rate for that encoding is the
+        # measured ceiling beside the nominal one.
+        measured = probe["vop3p_pk16"]
         return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3),
                 "peak": round(VALU_ISSUE_PEAK_T, 1),
                 "unit": "T VALU lane-instructions/s issued (SQ_INSTS_VALU per cell x cells / launch time)",
                 "frac": round(achieved / VALU_ISSUE_PEAK_T, 4), "traffic": traffic,
+                "peak_measured_vop3p": measured, "frac_of_measured": round(achieved / measured, 4),
                 "valu_insts_per_cell": round(ipc, 5), "cells_per_launch": int(cells_per_launch),
                 "avg_launch_ms": round(per_launch_ms, 3),
                 "hbm_gbs": round(traffic / secs / 1e9, 1) if traffic else None,
@@ -181,6 +204,8 @@ def bench_uniform(args):
                                   "tops": round(cells_per_launch * ops / secs / 1e12, 3),
                                   "packed_i16_peak_tops": round(VALU16_PEAK_TOPS, 1)}}
 
+    probe = issue_probe()
+    log(f"issue probe (T lane-instructions/s): {probe}")
     wall, tm = run(False, args.steps, args.warmup, "scores")
     cells = tm.myers_cells + tm.affine_cells
     value = cells / wall / 1e9
@@ -209,6 +234,7 @@ def bench_uniform(args):
                                "affine": round(tm.affine_ms / args.steps, 2)},
         "roofline": roofline(tm, "score", "affine_pk16_kernel<8,17|18,true> (8 lanes x 34 or 36 slots a "
                              "pair, 8 pairs a wave; the score step's affine phase)", AFFINE_OPS_PER_CELL, 1),
+        "issue_probe_t": probe,
     }
     wall_c, tm_c = run(True, args.steps, 1, "with cigar")
     out["with_cigar"] = {
@@ -411,6 +437,11 @@ def main():
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "probe":
+        # The issue-rate probe alone (the denominators of the VALU rooflines).
+        print(json.dumps({"issue_probe_t_lane_instructions_per_s": issue_probe(),
+                          "nominal_one_wave_instruction_per_4_cycles": round(VALU_ISSUE_PEAK_T, 1)}))
+        return
     {"uniform": bench_uniform, "uneven": bench_uneven, "long": bench_long, "serve": bench_serve}[args.workload](args)
 
 

@@ -274,6 +274,13 @@ int sf_set_affine_lp8(int policy);
  * the DPP and shuffle encodings the kernels rely on, 5 x 64 values into `out` (320 int32). */
 int sf_selftest_lanes(int32_t *out);
 
+/* Diagnostic (no reference counterpart): the VALU issue rate the kernels' rooflines are measured
+ * against.  Runs `blocks` workgroups of 4 waves, each wave `iters` x 32 independent VALU
+ * instructions of one kind (0: two-operand 32-bit integer ops; 1: packed 16-bit ops as in the
+ * affine kernels; 2: DPP lane moves; 3: three-operand 32-bit integer ops as in the Myers step),
+ * once untimed, then once timed: *ms. */
+int sf_probe_issue(int kind, uint32_t iters, uint32_t blocks, float *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -164,6 +164,7 @@ SIGNATURES = {
     "sf_set_affine_lp16_max": ([C.c_uint64], _int),
     "sf_set_affine_lp8": ([C.c_int], _int),
     "sf_selftest_lanes": ([_vp], _int),
+    "sf_probe_issue": ([_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)], _int),
 }
 
 _lib = None

@@ -367,3 +367,15 @@ def test_myers_long_patterns_class_boundaries():
 def test_myers_long_workload(mode):
     ps, _ = W.generate(4, 24, 5)
     check_myers(ps, mode, U32)
+
+
+def test_issue_probe_runs_every_kind():
+    # The VALU issue-rate probe behind the rooflines: every kind runs and takes measurable time;
+    # an unknown kind is refused.
+    import ctypes as C
+    lib = A.N.load()
+    ms = C.c_float()
+    for kind in range(4):
+        A.N.check(lib.sf_probe_issue(kind, 200, 64, C.byref(ms)))
+        assert ms.value > 0
+    assert lib.sf_probe_issue(4, 200, 64, C.byref(ms)) != 0
