@@ -480,52 +480,76 @@ void choose_shapes(const sf_batch *b, const sf_scoring *sc, const std::vector<ui
     std::vector<std::array<int, 3>> cand(n);
     auto pair_row = [](int c) { return aff_row_cost(c) * aff_lp(c) / 64.0; };
     auto chain = [&](uint64_t i, int c) { return 1.5 * b->plen[i] * aff_row_cost(c); };
-    double spread = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-        const uint32_t m = b->plen[i], below = (uint32_t)ba[i];
-        const uint64_t width = below + (ba[i] >> 32) + 1;
-        int c64 = sf_affine_class(width, false);
-        if (int p = sf_affine_pk16_class(aff_k(c64), sc->mat, sc->mis, sc->open, sc->ext, m, below, cigar, 64)) c64 = p;
-        int c16 = 0;
-        if (width <= lp16max && width <= 16 * 32) {
-            c16 = sf_affine_class(width, true);
-            if (int p = sf_affine_pk16_class(aff_k(c16), sc->mat, sc->mis, sc->open, sc->ext, m, below, cigar, 16))
-                c16 = p;
+    parallel_for(n, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t i = lo; i < hi; ++i) {
+            const uint32_t m = b->plen[i], below = (uint32_t)ba[i];
+            const uint64_t width = below + (ba[i] >> 32) + 1;
+            int c64 = sf_affine_class(width, false);
+            if (int p = sf_affine_pk16_class(aff_k(c64), sc->mat, sc->mis, sc->open, sc->ext, m, below, cigar, 64))
+                c64 = p;
+            int c16 = 0;
+            if (width <= lp16max && width <= 16 * 32) {
+                c16 = sf_affine_class(width, true);
+                if (int p = sf_affine_pk16_class(aff_k(c16), sc->mat, sc->mis, sc->open, sc->ext, m, below, cigar, 16))
+                    c16 = p;
+            }
+            const int c8 =
+                (!cigar && lp8 >= 0) ? sf_affine_pk8_class(width, sc->mat, sc->mis, sc->open, sc->ext, m, below) : 0;
+            cand[i] = {c8, c16, c64};
+            int pick = 0;
+            for (int c : cand[i])
+                if (c && (!pick || (lp8 <= 0 && pair_row(c) < pair_row(pick)))) pick = c;
+            kcls[i] = pick;
         }
-        const int c8 = (!cigar && lp8 >= 0) ? sf_affine_pk8_class(width, sc->mat, sc->mis, sc->open, sc->ext, m, below) : 0;
-        cand[i] = {c8, c16, c64};
-        int pick = 0;
-        for (int c : cand[i])
-            if (c && (!pick || (lp8 <= 0 && pair_row(c) < pair_row(pick)))) pick = c;
-        kcls[i] = pick;
-        spread += m * pair_row(pick) / 1024.0;
-    }
+    });
     if (lp8 > 0) return;
-    for (uint64_t i = 0; i < n; ++i) {
-        if (chain(i, kcls[i]) <= spread) continue;
-        int pick = 0;
-        for (int c : cand[i])
-            if (c && chain(i, c) <= spread && (!pick || pair_row(c) < pair_row(pick))) pick = c;
-        kcls[i] = pick ? pick : cand[i][2];
-    }
-    struct Cls {
-        double chain = 0, spread = 0;
-    };
-    std::map<int, Cls> cls;
-    for (uint64_t i = 0; i < n; ++i) {
-        Cls &c = cls[kcls[i]];
-        c.chain = std::max(c.chain, b->plen[i] * aff_row_cost(kcls[i]));
-        c.spread += b->plen[i] * pair_row(kcls[i]) / 1024.0;
-    }
-    // (The move costs a class 4 to 8 times its spread: only where that stays small.)
-    std::map<int, double> spread64;
-    for (uint64_t i = 0; i < n; ++i)
-        if (aff_lp(kcls[i]) != 64) spread64[kcls[i]] += b->plen[i] * pair_row(cand[i][2]) / 1024.0;
-    for (uint64_t i = 0; i < n; ++i)
-        if (aff_lp(kcls[i]) != 64) {
-            const Cls &c = cls[kcls[i]];
-            if (c.chain > c.spread && spread64[kcls[i]] <= 0.05 * spread) kcls[i] = cand[i][2];
+    double spread = 0;
+    for (uint64_t i = 0; i < n; ++i) spread += b->plen[i] * pair_row(kcls[i]) / 1024.0;
+    parallel_for(n, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t i = lo; i < hi; ++i) {
+            if (chain(i, kcls[i]) <= spread) continue;
+            int pick = 0;
+            for (int c : cand[i])
+                if (c && chain(i, c) <= spread && (!pick || pair_row(c) < pair_row(pick))) pick = c;
+            kcls[i] = pick ? pick : cand[i][2];
         }
+    });
+    // Per class (codes are below 65536): its chain, its spread, and its spread on 64 lanes.
+    std::vector<double> cchain(65536, 0.0), cspread(65536, 0.0), cspread64(65536, 0.0);
+    for (uint64_t i = 0; i < n; ++i) {
+        const int k = kcls[i];
+        cchain[k] = std::max(cchain[k], b->plen[i] * aff_row_cost(k));
+        cspread[k] += b->plen[i] * pair_row(k) / 1024.0;
+        // (The move costs a class 4 to 8 times its spread: only where that stays small.)
+        if (aff_lp(k) != 64) cspread64[k] += b->plen[i] * pair_row(cand[i][2]) / 1024.0;
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+        const int k = kcls[i];
+        if (aff_lp(k) != 64 && cchain[k] > cspread[k] && cspread64[k] <= 0.05 * spread) kcls[i] = cand[i][2];
+    }
+}
+
+// The pairs in launch order: by shape class, longer patterns first inside a class, ties in index
+// order (a stable LSD radix sort on a 48-bit key: class code, then 2^31 - 1 - pattern length).
+std::vector<uint32_t> launch_order(const std::vector<int> &kcls, const std::vector<uint32_t> &plen) {
+    const uint64_t n = kcls.size();
+    std::vector<uint64_t> key(n);
+    for (uint64_t i = 0; i < n; ++i) key[i] = ((uint64_t)(uint32_t)kcls[i] << 31) | (0x7FFFFFFFu - (plen[i] & 0x7FFFFFFFu));
+    std::vector<uint32_t> order(n), tmp(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::vector<uint32_t> cnt(65536 + 1);
+    for (int pass = 0; pass < 3; ++pass) {
+        const int sh = 16 * pass;
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (uint64_t i = 0; i < n; ++i) ++cnt[((key[i] >> sh) & 0xFFFF) + 1];
+        for (int d = 0; d < 65536; ++d) cnt[d + 1] += cnt[d];
+        for (uint64_t q = 0; q < n; ++q) {
+            const uint32_t i = order[q];
+            tmp[cnt[(key[i] >> sh) & 0xFFFF]++] = i;
+        }
+        order.swap(tmp);
+    }
+    return order;
 }
 
 // Groups the pairs of a batch into bins of one shape class each (longer patterns first inside a
@@ -535,12 +559,7 @@ int build_plan(sf_batch *b, sf_batch::AffPlan &pl, const std::vector<uint64_t> &
                uint64_t key, uint64_t cells) {
     const uint64_t n = b->n;
     int st;
-    std::vector<uint32_t> order(n);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-        if (kcls[x] != kcls[y]) return kcls[x] < kcls[y];
-        return b->plen[x] > b->plen[y];
-    });
+    std::vector<uint32_t> order = launch_order(kcls, b->plen);
     pl.bins.clear();
     for (uint64_t q = 0; q < n;) {
         uint64_t e = q;
