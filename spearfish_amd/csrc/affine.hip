@@ -495,9 +495,10 @@ int sf_affine_k_for(uint64_t width) { return sf_affine_class(width, width <= sf_
 
 int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m, uint32_t below,
                          bool tb, int lanes) {
-    if (mis != 2 * ext || (lanes != 16 && lanes != 64)) return 0;
+    if (mis != 2 * ext || (lanes != 16 && lanes != 32 && lanes != 64)) return 0;
     const int kp = K + (K & 1); // two slots a register
     if (kp > 32) return 0; // (the 64-lane classes' slot counts, 2 .. 20, 24, 28, 32, are all instantiated)
+    if (lanes == 32 && (!tb || kp < 18)) return 0; // 32 lanes: traceback, 18 .. 32 slots a lane
     const int64_t dm = (int64_t)mat + mis;
     // The traceback kernel's offset is `open` higher (no band term saturates).
     const int64_t off = (int64_t)open * (tb ? 2 : 1) + (dm + ext) * below + 2;

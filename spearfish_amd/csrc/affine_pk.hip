@@ -1,5 +1,5 @@
-// affine_pk.hip -- instantiates and launches the 16-bit packed kernels (affine_pk.hpp) for 8 and
-// 16 lanes a pair; 64 lanes a pair are in affine_pk64.hip.
+// affine_pk.hip -- instantiates and launches the 16-bit packed kernels (affine_pk.hpp) for 8, 16
+// and (traceback) 32 lanes a pair; 64 lanes a pair are in affine_pk64.hip.
 #include "affine_pk.hpp"
 
 hipError_t sf_launch_affine_packed(const AffineArgs &a, int cls, bool tb, hipStream_t s) {
@@ -7,6 +7,19 @@ hipError_t sf_launch_affine_packed(const AffineArgs &a, int cls, bool tb, hipStr
     const int LP = aff_lp(cls);
     dim3 grid((unsigned)(((uint64_t)a.count * LP + 255) / 256)), block(256);
     if (LP == 64) return sf_launch_affine_packed64(a, cls, tb, s);
+    if (LP == 32) {
+        // 32 lanes a pair (two pairs a wave): traceback runs of bands over 512 slots.
+        if (!tb || cls % 1000 >= 200) return hipErrorInvalidValue;
+        switch (aff_k(cls)) {
+#define PK32(KK)                                                                                 \
+    case 2 * KK: hipLaunchKernelGGL((affine_pk16_tb_kernel<32, KK>), grid, block, 0, s, a); break;
+            PK32(9) PK32(10) PK32(11) PK32(12) PK32(13) PK32(14) PK32(15) PK32(16)
+#undef PK32
+        default:
+            return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (LP == 8) {
         if (tb || cls % 1000 < 200) return hipErrorInvalidValue; // 8 lanes: scores, three-term max
         switch (aff_k(cls)) {

@@ -242,16 +242,36 @@ __global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
     affine_pk16_body<LP, K2, M3>(a, blockIdx.x);
 }
 
-// Traceback variant of the packed kernel.  Same layout and bias; the offset is raised by `open`
-// so that no band cell's up-, left- or gap term saturates, and F opens from H as the reference's
-// does (the traceback reads F).  Per cell it writes the 32-bit kernel's bits nd, ne, fx, ex, each
+// A value from the next lane up (down) inside the pair's group of LP lanes, 0 past the group's
+// last (first) lane.  32 lanes: the wave shift, with the move across the two groups' boundary
+// (lanes 31 and 32) replaced by 0.
+template <int LP>
+__device__ __forceinline__ uint32_t from_up(uint32_t x) {
+    if constexpr (LP == 64) return dpp0<kWaveShl1>(x);
+    else if constexpr (LP == 32) {
+        const uint32_t y = dpp0<kWaveShl1>(x);
+        return (threadIdx.x & 31) == 31 ? 0u : y;
+    } else return dpp0<kRowShl1>(x);
+}
+
+template <int LP>
+__device__ __forceinline__ uint32_t from_down(uint32_t x) {
+    if constexpr (LP == 64) return dpp0<kWaveShr1>(x);
+    else if constexpr (LP == 32) {
+        const uint32_t y = dpp0<kWaveShr1>(x);
+        return (threadIdx.x & 31) == 0 ? 0u : y;
+    } else return dpp0<kRowShr1>(x);
+}
+
+// Traceback variant of the packed kernel (16, 32 or 64 lanes a pair).  Same layout and bias; the
+// offset is raised by `open` so that no band cell's up-, left- or gap term saturates, and F opens
+// from H as the reference's does (the traceback reads F).  Per cell it writes the 32-bit kernel's bits nd, ne, fx, ex, each
 // a saturating difference clamped to 1, in the 32-bit kernel's layout for K = 2*K2 slots a lane
 // (slot q < K2 is register q's low half, slot K2 + q register q's high half), so the same walk
 // reads them back.  Exact whenever off + dm*m + ext*slots <= 65535 (checked on the host).
 template <int LP, int K2>
 __global__ __launch_bounds__(256) void affine_pk16_tb_kernel(AffineArgs a) {
-    static_assert(LP == 16 || LP == 64, "16 or 64 lanes a pair");
-    constexpr int SHL = LP == 64 ? kWaveShl1 : kRowShl1, SHR = LP == 64 ? kWaveShr1 : kRowShr1;
+    static_assert(LP == 16 || LP == 32 || LP == 64, "16, 32 or 64 lanes a pair");
     constexpr int S = 2 * K2;
     constexpr int BLK = 16;
     constexpr int NF = K2 + BLK - 1;
@@ -329,8 +349,8 @@ __global__ __launch_bounds__(256) void affine_pk16_tb_kernel(AffineArgs a) {
             W[f] = ((wp[f / 8] >> (2 * (f % 8))) & 0x00030003u) | 0x0c000c00u;
         auto row = [&](const int r) {
             const uint32_t onehot = 1u << (__builtin_amdgcn_ubfe(pw, 2 * r, 2) << 3);
-            const uint32_t un = dpp0<SHL>(as1(Ub[0]));
-            const uint32_t fn = dpp0<SHL>(as1(Fb[0]));
+            const uint32_t un = from_up<LP>(as1(Ub[0]));
+            const uint32_t fn = from_up<LP>(as1(Fb[0]));
             const u16x2 uuK = as2(__builtin_amdgcn_alignbit(un, as1(Ub[0]), 16));
             const u16x2 fuK = as2(__builtin_amdgcn_alignbit(fn, as1(Fb[0]), 16));
             u16x2 X[K2], G[K2], D[K2], BF[K2];
@@ -364,7 +384,7 @@ __global__ __launch_bounds__(256) void affine_pk16_tb_kernel(AffineArgs a) {
             }
             // Left neighbours of register 0: slot s0 - 1 (the previous lane's last slot) in the
             // low half, slot s0 + K2 - 1 in the high half.
-            const uint32_t ulp = dpp0<SHR>(as1(Ub[K2 - 1]));
+            const uint32_t ulp = from_down<LP>(as1(Ub[K2 - 1]));
             const u16x2 ul0 = as2(__builtin_amdgcn_alignbit(as1(Ub[K2 - 1]), ulp, 16));
             u16x2 acc[NACC];
 #pragma unroll

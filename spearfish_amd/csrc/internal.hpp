@@ -206,8 +206,8 @@ __device__ __forceinline__ uint32_t wave_excl_umax(uint32_t x) {
 template <int K>
 constexpr int kTbWords = (4 * K + 31) / 32;
 
-// Exclusive max-scan of unsigned values over each group of LP lanes (16: one DPP row; 64: the
-// wave); the group's first lane gets 0.
+// Exclusive max-scan of unsigned values over each group of LP lanes (16: one DPP row; 32: two;
+// 64: the wave); the group's first lane gets 0.
 template <int LP>
 __device__ __forceinline__ uint32_t group_excl_umax(uint32_t x) {
     if constexpr (LP == 64) {
@@ -223,8 +223,19 @@ __device__ __forceinline__ uint32_t group_excl_umax(uint32_t x) {
         y = umax(y, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)y, 0x44, 0xf, 0xf, false));
         const uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)y, 0xff, 0xf, 0xf, false);
         return umax(y, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, kRowShr4, 0xf, 0xa, false));
+    } else if constexpr (LP == 32) {
+        // Two DPP rows: the rows' inclusive scans, then rows 1 and 3 take rows 0 and 2's totals
+        // (row_bcast:15 written to those rows only), then the exclusive shift, whose move across
+        // lanes 31 -> 32 (the next group) the group's first lane replaces by 0.
+        x = umax(x, dpp0<kRowShr1>(x));
+        x = umax(x, dpp0<kRowShr2>(x));
+        x = umax(x, dpp0<kRowShr4>(x));
+        x = umax(x, dpp0<kRowShr8>(x));
+        x = umax(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowBcast15, 0xa, 0xf, true));
+        const uint32_t y = dpp0<kWaveShr1>(x);
+        return (threadIdx.x & 31) ? y : 0u;
     } else {
-        static_assert(LP == 16, "groups of 8, 16 or 64 lanes");
+        static_assert(LP == 16, "groups of 8, 16, 32 or 64 lanes");
         x = umax(x, dpp0<kRowShr1>(x));
         x = umax(x, dpp0<kRowShr2>(x));
         x = umax(x, dpp0<kRowShr4>(x));

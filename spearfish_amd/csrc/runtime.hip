@@ -454,11 +454,11 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch, uint
 
 // Estimated VALU wave-instructions a row of one wave costs in shape class `cls`, shared by its
 // 64 / LP pairs: about 9 a slot and 15 besides for the 32-bit kernels; 3.5 a slot (two slots a
-// register) and 24 besides for the 16-bit packed ones, 30 with the 64-lane scan.
+// register) and 24 besides for the 16-bit packed ones, 27 on 32 lanes, 30 with the 64-lane scan.
 double aff_row_cost(int cls) {
     const double k = aff_k(cls);
     if (!aff_packed(cls)) return 9.0 * k + 15.0;
-    return 3.5 * k + (aff_lp(cls) == 64 ? 30.0 : 24.0);
+    return 3.5 * k + (aff_lp(cls) == 64 ? 30.0 : (aff_lp(cls) == 32 ? 27.0 : 24.0));
 }
 
 // Per pair, the shape an affine run uses.  A shape's cost per pair-row is its wave-row cost over
@@ -466,7 +466,8 @@ double aff_row_cost(int cls) {
 // times the issue interval (dependent VALU, DPP and waits); the run's spread is every pair's rows
 // at its cost per pair-row over the chip's 1024 SIMDs.  Every pair starts in its cheapest shape per
 // pair-row (8 lanes for scores where the 16-bit range holds and the policy allows; 16 up to the
-// 16-lane width limit; 16-bit packed where its range holds), and a pair whose chain would outlast
+// 16-lane width limit; 32 for traceback runs of wider bands up to 1,024 slots where the 16-bit
+// range holds; 16-bit packed where its range holds), and a pair whose chain would outlast
 // the spread moves to the cheapest shape whose chain does not (else 64 lanes, the shortest).  A
 // class left with too few waves to fill the chip (chain above its own spread) moves to 64 lanes
 // as a whole where that adds little to the run (its 64-lane spread within 5% of the run's): its
@@ -477,7 +478,7 @@ void choose_shapes(const sf_batch *b, const sf_scoring *sc, const std::vector<ui
     const uint64_t n = b->n;
     const int lp8 = sf_affine_lp8_policy();
     const uint64_t lp16max = sf_affine_lp16_max();
-    std::vector<std::array<int, 3>> cand(n);
+    std::vector<std::array<int, 4>> cand(n); // 8, 16, 32 and 64 lanes a pair; 0 where not allowed
     auto pair_row = [](int c) { return aff_row_cost(c) * aff_lp(c) / 64.0; };
     auto chain = [&](uint64_t i, int c) { return 1.5 * b->plen[i] * aff_row_cost(c); };
     parallel_for(n, [&](uint64_t lo, uint64_t hi) {
@@ -495,7 +496,12 @@ void choose_shapes(const sf_batch *b, const sf_scoring *sc, const std::vector<ui
             }
             const int c8 =
                 (!cigar && lp8 >= 0) ? sf_affine_pk8_class(width, sc->mat, sc->mis, sc->open, sc->ext, m, below) : 0;
-            cand[i] = {c8, c16, c64};
+            // 32 lanes: traceback runs of bands over 512 slots (2 * ceil(width / 64) slots a lane).
+            const int c32 = (cigar && width > 16 * 32 && width <= 32 * 32)
+                                ? sf_affine_pk16_class(2 * (int)((width + 63) / 64), sc->mat, sc->mis, sc->open, sc->ext,
+                                                       m, below, true, 32)
+                                : 0;
+            cand[i] = {c8, c16, c32, c64};
             int pick = 0;
             for (int c : cand[i])
                 if (c && (!pick || (lp8 <= 0 && pair_row(c) < pair_row(pick)))) pick = c;
@@ -511,7 +517,7 @@ void choose_shapes(const sf_batch *b, const sf_scoring *sc, const std::vector<ui
             int pick = 0;
             for (int c : cand[i])
                 if (c && chain(i, c) <= spread && (!pick || pair_row(c) < pair_row(pick))) pick = c;
-            kcls[i] = pick ? pick : cand[i][2];
+            kcls[i] = pick ? pick : cand[i][3];
         }
     });
     // Per class (codes are below 65536): its chain, its spread, and its spread on 64 lanes.
@@ -521,11 +527,11 @@ void choose_shapes(const sf_batch *b, const sf_scoring *sc, const std::vector<ui
         cchain[k] = std::max(cchain[k], b->plen[i] * aff_row_cost(k));
         cspread[k] += b->plen[i] * pair_row(k) / 1024.0;
         // (The move costs a class 4 to 8 times its spread: only where that stays small.)
-        if (aff_lp(k) != 64) cspread64[k] += b->plen[i] * pair_row(cand[i][2]) / 1024.0;
+        if (aff_lp(k) != 64) cspread64[k] += b->plen[i] * pair_row(cand[i][3]) / 1024.0;
     }
     for (uint64_t i = 0; i < n; ++i) {
         const int k = kcls[i];
-        if (aff_lp(k) != 64 && cchain[k] > cspread[k] && cspread64[k] <= 0.05 * spread) kcls[i] = cand[i][2];
+        if (aff_lp(k) != 64 && cchain[k] > cspread[k] && cspread64[k] <= 0.05 * spread) kcls[i] = cand[i][3];
     }
 }
 

@@ -255,6 +255,28 @@ def test_affine_scores_8_lane_kernel(mode, lanes8):
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_affine_traceback_32_lane_kernel(mode, lanes8):
+    # CIGAR runs of bands over 512 slots in their fewest-lanes shape: the 32-lane packed traceback
+    # kernel (two pairs a wave, 18 to 32 slots a lane), next to the oracle's alignment.
+    rng = random.Random(71 + int(mode))
+    P, T, B = [], [], []
+    nmax = 420 if mode == A.Mode.Infix else 1400  # Infix is unbanded: m + n + 1 slots a row
+    for _ in range(160):
+        n = rng.randrange(260 if mode == A.Mode.Infix else 0, nmax)
+        t = [rng.randrange(4) for _ in range(n)]
+        p = [x if rng.random() > 0.1 else rng.randrange(4) for x in t]
+        if rng.random() < 0.3:
+            k = rng.randrange(len(p) + 1)
+            p = p[:k] + [rng.randrange(4) for _ in range(rng.randrange(1, 30))] + p[k + rng.randrange(0, 30):]
+        P.append(p)
+        T.append(t)
+        B.append(rng.choice([256, 257, 300, 400, 500, 511]))
+    ps = A.PairSet.from_seqs(P, T)
+    for sc in (A.Affine.ONT, A.Affine.HIFI):
+        check_affine(ps, mode, sc, np.array(B, np.uint64), cigar=True)
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_affine_scores_16bit_gap_corners(mode):
     # The 16-bit kernel lets a vertical gap open only from H before the row's horizontal gaps (a
     # horizontal gap never turns straight into a vertical one), which leaves every H unchanged when
