@@ -320,6 +320,8 @@ hipError_t sf_launch_issue_probe(int kind, uint32_t iters, uint32_t blocks, uint
 
 // Supported shape classes.
 int sf_myers_maxb_for(uint32_t nblocks);     // 0 if unsupported
+// The 64-lane pipeline class with the shorter chain that holds a 32-lane class's patterns.
+int sf_myers_wide_class(int cls);
 int sf_affine_k_for(uint64_t width);         // 0 if unsupported
 
 namespace sf {

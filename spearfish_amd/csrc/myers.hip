@@ -191,8 +191,8 @@ __global__ __launch_bounds__(256) void myers_lane_kernel(MyersArgs a) {
     }
 }
 
-// Long patterns: G lanes per pair (G = 16: four pairs per wave, one DPP row each; G = 64: one
-// pair per wave), lane l holding R consecutive 64-row blocks.  The lanes form a pipeline over the
+// Long patterns: G lanes per pair (G = 16: four pairs per wave, one DPP row each; G = 32: two
+// pairs; G = 64: one pair per wave), lane l holding R consecutive 64-row blocks.  The lanes form a pipeline over the
 // text in steps of CB columns: at step t lane l works columns CB (t - l) .. CB (t - l) + CB - 1,
 // taking each column's horizontal delta out of lane l - 1's last block (computed one step
 // earlier) and the columns' text codes, both moved by DPP row_shr:1 / wave_shr:1; lane 0 takes
@@ -204,7 +204,9 @@ __global__ __launch_bounds__(256) void myers_lane_kernel(MyersArgs a) {
 // text's end (in its last step) run on unspecified codes and are left out of the score.
 template <int G, int R>
 __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
-    constexpr int SHR = G == 64 ? kWaveShr1 : kRowShr1;
+    // 16 lanes: one DPP row each; 32 and 64: the wave shift (with 32, lane 32 -- the second pair's
+    // lane 0 -- keeps its own deltas, as lane 0 does).
+    constexpr int SHR = G == 16 ? kRowShr1 : kWaveShr1;
     constexpr int CB = 4;       // text columns a lane takes per step
     constexpr int SW = 16 / CB; // steps per fetched text word
     const int lane = threadIdx.x & 63;
@@ -278,8 +280,10 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
         codes = l == 0 ? fresh : codes;
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
-            hin[c] = (uint32_t)__builtin_amdgcn_update_dpp((int32_t)hin[c], (int32_t)hp[c], SHR, 0xf, 0xf, false);
-            nin[c] = (uint32_t)__builtin_amdgcn_update_dpp((int32_t)nin[c], (int32_t)hn[c], SHR, 0xf, 0xf, false);
+            const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int32_t)hin[c], (int32_t)hp[c], SHR, 0xf, 0xf, false);
+            const uint32_t un = (uint32_t)__builtin_amdgcn_update_dpp((int32_t)nin[c], (int32_t)hn[c], SHR, 0xf, 0xf, false);
+            hin[c] = (G == 32 && l == 0) ? hin[c] : up;
+            nin[c] = (G == 32 && l == 0) ? nin[c] : un;
         }
         const int32_t j0 = CB * (t - l);
         if (!GD || (work && j0 >= 0 && j0 < (int32_t)n)) {
@@ -367,8 +371,10 @@ __global__ void probe_kernel(int32_t *out) {
 } // namespace
 
 // Shape classes: lane kernel MAXB (1..24; more planes would spill), then pipelined classes
-// encoded as 1000 * G + R: 16 lanes per pair up to 96 blocks, 64 lanes per pair beyond, which
-// keeps the per-lane chain of a long pattern (its critical path) at most 24 blocks.
+// encoded as 1000 * G + R: 16 lanes per pair up to 96 blocks, then 64 lanes per pair, which keeps
+// the per-lane chain of a long pattern (its critical path) at most 24 blocks, with 32-lane classes
+// of 160 and 224 blocks between 64 x 2, 64 x 3 and 64 x 4 (less padding for a class that fills
+// the chip; the batch moves a 32-lane bin too small for that to 64 lanes, sf_myers_wide_class).
 int sf_myers_maxb_for(uint32_t nblocks) {
     static const int lane_cls[] = {1, 2, 4, 8, 12, 16, 20, 24};
     for (int c : lane_cls)
@@ -376,11 +382,14 @@ int sf_myers_maxb_for(uint32_t nblocks) {
     static const int pipe16[] = {2, 3, 4, 5, 6};
     for (int r : pipe16)
         if (nblocks <= (uint32_t)(16 * r)) return 16000 + r;
-    static const int pipe64[] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24};
-    for (int r : pipe64)
-        if (nblocks <= (uint32_t)(64 * r)) return 64000 + r;
+    static const int pipe[] = {64002, 32005, 64003, 32007, 64004, 64005, 64006, 64008,
+                               64010, 64012, 64016, 64020, 64024};
+    for (int c : pipe)
+        if (nblocks <= (uint32_t)((c / 1000) * (c % 1000))) return c;
     return 0;
 }
+
+int sf_myers_wide_class(int cls) { return cls / 1000 == 32 ? 64000 + (cls % 1000 + 1) / 2 : cls; }
 
 hipError_t sf_launch_myers(const MyersArgs &a, int maxb, hipStream_t s) {
     if (a.count == 0) return hipSuccess;
@@ -392,7 +401,8 @@ hipError_t sf_launch_myers(const MyersArgs &a, int maxb, hipStream_t s) {
         switch (maxb) {
 #define PCASE(GG, RR) \
     case GG * 1000 + RR: hipLaunchKernelGGL((myers_pipe_kernel<GG, RR>), pg, block, 0, s, a); break;
-            PCASE(16, 2) PCASE(16, 3) PCASE(16, 4) PCASE(16, 5) PCASE(16, 6) PCASE(64, 2) PCASE(64, 3)
+            PCASE(16, 2) PCASE(16, 3) PCASE(16, 4) PCASE(16, 5) PCASE(16, 6) PCASE(32, 5) PCASE(32, 7)
+            PCASE(64, 2) PCASE(64, 3)
             PCASE(64, 4) PCASE(64, 5) PCASE(64, 6) PCASE(64, 8) PCASE(64, 10) PCASE(64, 12) PCASE(64, 16)
             PCASE(64, 20) PCASE(64, 24)
 #undef PCASE

@@ -697,13 +697,28 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
         sf_batch::Bin bin{cls, (uint32_t)q, (uint32_t)e};
         // Per wave: lane classes take 64 pairs of `cls` blocks a column; pipelined ones take
         // 64 / G pairs, R blocks a lane a column, G - 1 columns of skew.
-        const double G = cls >= 16000 ? cls / 1000 : 1, R = cls >= 16000 ? cls % 1000 : cls;
-        for (uint64_t x = q; x < e; ++x) {
-            const double cols = (double)b->tlen[order[x]] + (G - 1);
-            bin.chain = std::max(bin.chain, cols * R);
-            bin.spread += cols * R * G / 64.0 / 1024.0;
+        auto estimate = [&](sf_batch::Bin &x) {
+            const double G = x.cls >= 16000 ? x.cls / 1000 : 1, R = x.cls >= 16000 ? x.cls % 1000 : x.cls;
+            x.chain = x.spread = 0;
+            for (uint64_t y = x.begin; y < x.end; ++y) {
+                const double cols = (double)b->tlen[order[y]] + (G - 1);
+                x.chain = std::max(x.chain, cols * R);
+                x.spread += cols * R * G / 64.0 / 1024.0;
+            }
+        };
+        estimate(bin);
+        // A 32-lane bin that cannot fill the chip is bound by its chain: 64 lanes halve it.  It
+        // then joins a neighbouring bin of that class (one launch, not two in a row).
+        if (bin.chain > bin.spread && sf_myers_wide_class(cls) != cls) {
+            bin.cls = sf_myers_wide_class(cls);
+            estimate(bin);
         }
-        b->mbins.push_back(bin);
+        if (!b->mbins.empty() && b->mbins.back().cls == bin.cls && b->mbins.back().end == bin.begin) {
+            b->mbins.back().end = bin.end;
+            estimate(b->mbins.back());
+        } else {
+            b->mbins.push_back(bin);
+        }
         q = e;
     }
     for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];

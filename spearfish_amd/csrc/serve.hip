@@ -168,7 +168,7 @@ void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
     // Shape classes (as sf_batch_*): Myers by pattern blocks, affine by band width.
     std::vector<int> mcls(seg), acls(seg);
     for (uint32_t i = 0; i < seg; ++i) {
-        mcls[i] = sf_myers_maxb_for((s.h_plen[i] + 63) / 64);
+        mcls[i] = sf_myers_wide_class(sf_myers_maxb_for((s.h_plen[i] + 63) / 64)); // (small batches)
         uint64_t below, above;
         band_geom(s.h_plen[i], s.h_tlen[i], cfg.affine_mode, cfg.band, &below, &above);
         s.h_ba[i] = below | (above << 32);

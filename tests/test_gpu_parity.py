@@ -386,6 +386,27 @@ def test_myers_long_patterns_class_boundaries():
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_myers_32_lane_pipeline(mode):
+    # A bin of 129-160 (or 193-224) block patterns with pairs enough to fill the chip (2,048 at 32
+    # lanes a pair) runs the 32-lane pipeline, two pairs a wave, 5 (or 7) blocks a lane; smaller
+    # bins run 64 lanes.  Short texts keep the oracle quick.
+    rng = np.random.default_rng(83 + int(mode))
+    lo, hi = (8193, 10240) if mode != A.Mode.Prefix else (12289, 14336)
+    P, T = [], []
+    for i in range(2100):
+        m = int(rng.integers(lo, hi + 1))
+        n = int(rng.integers(0, 90))
+        t = rng.integers(0, 4, n, dtype=np.uint8)
+        p = rng.integers(0, 4, m, dtype=np.uint8)
+        if n and i % 2:
+            p[m - n:] = t  # the text matches the pattern's end: small distances too
+        P.append(p)
+        T.append(t)
+    ps = A.PairSet.from_seqs(P, T)
+    check_myers(ps, mode, U32)
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_myers_long_workload(mode):
     ps, _ = W.generate(4, 24, 5)
     check_myers(ps, mode, U32)
