@@ -270,7 +270,7 @@ __device__ __forceinline__ uint32_t from_down(uint32_t x) {
 // (slot q < K2 is register q's low half, slot K2 + q register q's high half), so the same walk
 // reads them back.  Exact whenever off + dm*m + ext*slots <= 65535 (checked on the host).
 template <int LP, int K2>
-__global__ __launch_bounds__(256) void affine_pk16_tb_kernel(AffineArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K2 == 9 ? 3 : 1))) void affine_pk16_tb_kernel(AffineArgs a) {
     static_assert(LP == 16 || LP == 32 || LP == 64, "16, 32 or 64 lanes a pair");
     constexpr int S = 2 * K2;
     constexpr int BLK = 16;
@@ -375,15 +375,13 @@ __global__ __launch_bounds__(256) void affine_pk16_tb_kernel(AffineArgs a) {
             const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
             const uint32_t C = group_excl_umax<LP>(tlo > thi ? tlo : thi);
             const u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16));
-            u16x2 EQ[K2];
-#pragma unroll
-            for (int q = 0; q < K2; ++q) {
-                EQ[q] = q ? pmax(Pre[q], E) : E; // the horizontal gap entering the slot
-                V[q] = pmax(X[q], EQ[q]);
-                Ub[q] = psub(V[q], dl[q]);
-            }
-            // Left neighbours of register 0: slot s0 - 1 (the previous lane's last slot) in the
-            // low half, slot s0 + K2 - 1 in the high half.
+            // The last register first: its new Ub holds the left neighbours of register 0 (slot
+            // s0 - 1, the previous lane's last slot, in the low half; slot s0 + K2 - 1 in the
+            // high half).  Then each register's new value and its four bits in one pass, so that
+            // D, X and the gap term of a register die with it (fewer live registers, more waves).
+            const u16x2 EQL = K2 > 1 ? pmax(Pre[K2 - 1], E) : E; // the horizontal gap entering
+            V[K2 - 1] = pmax(X[K2 - 1], EQL);
+            Ub[K2 - 1] = psub(V[K2 - 1], dl[K2 - 1]);
             const uint32_t ulp = from_down<LP>(as1(Ub[K2 - 1]));
             const u16x2 ul0 = as2(__builtin_amdgcn_alignbit(as1(Ub[K2 - 1]), ulp, 16));
             u16x2 acc[NACC];
@@ -392,9 +390,15 @@ __global__ __launch_bounds__(256) void affine_pk16_tb_kernel(AffineArgs a) {
 #pragma unroll
             for (int q = 0; q < K2; ++q) {
                 const u16x2 ul = q ? Ub[q - 1] : ul0;
+                u16x2 EQq = EQL;
+                if (q < K2 - 1) {
+                    EQq = q ? pmax(Pre[q], E) : E;
+                    V[q] = pmax(X[q], EQq);
+                    Ub[q] = psub(V[q], dl[q]);
+                }
                 const u16x2 nd = __builtin_elementwise_min(psub(V[q], D[q]), one);
                 const u16x2 ne = __builtin_elementwise_min(psub(V[q], X[q]), one);
-                const u16x2 ex = __builtin_elementwise_min(psub(EQ[q], ul), one);
+                const u16x2 ex = __builtin_elementwise_min(psub(EQq, ul), one);
                 u16x2 A = acc[q / 4];
                 A = A * two + nd;
                 A = A * two + ne;
