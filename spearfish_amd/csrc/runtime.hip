@@ -916,6 +916,39 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         if (cb < bin.end) chunks.push_back({bin.cls, cb, bin.end});
         need_max = std::max(need_max, used);
     }
+    // The last chunk's walk runs after every DP has ended.  So the largest bin's last chunk gives
+    // up about a third of itself (its shortest patterns, whose walks are the shortest too) as a
+    // piece that runs last: the walk of the rest -- about a third of its DP time -- then overlaps
+    // the piece's DP, and only the piece's short walk is left after the last DP.
+    if (!chunks.empty()) {
+        auto words = [&](int cls, uint32_t q) {
+            return (((uint64_t)b->plen[pl.order[q]] + 15) & ~15ull) * aff_tb_words_per_row(cls);
+        };
+        std::map<int, uint64_t> bin_words;
+        for (const Chunk &c : chunks)
+            for (uint32_t q = c.begin; q < c.end; ++q) bin_words[c.K] += words(c.K, q);
+        int big = chunks[0].K;
+        for (auto &e : bin_words)
+            if (e.second > bin_words[big]) big = e.first;
+        size_t li = 0;
+        for (size_t c = 0; c < chunks.size(); ++c)
+            if (chunks[c].K == big) li = c;
+        const Chunk last = chunks[li];
+        uint64_t lw = 0;
+        for (uint32_t q = last.begin; q < last.end; ++q) lw += words(last.K, q);
+        if (last.end - last.begin >= 2 && lw > budget / 4) {
+            uint32_t split = last.end;
+            uint64_t tail = 0;
+            while (split > last.begin + 1 && tail < lw / 3) tail += words(last.K, --split);
+            uint64_t u = 0;
+            for (uint32_t q = split; q < last.end; ++q) {
+                tboff[q] = u;
+                u += words(last.K, q);
+            }
+            chunks[li].end = split;
+            chunks.push_back({last.K, split, last.end});
+        }
+    }
     if (need_max > b->tb_words) {
         dfree(b->d_tb[0]);
         dfree(b->d_tb[1]);
