@@ -124,6 +124,15 @@ def lane_policy(request):
     A.N.check(lib.sf_set_affine_lp8(0))
 
 
+def test_affine_traceback_in_many_chunks(monkeypatch):
+    # Traceback bits in 16 MB chunks (SF_TB_BUDGET_MB, read at each call): the 2,048 pairs of the
+    # parity-set generator (band 64) need about ten, so the double-buffered bits, each chunk's walk
+    # beside the next chunk's DP and the largest bin's last piece run last all take part.
+    monkeypatch.setenv("SF_TB_BUDGET_MB", "16")
+    ps, band = W.generate(1, 2048, 3)
+    check_affine(ps, A.Mode.Global, A.Affine.ONT, band)
+
+
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("sc", [A.Affine.ONT, A.Affine.HIFI, A.Affine(1, 1, 0, 1), A.Affine(3, 2, 5, 1)])
 def test_affine_small_random(mode, sc, lane_policy):

@@ -34,7 +34,12 @@ def test_uneven_workload_subset():
     b.close()
 
 
-def test_long_workload_located_windows():
+@pytest.mark.parametrize("budget_mb", [None, "16"], ids=["one-chunk", "16MB-chunks"])
+def test_long_workload_located_windows(budget_mb, monkeypatch):
+    # With 16 MB of traceback bits a chunk (SF_TB_BUDGET_MB), the pairs' bits (about 40 MB) run in
+    # several double-buffered chunks, walks beside the next chunk's DP.
+    if budget_mb:
+        monkeypatch.setenv("SF_TB_BUDGET_MB", budget_mb)
     ps, bands = W.generate(4, 12, 0)
     b = A.Batch(ps)
     b.myers(A.Mode.Infix, locate=2)
