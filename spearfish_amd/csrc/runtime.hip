@@ -9,6 +9,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -340,8 +341,33 @@ inline uint32_t code_at(const void *buf, int packed, uint64_t k) {
     return (static_cast<const uint32_t *>(buf)[k >> 4] >> (2 * (k & 15))) & 3u;
 }
 
+// The low two bits of 8 bytes, compacted into 16 bits (byte k's bits at 2k).
+inline uint32_t pack8(uint64_t x) {
+    x &= 0x0303030303030303ull;
+    x = (x | (x >> 6)) & 0x000F000F000F000Full;
+    x = (x | (x >> 12)) & 0x000000FF000000FFull;
+    return (uint32_t)((x | (x >> 24)) & 0xFFFFull);
+}
+
 void pack_into(uint32_t *dst, const void *buf, int packed, uint64_t off, uint32_t len) {
     const uint64_t nw = (len + 15) / 16;
+    if (!packed) {
+        // One code a byte: 16 bases a word, 8 bytes at a time.
+        const uint8_t *src = static_cast<const uint8_t *>(buf) + off;
+        const uint64_t full = len / 16;
+        for (uint64_t w = 0; w < full; ++w) {
+            uint64_t lo, hi;
+            std::memcpy(&lo, src + 16 * w, 8);
+            std::memcpy(&hi, src + 16 * w + 8, 8);
+            dst[w] = pack8(lo) | (pack8(hi) << 16);
+        }
+        if (full < nw) {
+            uint32_t v = 0;
+            for (uint32_t k = 0; k < len - 16 * full; ++k) v |= (src[16 * full + k] & 3u) << (2 * k);
+            dst[full] = v;
+        }
+        return;
+    }
     for (uint64_t w = 0; w < nw; ++w) {
         uint32_t v = 0;
         const uint64_t k0 = w * 16;
@@ -535,12 +561,10 @@ void choose_shapes(const sf_batch *b, const sf_scoring *sc, const std::vector<ui
     }
 }
 
-// The pairs in launch order: by shape class, longer patterns first inside a class, ties in index
-// order (a stable LSD radix sort on a 48-bit key: class code, then 2^31 - 1 - pattern length).
-std::vector<uint32_t> launch_order(const std::vector<int> &kcls, const std::vector<uint32_t> &plen) {
-    const uint64_t n = kcls.size();
-    std::vector<uint64_t> key(n);
-    for (uint64_t i = 0; i < n; ++i) key[i] = ((uint64_t)(uint32_t)kcls[i] << 31) | (0x7FFFFFFFu - (plen[i] & 0x7FFFFFFFu));
+// The indices 0..n-1 ordered by a 48-bit key, ties in index order (a stable LSD radix sort, three
+// passes of 16 bits).
+std::vector<uint32_t> radix_order(const std::vector<uint64_t> &key) {
+    const uint64_t n = key.size();
     std::vector<uint32_t> order(n), tmp(n);
     std::iota(order.begin(), order.end(), 0u);
     std::vector<uint32_t> cnt(65536 + 1);
@@ -556,6 +580,15 @@ std::vector<uint32_t> launch_order(const std::vector<int> &kcls, const std::vect
         order.swap(tmp);
     }
     return order;
+}
+
+// The pairs in launch order: by shape class, longer patterns first inside a class, ties in index
+// order (key: class code, then 2^31 - 1 - pattern length).
+std::vector<uint32_t> launch_order(const std::vector<int> &kcls, const std::vector<uint32_t> &plen) {
+    const uint64_t n = kcls.size();
+    std::vector<uint64_t> key(n);
+    for (uint64_t i = 0; i < n; ++i) key[i] = ((uint64_t)(uint32_t)kcls[i] << 31) | (0x7FFFFFFFu - (plen[i] & 0x7FFFFFFFu));
+    return radix_order(key);
 }
 
 // Groups the pairs of a batch into bins of one shape class each (longer patterns first inside a
@@ -658,7 +691,11 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
         toff[i] = words;
         words += (b->tlen[i] + 15) / 16;
     }
-    std::vector<uint32_t> seq(std::max<uint64_t>(words, 1));
+    // Every word belongs to one pair's range and is written by its packing (no zero fill).
+    const uint64_t seq_words = std::max<uint64_t>(words, 1);
+    std::unique_ptr<uint32_t[]> seqbuf(new uint32_t[seq_words]);
+    uint32_t *seq = seqbuf.get();
+    seq[seq_words - 1] = 0; // (the one word of a batch without bases)
     parallel_for(n, [&](uint64_t a, uint64_t e) {
         for (uint64_t i = a; i < e; ++i) {
             pack_into(&seq[poff[i]], pairs->pat, pairs->packed, pairs->pat_off[i], b->plen[i]);
@@ -671,20 +708,19 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
     };
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(SF_EDEVICE, "hipStreamCreate failed"));
-    if ((st = dalloc(&b->d_seq, seq.size())) || (st = dalloc(&b->d_poff, n)) || (st = dalloc(&b->d_toff, n)) ||
+    if ((st = dalloc(&b->d_seq, seq_words)) || (st = dalloc(&b->d_poff, n)) || (st = dalloc(&b->d_toff, n)) ||
         (st = dalloc(&b->d_plen, n)) || (st = dalloc(&b->d_tlen, n)) || (st = dalloc(&b->d_dist, n)) ||
         (st = dalloc(&b->d_end, n)) || (st = dalloc(&b->d_start, n)) || (st = dalloc(&b->d_morder, n)) ||
         (st = dalloc(&b->d_mdesc, n)) ||
         (st = dalloc(&b->d_toff0, n)) || (st = dalloc(&b->d_tlen0, n)))
         return bail(st);
     // Myers bins: by blocks per pattern (the lane kernel's register class), then text length.
-    std::vector<uint32_t> order(n);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-        uint32_t bx = (b->plen[x] + 63) / 64, by = (b->plen[y] + 63) / 64;
-        if (bx != by) return bx < by;
-        return b->tlen[x] < b->tlen[y];
-    });
+    // Key: blocks (clamped to 2^17: patterns past 98,304 bases have no class anyway) over 30 bits
+    // of text length, 47 bits.
+    std::vector<uint64_t> mkey(n);
+    for (uint64_t i = 0; i < n; ++i)
+        mkey[i] = ((uint64_t)std::min<uint32_t>((b->plen[i] + 63) / 64, 1u << 17) << 30) | b->tlen[i];
+    std::vector<uint32_t> order = radix_order(mkey);
     for (uint64_t q = 0; q < n;) {
         int cls = sf_myers_maxb_for((b->plen[order[q]] + 63) / 64);
         uint64_t e = q;
@@ -722,7 +758,7 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
         q = e;
     }
     for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];
-    hipError_t e1 = hipMemcpy(b->d_seq, seq.data(), seq.size() * 4, hipMemcpyHostToDevice);
+    hipError_t e1 = hipMemcpy(b->d_seq, seq, seq_words * 4, hipMemcpyHostToDevice);
     // The kernels address bases: word offsets * 16.
     std::vector<uint64_t> pbase(n), tbase(n);
     for (uint64_t i = 0; i < n; ++i) {
@@ -745,11 +781,29 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
     // (align.rs:731-734); the other functions mask codes with & 3 as the reference does.
     if (!pairs->packed) {
         const uint8_t *pb = static_cast<const uint8_t *>(pairs->pat);
+        // First whether any pattern holds one: 8 codes at a time, on the host threads.
+        std::atomic<bool> any{false};
+        parallel_for(n, [&](uint64_t a, uint64_t e) {
+            bool found = false;
+            for (uint64_t i = a; i < e && !found; ++i) {
+                const uint8_t *src = pb + pairs->pat_off[i];
+                const uint32_t len = b->plen[i];
+                uint32_t k = 0;
+                for (; k + 8 <= len; k += 8) {
+                    uint64_t x;
+                    std::memcpy(&x, src + k, 8);
+                    if (x & 0xFCFCFCFCFCFCFCFCull) found = true;
+                }
+                for (; k < len; ++k)
+                    if (src[k] >= 4) found = true;
+            }
+            if (found) any.store(true, std::memory_order_relaxed);
+        });
         std::vector<uint32_t> inv;
-        for (uint64_t i = 0; i < n; ++i)
+        for (uint64_t i = 0; any.load() && i < n; ++i)
             for (uint32_t k = 0; k < b->plen[i]; ++k)
                 if (pb[pairs->pat_off[i] + k] >= 4) {
-                    if (inv.empty()) inv.assign((seq.size() * 16 + 64) / 32 + 2, 0u);
+                    if (inv.empty()) inv.assign((seq_words * 16 + 64) / 32 + 2, 0u);
                     const uint64_t bit = pbase[i] + k;
                     inv[bit >> 5] |= 1u << (bit & 31);
                 }
