@@ -68,7 +68,12 @@ struct Slot {
     std::vector<Req *> reqs;
     std::vector<uint32_t> seg0;
     uint32_t nseg = 0;
-    bool busy = false;
+    bool busy = false;   // a batch is staged or on the device (the scheduler thread's own state)
+    int fail = SF_OK;    // set when the batch could not be queued: it completes with this status
+    int status = SF_OK;  // the finished batch's outcome, once `ended`
+    bool ended = false;  // results are in the callers' arrays; not yet published under the lock
+    uint64_t cells = 0;  // DP cells of the batch (both kernels), counted when it is published
+    uint64_t launch_no = 0; // 1-based count of batches the server has taken in (never reset)
 };
 
 template <class T>
@@ -101,7 +106,7 @@ struct sf_server {
     std::condition_variable cv_sched, cv_done;
     std::deque<Req *> pending;
     std::unordered_map<uint64_t, Req *> reqs;
-    uint64_t next_ticket = 1;
+    uint64_t next_ticket = 1, launches = 0;
     bool stop = false;
     std::thread th;
     // stats
@@ -112,47 +117,65 @@ struct sf_server {
     bool any = false;
 
     void loop();
-    void launch(Slot &s, std::vector<Req *> &batch);
-    void finish(Slot &s, int status);
+    void launch(Slot &s);
+    void copy_out(Slot &s, int status);
+    void publish(Slot &s);
 };
 
-void sf_server::finish(Slot &s, int status) {
-    const auto now = Clock::now();
+// Results of a finished batch into the callers' arrays.  Runs without the lock: a request is
+// not freed before it is marked done (sf_server_wait), and its fields do not change after
+// submission.
+void sf_server::copy_out(Slot &s, int status) {
     const uint32_t *rd = s.h_res, *re = s.h_res + cap_seg, *rc = s.h_res + 3 * cap_seg;
     const int32_t *rs = reinterpret_cast<const int32_t *>(s.h_res + 2 * cap_seg);
-    for (size_t q = 0; q < s.reqs.size(); ++q) {
-        Req *r = s.reqs[q];
-        const uint32_t k0 = s.seg0[q];
-        if (status == SF_OK) {
-            for (uint32_t k = 0; k < (uint32_t)r->rlen.size(); ++k) {
-                if (r->dist) r->dist[k] = rd[k0 + k];
-                if (r->end) r->end[k] = re[k0 + k];
-                if (r->score) r->score[k] = rs[k0 + k];
-                if (r->col) r->col[k] = rc[k0 + k];
-            }
+    if (status == SF_OK) {
+        for (size_t q = 0; q < s.reqs.size(); ++q) {
+            Req *r = s.reqs[q];
+            const uint32_t k0 = s.seg0[q], n = (uint32_t)r->rlen.size();
+            if (r->dist) std::memcpy(r->dist, rd + k0, n * 4);
+            if (r->end) std::memcpy(r->end, re + k0, n * 4);
+            if (r->score) std::memcpy(r->score, rs + k0, n * 4);
+            if (r->col) std::memcpy(r->col, rc + k0, n * 4);
+        }
+    }
+    s.status = status;
+    s.ended = true;
+}
+
+// Marks a finished batch's requests done and counts it.  Called with the lock held.
+void sf_server::publish(Slot &s) {
+    const auto now = Clock::now();
+    for (Req *r : s.reqs) {
+        if (s.status == SF_OK) {
             ++reads_done;
             lat_ms.push_back(std::chrono::duration<double, std::milli>(now - r->t_submit).count());
         } else {
             ++reads_failed;
         }
-        r->status = status;
+        r->status = s.status;
         r->t_done = now;
         r->done = true;
     }
+    ++batches;
+    segments += s.nseg;
+    batch_reads += s.reqs.size();
+    cells += s.cells;
     t_last = now;
     s.reqs.clear();
     s.seg0.clear();
+    s.ended = false;
     s.busy = false;
-    cv_done.notify_all();
 }
 
-void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
+// Stages the slot's batch (s.reqs), copies it over and queues the kernels and the copy of the
+// results back on the slot's stream.  Runs without the lock.
+void sf_server::launch(Slot &s) {
     // Stage: reads, then one descriptor per segment.
     uint64_t words = 0;
     uint32_t seg = 0;
-    s.reqs = batch;
     s.seg0.clear();
-    for (Req *r : batch) {
+    s.cells = 0;
+    for (Req *r : s.reqs) {
         std::memcpy(s.h_words + words, r->words.data(), r->words.size() * 4);
         const uint64_t rbase = (store_words + s.words_base + words) * 16;
         s.seg0.push_back(seg);
@@ -178,7 +201,7 @@ void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
                                                 s.h_plen[i], (uint32_t)below, false, 16);
             if (kp) acls[i] = kp;
         }
-        cells += (uint64_t)s.h_plen[i] * s.h_tlen[i] + band_cells(s.h_plen[i], s.h_tlen[i], below, above);
+        s.cells += (uint64_t)s.h_plen[i] * s.h_tlen[i] + band_cells(s.h_plen[i], s.h_tlen[i], below, above);
     }
     std::iota(s.h_morder, s.h_morder + seg, 0u);
     std::iota(s.h_aorder, s.h_aorder + seg, 0u);
@@ -210,7 +233,7 @@ void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
     ok(sf_launch_desc(seg, s.d_aorder, pv, s.d_ba, s.d_adesc, st));
     uint32_t *dist = s.d_res, *end = s.d_res + cap_seg, *col = s.d_res + 3 * cap_seg;
     int32_t *score = reinterpret_cast<int32_t *>(s.d_res + 2 * cap_seg);
-    const bool inject = cfg.fail_batch && batches + 1 == cfg.fail_batch;
+    const bool inject = cfg.fail_batch && s.launch_no == cfg.fail_batch;
     if (!inject) {
         for (uint32_t a = 0; a < seg;) {
             uint32_t e = a;
@@ -248,60 +271,72 @@ void sf_server::launch(Slot &s, std::vector<Req *> &batch) {
     for (int q = 0; q < 4; ++q)
         ok(hipMemcpyAsync(s.h_res + q * cap_seg, s.d_res + q * cap_seg, seg * 4, hipMemcpyDeviceToHost, st));
     ok(hipEventRecord(s.ev, st));
-    ++batches;
-    segments += seg;
-    batch_reads += batch.size();
-    s.busy = true;
-    if (status != SF_OK) {
-        (void)hipStreamSynchronize(st);
-        finish(s, status);
-    }
+    s.fail = status;
 }
 
+// The scheduler.  The lock guards only the hand-over: taking waiting requests into free slots
+// and marking finished ones done.  Staging, launching and copying results out run without it,
+// so submitters and waiters never wait behind a batch's host work.
 void sf_server::loop() {
     (void)hipSetDevice(sf_runtime_device());
-    std::unique_lock<std::mutex> lk(mu);
+    std::vector<Slot *> fresh;
     for (;;) {
-        bool progress = false;
+        // Batches that ended: results out to the callers (no lock).
+        bool ended = false;
         for (Slot &s : slots) {
-            if (s.busy) {
+            if (!s.busy || s.ended || s.reqs.empty()) continue;
+            int st = SF_OK;
+            if (s.fail != SF_OK) {
+                (void)hipStreamSynchronize(s.stream);
+                st = s.fail;
+            } else {
                 const hipError_t q = hipEventQuery(s.ev);
-                if (q == hipSuccess) {
-                    finish(s, SF_OK);
-                    progress = true;
-                } else if (q != hipErrorNotReady) {
+                if (q == hipErrorNotReady) continue;
+                if (q != hipSuccess) {
                     (void)hipGetLastError();
-                    finish(s, SF_EDEVICE);
-                    progress = true;
+                    st = SF_EDEVICE;
                 }
             }
+            copy_out(s, st);
+            ended = true;
         }
-        if (!pending.empty()) {
-            for (Slot &s : slots) {
-                if (s.busy || pending.empty()) continue;
-                std::vector<Req *> batch;
-                uint64_t words = 0, seg = 0;
-                while (!pending.empty()) {
-                    Req *r = pending.front();
-                    if (!batch.empty() && (words + r->words.size() > cap_words || seg + r->rlen.size() > cap_seg ||
-                                           (cfg.max_reads_per_batch && batch.size() >= cfg.max_reads_per_batch)))
-                        break;
-                    words += r->words.size();
-                    seg += r->rlen.size();
-                    batch.push_back(r);
-                    pending.pop_front();
-                }
-                launch(s, batch);
-                progress = true;
+        std::unique_lock<std::mutex> lk(mu);
+        if (ended) {
+            for (Slot &s : slots)
+                if (s.ended) publish(s);
+            cv_done.notify_all();
+        }
+        // Free slots take the waiting requests, oldest first, while they fit the slot's share.
+        fresh.clear();
+        for (Slot &s : slots) {
+            if (s.busy || pending.empty()) continue;
+            uint64_t words = 0, seg = 0;
+            while (!pending.empty()) {
+                Req *r = pending.front();
+                if (!s.reqs.empty() && (words + r->words.size() > cap_words || seg + r->rlen.size() > cap_seg ||
+                                        (cfg.max_reads_per_batch && s.reqs.size() >= cfg.max_reads_per_batch)))
+                    break;
+                words += r->words.size();
+                seg += r->rlen.size();
+                s.reqs.push_back(r);
+                pending.pop_front();
             }
+            s.busy = true;
+            s.launch_no = ++launches;
+            fresh.push_back(&s);
         }
         bool busy = false;
         for (Slot &s : slots) busy |= s.busy;
         if (stop && pending.empty() && !busy) break;
-        if (!progress) {
-            if (busy) cv_sched.wait_for(lk, std::chrono::microseconds(20));
-            else cv_sched.wait(lk, [&] { return stop || !pending.empty(); });
+        if (fresh.empty()) {
+            if (!ended) {
+                if (busy) cv_sched.wait_for(lk, std::chrono::microseconds(20));
+                else cv_sched.wait(lk, [&] { return stop || !pending.empty(); });
+            }
+            continue;
         }
+        lk.unlock();
+        for (Slot *s : fresh) launch(*s);
     }
 }
 
@@ -384,8 +419,14 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
                                                                     : "segment beyond this build's shape classes");
         }
     }
-    r->words.assign((rq->read_len + 15) / 16, 0u);
-    for (uint32_t k = 0; k < rq->read_len; ++k) r->words[k >> 4] |= (uint32_t)(rq->read[k] & 3u) << (2 * (k & 15));
+    r->words.resize((rq->read_len + 15) / 16);
+    for (size_t w = 0; w < r->words.size(); ++w) {
+        const uint8_t *b = rq->read + 16 * w;
+        const uint32_t n = std::min<uint32_t>(16, rq->read_len - 16 * (uint32_t)w);
+        uint32_t x = 0;
+        for (uint32_t k = 0; k < n; ++k) x |= (uint32_t)(b[k] & 3u) << (2 * k);
+        r->words[w] = x;
+    }
     if (r->words.size() > sv->cap_words || r->rlen.size() > sv->cap_seg) {
         delete r;
         return sf_runtime_fail(SF_ENOMEM, "request exceeds one slot's share of the HBM budget");
