@@ -401,7 +401,9 @@ hipError_t sf_launch_myers(const MyersArgs &a, int maxb, hipStream_t s) {
         switch (maxb) {
 #define PCASE(GG, RR) \
     case GG * 1000 + RR: hipLaunchKernelGGL((myers_pipe_kernel<GG, RR>), pg, block, 0, s, a); break;
-            PCASE(16, 2) PCASE(16, 3) PCASE(16, 4) PCASE(16, 5) PCASE(16, 6) PCASE(32, 5) PCASE(32, 7)
+            // (16, 1): one block a lane, for patterns of up to 16 blocks when a segment's chain, not
+            // the issue rate, sets the time (small serving batches); sf_myers_maxb_for never picks it.
+            PCASE(16, 1) PCASE(16, 2) PCASE(16, 3) PCASE(16, 4) PCASE(16, 5) PCASE(16, 6) PCASE(32, 5) PCASE(32, 7)
             PCASE(64, 2) PCASE(64, 3)
             PCASE(64, 4) PCASE(64, 5) PCASE(64, 6) PCASE(64, 8) PCASE(64, 10) PCASE(64, 12) PCASE(64, 16)
             PCASE(64, 20) PCASE(64, 24)

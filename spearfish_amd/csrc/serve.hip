@@ -13,8 +13,9 @@
 //
 // Per segment: Myers Infix distance and end column (align.rs:92-168 semantics, k unbounded)
 // and the banded affine Global score and end column (align.rs:699-701, the server's band and
-// scoring) -- the same kernels as sf_batch_*, so a segment's results do not depend on which
-// other requests share its batch.
+// scoring) -- the kernels of sf_batch_*, each bit-exact against the same recurrences, so a
+// segment's results do not depend on which other requests share its batch (the batch's size may
+// pick another kernel for a Myers segment: the 16-lane pipeline for small batches).
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -52,21 +53,37 @@ struct Req {
 };
 
 struct Slot {
-    hipStream_t stream = nullptr;
-    hipEvent_t ev = nullptr;
+    hipStream_t stream = nullptr, aux = nullptr; // Myers and the copies; affine
+    hipEvent_t ev = nullptr, ev_fork = nullptr, ev_join = nullptr;
     uint64_t words_base = 0; // word offset of this slot's read area in the device sequence buffer
-    // device
+    // A batch's segment records, one block of 40 B a segment laid out by the batch's own segment
+    // count (poff, toff, ba: u64; plen, tlen, Myers order, affine order: u32), so one copy moves
+    // them; the same layout in pinned host memory and on the device.  Results likewise: dist,
+    // end, score, col, each the batch's segment count long, one copy back.
+    uint64_t *d_stage = nullptr, *h_stage = nullptr; // 5 x cap_seg words
+    uint32_t *d_res = nullptr, *h_res = nullptr;     // 4 x cap_seg
+    uint32_t *h_words = nullptr;                     // pinned: the batch's reads
+    PairDesc *d_mdesc = nullptr, *d_adesc = nullptr; // segments in Myers and affine launch order
+    // views into the stage blocks for the current batch (set by views())
     uint64_t *d_poff = nullptr, *d_toff = nullptr, *d_ba = nullptr;
     uint32_t *d_plen = nullptr, *d_tlen = nullptr, *d_morder = nullptr, *d_aorder = nullptr;
-    PairDesc *d_mdesc = nullptr, *d_adesc = nullptr; // segments in Myers and affine launch order
-    uint32_t *d_res = nullptr; // dist | end | score | col, 4 x cap_seg
-    // host (pinned)
-    uint32_t *h_words = nullptr;
     uint64_t *h_poff = nullptr, *h_toff = nullptr, *h_ba = nullptr;
     uint32_t *h_plen = nullptr, *h_tlen = nullptr, *h_morder = nullptr, *h_aorder = nullptr;
-    uint32_t *h_res = nullptr;
+
+    void views(uint32_t seg) {
+        auto cut = [seg](uint64_t *b, uint64_t *&poff, uint64_t *&toff, uint64_t *&ba, uint32_t *&plen,
+                         uint32_t *&tlen, uint32_t *&mo, uint32_t *&ao) {
+            poff = b, toff = b + seg, ba = b + 2 * (uint64_t)seg;
+            plen = reinterpret_cast<uint32_t *>(b + 3 * (uint64_t)seg);
+            tlen = plen + seg, mo = tlen + seg, ao = mo + seg;
+        };
+        cut(d_stage, d_poff, d_toff, d_ba, d_plen, d_tlen, d_morder, d_aorder);
+        cut(h_stage, h_poff, h_toff, h_ba, h_plen, h_tlen, h_morder, h_aorder);
+    }
     std::vector<Req *> reqs;
     std::vector<uint32_t> seg0;
+    std::vector<uint64_t> key; // launch-order sort keys (host work space)
+    std::vector<uint32_t> tmp;
     uint32_t nseg = 0;
     bool busy = false;   // a batch is staged or on the device (the scheduler thread's own state)
     int fail = SF_OK;    // set when the batch could not be queued: it completes with this status
@@ -83,6 +100,40 @@ bool dev_alloc(T **p, uint64_t count) {
 template <class T>
 bool host_alloc(T **p, uint64_t count) {
     return hipHostMalloc((void **)p, std::max<uint64_t>(count, 1) * sizeof(T), hipHostMallocDefault) == hipSuccess;
+}
+
+constexpr uint32_t kPipeBinMax = 8192; // segments of a Myers lane bin that still run the 16-lane pipeline
+constexpr uint32_t kWideAffMax = 2048; // segments of a batch whose affine runs 64 lanes a pair, in one launch
+
+// Fills order[0, n) with 0..n-1 sorted by key, ties in index order: LSD radix over 11-bit digits,
+// skipping a digit that is the same in every key (the high digits, mostly), so a batch of a few
+// segments costs microseconds and one of 10^5 about a millisecond.
+void key_order(const std::vector<uint64_t> &key, uint32_t *order, std::vector<uint32_t> &tmp) {
+    constexpr int D = 11, NB = 1 << D, P = (64 + D - 1) / D;
+    const uint32_t n = (uint32_t)key.size();
+    std::iota(order, order + n, 0u);
+    if (n < 2) return;
+    std::vector<uint32_t> cnt((size_t)P * NB, 0u);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int p = 0; p < P; ++p) ++cnt[(size_t)p * NB + ((key[i] >> (D * p)) & (NB - 1))];
+    tmp.resize(n);
+    uint32_t *src = order, *dst = tmp.data();
+    for (int p = 0; p < P; ++p) {
+        uint32_t *c = &cnt[(size_t)p * NB];
+        if (c[(key[0] >> (D * p)) & (NB - 1)] == n) continue;
+        uint32_t sum = 0;
+        for (int d = 0; d < NB; ++d) {
+            const uint32_t x = c[d];
+            c[d] = sum;
+            sum += x;
+        }
+        for (uint32_t q = 0; q < n; ++q) {
+            const uint32_t i = src[q];
+            dst[c[(key[i] >> (D * p)) & (NB - 1)]++] = i;
+        }
+        std::swap(src, dst);
+    }
+    if (src != order) std::memcpy(order, src, (size_t)n * 4);
 }
 
 double pct(std::vector<double> v, double q) {
@@ -126,8 +177,9 @@ struct sf_server {
 // not freed before it is marked done (sf_server_wait), and its fields do not change after
 // submission.
 void sf_server::copy_out(Slot &s, int status) {
-    const uint32_t *rd = s.h_res, *re = s.h_res + cap_seg, *rc = s.h_res + 3 * cap_seg;
-    const int32_t *rs = reinterpret_cast<const int32_t *>(s.h_res + 2 * cap_seg);
+    const uint32_t ns = s.nseg;
+    const uint32_t *rd = s.h_res, *re = s.h_res + ns, *rc = s.h_res + 3 * (uint64_t)ns;
+    const int32_t *rs = reinterpret_cast<const int32_t *>(s.h_res + 2 * (uint64_t)ns);
     if (status == SF_OK) {
         for (size_t q = 0; q < s.reqs.size(); ++q) {
             Req *r = s.reqs[q];
@@ -175,6 +227,9 @@ void sf_server::launch(Slot &s) {
     uint32_t seg = 0;
     s.seg0.clear();
     s.cells = 0;
+    uint32_t nseg = 0;
+    for (Req *r : s.reqs) nseg += (uint32_t)r->rlen.size();
+    s.views(nseg);
     for (Req *r : s.reqs) {
         std::memcpy(s.h_words + words, r->words.data(), r->words.size() * 4);
         const uint64_t rbase = (store_words + s.words_base + words) * 16;
@@ -190,51 +245,71 @@ void sf_server::launch(Slot &s) {
     s.nseg = seg;
     // Shape classes (as sf_batch_*): Myers by pattern blocks, affine by band width.
     std::vector<int> mcls(seg), acls(seg);
+    uint32_t lane_bin[25] = {};
+    // Up to kWideAffMax segments (a wave each: two a SIMD) the affine run is chain-bound: each
+    // pair takes its 64-lane class (a row costs 3.5 kp + 30 VALU at kp slots a lane, against
+    // 3.5 kp + 24 at 16 lanes with 4 times the slots), and the classes run side by side in one
+    // mixed launch, so the run takes its longest pair's chain rather than the sum over classes.
+    const bool wide = seg <= kWideAffMax;
     for (uint32_t i = 0; i < seg; ++i) {
         mcls[i] = sf_myers_wide_class(sf_myers_maxb_for((s.h_plen[i] + 63) / 64)); // (small batches)
+        if (mcls[i] <= 24) ++lane_bin[mcls[i]];
         uint64_t below, above;
         band_geom(s.h_plen[i], s.h_tlen[i], cfg.affine_mode, cfg.band, &below, &above);
         s.h_ba[i] = below | (above << 32);
         acls[i] = sf_affine_k_for(below + above + 1);
-        if (aff_lp(acls[i]) == 16) { // scores only: the 16-bit kernel where its range holds
+        int c64 = wide ? sf_affine_class(below + above + 1, false) : 0;
+        if (c64) {
+            const int kp = sf_affine_pk16_class(aff_k(c64), cfg.sc.mat, cfg.sc.mis, cfg.sc.open, cfg.sc.ext,
+                                                s.h_plen[i], (uint32_t)below, false, 64);
+            if (kp) c64 = kp;
+            if (!sf_affine_mixable(c64) || !scores_in_range(cfg.sc.mat, cfg.sc.mis, cfg.sc.open, cfg.sc.ext,
+                                                            s.h_plen[i], s.h_tlen[i], aff_slots(c64)))
+                c64 = 0;
+        }
+        if (c64) {
+            acls[i] = c64;
+        } else if (aff_lp(acls[i]) == 16) { // scores only: the 16-bit kernel where its range holds
             const int kp = sf_affine_pk16_class(aff_k(acls[i]), cfg.sc.mat, cfg.sc.mis, cfg.sc.open, cfg.sc.ext,
                                                 s.h_plen[i], (uint32_t)below, false, 16);
             if (kp) acls[i] = kp;
         }
         s.cells += (uint64_t)s.h_plen[i] * s.h_tlen[i] + band_cells(s.h_plen[i], s.h_tlen[i], below, above);
     }
-    std::iota(s.h_morder, s.h_morder + seg, 0u);
-    std::iota(s.h_aorder, s.h_aorder + seg, 0u);
-    // Within a class, similar lengths side by side: the lanes (Myers) and the pairs sharing a
-    // wave (affine) then run about as many columns and rows.
-    std::stable_sort(s.h_morder, s.h_morder + seg, [&](uint32_t x, uint32_t y) {
-        if (mcls[x] != mcls[y]) return mcls[x] < mcls[y];
-        return s.h_tlen[x] < s.h_tlen[y];
-    });
-    std::stable_sort(s.h_aorder, s.h_aorder + seg, [&](uint32_t x, uint32_t y) {
-        if (acls[x] != acls[y]) return acls[x] < acls[y];
-        return s.h_plen[x] > s.h_plen[y];
-    });
+    // A serving batch is small, so a segment's own chain sets its time, not the chip's issue rate.
+    // The lane kernel walks a pattern's blocks one after another in one lane (20 blocks: 20 block
+    // steps a column); the 16-lane pipeline gives each lane 1 block (patterns of up to 16 blocks)
+    // or 2 and moves 4 columns a step, a chain a fifth to a tenth as long for 3 times the
+    // lane-instructions.  A lane bin of 5-24 blocks (classes 8-24) therefore runs the pipeline
+    // while it is at most kPipeBinMax segments (2,048 waves: two a SIMD), beyond which the extra
+    // instructions cost more than the chain saves.
+    for (uint32_t i = 0; i < seg; ++i)
+        if (mcls[i] >= 8 && mcls[i] <= 24 && lane_bin[mcls[i]] <= kPipeBinMax) mcls[i] = mcls[i] <= 16 ? 16001 : 16002;
+    // Within a class, similar lengths side by side: the lanes (Myers, shorter texts first) and the
+    // pairs sharing a wave (affine, longer patterns first) then run about as many columns and rows.
+    s.key.resize(seg);
+    for (uint32_t i = 0; i < seg; ++i) s.key[i] = ((uint64_t)(uint32_t)mcls[i] << 32) | s.h_tlen[i];
+    key_order(s.key, s.h_morder, s.tmp);
+    for (uint32_t i = 0; i < seg; ++i) s.key[i] = ((uint64_t)(uint32_t)acls[i] << 32) | (0xFFFFFFFFu - s.h_plen[i]);
+    key_order(s.key, s.h_aorder, s.tmp);
     hipStream_t st = s.stream;
     int status = SF_OK;
+    // Myers on the slot's stream and affine on its second one, side by side; the results go back
+    // once both are done.
     auto ok = [&](hipError_t e) {
         if (e != hipSuccess && status == SF_OK) status = SF_EDEVICE;
     };
     ok(hipMemcpyAsync(d_seq + store_words + s.words_base, s.h_words, words * 4, hipMemcpyHostToDevice, st));
-    ok(hipMemcpyAsync(s.d_poff, s.h_poff, seg * 8, hipMemcpyHostToDevice, st));
-    ok(hipMemcpyAsync(s.d_toff, s.h_toff, seg * 8, hipMemcpyHostToDevice, st));
-    ok(hipMemcpyAsync(s.d_plen, s.h_plen, seg * 4, hipMemcpyHostToDevice, st));
-    ok(hipMemcpyAsync(s.d_tlen, s.h_tlen, seg * 4, hipMemcpyHostToDevice, st));
-    ok(hipMemcpyAsync(s.d_ba, s.h_ba, seg * 8, hipMemcpyHostToDevice, st));
-    ok(hipMemcpyAsync(s.d_morder, s.h_morder, seg * 4, hipMemcpyHostToDevice, st));
-    ok(hipMemcpyAsync(s.d_aorder, s.h_aorder, seg * 4, hipMemcpyHostToDevice, st));
+    ok(hipMemcpyAsync(s.d_stage, s.h_stage, (uint64_t)seg * 40, hipMemcpyHostToDevice, st));
     const PairsView pv{d_seq, s.d_poff, s.d_toff, s.d_plen, s.d_tlen};
     ok(sf_launch_desc(seg, s.d_morder, pv, nullptr, s.d_mdesc, st));
     ok(sf_launch_desc(seg, s.d_aorder, pv, s.d_ba, s.d_adesc, st));
-    uint32_t *dist = s.d_res, *end = s.d_res + cap_seg, *col = s.d_res + 3 * cap_seg;
-    int32_t *score = reinterpret_cast<int32_t *>(s.d_res + 2 * cap_seg);
+    uint32_t *dist = s.d_res, *end = s.d_res + seg, *col = s.d_res + 3 * (uint64_t)seg;
+    int32_t *score = reinterpret_cast<int32_t *>(s.d_res + 2 * (uint64_t)seg);
     const bool inject = cfg.fail_batch && s.launch_no == cfg.fail_batch;
     if (!inject) {
+        ok(hipEventRecord(s.ev_fork, st));
+        ok(hipStreamWaitEvent(s.aux, s.ev_fork, 0));
         for (uint32_t a = 0; a < seg;) {
             uint32_t e = a;
             while (e < seg && mcls[s.h_morder[e]] == mcls[s.h_morder[a]]) ++e;
@@ -250,26 +325,45 @@ void sf_server::launch(Slot &s) {
             ok(sf_launch_myers(ma, mcls[s.h_morder[a]], st));
             a = e;
         }
+        AffineArgs aa{};
+        aa.pv = pv;
+        aa.mode = cfg.affine_mode;
+        aa.mat = cfg.sc.mat, aa.mis = cfg.sc.mis, aa.open = cfg.sc.open, aa.ext = cfg.sc.ext;
+        aa.score = score;
+        aa.score_col = col;
+        MixArgs mx{};
         for (uint32_t a = 0; a < seg;) {
             uint32_t e = a;
-            while (e < seg && acls[s.h_aorder[e]] == acls[s.h_aorder[a]]) ++e;
-            AffineArgs aa{};
-            aa.pv = pv;
-            aa.order = s.d_aorder + a;
-            aa.desc = s.d_adesc + a;
-            aa.count = e - a;
-            aa.mode = cfg.affine_mode;
-            aa.mat = cfg.sc.mat, aa.mis = cfg.sc.mis, aa.open = cfg.sc.open, aa.ext = cfg.sc.ext;
-            aa.score = score;
-            aa.score_col = col;
-            ok(sf_launch_affine(aa, acls[s.h_aorder[a]], false, st));
+            const int c = acls[s.h_aorder[a]];
+            while (e < seg && acls[s.h_aorder[e]] == c) ++e;
+            if (wide && sf_affine_mixable(c) && mx.nseg < MixArgs::kMax) {
+                mx.cls[mx.nseg] = c;
+                mx.off[mx.nseg] = a;
+                mx.count[mx.nseg] = e - a;
+                mx.blk0[mx.nseg] = mx.blocks;
+                mx.blocks += (uint32_t)(((uint64_t)(e - a) * 64 + 255) / 256);
+                ++mx.nseg;
+            } else {
+                AffineArgs x = aa;
+                x.order = s.d_aorder + a;
+                x.desc = s.d_adesc + a;
+                x.count = e - a;
+                ok(sf_launch_affine(x, c, false, s.aux));
+            }
             a = e;
         }
+        if (mx.nseg) {
+            AffineArgs x = aa;
+            x.order = s.d_aorder;
+            x.desc = s.d_adesc;
+            ok(sf_launch_affine_mix(x, mx, s.aux));
+        }
+        ok(hipEventRecord(s.ev_join, s.aux));
+        ok(hipStreamWaitEvent(st, s.ev_join, 0));
     } else {
         status = SF_EDEVICE; // fault injected from the host: the batch is failed without a launch
     }
-    for (int q = 0; q < 4; ++q)
-        ok(hipMemcpyAsync(s.h_res + q * cap_seg, s.d_res + q * cap_seg, seg * 4, hipMemcpyDeviceToHost, st));
+    ok(hipMemcpyAsync(s.h_res, s.d_res, (uint64_t)seg * 16, hipMemcpyDeviceToHost, st));
     ok(hipEventRecord(s.ev, st));
     s.fail = status;
 }
@@ -287,6 +381,7 @@ void sf_server::loop() {
             if (!s.busy || s.ended || s.reqs.empty()) continue;
             int st = SF_OK;
             if (s.fail != SF_OK) {
+                (void)hipStreamSynchronize(s.aux);
                 (void)hipStreamSynchronize(s.stream);
                 st = s.fail;
             } else {
@@ -378,15 +473,15 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
         const uint64_t c = sv->cap_seg;
         s.words_base = q * sv->cap_words;
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess)
+            hipStreamCreateWithFlags(&s.aux, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s.ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s.ev_join, hipEventDisableTiming) != hipSuccess)
             return bail("stream");
-        if (!dev_alloc(&s.d_poff, c) || !dev_alloc(&s.d_toff, c) || !dev_alloc(&s.d_ba, c) || !dev_alloc(&s.d_plen, c) ||
-            !dev_alloc(&s.d_tlen, c) || !dev_alloc(&s.d_morder, c) || !dev_alloc(&s.d_aorder, c) ||
-            !dev_alloc(&s.d_mdesc, c) || !dev_alloc(&s.d_adesc, c) || !dev_alloc(&s.d_res, 4 * c))
+        if (!dev_alloc(&s.d_stage, 5 * c) || !dev_alloc(&s.d_mdesc, c) || !dev_alloc(&s.d_adesc, c) ||
+            !dev_alloc(&s.d_res, 4 * c))
             return bail("slot device buffers");
-        if (!host_alloc(&s.h_words, sv->cap_words) || !host_alloc(&s.h_poff, c) || !host_alloc(&s.h_toff, c) ||
-            !host_alloc(&s.h_ba, c) || !host_alloc(&s.h_plen, c) || !host_alloc(&s.h_tlen, c) ||
-            !host_alloc(&s.h_morder, c) || !host_alloc(&s.h_aorder, c) || !host_alloc(&s.h_res, 4 * c))
+        if (!host_alloc(&s.h_words, sv->cap_words) || !host_alloc(&s.h_stage, 5 * c) || !host_alloc(&s.h_res, 4 * c))
             return bail("slot pinned buffers");
     }
     sv->th = std::thread([sv] { sv->loop(); });
@@ -506,14 +601,15 @@ void sf_server_destroy(sf_server *sv) {
     if (sv->th.joinable()) sv->th.join();
     for (auto &kv : sv->reqs) delete kv.second;
     for (Slot &s : sv->slots) {
+        if (s.aux) (void)hipStreamSynchronize(s.aux);
         if (s.stream) (void)hipStreamSynchronize(s.stream);
-        (void)hipFree(s.d_poff), (void)hipFree(s.d_toff), (void)hipFree(s.d_ba), (void)hipFree(s.d_plen);
-        (void)hipFree(s.d_tlen), (void)hipFree(s.d_morder), (void)hipFree(s.d_aorder), (void)hipFree(s.d_res);
+        (void)hipFree(s.d_stage), (void)hipFree(s.d_res);
         (void)hipFree(s.d_mdesc), (void)hipFree(s.d_adesc);
-        (void)hipHostFree(s.h_words), (void)hipHostFree(s.h_poff), (void)hipHostFree(s.h_toff);
-        (void)hipHostFree(s.h_ba), (void)hipHostFree(s.h_plen), (void)hipHostFree(s.h_tlen);
-        (void)hipHostFree(s.h_morder), (void)hipHostFree(s.h_aorder), (void)hipHostFree(s.h_res);
+        (void)hipHostFree(s.h_words), (void)hipHostFree(s.h_stage), (void)hipHostFree(s.h_res);
         if (s.ev) (void)hipEventDestroy(s.ev);
+        if (s.ev_fork) (void)hipEventDestroy(s.ev_fork);
+        if (s.ev_join) (void)hipEventDestroy(s.ev_join);
+        if (s.aux) (void)hipStreamDestroy(s.aux);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
     (void)hipFree(sv->d_seq);

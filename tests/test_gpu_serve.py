@@ -81,3 +81,73 @@ def test_injected_batch_failure_fails_only_its_reads(workload):
         assert status == [N.SF_OK, N.SF_OK, N.SF_EDEVICE, N.SF_OK, N.SF_OK, N.SF_OK]
         st = s.stats()
         assert st.reads_done == 5 and st.reads_failed == 1
+
+
+def block_count_reads(store, seed):
+    """Reads whose segments have patterns of every block count from 1 to 33 (and the lengths at
+    the class edges), each against a window of the store holding a mutated copy of it."""
+    rng = np.random.default_rng(seed)
+    codes = store_codes(store)
+    lens = sorted({1, 63, 65, 255, 257, 319, 320, 321, 1535, 1537, 2047, 2048, 2049}
+                  | {64 * b for b in range(1, 34)})
+    rc, roff, rlen, nseg, sro, srl, sto, stl = [], [], [], [], [], [], [], []
+    pos = 0
+    for i in range(0, len(lens), 4):
+        group = lens[i:i + 4]
+        read = []
+        for m in group:
+            n = m + int(rng.integers(0, 60))
+            t0 = int(rng.integers(0, store.bases - n))
+            window = codes[t0:t0 + n]
+            a = int(rng.integers(0, n - m + 1))
+            p = window[a:a + m].copy()
+            flip = rng.random(m) < 0.06
+            p[flip] = (p[flip] + rng.integers(1, 4, int(flip.sum()))) % 4
+            sro.append(len(read))
+            srl.append(m)
+            sto.append(t0)
+            stl.append(n)
+            read.extend(p.tolist())
+        rc.extend(read)
+        roff.append(pos)
+        rlen.append(len(read))
+        nseg.append(len(group))
+        pos += len(read)
+    nseg = np.array(nseg, np.uint32)
+    seg0 = np.zeros(len(nseg), np.uint64)
+    seg0[1:] = np.cumsum(nseg[:-1], dtype=np.uint64)
+    return S.Reads(np.array(rc, np.uint8), np.array(roff, np.uint64), np.array(rlen, np.uint32), nseg, seg0,
+                   np.array(sro, np.uint32), np.array(srl, np.uint32), np.array(sto, np.uint64),
+                   np.array(stl, np.uint32))
+
+
+@pytest.mark.parametrize("mode", [S.Mode.Infix, S.Mode.Prefix, S.Mode.Global])
+def test_myers_every_block_count_alone_and_together(workload, mode):
+    # A small batch runs patterns of 5-16 blocks on the 16-lane pipeline at one block a lane and
+    # 17-24 at two (blocks in the last lanes, the leading lanes passing the top row through), 1-4
+    # on the lane kernel, and every affine class at 64 lanes in one mixed launch beside them:
+    # every segment's distance and end, score and score column equal the oracle's, served a read
+    # at a time and all together.
+    store, _ = workload
+    reads = block_count_reads(store, seed=5 + int(mode))
+    n = len(reads)
+    alone, mix = S.Results(reads), S.Results(reads)
+    with S.Server(store, band=BAND, myers_mode=mode) as s:
+        ra = S.requests(reads, alone)
+        for i in range(n):
+            assert s.wait(s.submit(ra[i])) == N.SF_OK
+        s.replay(S.requests(reads, mix), np.zeros(n))
+    for a in ("dist", "end", "score", "score_col"):
+        assert np.array_equal(getattr(alone, a), getattr(mix, a)), a
+    codes = store_codes(store)
+    for i in range(n):
+        s0 = int(reads.seg0[i])
+        r = reads.codes[int(reads.read_off[i]):int(reads.read_off[i]) + int(reads.read_len[i])]
+        for k in range(int(reads.nseg[i])):
+            q = s0 + k
+            p = r[int(reads.seg_read_off[q]):int(reads.seg_read_off[q]) + int(reads.seg_read_len[q])]
+            t = codes[int(reads.seg_tx_off[q]):int(reads.seg_tx_off[q]) + int(reads.seg_tx_len[q])]
+            d, e, _ = O.locate(p, t, int(mode))
+            assert (int(alone.dist[q]), int(alone.end[q])) == (d, e), (len(p), len(t))
+            dp = O.affine_dp(p, t, O.GLOBAL, O.ONT, BAND)
+            assert (int(alone.score[q]), int(alone.score_col[q])) == (dp.score, dp.score_col), (len(p), len(t))
