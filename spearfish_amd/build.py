@@ -22,8 +22,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
-HIP_SOURCES = ["myers.hip", "affine.hip", "affine_pk.hip", "affine_pk64.hip", "affine_mix.hip", "probe.hip",
-               "runtime.hip", "serve.hip"]
+HIP_SOURCES = ["myers.hip", "myers_mix.hip", "affine.hip", "affine_pk.hip", "affine_pk64.hip", "affine_mix.hip",
+               "probe.hip", "runtime.hip", "serve.hip"]
 ALIGN_LIB = os.path.join(HERE, "libsf_align.so")
 WORKLOAD_LIB = os.path.join(HERE, "libsf_workload.so")
 

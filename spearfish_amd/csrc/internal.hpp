@@ -307,6 +307,12 @@ hipError_t sf_launch_affine_packed64(const sf::AffineArgs &a, int cls, bool tb, 
 // Score-only 64-lane classes that can share one launch, and that launch.
 bool sf_affine_mixable(int cls);
 hipError_t sf_launch_affine_mix(const sf::AffineArgs &a, const sf::MixArgs &mx, hipStream_t s);
+// Myers classes sharing one launch (myers_mix.hip): the register group of a class (1-3; 0 for a
+// class with launches of its own); workgroups a class takes for `count` pairs; the launch of one
+// group's segments (as for sf_launch_affine_mix: offsets into a.order / a.desc).
+int sf_myers_mix_group(int cls);
+uint32_t sf_myers_blocks_for(int cls, uint32_t count);
+hipError_t sf_launch_myers_mix(const sf::MyersArgs &a, const sf::MixArgs &mx, int group, hipStream_t s);
 hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
 // cls: a 32-bit affine class (16 or 64 lanes); local selects local_band_core.
 hipError_t sf_launch_pass(const sf::PassArgs &a, int cls, bool local, hipStream_t s);

@@ -413,6 +413,17 @@ __global__ void restrict_kernel(uint64_t n, const uint32_t *dist, const uint32_t
 // stream takes chain-bound ones.  The main stream then waits for every stream used.  Adds the
 // number of launches to *launches.
 using MixFn = std::function<hipError_t(const std::vector<size_t> &, hipStream_t)>;
+// The extra streams (and their fork and join events), made on first use.
+int ensure_aux(sf_batch *b) {
+    if (b->fork) return SF_OK;
+    HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (int i = 0; i < sf_batch::kAux; ++i) {
+        HIP_TRY(hipStreamCreateWithFlags(&b->aux[i], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming));
+    }
+    return SF_OK;
+}
+
 template <class F>
 int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch, uint64_t *launches = nullptr,
              const MixFn &mix = nullptr, bool (*mixable)(int) = nullptr) {
@@ -422,13 +433,7 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch, uint
         if (launches) *launches += bins.size();
         return SF_OK;
     }
-    if (!b->fork) {
-        HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-        for (int i = 0; i < sf_batch::kAux; ++i) {
-            HIP_TRY(hipStreamCreateWithFlags(&b->aux[i], hipStreamNonBlocking));
-            HIP_TRY(hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming));
-        }
-    }
+    if (const int st = ensure_aux(b)) return st;
     std::vector<size_t> big;
     std::vector<std::vector<size_t>> small; // launch units: one bin, or the mixed bins together
     std::vector<size_t> mixed;
@@ -594,10 +599,20 @@ std::vector<uint32_t> launch_order(const std::vector<int> &kcls, const std::vect
 // Groups the pairs of a batch into bins of one shape class each (longer patterns first inside a
 // class), estimates each bin's time for the stream schedule, and uploads the launch order, the
 // band geometry and the launch-order pair records.
+// Waits for the batch's queued work.  Blocking uploads go on the null stream, which the batch's
+// non-blocking streams do not order with, so one into a buffer that queued kernels may still read
+// (a plan being replaced, the traceback offsets) waits first.  The main stream covers the aux and
+// walk streams: every run joins them back before it returns.
+int quiesce(sf_batch *b) {
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return SF_OK;
+}
+
 int build_plan(sf_batch *b, sf_batch::AffPlan &pl, const std::vector<uint64_t> &ba, const std::vector<int> &kcls,
                uint64_t key, uint64_t cells) {
     const uint64_t n = b->n;
     int st;
+    if (pl.d_ba && (st = quiesce(b))) return st; // the plan being replaced may still be in use
     std::vector<uint32_t> order = launch_order(kcls, b->plen);
     pl.bins.clear();
     for (uint64_t q = 0; q < n;) {
@@ -818,6 +833,88 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
     return SF_OK;
 }
 
+// Runs every Myers bin.  The bins whose classes share a register group (myers_mix.hip) run as one
+// launch per group, the longest chain's workgroups first, so the chip refills as waves end instead
+// of draining at each bin's tail; the launches (and bins of classes outside the groups, or alone
+// in theirs, on their own kernels) go to up to three streams side by side, the most work on the
+// main stream.  Three: a fourth stream shares a hardware queue with another (four queues a
+// process, one taken) and would only serialise behind it.  A single bin is one plain launch.
+int run_myers(sf_batch *b, const MyersArgs &a) {
+    const std::vector<sf_batch::Bin> &bins = b->mbins;
+    auto one = [&](const sf_batch::Bin &bin, hipStream_t s) {
+        MyersArgs x = a;
+        x.order = b->d_morder + bin.begin;
+        x.desc = b->d_mdesc + bin.begin;
+        x.count = bin.end - bin.begin;
+        return sf_launch_myers(x, bin.cls, s);
+    };
+    if (bins.size() <= 1) {
+        for (const auto &bin : bins) HIP_TRY(one(bin, b->stream));
+        return SF_OK;
+    }
+    struct Unit {
+        int group;
+        std::vector<size_t> ids;
+        double est;
+    };
+    std::vector<Unit> units;
+    std::vector<size_t> grp[4];
+    for (size_t i = 0; i < bins.size(); ++i) grp[sf_myers_mix_group(bins[i].cls)].push_back(i);
+    for (int g = 1; g <= 3; ++g)
+        if (!grp[g].empty()) {
+            Unit u{grp[g].size() > 1 ? g : 0, grp[g], 0};
+            double chain = 0, spread = 0;
+            for (size_t i : u.ids) chain = std::max(chain, bins[i].chain), spread += bins[i].spread;
+            u.est = std::max(chain, spread);
+            units.push_back(std::move(u));
+        }
+    for (size_t i : grp[0]) units.push_back(Unit{0, {i}, bins[i].est()});
+    std::stable_sort(units.begin(), units.end(), [](const Unit &x, const Unit &y) { return x.est > y.est; });
+    if (const int st = ensure_aux(b)) return st;
+    constexpr int S = 3;
+    hipStream_t ss[S] = {b->stream, b->aux[0], b->aux[1]};
+    double load[S] = {0, 0, 0};
+    bool used[S] = {true, false, false};
+    HIP_TRY(hipEventRecord(b->fork, b->stream));
+    for (const Unit &u : units) {
+        int s = 0;
+        for (int t = 1; t < S; ++t)
+            if (load[t] < load[s]) s = t;
+        if (!used[s]) {
+            HIP_TRY(hipStreamWaitEvent(ss[s], b->fork, 0));
+            used[s] = true;
+        }
+        if (u.group == 0) {
+            HIP_TRY(one(bins[u.ids[0]], ss[s]));
+        } else {
+            std::vector<size_t> o(u.ids);
+            std::stable_sort(o.begin(), o.end(), [&](size_t x, size_t y) { return bins[x].chain > bins[y].chain; });
+            MixArgs mx{};
+            for (size_t i : o) {
+                if (mx.nseg == MixArgs::kMax) return fail(SF_EDEVICE, "more Myers bins than a mixed launch holds");
+                const sf_batch::Bin &bin = bins[i];
+                mx.cls[mx.nseg] = bin.cls;
+                mx.off[mx.nseg] = bin.begin;
+                mx.count[mx.nseg] = bin.end - bin.begin;
+                mx.blk0[mx.nseg] = mx.blocks;
+                mx.blocks += sf_myers_blocks_for(bin.cls, bin.end - bin.begin);
+                ++mx.nseg;
+            }
+            MyersArgs x = a;
+            x.order = b->d_morder;
+            x.desc = b->d_mdesc;
+            HIP_TRY(sf_launch_myers_mix(x, mx, u.group, ss[s]));
+        }
+        load[s] += u.est;
+    }
+    for (int t = 1; t < S; ++t)
+        if (used[t]) {
+            HIP_TRY(hipEventRecord(b->joins[t - 1], ss[t]));
+            HIP_TRY(hipStreamWaitEvent(b->stream, b->joins[t - 1], 0));
+        }
+    return SF_OK;
+}
+
 int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
     if (!b) return fail(SF_EINVAL, "batch is null");
     if (mode < SF_INFIX || mode > SF_GLOBAL) return fail(SF_EINVAL, "mode %d", mode);
@@ -835,28 +932,14 @@ int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
     hipEvent_t e0;
     if ((st = timed_begin(b, 0, &e0))) return st;
     a.reverse = 0;
-    if ((st = run_bins(b, b->mbins, [&](const sf_batch::Bin &bin, hipStream_t s) {
-             MyersArgs x = a;
-             x.order = b->d_morder + bin.begin;
-             x.desc = b->d_mdesc + bin.begin;
-             x.count = bin.end - bin.begin;
-             return sf_launch_myers(x, bin.cls, s);
-         })))
-        return st;
+    if ((st = run_myers(b, a))) return st;
     if ((st = timed_end(b, 0, e0))) return st;
     b->myers_cells += b->myers_cells_per_run;
     b->located_ready = locate == 2;
     if (locate == 2 && mode == SF_INFIX) {
         if ((st = timed_begin(b, 1, &e0))) return st;
         a.reverse = 1;
-        if ((st = run_bins(b, b->mbins, [&](const sf_batch::Bin &bin, hipStream_t s) {
-                 MyersArgs x = a;
-                 x.order = b->d_morder + bin.begin;
-                 x.desc = b->d_mdesc + bin.begin;
-                 x.count = bin.end - bin.begin;
-                 return sf_launch_myers(x, bin.cls, s);
-             })))
-            return st;
+        if ((st = run_myers(b, a))) return st;
         if ((st = timed_end(b, 1, e0))) return st;
     }
     return SF_OK;
@@ -1018,6 +1101,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         for (hipEvent_t *e : {&b->dp_done[0], &b->dp_done[1], &b->walk_done[0], &b->walk_done[1], &b->walks_end})
             HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
+    if (b->d_tboff && (st = quiesce(b))) return st; // the last traceback run may still read the offsets
     if (!b->d_tboff) {
         if ((st = dalloc(&b->d_tboff, n)) || (st = dalloc(&b->d_tstart, n)) || (st = dalloc(&b->d_nruns, n)) ||
             (st = dalloc(&b->d_runs_off, n)) || (st = dalloc(&b->d_out_off, n + 1)) ||
