@@ -257,9 +257,11 @@ struct sf_batch {
     uint64_t tb_words = 0;
     hipStream_t wstream = nullptr;           // traceback walks, overlapping the next chunk's DP
     // Extra streams on which independent bins run beside the main stream's.
-    static constexpr int kAux = 3;
-    hipStream_t aux[kAux] = {nullptr, nullptr, nullptr};
-    hipEvent_t fork = nullptr, joins[kAux] = {nullptr, nullptr, nullptr};
+    // Two: with the main stream that is three, one a hardware queue (a process gets four and one is
+    // taken); a third would share a queue with another and only serialise behind it.
+    static constexpr int kAux = 2;
+    hipStream_t aux[kAux] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, joins[kAux] = {nullptr, nullptr};
     hipEvent_t dp_done[2] = {nullptr, nullptr}, walk_done[2] = {nullptr, nullptr}, walks_end = nullptr;
     bool cigar_valid = false;
     // timing
@@ -307,8 +309,7 @@ struct sf_batch {
         dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0);
         for (hipStream_t x : aux)
             if (x) (void)hipStreamSynchronize(x);
-        for (hipEvent_t e : {dp_done[0], dp_done[1], walk_done[0], walk_done[1], walks_end, fork, joins[0], joins[1],
-                             joins[2]})
+        for (hipEvent_t e : {dp_done[0], dp_done[1], walk_done[0], walk_done[1], walks_end, fork, joins[0], joins[1]})
             if (e) (void)hipEventDestroy(e);
         for (hipStream_t x : aux)
             if (x) (void)hipStreamDestroy(x);
@@ -453,9 +454,9 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch, uint
     };
     std::stable_sort(small.begin(), small.end(),
                      [&](const std::vector<size_t> &x, const std::vector<size_t> &y) { return est(x) > est(y); });
-    hipStream_t ss[S] = {b->stream, b->aux[0], b->aux[1], b->aux[2]};
-    double load[S] = {0, 0, 0, 0};
-    bool used[S] = {true, false, false, false};
+    hipStream_t ss[S] = {b->stream, b->aux[0], b->aux[1]};
+    double load[S] = {0, 0, 0};
+    bool used[S] = {true, false, false};
     HIP_TRY(hipEventRecord(b->fork, b->stream));
     for (size_t i : big) {
         HIP_TRY(launch(bins[i], b->stream));
@@ -836,9 +837,8 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
 // Runs every Myers bin.  The bins whose classes share a register group (myers_mix.hip) run as one
 // launch per group, the longest chain's workgroups first, so the chip refills as waves end instead
 // of draining at each bin's tail; the launches (and bins of classes outside the groups, or alone
-// in theirs, on their own kernels) go to up to three streams side by side, the most work on the
-// main stream.  Three: a fourth stream shares a hardware queue with another (four queues a
-// process, one taken) and would only serialise behind it.  A single bin is one plain launch.
+// in theirs, on their own kernels) go to the main and the two extra streams side by side, the most
+// work on the main stream.  A single bin is one plain launch.
 int run_myers(sf_batch *b, const MyersArgs &a) {
     const std::vector<sf_batch::Bin> &bins = b->mbins;
     auto one = [&](const sf_batch::Bin &bin, hipStream_t s) {
@@ -871,7 +871,7 @@ int run_myers(sf_batch *b, const MyersArgs &a) {
     for (size_t i : grp[0]) units.push_back(Unit{0, {i}, bins[i].est()});
     std::stable_sort(units.begin(), units.end(), [](const Unit &x, const Unit &y) { return x.est > y.est; });
     if (const int st = ensure_aux(b)) return st;
-    constexpr int S = 3;
+    constexpr int S = 1 + sf_batch::kAux;
     hipStream_t ss[S] = {b->stream, b->aux[0], b->aux[1]};
     double load[S] = {0, 0, 0};
     bool used[S] = {true, false, false};
