@@ -1,4 +1,6 @@
-// affine_mix.hip -- one launch for several shape classes of a score-only run on 64 lanes a pair.
+// affine_mix.hip -- one launch for several shape classes of a score-only run: the 64-lane classes
+// and the widest 16-bit packed 8- and 16-lane ones (two waves a SIMD); affine_mix1.hip holds the
+// narrower 8- and 16-lane classes.
 //
 // Pairs whose rows are long run one pair a wave, and a class holding only a few of them has too few
 // waves to fill the chip: its time is its longest pair's chain of rows.  Launched one class after
@@ -28,8 +30,16 @@ __global__ __launch_bounds__(256) void affine_mix_kernel(AffineArgs a, MixArgs m
     case 64200 + 2 * KK: affine_pk16_body<64, KK, true>(x, blk); break;
         P(1) P(2) P(3) P(4) P(5) P(6) P(7) P(8) P(9) P(10) P(12) P(14) P(16)
 #undef P
+#define P8(KK)                                                                                   \
+    case 8200 + 2 * KK: affine_pk16_body<8, KK, true>(x, blk); break;
+#define P16(KK)                                                                                  \
+    case 16100 + 2 * KK: affine_pk16_body<16, KK, false>(x, blk); break;                          \
+    case 16200 + 2 * KK: affine_pk16_body<16, KK, true>(x, blk); break;
+        P8(14) P8(15) P8(16) P8(17) P8(18) P16(14) P16(15) P16(16)
+#undef P8
+#undef P16
     default:
-        break; // the host launches only the classes above (sf_affine_mixable)
+        break; // the host launches only the classes above (sf_affine_mix_group)
     }
 }
 
@@ -44,6 +54,42 @@ bool sf_affine_mixable(int cls) {
     if (aff_lp(cls) != 64 || !aff_packed(cls)) return false;
     const int kp = aff_k(cls);
     return kp >= 2 && kp % 2 == 0 && (kp <= 20 || kp == 24 || kp == 28 || kp == 32);
+}
+
+// Register groups of the score classes that share a launch: 1 for the packed 8- and 16-lane
+// classes of at most 13 registers of slot pairs a lane (at most 168 VGPRs: three waves a SIMD;
+// affine_mix1.hip), 2 for the wider ones and the 64-lane classes (affine_mix_kernel, two waves),
+// 0 for classes that launch alone (the 32-bit 16-lane kernels).
+int sf_affine_mix_group(int cls) {
+    if (!aff_packed(cls)) return sf_affine_mixable(cls) ? 2 : 0;
+    const int lp = aff_lp(cls), kp = aff_k(cls), k2 = kp / 2;
+    const bool m3 = cls % 1000 >= 200;
+    if (kp % 2 || k2 < 1) return 0;
+    if (lp == 8) return !m3 ? 0 : (k2 <= 13 ? 1 : (k2 <= 18 ? 2 : 0));
+    if (lp == 16) return k2 <= 13 ? 1 : (k2 <= 16 ? 2 : 0);
+    return sf_affine_mixable(cls) ? 2 : 0;
+}
+
+uint32_t sf_affine_blocks_for(int cls, uint32_t count) {
+    return (uint32_t)(((uint64_t)count * aff_lp(cls) + 255) / 256);
+}
+
+static bool mix_segments_ok(const MixArgs &mx, int group) {
+    if (mx.nseg == 0 || mx.nseg > MixArgs::kMax) return false;
+    for (uint32_t i = 0; i < mx.nseg; ++i)
+        if (sf_affine_mix_group(mx.cls[i]) != group ||
+            (uint64_t)mx.blk0[i] + sf_affine_blocks_for(mx.cls[i], mx.count[i]) !=
+                (i + 1 < mx.nseg ? mx.blk0[i + 1] : mx.blocks))
+            return false;
+    return true;
+}
+
+hipError_t sf_launch_affine_mix_group(const AffineArgs &a, const MixArgs &mx, int group, hipStream_t s) {
+    if ((group != 1 && group != 2) || !mix_segments_ok(mx, group)) return hipErrorInvalidValue;
+    if (mx.blocks == 0) return hipSuccess;
+    if (group == 1) return sf_launch_affine_mix1(a, mx, s);
+    hipLaunchKernelGGL(affine_mix_kernel, dim3(mx.blocks), dim3(256), 0, s, a, mx);
+    return hipGetLastError();
 }
 
 hipError_t sf_launch_affine_mix(const AffineArgs &a, const MixArgs &mx, hipStream_t s) {

@@ -307,6 +307,13 @@ hipError_t sf_launch_affine_packed64(const sf::AffineArgs &a, int cls, bool tb, 
 // Score-only 64-lane classes that can share one launch, and that launch.
 bool sf_affine_mixable(int cls);
 hipError_t sf_launch_affine_mix(const sf::AffineArgs &a, const sf::MixArgs &mx, hipStream_t s);
+// Score classes sharing one launch by register group (affine_mix.hip, affine_mix1.hip): the group
+// of a class (1-2; 0 for a class with launches of its own); workgroups a class takes for `count`
+// pairs; the launch of one group's segments.
+int sf_affine_mix_group(int cls);
+uint32_t sf_affine_blocks_for(int cls, uint32_t count);
+hipError_t sf_launch_affine_mix_group(const sf::AffineArgs &a, const sf::MixArgs &mx, int group, hipStream_t s);
+hipError_t sf_launch_affine_mix1(const sf::AffineArgs &a, const sf::MixArgs &mx, hipStream_t s);
 // Myers classes sharing one launch (myers_mix.hip): the register group of a class (1-3; 0 for a
 // class with launches of its own); workgroups a class takes for `count` pairs; the launch of one
 // group's segments (as for sf_launch_affine_mix: offsets into a.order / a.desc).

@@ -484,6 +484,111 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch, uint
     return SF_OK;
 }
 
+// Runs a run's bins grouped by register footprint.  The bins whose classes share a register group
+// (group_of > 0; myers_mix.hip, affine_mix.hip, affine_mix1.hip) run as one launch per group, the
+// longest chain's workgroups first, so the chip refills as waves end instead of draining at each
+// bin's tail; the launches (and bins of classes outside the groups, or alone in theirs, on their
+// own kernels) go to the main and the two extra streams side by side, the most work on the main
+// stream.  A single bin is one plain launch.
+template <class One, class Mix>
+int run_grouped(sf_batch *b, const std::vector<sf_batch::Bin> &bins, One one, int (*group_of)(int),
+                uint32_t (*blocks_for)(int, uint32_t), Mix mix, uint64_t *launches) {
+    if (bins.size() <= 1) {
+        for (const auto &bin : bins) HIP_TRY(one(bin, b->stream));
+        if (launches) *launches += bins.size();
+        return SF_OK;
+    }
+    struct Unit {
+        int group;
+        std::vector<size_t> ids;
+        double est;
+    };
+    std::vector<Unit> units;
+    std::vector<std::vector<size_t>> grp;
+    for (size_t i = 0; i < bins.size(); ++i) {
+        const int g = group_of(bins[i].cls);
+        if ((size_t)g >= grp.size()) grp.resize(g + 1);
+        grp[g].push_back(i);
+    }
+    for (size_t g = 1; g < grp.size(); ++g) {
+        // A group's launch holds at most MixArgs::kMax bins; more run as further launches.
+        for (size_t q = 0; q < grp[g].size(); q += MixArgs::kMax) {
+            Unit u{0, {}, 0};
+            u.ids.assign(grp[g].begin() + q, grp[g].begin() + std::min(grp[g].size(), q + MixArgs::kMax));
+            if (u.ids.size() > 1) u.group = (int)g;
+            double chain = 0, spread = 0;
+            for (size_t i : u.ids) chain = std::max(chain, bins[i].chain), spread += bins[i].spread;
+            u.est = std::max(chain, spread);
+            if (u.group == 0) {
+                for (size_t i : u.ids) units.push_back(Unit{0, {i}, bins[i].est()});
+            } else {
+                units.push_back(std::move(u));
+            }
+        }
+    }
+    if (!grp.empty())
+        for (size_t i : grp[0]) units.push_back(Unit{0, {i}, bins[i].est()});
+    std::stable_sort(units.begin(), units.end(), [](const Unit &x, const Unit &y) { return x.est > y.est; });
+    if (const int st = ensure_aux(b)) return st;
+    constexpr int S = 1 + sf_batch::kAux;
+    hipStream_t ss[S] = {b->stream, b->aux[0], b->aux[1]};
+    double load[S] = {0, 0, 0};
+    bool used[S] = {true, false, false};
+    HIP_TRY(hipEventRecord(b->fork, b->stream));
+    for (const Unit &u : units) {
+        int s = 0;
+        for (int t = 1; t < S; ++t)
+            if (load[t] < load[s]) s = t;
+        if (!used[s]) {
+            HIP_TRY(hipStreamWaitEvent(ss[s], b->fork, 0));
+            used[s] = true;
+        }
+        if (u.group == 0) {
+            HIP_TRY(one(bins[u.ids[0]], ss[s]));
+        } else {
+            std::vector<size_t> o(u.ids);
+            std::stable_sort(o.begin(), o.end(), [&](size_t x, size_t y) { return bins[x].chain > bins[y].chain; });
+            MixArgs mx{};
+            for (size_t i : o) {
+                const sf_batch::Bin &bin = bins[i];
+                mx.cls[mx.nseg] = bin.cls;
+                mx.off[mx.nseg] = bin.begin;
+                mx.count[mx.nseg] = bin.end - bin.begin;
+                mx.blk0[mx.nseg] = mx.blocks;
+                mx.blocks += blocks_for(bin.cls, bin.end - bin.begin);
+                ++mx.nseg;
+            }
+            HIP_TRY(mix(mx, u.group, ss[s]));
+        }
+        load[s] += u.est;
+    }
+    if (launches) *launches += units.size();
+    for (int t = 1; t < S; ++t)
+        if (used[t]) {
+            HIP_TRY(hipEventRecord(b->joins[t - 1], ss[t]));
+            HIP_TRY(hipStreamWaitEvent(b->stream, b->joins[t - 1], 0));
+        }
+    return SF_OK;
+}
+
+// Every Myers bin, grouped (myers_mix.hip).
+int run_myers(sf_batch *b, const MyersArgs &a) {
+    auto one = [&](const sf_batch::Bin &bin, hipStream_t s) {
+        MyersArgs x = a;
+        x.order = b->d_morder + bin.begin;
+        x.desc = b->d_mdesc + bin.begin;
+        x.count = bin.end - bin.begin;
+        return sf_launch_myers(x, bin.cls, s);
+    };
+    auto mix = [&](const MixArgs &mx, int group, hipStream_t s) {
+        MyersArgs x = a;
+        x.order = b->d_morder;
+        x.desc = b->d_mdesc;
+        return sf_launch_myers_mix(x, mx, group, s);
+    };
+    return run_grouped(b, b->mbins, one, sf_myers_mix_group, sf_myers_blocks_for, mix, nullptr);
+}
+
 // Estimated VALU wave-instructions a row of one wave costs in shape class `cls`, shared by its
 // 64 / LP pairs: about 9 a slot and 15 besides for the 32-bit kernels; 3.5 a slot (two slots a
 // register) and 24 besides for the 16-bit packed ones, 27 on 32 lanes, 30 with the 64-lane scan.
@@ -834,87 +939,6 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
     return SF_OK;
 }
 
-// Runs every Myers bin.  The bins whose classes share a register group (myers_mix.hip) run as one
-// launch per group, the longest chain's workgroups first, so the chip refills as waves end instead
-// of draining at each bin's tail; the launches (and bins of classes outside the groups, or alone
-// in theirs, on their own kernels) go to the main and the two extra streams side by side, the most
-// work on the main stream.  A single bin is one plain launch.
-int run_myers(sf_batch *b, const MyersArgs &a) {
-    const std::vector<sf_batch::Bin> &bins = b->mbins;
-    auto one = [&](const sf_batch::Bin &bin, hipStream_t s) {
-        MyersArgs x = a;
-        x.order = b->d_morder + bin.begin;
-        x.desc = b->d_mdesc + bin.begin;
-        x.count = bin.end - bin.begin;
-        return sf_launch_myers(x, bin.cls, s);
-    };
-    if (bins.size() <= 1) {
-        for (const auto &bin : bins) HIP_TRY(one(bin, b->stream));
-        return SF_OK;
-    }
-    struct Unit {
-        int group;
-        std::vector<size_t> ids;
-        double est;
-    };
-    std::vector<Unit> units;
-    std::vector<size_t> grp[4];
-    for (size_t i = 0; i < bins.size(); ++i) grp[sf_myers_mix_group(bins[i].cls)].push_back(i);
-    for (int g = 1; g <= 3; ++g)
-        if (!grp[g].empty()) {
-            Unit u{grp[g].size() > 1 ? g : 0, grp[g], 0};
-            double chain = 0, spread = 0;
-            for (size_t i : u.ids) chain = std::max(chain, bins[i].chain), spread += bins[i].spread;
-            u.est = std::max(chain, spread);
-            units.push_back(std::move(u));
-        }
-    for (size_t i : grp[0]) units.push_back(Unit{0, {i}, bins[i].est()});
-    std::stable_sort(units.begin(), units.end(), [](const Unit &x, const Unit &y) { return x.est > y.est; });
-    if (const int st = ensure_aux(b)) return st;
-    constexpr int S = 1 + sf_batch::kAux;
-    hipStream_t ss[S] = {b->stream, b->aux[0], b->aux[1]};
-    double load[S] = {0, 0, 0};
-    bool used[S] = {true, false, false};
-    HIP_TRY(hipEventRecord(b->fork, b->stream));
-    for (const Unit &u : units) {
-        int s = 0;
-        for (int t = 1; t < S; ++t)
-            if (load[t] < load[s]) s = t;
-        if (!used[s]) {
-            HIP_TRY(hipStreamWaitEvent(ss[s], b->fork, 0));
-            used[s] = true;
-        }
-        if (u.group == 0) {
-            HIP_TRY(one(bins[u.ids[0]], ss[s]));
-        } else {
-            std::vector<size_t> o(u.ids);
-            std::stable_sort(o.begin(), o.end(), [&](size_t x, size_t y) { return bins[x].chain > bins[y].chain; });
-            MixArgs mx{};
-            for (size_t i : o) {
-                if (mx.nseg == MixArgs::kMax) return fail(SF_EDEVICE, "more Myers bins than a mixed launch holds");
-                const sf_batch::Bin &bin = bins[i];
-                mx.cls[mx.nseg] = bin.cls;
-                mx.off[mx.nseg] = bin.begin;
-                mx.count[mx.nseg] = bin.end - bin.begin;
-                mx.blk0[mx.nseg] = mx.blocks;
-                mx.blocks += sf_myers_blocks_for(bin.cls, bin.end - bin.begin);
-                ++mx.nseg;
-            }
-            MyersArgs x = a;
-            x.order = b->d_morder;
-            x.desc = b->d_mdesc;
-            HIP_TRY(sf_launch_myers_mix(x, mx, u.group, ss[s]));
-        }
-        load[s] += u.est;
-    }
-    for (int t = 1; t < S; ++t)
-        if (used[t]) {
-            HIP_TRY(hipEventRecord(b->joins[t - 1], ss[t]));
-            HIP_TRY(hipStreamWaitEvent(b->stream, b->joins[t - 1], 0));
-        }
-    return SF_OK;
-}
-
 int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
     if (!b) return fail(SF_EINVAL, "batch is null");
     if (mode < SF_INFIX || mode > SF_GLOBAL) return fail(SF_EINVAL, "mode %d", mode);
@@ -1012,7 +1036,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
                 mx.off[t] = bin.begin;
                 mx.count[t] = bin.end - bin.begin;
                 mx.blk0[t] = mx.blocks;
-                mx.blocks += (uint32_t)(((uint64_t)mx.count[t] * 64 + 255) / 256);
+                mx.blocks += sf_affine_blocks_for(bin.cls, mx.count[t]);
             }
             mx.nseg = (uint32_t)ids.size();
             AffineArgs x = a;
@@ -1020,7 +1044,21 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
             x.desc = pl.d_desc;
             return sf_launch_affine_mix(x, mx, s);
         };
-        if ((st = run_bins(b, pl.bins, one, &b->affine_launches, mixed, sf_affine_mixable))) return st;
+        // Many bins (a length-skewed batch): grouped launches, no tail between classes.  A few
+        // (uniform lengths: one dominant class) keep a kernel each at that class's own occupancy.
+        if (pl.bins.size() >= 4) {
+            auto grouped = [&](const MixArgs &mx, int group, hipStream_t s) {
+                AffineArgs x = a;
+                x.order = pl.d_order;
+                x.desc = pl.d_desc;
+                return sf_launch_affine_mix_group(x, mx, group, s);
+            };
+            if ((st = run_grouped(b, pl.bins, one, sf_affine_mix_group, sf_affine_blocks_for, grouped,
+                                  &b->affine_launches)))
+                return st;
+        } else if ((st = run_bins(b, pl.bins, one, &b->affine_launches, mixed, sf_affine_mixable))) {
+            return st;
+        }
         if ((st = timed_end(b, 2, e0))) return st;
         b->affine_cells += pl.cells;
         b->cigar_valid = false;
