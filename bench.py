@@ -55,8 +55,9 @@ AFFINE_TB_OPS_PER_CELL = 11 + 4
 VALU_INSTS_PER_CELL = {"score": 1.9077e10 / 2.46731e11, "traceback": 5.6398e10 / 2.46731e11}
 # HBM bytes per launch of the dominant kernels on the uniform workload, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the same counters on a 1 GiB copy
-# (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.
-TRAFFIC_BYTES = {"score": 728_408_679, "traceback": 65_583_623_475}
+# (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.  Score: the
+# step's affine phase; traceback: the average of the DP's four launches a step.
+TRAFFIC_BYTES = {"score": 722_270_235, "traceback": 49_738_133_876}
 
 
 # Issue-rate probe (sf_probe_issue): waves of 32 independent VALU instructions of one encoding.
