@@ -310,20 +310,43 @@ void sf_server::launch(Slot &s) {
     if (!inject) {
         ok(hipEventRecord(s.ev_fork, st));
         ok(hipStreamWaitEvent(s.aux, s.ev_fork, 0));
+        MyersArgs ma{};
+        ma.pv = pv;
+        ma.mode = cfg.myers_mode;
+        ma.k = UINT32_MAX;
+        ma.dist = dist;
+        ma.end = end;
+        // Small batches: the classes of a register group side by side in one mixed launch (the
+        // Myers side then takes its longest chain, not the sum over classes).
+        MixArgs mg[4] = {};
         for (uint32_t a = 0; a < seg;) {
             uint32_t e = a;
-            while (e < seg && mcls[s.h_morder[e]] == mcls[s.h_morder[a]]) ++e;
-            MyersArgs ma{};
-            ma.pv = pv;
-            ma.order = s.d_morder + a;
-            ma.desc = s.d_mdesc + a;
-            ma.count = e - a;
-            ma.mode = cfg.myers_mode;
-            ma.k = UINT32_MAX;
-            ma.dist = dist;
-            ma.end = end;
-            ok(sf_launch_myers(ma, mcls[s.h_morder[a]], st));
+            const int c = mcls[s.h_morder[a]];
+            while (e < seg && mcls[s.h_morder[e]] == c) ++e;
+            const int g = wide ? sf_myers_mix_group(c) : 0;
+            if (g > 0 && mg[g].nseg < MixArgs::kMax) {
+                MixArgs &mx = mg[g];
+                mx.cls[mx.nseg] = c;
+                mx.off[mx.nseg] = a;
+                mx.count[mx.nseg] = e - a;
+                mx.blk0[mx.nseg] = mx.blocks;
+                mx.blocks += sf_myers_blocks_for(c, e - a);
+                ++mx.nseg;
+            } else {
+                MyersArgs x = ma;
+                x.order = s.d_morder + a;
+                x.desc = s.d_mdesc + a;
+                x.count = e - a;
+                ok(sf_launch_myers(x, c, st));
+            }
             a = e;
+        }
+        for (int g = 1; g <= 3; ++g) {
+            if (!mg[g].nseg) continue;
+            MyersArgs x = ma;
+            x.order = s.d_morder;
+            x.desc = s.d_mdesc;
+            ok(sf_launch_myers_mix(x, mg[g], g, st));
         }
         AffineArgs aa{};
         aa.pv = pv;
@@ -460,6 +483,10 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
     sv->store_words = (store_bases + 15) / 16;
     sv->cap_words = share / 2 / 4;
     sv->cap_seg = share / 2 / 128;
+    // Sized up front: growing the ticket table rehashes it under the server's lock (the scheduler
+    // then waits milliseconds to hand over a batch), and the latency record grows by copying.
+    sv->reqs.reserve(1u << 18);
+    sv->lat_ms.reserve(1u << 18);
     auto bail = [&](const char *m) {
         delete sv;
         return sf_runtime_fail(SF_ENOMEM, m);
