@@ -314,6 +314,12 @@ int sf_affine_mix_group(int cls);
 uint32_t sf_affine_blocks_for(int cls, uint32_t count);
 hipError_t sf_launch_affine_mix_group(const sf::AffineArgs &a, const sf::MixArgs &mx, int group, hipStream_t s);
 hipError_t sf_launch_affine_mix1(const sf::AffineArgs &a, const sf::MixArgs &mx, hipStream_t s);
+// A small serving batch's Myers and affine class runs in one launch (serve_mix.hip): the classes
+// it holds, and the launch (affine segments first, offsets into aa.order / ma.order).
+bool sf_serve_mix_affine(int cls);
+bool sf_serve_mix_myers(int cls);
+hipError_t sf_launch_serve_mix(const sf::AffineArgs &aa, const sf::MixArgs &amx, const sf::MyersArgs &ma,
+                               const sf::MixArgs &mmx, hipStream_t s);
 // Myers classes sharing one launch (myers_mix.hip): the register group of a class (1-3; 0 for a
 // class with launches of its own); workgroups a class takes for `count` pairs; the launch of one
 // group's segments (as for sf_launch_affine_mix: offsets into a.order / a.desc).

@@ -53,8 +53,8 @@ struct Req {
 };
 
 struct Slot {
-    hipStream_t stream = nullptr, aux = nullptr; // Myers and the copies; affine
-    hipEvent_t ev = nullptr, ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
     uint64_t words_base = 0; // word offset of this slot's read area in the device sequence buffer
     // A batch's segment records, one block of 40 B a segment laid out by the batch's own segment
     // count (poff, toff, ba: u64; plen, tlen, Myers order, affine order: u32), so one copy moves
@@ -294,8 +294,6 @@ void sf_server::launch(Slot &s) {
     key_order(s.key, s.h_aorder, s.tmp);
     hipStream_t st = s.stream;
     int status = SF_OK;
-    // Myers on the slot's stream and affine on its second one, side by side; the results go back
-    // once both are done.
     auto ok = [&](hipError_t e) {
         if (e != hipSuccess && status == SF_OK) status = SF_EDEVICE;
     };
@@ -308,81 +306,70 @@ void sf_server::launch(Slot &s) {
     int32_t *score = reinterpret_cast<int32_t *>(s.d_res + 2 * (uint64_t)seg);
     const bool inject = cfg.fail_batch && s.launch_no == cfg.fail_batch;
     if (!inject) {
-        ok(hipEventRecord(s.ev_fork, st));
-        ok(hipStreamWaitEvent(s.aux, s.ev_fork, 0));
         MyersArgs ma{};
         ma.pv = pv;
         ma.mode = cfg.myers_mode;
         ma.k = UINT32_MAX;
         ma.dist = dist;
         ma.end = end;
-        // Small batches: the classes of a register group side by side in one mixed launch (the
-        // Myers side then takes its longest chain, not the sum over classes).
-        MixArgs mg[4] = {};
-        for (uint32_t a = 0; a < seg;) {
-            uint32_t e = a;
-            const int c = mcls[s.h_morder[a]];
-            while (e < seg && mcls[s.h_morder[e]] == c) ++e;
-            const int g = wide ? sf_myers_mix_group(c) : 0;
-            if (g > 0 && mg[g].nseg < MixArgs::kMax) {
-                MixArgs &mx = mg[g];
-                mx.cls[mx.nseg] = c;
-                mx.off[mx.nseg] = a;
-                mx.count[mx.nseg] = e - a;
-                mx.blk0[mx.nseg] = mx.blocks;
-                mx.blocks += sf_myers_blocks_for(c, e - a);
-                ++mx.nseg;
-            } else {
-                MyersArgs x = ma;
-                x.order = s.d_morder + a;
-                x.desc = s.d_mdesc + a;
-                x.count = e - a;
-                ok(sf_launch_myers(x, c, st));
-            }
-            a = e;
-        }
-        for (int g = 1; g <= 3; ++g) {
-            if (!mg[g].nseg) continue;
-            MyersArgs x = ma;
-            x.order = s.d_morder;
-            x.desc = s.d_mdesc;
-            ok(sf_launch_myers_mix(x, mg[g], g, st));
-        }
         AffineArgs aa{};
         aa.pv = pv;
         aa.mode = cfg.affine_mode;
         aa.mat = cfg.sc.mat, aa.mis = cfg.sc.mis, aa.open = cfg.sc.open, aa.ext = cfg.sc.ext;
         aa.score = score;
         aa.score_col = col;
-        MixArgs mx{};
+        // Class runs (a, e) of each launch order.  A small batch puts the runs the combined kernel
+        // holds into its one launch (Myers and affine side by side, each taking its longest
+        // chain) and launches the rest after it; a large one launches every run on its own.
+        std::vector<std::pair<uint32_t, uint32_t>> mrest, arest;
+        MixArgs mmx{}, amx{};
+        auto add = [](MixArgs &mx, int c, uint32_t a, uint32_t e, uint32_t blocks) {
+            mx.cls[mx.nseg] = c;
+            mx.off[mx.nseg] = a;
+            mx.count[mx.nseg] = e - a;
+            mx.blk0[mx.nseg] = mx.blocks;
+            mx.blocks += blocks;
+            ++mx.nseg;
+        };
+        for (uint32_t a = 0; a < seg;) {
+            uint32_t e = a;
+            const int c = mcls[s.h_morder[a]];
+            while (e < seg && mcls[s.h_morder[e]] == c) ++e;
+            if (wide && sf_serve_mix_myers(c) && mmx.nseg < MixArgs::kMax) add(mmx, c, a, e, sf_myers_blocks_for(c, e - a));
+            else mrest.push_back({a, e});
+            a = e;
+        }
         for (uint32_t a = 0; a < seg;) {
             uint32_t e = a;
             const int c = acls[s.h_aorder[a]];
             while (e < seg && acls[s.h_aorder[e]] == c) ++e;
-            if (wide && sf_affine_mixable(c) && mx.nseg < MixArgs::kMax) {
-                mx.cls[mx.nseg] = c;
-                mx.off[mx.nseg] = a;
-                mx.count[mx.nseg] = e - a;
-                mx.blk0[mx.nseg] = mx.blocks;
-                mx.blocks += (uint32_t)(((uint64_t)(e - a) * 64 + 255) / 256);
-                ++mx.nseg;
-            } else {
-                AffineArgs x = aa;
-                x.order = s.d_aorder + a;
-                x.desc = s.d_adesc + a;
-                x.count = e - a;
-                ok(sf_launch_affine(x, c, false, s.aux));
-            }
+            if (wide && sf_serve_mix_affine(c) && amx.nseg < MixArgs::kMax) add(amx, c, a, e, sf_affine_blocks_for(c, e - a));
+            else arest.push_back({a, e});
             a = e;
         }
-        if (mx.nseg) {
-            AffineArgs x = aa;
-            x.order = s.d_aorder;
-            x.desc = s.d_adesc;
-            ok(sf_launch_affine_mix(x, mx, s.aux));
+        if (mmx.nseg || amx.nseg) {
+            MyersArgs xm = ma;
+            xm.order = s.d_morder;
+            xm.desc = s.d_mdesc;
+            AffineArgs xa = aa;
+            xa.order = s.d_aorder;
+            xa.desc = s.d_adesc;
+            ok(sf_launch_serve_mix(xa, amx, xm, mmx, st));
         }
-        ok(hipEventRecord(s.ev_join, s.aux));
-        ok(hipStreamWaitEvent(st, s.ev_join, 0));
+        for (auto [a, e] : mrest) {
+            MyersArgs x = ma;
+            x.order = s.d_morder + a;
+            x.desc = s.d_mdesc + a;
+            x.count = e - a;
+            ok(sf_launch_myers(x, mcls[s.h_morder[a]], st));
+        }
+        for (auto [a, e] : arest) {
+            AffineArgs x = aa;
+            x.order = s.d_aorder + a;
+            x.desc = s.d_adesc + a;
+            x.count = e - a;
+            ok(sf_launch_affine(x, acls[s.h_aorder[a]], false, st));
+        }
     } else {
         status = SF_EDEVICE; // fault injected from the host: the batch is failed without a launch
     }
@@ -404,7 +391,6 @@ void sf_server::loop() {
             if (!s.busy || s.ended || s.reqs.empty()) continue;
             int st = SF_OK;
             if (s.fail != SF_OK) {
-                (void)hipStreamSynchronize(s.aux);
                 (void)hipStreamSynchronize(s.stream);
                 st = s.fail;
             } else {
@@ -500,10 +486,7 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
         const uint64_t c = sv->cap_seg;
         s.words_base = q * sv->cap_words;
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&s.aux, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&s.ev_fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&s.ev_join, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess)
             return bail("stream");
         if (!dev_alloc(&s.d_stage, 5 * c) || !dev_alloc(&s.d_mdesc, c) || !dev_alloc(&s.d_adesc, c) ||
             !dev_alloc(&s.d_res, 4 * c))
@@ -628,15 +611,11 @@ void sf_server_destroy(sf_server *sv) {
     if (sv->th.joinable()) sv->th.join();
     for (auto &kv : sv->reqs) delete kv.second;
     for (Slot &s : sv->slots) {
-        if (s.aux) (void)hipStreamSynchronize(s.aux);
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         (void)hipFree(s.d_stage), (void)hipFree(s.d_res);
         (void)hipFree(s.d_mdesc), (void)hipFree(s.d_adesc);
         (void)hipHostFree(s.h_words), (void)hipHostFree(s.h_stage), (void)hipHostFree(s.h_res);
         if (s.ev) (void)hipEventDestroy(s.ev);
-        if (s.ev_fork) (void)hipEventDestroy(s.ev_fork);
-        if (s.ev_join) (void)hipEventDestroy(s.ev_join);
-        if (s.aux) (void)hipStreamDestroy(s.aux);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
     (void)hipFree(sv->d_seq);
