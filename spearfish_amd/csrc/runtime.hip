@@ -262,12 +262,17 @@ struct sf_batch {
     static constexpr int kAux = 2;
     hipStream_t aux[kAux] = {nullptr, nullptr};
     hipEvent_t fork = nullptr, joins[kAux] = {nullptr, nullptr};
+    // Extra streams whose work the main stream has not waited for (bit t: aux[t], its last work
+    // marked by joins[t]).  A Myers run leaves them open so that a following independent run
+    // (affine scores) starts beside its last bins; every other call joins them first (join_aux).
+    uint32_t aux_open = 0;
     hipEvent_t dp_done[2] = {nullptr, nullptr}, walk_done[2] = {nullptr, nullptr}, walks_end = nullptr;
     bool cigar_valid = false;
     // timing
-    struct Ev {
+    struct Ev { // an interval from a to the latest of its ends (more: other streams' ends, or null)
         int cat;
         hipEvent_t a, b;
+        hipEvent_t more[kAux];
     };
     std::vector<Ev> pending;
     std::vector<hipEvent_t> pool;
@@ -287,14 +292,22 @@ struct sf_batch {
     }
     void collect() {
         for (auto &e : pending) {
-            float t = 0;
-            if (hipEventElapsedTime(&t, e.a, e.b) == hipSuccess) ms[e.cat] += t;
+            float t = 0, u = 0;
+            if (hipEventElapsedTime(&t, e.a, e.b) != hipSuccess) t = 0;
+            for (hipEvent_t m : e.more)
+                if (m) {
+                    if (hipEventElapsedTime(&u, e.a, m) == hipSuccess) t = std::max(t, u);
+                    pool.push_back(m);
+                }
+            ms[e.cat] += t;
             pool.push_back(e.a);
             pool.push_back(e.b);
         }
         pending.clear();
     }
     ~sf_batch() {
+        for (hipStream_t x : aux)
+            if (x) (void)hipStreamSynchronize(x);
         if (wstream) (void)hipStreamSynchronize(wstream);
         if (stream) (void)hipStreamSynchronize(stream);
         collect();
@@ -388,7 +401,28 @@ int timed_begin(sf_batch *b, int cat, hipEvent_t *a, hipStream_t s = nullptr) {
 int timed_end(sf_batch *b, int cat, hipEvent_t a, hipStream_t s = nullptr) {
     hipEvent_t e = b->ev();
     HIP_TRY(hipEventRecord(e, s ? s : b->stream));
-    b->pending.push_back({cat, a, e});
+    b->pending.push_back({cat, a, e, {}});
+    return SF_OK;
+}
+
+// Makes the main stream wait for the extra streams' open work.
+int join_aux(sf_batch *b) {
+    for (int t = 0; t < sf_batch::kAux; ++t)
+        if (b->aux_open & (1u << t)) HIP_TRY(hipStreamWaitEvent(b->stream, b->joins[t], 0));
+    b->aux_open = 0;
+    return SF_OK;
+}
+
+// timed_end for a run that left extra streams open: an end on the main stream and one on each open
+// stream, the interval running to the latest (no stream waits for another).
+int timed_end_open(sf_batch *b, int cat, hipEvent_t a) {
+    if (const int st = timed_end(b, cat, a)) return st;
+    sf_batch::Ev &ev = b->pending.back();
+    for (int t = 0; t < sf_batch::kAux; ++t)
+        if (b->aux_open & (1u << t)) {
+            ev.more[t] = b->ev();
+            HIP_TRY(hipEventRecord(ev.more[t], b->aux[t]));
+        }
     return SF_OK;
 }
 
@@ -480,6 +514,7 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch, uint
         if (used[t]) {
             HIP_TRY(hipEventRecord(b->joins[t - 1], ss[t]));
             HIP_TRY(hipStreamWaitEvent(b->stream, b->joins[t - 1], 0));
+            b->aux_open &= ~(1u << (t - 1));
         }
     return SF_OK;
 }
@@ -492,7 +527,7 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch, uint
 // stream.  A single bin is one plain launch.
 template <class One, class Mix>
 int run_grouped(sf_batch *b, const std::vector<sf_batch::Bin> &bins, One one, int (*group_of)(int),
-                uint32_t (*blocks_for)(int, uint32_t), Mix mix, uint64_t *launches) {
+                uint32_t (*blocks_for)(int, uint32_t), Mix mix, uint64_t *launches, bool leave_open = false) {
     if (bins.size() <= 1) {
         for (const auto &bin : bins) HIP_TRY(one(bin, b->stream));
         if (launches) *launches += bins.size();
@@ -566,7 +601,12 @@ int run_grouped(sf_batch *b, const std::vector<sf_batch::Bin> &bins, One one, in
     for (int t = 1; t < S; ++t)
         if (used[t]) {
             HIP_TRY(hipEventRecord(b->joins[t - 1], ss[t]));
-            HIP_TRY(hipStreamWaitEvent(b->stream, b->joins[t - 1], 0));
+            if (leave_open) {
+                b->aux_open |= 1u << (t - 1);
+            } else {
+                HIP_TRY(hipStreamWaitEvent(b->stream, b->joins[t - 1], 0));
+                b->aux_open &= ~(1u << (t - 1));
+            }
         }
     return SF_OK;
 }
@@ -586,7 +626,7 @@ int run_myers(sf_batch *b, const MyersArgs &a) {
         x.desc = b->d_mdesc;
         return sf_launch_myers_mix(x, mx, group, s);
     };
-    return run_grouped(b, b->mbins, one, sf_myers_mix_group, sf_myers_blocks_for, mix, nullptr);
+    return run_grouped(b, b->mbins, one, sf_myers_mix_group, sf_myers_blocks_for, mix, nullptr, true);
 }
 
 // Estimated VALU wave-instructions a row of one wave costs in shape class `cls`, shared by its
@@ -710,6 +750,7 @@ std::vector<uint32_t> launch_order(const std::vector<int> &kcls, const std::vect
 // (a plan being replaced, the traceback offsets) waits first.  The main stream covers the aux and
 // walk streams: every run joins them back before it returns.
 int quiesce(sf_batch *b) {
+    if (const int st = join_aux(b)) return st;
     HIP_TRY(hipStreamSynchronize(b->stream));
     return SF_OK;
 }
@@ -946,6 +987,7 @@ int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
     if (!b->myers_unsupported.empty()) return fail(SF_ERANGE, "%s", b->myers_unsupported.c_str());
     int st;
     if ((st = ensure_device())) return st;
+    if ((st = join_aux(b))) return st; // the last Myers run may still write the outputs
     MyersArgs a{};
     a.pv = view(b);
     a.mode = mode;
@@ -957,14 +999,15 @@ int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
     if ((st = timed_begin(b, 0, &e0))) return st;
     a.reverse = 0;
     if ((st = run_myers(b, a))) return st;
-    if ((st = timed_end(b, 0, e0))) return st;
+    if ((st = timed_end_open(b, 0, e0))) return st;
     b->myers_cells += b->myers_cells_per_run;
     b->located_ready = locate == 2;
     if (locate == 2 && mode == SF_INFIX) {
+        if ((st = join_aux(b))) return st; // the start pass reads the forward pass's results
         if ((st = timed_begin(b, 1, &e0))) return st;
         a.reverse = 1;
         if ((st = run_myers(b, a))) return st;
-        if ((st = timed_end(b, 1, e0))) return st;
+        if ((st = timed_end_open(b, 1, e0))) return st;
     }
     return SF_OK;
 }
@@ -979,6 +1022,9 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         return fail(SF_ERANGE, "scores above 65535 are outside this build's bound");
     int st;
     if ((st = ensure_device())) return st;
+    // A score run needs nothing from a Myers run: it starts beside that run's last bins.  The
+    // traceback path (scratch, walks, uploads) starts from a joined state.
+    if (cigar && (st = join_aux(b))) return st;
     const uint64_t n = b->n;
     // Shape classes, cached across runs with the same mode, bands, scoring and tuning.
     sf_batch::AffPlan &pl = b->aplan[cigar ? 1 : 0];
@@ -1210,6 +1256,7 @@ int sf_batch_use_located(sf_batch *b, int on) {
     if (!b) return fail(SF_EINVAL, "batch is null");
     int st;
     if ((st = ensure_device())) return st;
+    if ((st = join_aux(b))) return st; // reads the Myers results
     const uint64_t n = b->n;
     if (on) {
         if (!b->located_ready) return fail(SF_EINVAL, "no Myers run with start and end on this batch");
@@ -1240,8 +1287,7 @@ int sf_set_affine_lp8(int policy) {
 
 int sf_batch_sync(sf_batch *b) {
     if (!b) return fail(SF_EINVAL, "batch is null");
-    HIP_TRY(hipStreamSynchronize(b->stream));
-    return SF_OK;
+    return quiesce(b);
 }
 
 int sf_batch_myers_results(sf_batch *b, uint32_t *dist, uint32_t *end, uint32_t *start) {
@@ -1299,7 +1345,7 @@ int sf_batch_timing(sf_batch *b, sf_timing *t) {
 
 void sf_batch_reset_timing(sf_batch *b) {
     if (!b) return;
-    (void)hipStreamSynchronize(b->stream);
+    (void)quiesce(b);
     b->collect();
     for (double &x : b->ms) x = 0;
     b->myers_cells = b->affine_cells = b->affine_launches = 0;
@@ -1320,6 +1366,7 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
         return fail(SF_ERANGE, "scores above 65535 are outside this build's bound");
     int st;
     if ((st = ensure_device())) return st;
+    if ((st = join_aux(b))) return st;
     const uint64_t n = b->n;
     sf_batch::AffPlan &pl = b->aplan[2];
     uint64_t key = (0x51ed270b27f1a3c5ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8)) * 1099511628211ull;
@@ -1408,6 +1455,7 @@ int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, c
         return fail(SF_ERANGE, "end bonuses beyond +-2^24 are outside this build's bound");
     int st;
     if ((st = ensure_device())) return st;
+    if ((st = join_aux(b))) return st;
     const uint64_t n = b->n;
     sf_batch::AffPlan &pl = b->aplan[3];
     uint64_t key = (0x2545f4914f6cdd1dull ^ (sf_affine_lp16_max() << 8)) * 1099511628211ull;
