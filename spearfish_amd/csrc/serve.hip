@@ -474,7 +474,7 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
     sv->reqs.reserve(1u << 18);
     sv->lat_ms.reserve(1u << 18);
     auto bail = [&](const char *m) {
-        delete sv;
+        sf_server_destroy(sv); // frees what was made so far (no scheduler thread yet)
         return sf_runtime_fail(SF_ENOMEM, m);
     };
     if (!dev_alloc(&sv->d_seq, sv->store_words + cfg->slots * sv->cap_words)) return bail("device sequence buffer");
