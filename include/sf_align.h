@@ -116,7 +116,9 @@ int sf_affine_cigar(const sf_pairs *pairs, int mode, const sf_scoring *sc, const
 
 /* ---- device-resident batches ---- */
 /* A batch packs its pairs 2-bit into HBM once, bins them by shape, and can then be run many
- * times; runs are asynchronous on the batch's own HIP stream until sf_batch_sync. */
+ * times; runs are asynchronous on the batch's own HIP streams (a main one and two more) until
+ * sf_batch_sync.  A run that needs another's results, or rewrites its outputs, waits for it; a
+ * score-only affine run may overlap the Myers run queued before it. */
 typedef struct sf_batch sf_batch;
 
 int sf_batch_create(const sf_pairs *pairs, sf_batch **out);
