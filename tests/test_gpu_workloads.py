@@ -58,3 +58,39 @@ def test_long_workload_located_windows(budget_mb, monkeypatch):
     b.myers(A.Mode.Infix, locate=2)
     assert np.array_equal(b.myers_results(2)[0], dist)
     b.close()
+
+
+def test_uneven_steps_queued_back_to_back():
+    # A Myers run leaves the batch's extra streams open and the score run after it starts beside
+    # its last bins (both read only the sequences); the next Myers run, and the results, wait for
+    # them.  Steps queued without a sync in between (as the bench runs them), a Prefix run queued
+    # right after an Infix one, and a CIGAR run after a score run all give the results of runs
+    # made one at a time, which match the oracle.
+    ps, bands = W.generate(3, 3000, 4)
+    b = A.Batch(ps)
+    b.myers(A.Mode.Infix, locate=1)
+    dist0, end0, _ = b.myers_results(1)
+    b.affine(A.Mode.Global, A.Affine.ONT, bands=bands)
+    r0 = b.affine_results()
+    for _ in range(3):
+        b.myers(A.Mode.Infix, locate=1)
+        b.affine(A.Mode.Global, A.Affine.ONT, bands=bands)
+    dist1, end1, _ = b.myers_results(1)
+    r1 = b.affine_results()
+    assert np.array_equal(dist0, dist1) and np.array_equal(end0, end1)
+    assert np.array_equal(r0["score"], r1["score"]) and np.array_equal(r0["score_col"], r1["score_col"])
+    b.myers(A.Mode.Infix, locate=1)
+    b.myers(A.Mode.Prefix, locate=1)
+    b.affine(A.Mode.Global, A.Affine.ONT, bands=bands)
+    b.affine(A.Mode.Global, A.Affine.ONT, bands=bands, cigar=True)
+    dist2, end2, _ = b.myers_results(1)
+    r2 = b.affine_results(cigar=True)
+    assert np.array_equal(r0["score"], r2["score"])
+    for i in range(0, len(ps), 37):
+        p, t = ps.pattern(i), ps.text(i)
+        assert (int(dist0[i]), int(end0[i])) == O.locate(p, t, O.INFIX)[:2], i
+        assert (int(dist2[i]), int(end2[i])) == O.locate(p, t, O.PREFIX)[:2], i
+        s, col, ts, c = O.align(p, t, O.GLOBAL, O.ONT, int(bands[i]))
+        assert (int(r0["score"][i]), int(r0["score_col"][i])) == (s, col), i
+        assert cig(r2, i) == c, i
+    b.close()
