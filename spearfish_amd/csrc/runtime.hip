@@ -546,6 +546,22 @@ int run_grouped(sf_batch *b, const std::vector<sf_batch::Bin> &bins, One one, in
         grp[g].push_back(i);
     }
     for (size_t g = 1; g < grp.size(); ++g) {
+        // A bin holding nearly all of its group's work (a batch of one dominant length class) that
+        // fills the chip alone launches on its own kernel, and the group's other bins (its few
+        // chain-bound stragglers) beside it: they then end on an extra stream, which a following
+        // independent run does not wait for.
+        if (grp[g].size() >= 2) {
+            double total = 0;
+            size_t top = grp[g][0];
+            for (size_t i : grp[g]) {
+                total += bins[i].spread;
+                if (bins[i].spread > bins[top].spread) top = i;
+            }
+            if (bins[top].spread >= 0.9 * total && bins[top].spread >= bins[top].chain) {
+                units.push_back(Unit{0, {top}, bins[top].est()});
+                grp[g].erase(std::find(grp[g].begin(), grp[g].end(), top));
+            }
+        }
         // A group's launch holds at most MixArgs::kMax bins; more run as further launches.
         for (size_t q = 0; q < grp[g].size(); q += MixArgs::kMax) {
             Unit u{0, {}, 0};
