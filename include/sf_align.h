@@ -5,8 +5,8 @@
  * Myers bit-parallel edit distance and banded affine-gap (Gotoh) alignment, computed by HIP
  * kernels with results bit-identical to the reference's CPU kernels.  Every entry point is a
  * batch form of one or more of the reference's functions; the comment on each cites the
- * function(s) it replaces.  The reference binds it from Rust through a thin `-sys` crate
- * (see INTEGRATION.md in the source tree).
+ * function(s) it replaces.  The reference binds it from Rust through a thin `-sys` crate of
+ * `extern "C"` declarations mirroring the ones below.
  *
  * Conventions
  *   - Plain C types only; the caller owns every buffer it passes, the library keeps no pointer
