@@ -450,7 +450,7 @@ def bench_serve(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested)",
+        print(f"bench.py runs on one GPU: the library has no multi-GPU path (--gpus {args.gpus} requested)",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "probe":

@@ -37,7 +37,7 @@ extern "C" {
 #define SF_ENOMEM 2  /* host or device memory exhausted */
 #define SF_EDEVICE 3 /* HIP runtime or kernel failure */
 #define SF_ENODEV 4  /* no gfx950 device, or the library was built without device code */
-#define SF_ERANGE 5  /* an input exceeds this build's limits, or an output buffer is too small */
+#define SF_ERANGE 5  /* an input exceeds this library's limits, or an output buffer is too small */
 
 /* align.rs:40-45 `Mode`. */
 #define SF_INFIX 0

@@ -2,8 +2,8 @@
  * sf_oracle.h -- CPU oracle for the alignment hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
- * leg may load this library, and only as the checker (or, for the baseline, as the timed CPU
- * restatement).  The product (libsf_align.so) never links, loads or calls it.
+ * leg may load this library, and only to check the product's results against (or, for the
+ * baseline, as the timed CPU restatement).  The product (libsf_align.so) never links, loads or calls it.
  *
  * A scalar C restatement of spearfish's crates/map/src/align.rs (Rust, v0.3.2), which cannot be
  * built here (no Rust toolchain).  Every function cites the reference lines it follows.  Two

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (libsf_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
-leg, as the checker.  The product package (spearfish_amd) never imports it.
+leg, to check the product's results against.  The product package (spearfish_amd) never imports it.
 
 The functions mirror crates/map/src/align.rs (the reference, Rust) one to one; see sf_oracle.h for
 the line citations.  Sequences are bytes / bytearray / numpy uint8 arrays of base codes (0..3).

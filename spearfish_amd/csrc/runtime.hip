@@ -1035,7 +1035,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0)
         return fail(SF_EINVAL, "scores must be non-negative (align.rs:2511 requires open, ext >= 0)");
     if (sc->mat > 65535 || sc->mis > 65535 || sc->open > 65535 || sc->ext > 65535)
-        return fail(SF_ERANGE, "scores above 65535 are outside this build's bound");
+        return fail(SF_ERANGE, "scores above 65535 are outside this library's bound");
     int st;
     if ((st = ensure_device())) return st;
     // A score run needs nothing from a Myers run: it starts beside that run's last bins.  The
@@ -1061,7 +1061,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
             sf::band_geom(m, t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
             const uint64_t width = below + above + 1;
             if (sf_affine_class(width, false) == 0)
-                return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this build handles up to 2048",
+                return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this library handles up to 2048",
                             (unsigned long long)i, (unsigned long long)width);
             ba[i] = below | (above << 32);
             cells += sf::band_cells(m, t, below, above); // DP_WORK (align.rs:2573-2578)
@@ -1379,7 +1379,7 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
     if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0)
         return fail(SF_EINVAL, "scores must be non-negative (align.rs:2511 requires open, ext >= 0)");
     if (sc->mat > 65535 || sc->mis > 65535 || sc->open > 65535 || sc->ext > 65535)
-        return fail(SF_ERANGE, "scores above 65535 are outside this build's bound");
+        return fail(SF_ERANGE, "scores above 65535 are outside this library's bound");
     int st;
     if ((st = ensure_device())) return st;
     if ((st = join_aux(b))) return st;
@@ -1402,7 +1402,7 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
             const uint64_t width = below + above + 1;
             kcls[i] = sf_affine_pass_class(width, width <= sf_affine_lp16_max());
             if (kcls[i] == 0)
-                return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this build handles up to 2048",
+                return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this library handles up to 2048",
                             (unsigned long long)i, (unsigned long long)width);
             if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t, aff_slots(kcls[i])))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
@@ -1466,9 +1466,9 @@ int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, c
     if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0)
         return fail(SF_EINVAL, "scores must be non-negative (align.rs:2511 requires open, ext >= 0)");
     if (sc->mat > 65535 || sc->mis > 65535 || sc->open > 65535 || sc->ext > 65535)
-        return fail(SF_ERANGE, "scores above 65535 are outside this build's bound");
+        return fail(SF_ERANGE, "scores above 65535 are outside this library's bound");
     if (bonus_start < -(1 << 24) || bonus_start > (1 << 24) || bonus_end < -(1 << 24) || bonus_end > (1 << 24))
-        return fail(SF_ERANGE, "end bonuses beyond +-2^24 are outside this build's bound");
+        return fail(SF_ERANGE, "end bonuses beyond +-2^24 are outside this library's bound");
     int st;
     if ((st = ensure_device())) return st;
     if ((st = join_aux(b))) return st;
@@ -1493,7 +1493,7 @@ int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, c
             const uint64_t width = (uint64_t)(hi - lo + 1);
             kcls[i] = sf_affine_pass_class(width, width <= sf_affine_lp16_max());
             if (kcls[i] == 0)
-                return fail(SF_ERANGE, "pair %llu: band of %llu diagonals; this build handles up to 2048",
+                return fail(SF_ERANGE, "pair %llu: band of %llu diagonals; this library handles up to 2048",
                             (unsigned long long)i, (unsigned long long)width);
             if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t, aff_slots(kcls[i])))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);

@@ -521,7 +521,7 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
         if (bad || !shape) {
             delete r;
             return sf_runtime_fail(bad ? SF_EINVAL : SF_ERANGE, bad ? "segment outside the read or the store"
-                                                                    : "segment beyond this build's shape classes");
+                                                                    : "segment beyond this library's shape classes");
         }
     }
     r->words.resize((rq->read_len + 15) / 16);
