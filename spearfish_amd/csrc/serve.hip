@@ -385,8 +385,8 @@ void sf_server::launch(Slot &s) {
 // so submitters and waiters never wait behind a batch's host work.
 void sf_server::loop() {
     (void)hipSetDevice(sf_runtime_device());
-    // The loop sleeps 20 us between polls of the running batches; with the default 50 us of timer
-    // slack such a sleep lasts up to 70 us, most of a small batch's latency.
+    // The loop sleeps 10 us between polls of the running batches; with the default 50 us of timer
+    // slack such a sleep lasts up to 60 us, much of a small batch's latency.
     (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
     std::vector<Slot *> fresh;
     for (;;) {
@@ -439,7 +439,7 @@ void sf_server::loop() {
         if (stop && pending.empty() && !busy) break;
         if (fresh.empty()) {
             if (!ended) {
-                if (busy) cv_sched.wait_for(lk, std::chrono::microseconds(20));
+                if (busy) cv_sched.wait_for(lk, std::chrono::microseconds(10));
                 else cv_sched.wait(lk, [&] { return stop || !pending.empty(); });
             }
             continue;
