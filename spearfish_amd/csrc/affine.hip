@@ -612,3 +612,9 @@ hipError_t sf_launch_compact(const uint32_t *order, uint32_t count, const uint32
                        count, runs, runs_off, nruns, out_off, out);
     return hipGetLastError();
 }
+
+// Loads this file's code object now (see sf_load_code_objects).
+hipError_t sf_load_code_affine() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&desc_kernel));
+}

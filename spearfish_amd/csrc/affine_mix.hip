@@ -102,3 +102,9 @@ hipError_t sf_launch_affine_mix(const AffineArgs &a, const MixArgs &mx, hipStrea
     hipLaunchKernelGGL(affine_mix_kernel, dim3(mx.blocks), dim3(256), 0, s, a, mx);
     return hipGetLastError();
 }
+
+// Loads this file's code object now (see sf_load_code_objects).
+hipError_t sf_load_code_affine_mix() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&affine_mix_kernel));
+}

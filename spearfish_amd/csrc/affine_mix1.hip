@@ -40,3 +40,9 @@ hipError_t sf_launch_affine_mix1(const AffineArgs &a, const MixArgs &mx, hipStre
     hipLaunchKernelGGL(affine_mix1_kernel, dim3(mx.blocks), dim3(256), 0, s, a, mx);
     return hipGetLastError();
 }
+
+// Loads this file's code object now (see sf_load_code_objects).
+hipError_t sf_load_code_affine_mix1() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&affine_mix1_kernel));
+}

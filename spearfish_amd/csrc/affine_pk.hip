@@ -52,3 +52,9 @@ hipError_t sf_launch_affine_packed(const AffineArgs &a, int cls, bool tb, hipStr
         return hipGetLastError();
     }
 }
+
+// Loads this file's code object now (see sf_load_code_objects).
+hipError_t sf_load_code_affine_pk() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&affine_pk16_kernel<16, 1, false>));
+}

@@ -21,3 +21,9 @@ hipError_t sf_launch_affine_packed64(const AffineArgs &a, int cls, bool tb, hipS
     }
     return hipGetLastError();
 }
+
+// Loads this file's code object now (see sf_load_code_objects).
+hipError_t sf_load_code_affine_pk64() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&affine_pk16_kernel<64, 1, false>));
+}

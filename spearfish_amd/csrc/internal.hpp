@@ -320,6 +320,18 @@ bool sf_serve_mix_affine(int cls);
 bool sf_serve_mix_myers(int cls);
 hipError_t sf_launch_serve_mix(const sf::AffineArgs &aa, const sf::MixArgs &amx, const sf::MyersArgs &ma,
                                const sf::MixArgs &mmx, hipStream_t s);
+// The runtime loads a file's code object when one of its kernels is first used, which holds up the
+// launching thread for milliseconds.  Each of these loads one file's object (a kernel's attributes
+// need it); sf_load_code_objects calls them all, for callers that must not pause later.
+hipError_t sf_load_code_myers();
+hipError_t sf_load_code_myers_mix();
+hipError_t sf_load_code_affine();
+hipError_t sf_load_code_affine_pk();
+hipError_t sf_load_code_affine_pk64();
+hipError_t sf_load_code_affine_mix();
+hipError_t sf_load_code_affine_mix1();
+hipError_t sf_load_code_serve_mix();
+hipError_t sf_load_code_objects();
 // Myers classes sharing one launch (myers_mix.hip): the register group of a class (1-3; 0 for a
 // class with launches of its own); workgroups a class takes for `count` pairs; the launch of one
 // group's segments (as for sf_launch_affine_mix: offsets into a.order / a.desc).

@@ -104,3 +104,9 @@ hipError_t sf_launch_probe(int32_t *out, hipStream_t s) {
     hipLaunchKernelGGL(probe_kernel, dim3(1), dim3(64), 0, s, out);
     return hipGetLastError();
 }
+
+// Loads this file's code object now (see sf_load_code_objects).
+hipError_t sf_load_code_myers() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&myers_lane_kernel<1>));
+}

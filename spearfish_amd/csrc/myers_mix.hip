@@ -86,3 +86,9 @@ hipError_t sf_launch_myers_mix(const MyersArgs &a, const MixArgs &mx, int group,
     }
     return hipGetLastError();
 }
+
+// Loads this file's code object now (see sf_load_code_objects).
+hipError_t sf_load_code_myers_mix() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&myers_mix_kernel<1>));
+}

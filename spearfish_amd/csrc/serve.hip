@@ -37,6 +37,14 @@ using namespace sf;
 using Clock = std::chrono::steady_clock;
 
 int sf_runtime_fail(int code, const char *msg); // runtime.hip
+
+hipError_t sf_load_code_objects() {
+    for (hipError_t (*f)() : {sf_load_code_myers, sf_load_code_myers_mix, sf_load_code_affine, sf_load_code_affine_pk,
+                              sf_load_code_affine_pk64, sf_load_code_affine_mix, sf_load_code_affine_mix1,
+                              sf_load_code_serve_mix})
+        if (const hipError_t e = f()) return e;
+    return hipSuccess;
+}
 int sf_runtime_device();                        // runtime.hip
 int sf_runtime_ensure();                        // runtime.hip
 
@@ -58,29 +66,36 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     uint64_t words_base = 0; // word offset of this slot's read area in the device sequence buffer
-    // A batch's segment records, one block of 40 B a segment laid out by the batch's own segment
-    // count (poff, toff, ba: u64; plen, tlen, Myers order, affine order: u32), so one copy moves
-    // them; the same layout in pinned host memory and on the device.  Results likewise: dist,
-    // end, score, col, each the batch's segment count long, one copy back.
-    uint64_t *d_stage = nullptr, *h_stage = nullptr; // 5 x cap_seg words
+    // A batch's segment records, one block of 104 B a segment laid out by the batch's own segment
+    // count (poff, toff, ba: u64; plen, tlen, Myers order, affine order: u32; then the pair
+    // records in Myers and in affine launch order), the same layout in pinned host memory and on
+    // the device.  A small batch's pair records are written here on the host, so one copy moves
+    // all of it and no kernel builds them; a large one copies the first 40 B a segment and builds
+    // them on the device.  Results: dist, end, score, col, each the batch's segment count long,
+    // one copy back.
+    static constexpr uint64_t kStageWords = 13;      // 104 B a segment
+    static_assert(sizeof(PairDesc) == 32, "the stage block holds two 32 B pair records a segment");
+    uint64_t *d_stage = nullptr, *h_stage = nullptr; // kStageWords x cap_seg words
     uint32_t *d_res = nullptr, *h_res = nullptr;     // 4 x cap_seg
     uint32_t *h_words = nullptr;                     // pinned: the batch's reads
-    PairDesc *d_mdesc = nullptr, *d_adesc = nullptr; // segments in Myers and affine launch order
     // views into the stage blocks for the current batch (set by views())
     uint64_t *d_poff = nullptr, *d_toff = nullptr, *d_ba = nullptr;
     uint32_t *d_plen = nullptr, *d_tlen = nullptr, *d_morder = nullptr, *d_aorder = nullptr;
     uint64_t *h_poff = nullptr, *h_toff = nullptr, *h_ba = nullptr;
     uint32_t *h_plen = nullptr, *h_tlen = nullptr, *h_morder = nullptr, *h_aorder = nullptr;
+    PairDesc *d_mdesc = nullptr, *d_adesc = nullptr, *h_mdesc = nullptr, *h_adesc = nullptr;
 
     void views(uint32_t seg) {
         auto cut = [seg](uint64_t *b, uint64_t *&poff, uint64_t *&toff, uint64_t *&ba, uint32_t *&plen,
-                         uint32_t *&tlen, uint32_t *&mo, uint32_t *&ao) {
+                         uint32_t *&tlen, uint32_t *&mo, uint32_t *&ao, PairDesc *&md, PairDesc *&ad) {
             poff = b, toff = b + seg, ba = b + 2 * (uint64_t)seg;
             plen = reinterpret_cast<uint32_t *>(b + 3 * (uint64_t)seg);
             tlen = plen + seg, mo = tlen + seg, ao = mo + seg;
+            md = reinterpret_cast<PairDesc *>(b + 5 * (uint64_t)seg);
+            ad = md + seg;
         };
-        cut(d_stage, d_poff, d_toff, d_ba, d_plen, d_tlen, d_morder, d_aorder);
-        cut(h_stage, h_poff, h_toff, h_ba, h_plen, h_tlen, h_morder, h_aorder);
+        cut(d_stage, d_poff, d_toff, d_ba, d_plen, d_tlen, d_morder, d_aorder, d_mdesc, d_adesc);
+        cut(h_stage, h_poff, h_toff, h_ba, h_plen, h_tlen, h_morder, h_aorder, h_mdesc, h_adesc);
     }
     std::vector<Req *> reqs;
     std::vector<uint32_t> seg0;
@@ -300,10 +315,21 @@ void sf_server::launch(Slot &s) {
         if (e != hipSuccess && status == SF_OK) status = SF_EDEVICE;
     };
     ok(hipMemcpyAsync(d_seq + store_words + s.words_base, s.h_words, words * 4, hipMemcpyHostToDevice, st));
-    ok(hipMemcpyAsync(s.d_stage, s.h_stage, (uint64_t)seg * 40, hipMemcpyHostToDevice, st));
     const PairsView pv{d_seq, s.d_poff, s.d_toff, s.d_plen, s.d_tlen};
-    ok(sf_launch_desc(seg, s.d_morder, pv, nullptr, s.d_mdesc, st));
-    ok(sf_launch_desc(seg, s.d_aorder, pv, s.d_ba, s.d_adesc, st));
+    if (wide) {
+        // The pair records in both launch orders (as desc_kernel makes them), sent with the rest.
+        for (uint32_t w = 0; w < seg; ++w) {
+            const uint32_t i = s.h_morder[w], k = s.h_aorder[w];
+            s.h_mdesc[w] = PairDesc{s.h_poff[i], s.h_toff[i], s.h_plen[i], s.h_tlen[i], 0u, 0u};
+            s.h_adesc[w] = PairDesc{s.h_poff[k], s.h_toff[k], s.h_plen[k], s.h_tlen[k], (uint32_t)s.h_ba[k],
+                                    (uint32_t)(s.h_ba[k] >> 32)};
+        }
+        ok(hipMemcpyAsync(s.d_stage, s.h_stage, (uint64_t)seg * Slot::kStageWords * 8, hipMemcpyHostToDevice, st));
+    } else {
+        ok(hipMemcpyAsync(s.d_stage, s.h_stage, (uint64_t)seg * 40, hipMemcpyHostToDevice, st));
+        ok(sf_launch_desc(seg, s.d_morder, pv, nullptr, s.d_mdesc, st));
+        ok(sf_launch_desc(seg, s.d_aorder, pv, s.d_ba, s.d_adesc, st));
+    }
     uint32_t *dist = s.d_res, *end = s.d_res + seg, *col = s.d_res + 3 * (uint64_t)seg;
     int32_t *score = reinterpret_cast<int32_t *>(s.d_res + 2 * (uint64_t)seg);
     const bool inject = cfg.fail_batch && s.launch_no == cfg.fail_batch;
@@ -457,6 +483,10 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
     *out = nullptr;
     if (const int st = sf_runtime_ensure()) return st;
     if (store_bases && !store_words) return sf_runtime_fail(SF_EINVAL, "store is null");
+    // Every kernel the loop may launch is loaded now, so that no first use mid-run (a class that
+    // turns up late, the first batch over the small-batch limit) holds up the scheduler while the
+    // runtime loads its code object.
+    if (sf_load_code_objects() != hipSuccess) return sf_runtime_fail(SF_EDEVICE, "loading the kernels failed");
     if (cfg->slots == 0 || cfg->slots > 16) return sf_runtime_fail(SF_EINVAL, "slots must be 1..16");
     if (cfg->myers_mode < SF_INFIX || cfg->myers_mode > SF_GLOBAL || cfg->affine_mode < SF_INFIX ||
         cfg->affine_mode > SF_GLOBAL)
@@ -493,10 +523,10 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess)
             return bail("stream");
-        if (!dev_alloc(&s.d_stage, 5 * c) || !dev_alloc(&s.d_mdesc, c) || !dev_alloc(&s.d_adesc, c) ||
-            !dev_alloc(&s.d_res, 4 * c))
+        if (!dev_alloc(&s.d_stage, Slot::kStageWords * c) || !dev_alloc(&s.d_res, 4 * c))
             return bail("slot device buffers");
-        if (!host_alloc(&s.h_words, sv->cap_words) || !host_alloc(&s.h_stage, 5 * c) || !host_alloc(&s.h_res, 4 * c))
+        if (!host_alloc(&s.h_words, sv->cap_words) || !host_alloc(&s.h_stage, Slot::kStageWords * c) ||
+            !host_alloc(&s.h_res, 4 * c))
             return bail("slot pinned buffers");
     }
     sv->th = std::thread([sv] { sv->loop(); });
@@ -618,7 +648,6 @@ void sf_server_destroy(sf_server *sv) {
     for (Slot &s : sv->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         (void)hipFree(s.d_stage), (void)hipFree(s.d_res);
-        (void)hipFree(s.d_mdesc), (void)hipFree(s.d_adesc);
         (void)hipHostFree(s.h_words), (void)hipHostFree(s.h_stage), (void)hipHostFree(s.h_res);
         if (s.ev) (void)hipEventDestroy(s.ev);
         if (s.stream) (void)hipStreamDestroy(s.stream);

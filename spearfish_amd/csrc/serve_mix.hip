@@ -80,3 +80,9 @@ hipError_t sf_launch_serve_mix(const AffineArgs &aa, const MixArgs &amx, const M
     hipLaunchKernelGGL(serve_mix_kernel, dim3((unsigned)blocks), dim3(256), 0, s, aa, amx, ma, mmx);
     return hipGetLastError();
 }
+
+// Loads this file's code object now (see sf_load_code_objects).
+hipError_t sf_load_code_serve_mix() {
+    hipFuncAttributes fa;
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&serve_mix_kernel));
+}

@@ -151,3 +151,34 @@ def test_myers_every_block_count_alone_and_together(workload, mode):
             assert (int(alone.dist[q]), int(alone.end[q])) == (d, e), (len(p), len(t))
             dp = O.affine_dp(p, t, O.GLOBAL, O.ONT, BAND)
             assert (int(alone.score[q]), int(alone.score_col[q])) == (dp.score, dp.score_col), (len(p), len(t))
+
+
+def test_large_batches_equal_reads_alone():
+    # Every read offered at once to one slot: batches grow past the small-batch limit (2,048
+    # segments: one launch per class, pair records built on the device), and each read's results
+    # equal those it gets served alone (pair records written on the host, one combined launch).
+    store, reads = S.generate_serve_workload(4000, seed=11)
+    n = len(reads)
+    alone, mix = S.Results(reads), S.Results(reads)
+    with S.Server(store, slots=1, band=BAND) as s:
+        ra = S.requests(reads, alone)
+        for i in range(n):
+            assert s.wait(s.submit(ra[i])) == N.SF_OK
+        s.stats(reset=True)
+        s.replay(S.requests(reads, mix), np.zeros(n))
+        st = s.stats()
+        assert st.reads_done == n and st.mean_reads_per_batch * 8 > 2048, st.mean_reads_per_batch
+    for a in ("dist", "end", "score", "score_col"):
+        assert np.array_equal(getattr(alone, a), getattr(mix, a)), a
+    codes = store_codes(store)
+    for i in range(0, n, 97):
+        s0 = int(reads.seg0[i])
+        r = reads.codes[int(reads.read_off[i]):int(reads.read_off[i]) + int(reads.read_len[i])]
+        for k in range(int(reads.nseg[i])):
+            q = s0 + k
+            p = r[int(reads.seg_read_off[q]):int(reads.seg_read_off[q]) + int(reads.seg_read_len[q])]
+            t = codes[int(reads.seg_tx_off[q]):int(reads.seg_tx_off[q]) + int(reads.seg_tx_len[q])]
+            d, e, _ = O.locate(p, t, O.INFIX)
+            assert (int(mix.dist[q]), int(mix.end[q])) == (d, e)
+            dp = O.affine_dp(p, t, O.GLOBAL, O.ONT, BAND)
+            assert (int(mix.score[q]), int(mix.score_col[q])) == (dp.score, dp.score_col)
