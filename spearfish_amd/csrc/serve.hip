@@ -19,6 +19,8 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -183,6 +185,12 @@ struct sf_server {
     uint64_t cells = 0;
     Clock::time_point t_first, t_last;
     bool any = false;
+    // SF_SERVE_TRACE set: the longest scheduler iteration, launch and gap between submissions
+    // (with when they ended, from creation), reported on stderr when the server is destroyed.
+    bool trace = std::getenv("SF_SERVE_TRACE") != nullptr;
+    Clock::time_point t_created = Clock::now(), t_last_submit{};
+    double max_iter_ms = 0, max_iter_at = 0, max_launch_ms = 0, max_launch_at = 0, max_submit_gap_ms = 0,
+           max_submit_gap_at = 0, max_build_ms = 0, max_lock_ms = 0;
 
     void loop();
     void launch(Slot &s);
@@ -415,7 +423,15 @@ void sf_server::loop() {
     // slack such a sleep lasts up to 60 us, much of a small batch's latency.
     (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
     std::vector<Slot *> fresh;
+    auto t_iter = Clock::now();
     for (;;) {
+        if (trace) {
+            const auto now = Clock::now();
+            const double ms = std::chrono::duration<double, std::milli>(now - t_iter).count();
+            if (ms > max_iter_ms)
+                max_iter_ms = ms, max_iter_at = std::chrono::duration<double>(now - t_created).count();
+            t_iter = now;
+        }
         // Batches that ended: results out to the callers (no lock).
         bool ended = false;
         for (Slot &s : slots) {
@@ -471,7 +487,16 @@ void sf_server::loop() {
             continue;
         }
         lk.unlock();
-        for (Slot *s : fresh) launch(*s);
+        for (Slot *s : fresh) {
+            const auto t0 = Clock::now();
+            launch(*s);
+            if (trace) {
+                const auto t1 = Clock::now();
+                const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+                if (ms > max_launch_ms)
+                    max_launch_ms = ms, max_launch_at = std::chrono::duration<double>(t1 - t_created).count();
+            }
+        }
     }
 }
 
@@ -539,6 +564,7 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
     if (rq->nseg && (!rq->seg_read_off || !rq->seg_read_len || !rq->seg_tx_off || !rq->seg_tx_len))
         return sf_runtime_fail(SF_EINVAL, "null segment arrays");
     if (rq->read_len && !rq->read) return sf_runtime_fail(SF_EINVAL, "read is null");
+    const auto t_enter = Clock::now();
     auto *r = new Req();
     r->roff.assign(rq->seg_read_off, rq->seg_read_off + rq->nseg);
     r->rlen.assign(rq->seg_read_len, rq->seg_read_len + rq->nseg);
@@ -578,9 +604,23 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
     r->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(sv->mu);
+        if (sv->trace) {
+            const auto t_locked = Clock::now();
+            sv->max_build_ms = std::max(sv->max_build_ms, std::chrono::duration<double, std::milli>(r->t_submit - t_enter).count());
+            sv->max_lock_ms = std::max(sv->max_lock_ms, std::chrono::duration<double, std::milli>(t_locked - r->t_submit).count());
+        }
         if (sv->stop) {
             delete r;
             return sf_runtime_fail(SF_EINVAL, "server is shutting down");
+        }
+        if (sv->trace) {
+            if (sv->t_last_submit != Clock::time_point{}) {
+                const double ms = std::chrono::duration<double, std::milli>(r->t_submit - sv->t_last_submit).count();
+                if (ms > sv->max_submit_gap_ms)
+                    sv->max_submit_gap_ms = ms,
+                    sv->max_submit_gap_at = std::chrono::duration<double>(r->t_submit - sv->t_created).count();
+            }
+            sv->t_last_submit = r->t_submit;
         }
         r->ticket = sv->next_ticket++;
         sv->reqs[r->ticket] = r;
@@ -632,6 +672,17 @@ int sf_server_get_stats(sf_server *sv, sf_server_stats *out, int reset) {
         sv->lat_ms.clear();
         sv->reads_done = sv->reads_failed = sv->batches = sv->segments = sv->batch_reads = sv->cells = 0;
         sv->any = false;
+        if (sv->trace) {
+            if (sv->max_iter_ms > 0)
+                std::fprintf(stderr, "sf_server: longest scheduler iteration %.3f ms (ended %.3f s), launch %.3f ms "
+                                     "(%.3f s), gap between submissions %.3f ms (%.3f s), request build %.3f ms, "
+                                     "lock wait %.3f ms\n",
+                             sv->max_iter_ms, sv->max_iter_at, sv->max_launch_ms, sv->max_launch_at,
+                             sv->max_submit_gap_ms, sv->max_submit_gap_at, sv->max_build_ms, sv->max_lock_ms);
+            sv->max_iter_ms = sv->max_launch_ms = sv->max_submit_gap_ms = 0; // (the scheduler's own, read racily)
+            sv->max_build_ms = sv->max_lock_ms = 0;
+            sv->t_last_submit = Clock::time_point{};
+        }
     }
     return SF_OK;
 }
@@ -644,6 +695,12 @@ void sf_server_destroy(sf_server *sv) {
     }
     sv->cv_sched.notify_all();
     if (sv->th.joinable()) sv->th.join();
+    if (sv->trace)
+        std::fprintf(stderr,
+                     "sf_server: longest scheduler iteration %.3f ms (ended %.3f s), launch %.3f ms (%.3f s), "
+                     "gap between submissions %.3f ms (%.3f s)\n",
+                     sv->max_iter_ms, sv->max_iter_at, sv->max_launch_ms, sv->max_launch_at, sv->max_submit_gap_ms,
+                     sv->max_submit_gap_at);
     for (auto &kv : sv->reqs) delete kv.second;
     for (Slot &s : sv->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
