@@ -249,8 +249,8 @@ int sf_server_submit(sf_server *s, const sf_read_request *req, uint64_t *ticket)
 /* Waits for a ticket (timeout_s < 0: forever); returns the request's status. */
 int sf_server_wait(sf_server *s, uint64_t ticket, double timeout_s);
 int sf_server_get_stats(sf_server *s, sf_server_stats *out, int reset);
-/* Drains the queue, then frees everything. */
-/* Finishes every request already submitted, then frees the server.  With SF_SERVE_TRACE set in
+/* Drains the queue (every request already submitted finishes), then frees the server.  With
+ * SF_SERVE_TRACE set in
  * the environment the server reports on stderr, at each stats reset and here, its longest
  * scheduler iteration and launch and the longest gap between submissions (a diagnostic). */
 void sf_server_destroy(sf_server *s);
