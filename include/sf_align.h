@@ -250,9 +250,9 @@ int sf_server_submit(sf_server *s, const sf_read_request *req, uint64_t *ticket)
 int sf_server_wait(sf_server *s, uint64_t ticket, double timeout_s);
 int sf_server_get_stats(sf_server *s, sf_server_stats *out, int reset);
 /* Drains the queue (every request already submitted finishes), then frees the server.  With
- * SF_SERVE_TRACE set in
- * the environment the server reports on stderr, at each stats reset and here, its longest
- * scheduler iteration and launch and the longest gap between submissions (a diagnostic). */
+ * SF_SERVE_TRACE set in the environment the server reports on stderr, at each stats reset and
+ * here, its longest scheduler iteration and launch and the longest gap between submissions
+ * (a diagnostic). */
 void sf_server_destroy(sf_server *s);
 /* Load generator for measurements: submits req[i] at t_arrive[i] seconds after the call, waits
  * for all; *wall_s = first arrival to last completion, *refused = requests refused at admission
