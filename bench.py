@@ -12,10 +12,12 @@ stay in HBM.
 value = DP cells of a step (sum of m*n for Myers plus the affine band cells, counted as the
 reference's DP_WORK does) / wall time per step, in GCUPS.
 
-    python bench.py [--steps K] [--warmup W] [--pairs N] [--no-cpu]
-    python bench.py --workload uneven|long|serve      # the other measured workloads
+    python bench.py [--steps K] [--warmup W] [--pairs N] [--no-cpu] [--no-extra]
+    python bench.py --workload uneven|long|serve      # one of the other workloads alone
 
-Prints one JSON line.
+Prints one JSON line.  By default the line also carries the other workloads under
+"other_workloads" (about a minute more): the uneven batch with its CPU baseline and the ratio to
+it, the long-segment batch, and the serving loop.
 
 Other workloads (same JSON shape, their own config):
   uneven  5e5 pairs, targets LogNormal(median 800 bp, sigma 0.8) clipped to 100..20,000, 5-15%
@@ -70,11 +72,11 @@ def issue_probe(blocks=8192, iters=20000):
     32-bit ops (VOP3).  About 35 ms of GPU time per kind."""
     import ctypes as C
     from spearfish_amd import _native as N
-    lib = N.load()
+    lib = N.load_diag()
     ms = C.c_float()
     out = {}
     for name, kind in PROBE_KINDS.items():
-        N.check(lib.sf_probe_issue(kind, iters, blocks, C.byref(ms)))
+        N.check_diag(lib.sf_probe_issue(kind, iters, blocks, C.byref(ms)))
         out[name] = round(blocks * 4 * iters * 32 * 64 / (ms.value * 1e-3) / 1e12, 2)
     return out
 
@@ -91,7 +93,16 @@ def parse():
     ap.add_argument("--per-pair", type=int, default=256, help="pairs for the per-pair-launch comparison")
     ap.add_argument("--workload", default="uniform", choices=["uniform", "uneven", "long", "serve", "probe"])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--no-extra", action="store_true", help="the uniform workload only")
     return ap.parse_args()
+
+
+def host_threads():
+    """The host cores this process may run on (its CPU affinity), capped by OMP_NUM_THREADS where
+    the environment sets a share: the CPU baseline runs on all of them."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(share))) if share.isdigit() and int(share) > 0 else max(1, n)
 
 
 def log(msg):
@@ -280,7 +291,7 @@ def bench_uniform(args):
         except Exception as e:  # report, don't hide
             out["per_pair_launch_gcups"] = f"error: {e}"
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         # The whole workload: about 5 s of host work at 16 threads with the AVX-512 rows.
         k = args.cpu_pairs or len(pairs)
         from spearfish_amd.align import PairSet
@@ -295,7 +306,7 @@ def bench_uniform(args):
                                          f"{cb['affine_s']:.1f}s)",
                                "sample_bit_exact_vs_gpu": cb["bit_exact"]}
     batch.close()
-    print(json.dumps(out))
+    return out
 
 
 def common(args, value, unit, ms, dtype, config, higher=True):
@@ -304,39 +315,42 @@ def common(args, value, unit, ms, dtype, config, higher=True):
             "data": "synthetic", "config": config}
 
 
-def bench_uneven(args):
+def bench_uneven(args, steps=None, warmup=None):
     from spearfish_amd import align as A
     from spearfish_amd import workload as W
-    n = args.pairs if args.pairs != 1_000_000 else 500_000
+    steps = steps or args.steps
+    warmup = args.warmup if warmup is None else warmup
+    n = args.pairs if (args.pairs != 1_000_000 and args.workload == "uneven") else 500_000
     pairs, bands = W.generate(3, n, args.seed)
     log(f"generated {n} uneven pairs")
     b = A.Batch(pairs)
-    for w in range(args.warmup):
+    for w in range(warmup):
         b.myers(A.Mode.Infix, locate=1)
         b.affine(A.Mode.Global, A.Affine.ONT, bands=bands)
         b.sync()
-        log(f"warmup {w + 1} done")
+        log(f"uneven: warmup {w + 1} done")
     b.reset_timing()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         b.myers(A.Mode.Infix, locate=1)
         b.affine(A.Mode.Global, A.Affine.ONT, bands=bands)
     b.sync()
     wall = time.perf_counter() - t0
     tm = b.timing()
-    log(f"timed {args.steps} steps in {wall:.3f}s")
+    log(f"uneven: timed {steps} steps in {wall:.3f}s")
     cells = tm.myers_cells + tm.affine_cells
-    out = common(args, round(cells / wall / 1e9, 2), "GCUPS", round(wall / args.steps * 1e3, 2), "int32",
+    out = common(args, round(cells / wall / 1e9, 2), "GCUPS", round(wall / steps * 1e3, 2), "int32",
                  {"workload": "uneven-5e5-lognormal800-5to15pct-adaptive-band", "pairs": n,
                   "step": "myers infix distance+end, affine global ONT adaptive band score"})
+    out["steps"], out["warmup"] = steps, warmup
     out["myers_gcups"] = round(tm.myers_cells / (tm.myers_ms * 1e-3) / 1e9, 1)
     out["affine_gcups"] = round(tm.affine_cells / (tm.affine_ms * 1e-3) / 1e9, 1)
-    out["kernel_ms_per_step"] = {"myers": round(tm.myers_ms / args.steps, 2), "affine": round(tm.affine_ms / args.steps, 2)}
-    out["affine_launches_per_step"] = tm.affine_launches // args.steps
+    out["kernel_ms_per_step"] = {"myers": round(tm.myers_ms / steps, 2), "affine": round(tm.affine_ms / steps, 2)}
+    out["affine_launches_per_step"] = tm.affine_launches // steps
     if not args.no_cpu:
         dist, _, _ = b.myers_results(0)
         res = b.affine_results()
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         k = args.cpu_pairs or len(pairs)  # the whole workload: about 4 s at 16 threads with AVX-512 rows
         from spearfish_amd.align import PairSet
         sub = PairSet(pairs.pat, pairs.pat_off[:k], pairs.pat_len[:k], pairs.txt, pairs.txt_off[:k], pairs.txt_len[:k])
@@ -346,14 +360,18 @@ def bench_uneven(args):
                                                          f"distance_with (scalar) + run_core with {cb['rows']} rows, "
                                                          f"{cb['seconds']:.1f}s",
                                "sample_bit_exact_vs_gpu": cb["bit_exact"]}
+        # The target: at least 20x the all-host-cores CPU path on this workload.
+        out["gpu_vs_cpu"] = {"ratio": round(out["value"] / out["cpu_baseline"]["value"], 1), "target": 20.0}
     b.close()
-    print(json.dumps(out))
+    return out
 
 
-def bench_long(args):
+def bench_long(args, steps=None, warmup=None):
     from spearfish_amd import align as A
     from spearfish_amd import workload as W
-    n = args.pairs if args.pairs != 1_000_000 else 200_000
+    steps = steps or args.steps
+    warmup = args.warmup if warmup is None else warmup
+    n = args.pairs if (args.pairs != 1_000_000 and args.workload == "long") else 200_000
     pairs, _ = W.generate(4, n, args.seed)
     log(f"generated {n} long pairs")
     b = A.Batch(pairs)
@@ -364,27 +382,28 @@ def bench_long(args):
         b.use_located(True)
         b.affine(A.Mode.Global, A.Affine.ONT, band=256, cigar=True)
 
-    for w in range(args.warmup):
+    for w in range(warmup):
         step()
         b.sync()
-        log(f"warmup {w + 1} done")
+        log(f"long: warmup {w + 1} done")
     b.reset_timing()
     t0 = time.perf_counter()
-    for s in range(args.steps):
+    for s in range(steps):
         step()
-        log(f"step {s + 1} queued")
+        log(f"long: step {s + 1} queued")
     b.sync()
     wall = time.perf_counter() - t0
     tm = b.timing()
     cells = tm.myers_cells + tm.affine_cells
-    out = common(args, round(cells / wall / 1e9, 2), "GCUPS", round(wall / args.steps * 1e3, 2), "int32",
+    out = common(args, round(cells / wall / 1e9, 2), "GCUPS", round(wall / steps * 1e3, 2), "int32",
                  {"workload": "long-2e5x2to10kb-12pct-band256", "pairs": n,
                   "step": "myers infix distance+end+start, affine global ONT band 256 + CIGAR on located window"})
-    out["kernel_ms_per_step"] = {"myers": round(tm.myers_ms / args.steps, 2), "myers_start": round(tm.locate_ms / args.steps, 2),
-                                 "affine_tb": round(tm.affine_ms / args.steps, 2), "walk": round(tm.walk_ms / args.steps, 2)}
+    out["steps"], out["warmup"] = steps, warmup
+    out["kernel_ms_per_step"] = {"myers": round(tm.myers_ms / steps, 2), "myers_start": round(tm.locate_ms / steps, 2),
+                                 "affine_tb": round(tm.affine_ms / steps, 2), "walk": round(tm.walk_ms / steps, 2)}
     out["myers_gcups"] = round(tm.myers_cells / ((tm.myers_ms) * 1e-3) / 1e9, 1)
     b.close()
-    print(json.dumps(out))
+    return out
 
 
 def bench_serve(args):
@@ -444,7 +463,7 @@ def bench_serve(args):
                   "server": "4 slots, 256 MiB HBM budget"})
     out["steps"], out["warmup"] = None, None
     out["serve"] = rows
-    print(json.dumps(out))
+    return out
 
 
 def main():
@@ -458,7 +477,28 @@ def main():
         print(json.dumps({"issue_probe_t_lane_instructions_per_s": issue_probe(),
                           "nominal_one_wave_instruction_per_4_cycles": round(VALU_ISSUE_PEAK_T, 1)}))
         return
-    {"uniform": bench_uniform, "uneven": bench_uneven, "long": bench_long, "serve": bench_serve}[args.workload](args)
+    if args.workload != "uniform":
+        print(json.dumps({"uneven": bench_uneven, "long": bench_long, "serve": bench_serve}[args.workload](args)))
+        return
+    out = bench_uniform(args)
+    if not args.no_extra:
+        # The other workloads at bounded step counts (about a minute together), each in its own
+        # record; `value` stays the uniform workload's.
+        extra = {}
+        for name, fn in (("uneven", lambda: bench_uneven(args, steps=5, warmup=1)),
+                         ("long", lambda: bench_long(args, steps=2, warmup=1)),
+                         ("serve", lambda: bench_serve(args))):
+            t0 = time.perf_counter()
+            try:
+                r = fn()
+                for key in ("metric", "n_gpus", "scaling", "vs_baseline", "data"):
+                    r.pop(key, None)
+                extra[name] = r
+            except Exception as e:  # report, don't hide
+                extra[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"{name}: done in {time.perf_counter() - t0:.1f}s")
+        out["other_workloads"] = extra
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define SF_ABI_VERSION 1
+#define SF_ABI_VERSION 2
 
 /* Status codes. */
 #define SF_OK 0
@@ -38,6 +38,8 @@ extern "C" {
 #define SF_EDEVICE 3 /* HIP runtime or kernel failure */
 #define SF_ENODEV 4  /* no gfx950 device, or the library was built without device code */
 #define SF_ERANGE 5  /* an input exceeds this library's limits, or an output buffer is too small */
+#define SF_ETIMEDOUT 6 /* sf_server_wait: the timeout passed before the request completed */
+#define SF_ECANCELED 7 /* the request was cancelled (sf_server_cancel) */
 
 /* align.rs:40-45 `Mode`. */
 #define SF_INFIX 0
@@ -216,7 +218,6 @@ typedef struct {
     int32_t affine_mode;
     sf_scoring sc;
     uint64_t band;
-    uint32_t fail_batch; /* test hook: the n-th batch (1-based) fails without a launch; 0 off */
 } sf_server_config;
 
 typedef struct {
@@ -238,16 +239,28 @@ typedef struct {
     double p50_ms, p99_ms, max_ms; /* submit-to-completion latency of the completed reads */
     double span_s;                 /* first submit to last completion */
     uint64_t cap_segments_per_batch, cap_read_bases_per_batch;
+    uint64_t max_segments_per_batch; /* the largest batch taken in since the last reset */
 } sf_server_stats;
 
 /* store_words: the transcripts, 2-bit packed as in sf_pairs, store_bases long; copied to HBM. */
 int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf_server_config *cfg,
                      sf_server **out);
-/* Queues one read; the request's input arrays may be reused after the call, its outputs must stay
- * valid until sf_server_wait returns for the ticket. */
+/* Queues one read; the request's input arrays may be reused after the call.  Its outputs must stay
+ * valid until the request completes: until a sf_server_wait on its ticket returns anything other
+ * than SF_ETIMEDOUT, or a sf_server_cancel of it returns SF_OK. */
 int sf_server_submit(sf_server *s, const sf_read_request *req, uint64_t *ticket);
-/* Waits for a ticket (timeout_s < 0: forever); returns the request's status. */
+/* Waits for a ticket (timeout_s < 0: forever).  Returns the request's status once it has completed
+ * (SF_OK, SF_ECANCELED, or the failure of its batch), after which the ticket is retired; or
+ * SF_ETIMEDOUT, after which the request is still queued or running and the ticket stays valid (wait
+ * again, or cancel it).  Several threads may wait on one ticket: each gets the status, and the
+ * ticket is retired when the last of them returns. */
 int sf_server_wait(sf_server *s, uint64_t ticket, double timeout_s);
+/* Cancels a request that has not completed.  SF_OK: it will never be computed or have its outputs
+ * written again (one already in a batch on the device finishes there, and its results are
+ * dropped); it completes with SF_ECANCELED, which a wait on its ticket returns.  SF_EINVAL for an
+ * unknown or retired ticket; a request that completed before the call keeps its status, and the
+ * call returns it. */
+int sf_server_cancel(sf_server *s, uint64_t ticket);
 int sf_server_get_stats(sf_server *s, sf_server_stats *out, int reset);
 /* Drains the queue (every request already submitted finishes), then frees the server.  With
  * SF_SERVE_TRACE set in the environment the server reports on stderr, at each stats reset and
@@ -274,17 +287,6 @@ int sf_set_affine_lp16_max(uint64_t slots);
  * 64); -1: never 8, else as 0.  Default 0 (or SF_AFFINE_LP8 from the environment).  Applies to
  * affine runs started afterwards in this process. */
 int sf_set_affine_lp8(int policy);
-
-/* Diagnostic (no reference counterpart): runs a one-wave kernel that records the lane moves of
- * the DPP and shuffle encodings the kernels rely on, 5 x 64 values into `out` (320 int32). */
-int sf_selftest_lanes(int32_t *out);
-
-/* Diagnostic (no reference counterpart): the VALU issue rate the kernels' rooflines are measured
- * against.  Runs `blocks` workgroups of 4 waves, each wave `iters` x 32 independent VALU
- * instructions of one kind (0: two-operand 32-bit integer ops; 1: packed 16-bit ops as in the
- * affine kernels; 2: DPP lane moves; 3: three-operand 32-bit integer ops as in the Myers step),
- * once untimed, then once timed: *ms. */
-int sf_probe_issue(int kind, uint32_t iters, uint32_t blocks, float *ms);
 
 #ifdef __cplusplus
 }

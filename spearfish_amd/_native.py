@@ -11,8 +11,10 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 ALIGN_LIB = os.path.join(HERE, "libsf_align.so")
 WORKLOAD_LIB = os.path.join(HERE, "libsf_workload.so")
+DIAG_LIB = os.path.join(HERE, "libsf_diag.so")
 
-SF_OK, SF_EINVAL, SF_ENOMEM, SF_EDEVICE, SF_ENODEV, SF_ERANGE = range(6)
+SF_OK, SF_EINVAL, SF_ENOMEM, SF_EDEVICE, SF_ENODEV, SF_ERANGE, SF_ETIMEDOUT, SF_ECANCELED = range(8)
+ABI_VERSION = 2
 
 
 class NativeError(RuntimeError):
@@ -87,7 +89,6 @@ class ServerConfig(C.Structure):
         ("affine_mode", C.c_int32),
         ("sc", Scoring),
         ("band", C.c_uint64),
-        ("fail_batch", C.c_uint32),
     ]
 
 
@@ -121,6 +122,7 @@ class ServerStats(C.Structure):
         ("span_s", C.c_double),
         ("cap_segments_per_batch", C.c_uint64),
         ("cap_read_bases_per_batch", C.c_uint64),
+        ("max_segments_per_batch", C.c_uint64),
     ]
 
 
@@ -148,6 +150,7 @@ SIGNATURES = {
     "sf_server_create": ([_vp, _u64, C.POINTER(ServerConfig), C.POINTER(_vp)], _int),
     "sf_server_submit": ([_vp, C.POINTER(ReadRequest), C.POINTER(_u64)], _int),
     "sf_server_wait": ([_vp, _u64, C.c_double], _int),
+    "sf_server_cancel": ([_vp, _u64], _int),
     "sf_server_get_stats": ([_vp, C.POINTER(ServerStats), _int], _int),
     "sf_server_destroy": ([_vp], None),
     "sf_server_replay": ([_vp, C.POINTER(ReadRequest), C.POINTER(C.c_double), _u64, C.POINTER(C.c_double),
@@ -163,6 +166,11 @@ SIGNATURES = {
     "sf_local_band_ends": ([_vp, _vp, _vp, _vp, C.c_int32, C.c_int32, _vp], _int),
     "sf_set_affine_lp16_max": ([C.c_uint64], _int),
     "sf_set_affine_lp8": ([C.c_int], _int),
+}
+
+# Every symbol include/sf_diag.h declares (libsf_diag.so: diagnostics for tests and measurements).
+DIAG_SIGNATURES = {
+    "sf_diag_last_error": ([], C.c_char_p),
     "sf_selftest_lanes": ([_vp], _int),
     "sf_probe_issue": ([_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)], _int),
 }
@@ -182,10 +190,34 @@ def load(path: str = ALIGN_LIB) -> C.CDLL:
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = res
-    if lib.sf_abi_version() != 1:
+    if lib.sf_abi_version() != ABI_VERSION:
         raise NativeError(SF_ENODEV, "ABI version mismatch")
     _lib = lib
     return lib
+
+
+_diag = None
+
+
+def load_diag(path: str = DIAG_LIB) -> C.CDLL:
+    """The diagnostics library (lane-move self-test, VALU issue probe)."""
+    global _diag
+    if _diag is not None:
+        return _diag
+    if not os.path.exists(path):
+        raise NativeError(SF_ENODEV, f"{path} is missing: run `python -m spearfish_amd.build`")
+    lib = C.CDLL(path)
+    for name, (args, res) in DIAG_SIGNATURES.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _diag = lib
+    return lib
+
+
+def check_diag(status: int) -> None:
+    if status != SF_OK:
+        raise NativeError(status, (load_diag().sf_diag_last_error() or b"").decode())
 
 
 def check(status: int) -> None:

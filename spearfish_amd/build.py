@@ -1,6 +1,7 @@
-"""Builds libsf_align.so (HIP, gfx950) and libsf_workload.so (host C++) in-tree.
+"""Builds libsf_align.so (HIP, gfx950), libsf_diag.so (HIP diagnostics for tests and
+measurements) and libsf_workload.so (host C++) in-tree.
 
-    python -m spearfish_amd.build          # both libraries, sources compiled in parallel
+    python -m spearfish_amd.build          # every library, sources compiled in parallel
 
 The libraries land next to this file so they travel with the source tree.
 """
@@ -24,8 +25,10 @@ HIP_FLAGS = [
 ]
 # Slowest to compile first: the build runs them on a pool of 8.
 HIP_SOURCES = ["affine_mix1.hip", "serve_mix.hip", "affine_pk.hip", "affine_mix.hip", "myers_mix.hip",
-               "affine_pk64.hip", "affine.hip", "myers.hip", "runtime.hip", "serve.hip", "probe.hip"]
+               "affine_pk64.hip", "affine.hip", "myers.hip", "runtime.hip", "serve.hip"]
+DIAG_SOURCES = ["probe.hip"]
 ALIGN_LIB = os.path.join(HERE, "libsf_align.so")
+DIAG_LIB = os.path.join(HERE, "libsf_diag.so")
 WORKLOAD_LIB = os.path.join(HERE, "libsf_workload.so")
 
 
@@ -37,21 +40,29 @@ def _run(cmd):
     return r
 
 
-def build_align() -> str:
+def _build_hip(sources, lib) -> str:
     os.makedirs(OBJ, exist_ok=True)
     objs = []
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(HIP_SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
         futs = []
-        for src in HIP_SOURCES:
+        for src in sources:
             obj = os.path.join(OBJ, src + ".o")
             objs.append(obj)
             futs.append(ex.submit(_run, [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]))
         for f in futs:
             f.result()
-    tmp = ALIGN_LIB + ".tmp"
+    tmp = lib + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, ALIGN_LIB)
-    return ALIGN_LIB
+    os.replace(tmp, lib)
+    return lib
+
+
+def build_align() -> str:
+    return _build_hip(HIP_SOURCES, ALIGN_LIB)
+
+
+def build_diag() -> str:
+    return _build_hip(DIAG_SOURCES, DIAG_LIB)
 
 
 def build_workload() -> str:
@@ -63,14 +74,13 @@ def build_workload() -> str:
 
 
 def build_all() -> None:
-    with cf.ThreadPoolExecutor(max_workers=2) as ex:
-        a = ex.submit(build_align)
-        w = ex.submit(build_workload)
-        a.result()
-        w.result()
+    with cf.ThreadPoolExecutor(max_workers=3) as ex:
+        for f in [ex.submit(build_align), ex.submit(build_diag), ex.submit(build_workload)]:
+            f.result()
 
 
 if __name__ == "__main__":
     build_all()
     print(ALIGN_LIB)
+    print(DIAG_LIB)
     print(WORKLOAD_LIB)

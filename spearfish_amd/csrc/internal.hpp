@@ -346,8 +346,6 @@ hipError_t sf_launch_desc(uint64_t n, const uint32_t *order, const sf::PairsView
                           sf::PairDesc *desc, hipStream_t s);
 hipError_t sf_launch_compact(const uint32_t *order, uint32_t count, const uint32_t *runs, const uint64_t *runs_off,
                              const uint32_t *nruns, const uint64_t *out_off, uint32_t *out, hipStream_t s);
-hipError_t sf_launch_probe(int32_t *out, hipStream_t s);
-hipError_t sf_launch_issue_probe(int kind, uint32_t iters, uint32_t blocks, uint32_t *out, hipStream_t s);
 
 // Supported shape classes.
 int sf_myers_maxb_for(uint32_t nblocks);     // 0 if unsupported

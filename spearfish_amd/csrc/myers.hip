@@ -32,16 +32,6 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
     myers_pipe_body<G, R>(a, blockIdx.x);
 }
 
-__global__ void probe_kernel(int32_t *out) {
-    // Lane-movement check of the DPP encodings the kernels rely on: 5 x 64 values.
-    const int32_t lane = threadIdx.x;
-    out[lane] = dpp<kWaveShl1>(-1, lane);
-    out[64 + lane] = dpp<kWaveShr1>(-1, lane);
-    out[128 + lane] = dpp<kRowShr1>(-1, lane);
-    out[192 + lane] = wave_excl_max(lane * 3 % 17, -100);
-    out[256 + lane] = __shfl_xor(lane, 5);
-}
-
 } // namespace
 
 // Shape classes: lane kernel MAXB (1..24; more planes would spill), then pipelined classes
@@ -100,10 +90,6 @@ hipError_t sf_launch_myers(const MyersArgs &a, int maxb, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t sf_launch_probe(int32_t *out, hipStream_t s) {
-    hipLaunchKernelGGL(probe_kernel, dim3(1), dim3(64), 0, s, out);
-    return hipGetLastError();
-}
 
 // Loads this file's code object now (see sf_load_code_objects).
 hipError_t sf_load_code_myers() {

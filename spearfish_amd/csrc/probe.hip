@@ -1,5 +1,11 @@
-// probe.hip -- issue-rate probes: kernels with a known number of VALU instructions, to measure the
-// chip's VALU issue rate per instruction kind (the denominator of the kernels' VALU rooflines).
+// probe.hip -- libsf_diag.so (include/sf_diag.h): diagnostics for tests and measurements, kept out
+// of the alignment library.
+//
+// sf_selftest_lanes: a one-wave kernel that records the lane moves of the DPP and shuffle
+// encodings the alignment kernels rely on.
+//
+// sf_probe_issue: issue-rate probes, kernels with a known number of VALU instructions, to measure
+// the chip's VALU issue rate per instruction kind (the denominator of the kernels' VALU rooflines).
 //
 // Each wave runs `iters` iterations of 32 VALU instructions: 8 independent register chains, each
 // register written every 8 instructions, so no dependency or DPP hazard stalls the wave, and 4
@@ -7,9 +13,27 @@
 // 1 = packed 16-bit ops of the affine kernels (VOP3P: v_pk_max_u16, v_pk_sub_u16, v_pk_mad_u16,
 // v_pk_min_u16), 2 = DPP moves (v_mov_b32_dpp row_shr:1), 3 = three-operand 32-bit integer ops of
 // the Myers block step (VOP3: v_bitop3, v_alignbit, v_lshl_or, v_add3).
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/sf_diag.h"
 #include "internal.hpp"
 
+using namespace sf;
+
 namespace {
+
+__global__ void lane_probe_kernel(int32_t *out) {
+    // Lane-movement check of the DPP encodings the kernels rely on: 5 x 64 values.
+    const int32_t lane = threadIdx.x;
+    out[lane] = dpp<kWaveShl1>(-1, lane);
+    out[64 + lane] = dpp<kWaveShr1>(-1, lane);
+    out[128 + lane] = dpp<kRowShr1>(-1, lane);
+    out[192 + lane] = wave_excl_max(lane * 3 % 17, -100);
+    out[256 + lane] = __shfl_xor(lane, 5);
+}
 
 #define SF_R8(OP) OP(0) OP(1) OP(2) OP(3) OP(4) OP(5) OP(6) OP(7)
 
@@ -68,9 +92,7 @@ __global__ __launch_bounds__(256) void issue_probe_kernel(uint32_t iters, uint32
 
 #undef SF_R8
 
-} // namespace
-
-hipError_t sf_launch_issue_probe(int kind, uint32_t iters, uint32_t blocks, uint32_t *out, hipStream_t s) {
+hipError_t launch_issue_probe(int kind, uint32_t iters, uint32_t blocks, uint32_t *out, hipStream_t s) {
     switch (kind) {
     case 0: hipLaunchKernelGGL(issue_probe_kernel<0>, dim3(blocks), dim3(256), 0, s, iters, out); break;
     case 1: hipLaunchKernelGGL(issue_probe_kernel<1>, dim3(blocks), dim3(256), 0, s, iters, out); break;
@@ -80,3 +102,69 @@ hipError_t sf_launch_issue_probe(int kind, uint32_t iters, uint32_t blocks, uint
     }
     return hipGetLastError();
 }
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int ensure_gfx950() {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(SF_DIAG_ENODEV, "no HIP device visible");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return fail(SF_DIAG_EDEVICE, "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SF_DIAG_ENODEV, "device 0 is %s, this library is built for gfx950", prop.gcnArchName);
+    if (hipSetDevice(0) != hipSuccess) return fail(SF_DIAG_EDEVICE, "hipSetDevice");
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *sf_diag_last_error(void) { return g_err.c_str(); }
+
+int sf_selftest_lanes(int32_t *out) {
+    if (!out) return fail(SF_DIAG_EINVAL, "out is null");
+    if (const int st = ensure_gfx950()) return st;
+    int32_t *d = nullptr;
+    if (hipMalloc((void **)&d, 320 * 4) != hipSuccess) return fail(SF_DIAG_EDEVICE, "hipMalloc");
+    hipLaunchKernelGGL(lane_probe_kernel, dim3(1), dim3(64), 0, nullptr, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, d, 320 * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(SF_DIAG_EDEVICE, "lane probe: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int sf_probe_issue(int kind, uint32_t iters, uint32_t blocks, float *ms) {
+    if (kind < 0 || kind > 3 || blocks == 0 || !ms) return fail(SF_DIAG_EINVAL, "probe: kind 0..3, blocks > 0, ms");
+    if (const int st = ensure_gfx950()) return st;
+    uint32_t *d = nullptr;
+    if (hipMalloc((void **)&d, (uint64_t)blocks * 256 * 4) != hipSuccess) return fail(SF_DIAG_EDEVICE, "hipMalloc");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_issue_probe(kind, iters, blocks, d, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = launch_issue_probe(kind, iters, blocks, d, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(SF_DIAG_EDEVICE, "probe: %s", hipGetErrorString(e));
+    return 0;
+}
+
+} // extern "C"

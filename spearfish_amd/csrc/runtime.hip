@@ -1684,42 +1684,4 @@ int sf_affine_cigar(const sf_pairs *pairs, int mode, const sf_scoring *sc, const
     return st;
 }
 
-// Diagnostic: lane movements of the DPP encodings the kernels use (320 values).
-int sf_selftest_lanes(int32_t *out) {
-    int st;
-    if ((st = ensure_device())) return st;
-    int32_t *d = nullptr;
-    if ((st = dalloc(&d, 320))) return st;
-    hipError_t e = sf_launch_probe(d, nullptr);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(out, d, 320 * 4, hipMemcpyDeviceToHost);
-    dfree(d);
-    if (e != hipSuccess) return fail(SF_EDEVICE, "probe: %s", hipGetErrorString(e));
-    return SF_OK;
-}
-
-// Diagnostic: `blocks` workgroups of 4 waves, each wave `iters` x 32 VALU instructions of one kind;
-// *ms = the launch's time (HIP events, after one untimed launch).
-int sf_probe_issue(int kind, uint32_t iters, uint32_t blocks, float *ms) {
-    if (kind < 0 || kind > 3 || blocks == 0 || !ms) return fail(SF_EINVAL, "probe: kind 0..3, blocks > 0, ms");
-    int st;
-    if ((st = ensure_device())) return st;
-    uint32_t *d = nullptr;
-    if ((st = dalloc(&d, (uint64_t)blocks * 256))) return st;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess) e = sf_launch_issue_probe(kind, iters, blocks, d, nullptr);
-    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
-    if (e == hipSuccess) e = sf_launch_issue_probe(kind, iters, blocks, d, nullptr);
-    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    dfree(d);
-    if (e != hipSuccess) return fail(SF_EDEVICE, "probe: %s", hipGetErrorString(e));
-    return SF_OK;
-}
-
 } // extern "C"

@@ -52,6 +52,8 @@ int sf_runtime_ensure();                        // runtime.hip
 
 namespace {
 
+// A request.  Requests are recycled through the server's pool (their vectors keep their
+// capacity), so that a submission in steady state allocates nothing.
 struct Req {
     uint64_t ticket = 0;
     std::vector<uint32_t> words; // the read, 2-bit packed
@@ -62,6 +64,19 @@ struct Req {
     Clock::time_point t_submit, t_done;
     int status = SF_OK;
     bool done = false;
+    bool queued = false;    // waiting in `pending` (not yet taken into a batch)
+    bool cancelled = false; // results are dropped (set under copy_mu, read by copy_out under it)
+    uint32_t waiters = 0;   // threads inside sf_server_wait for this ticket
+
+    void reset() {
+        ticket = 0;
+        words.clear(), roff.clear(), rlen.clear(), tlen.clear(), toff.clear();
+        dist = end = col = nullptr;
+        score = nullptr;
+        status = SF_OK;
+        done = queued = cancelled = false;
+        waiters = 0;
+    }
 };
 
 struct Slot {
@@ -155,42 +170,86 @@ void key_order(const std::vector<uint64_t> &key, uint32_t *order, std::vector<ui
     if (src != order) std::memcpy(order, src, (size_t)n * 4);
 }
 
-double pct(std::vector<double> v, double q) {
+// The q-quantile of v (linear between the two nearest ranks); reorders v.
+double pct(std::vector<double> &v, double q) {
     if (v.empty()) return 0;
-    std::sort(v.begin(), v.end());
     const double pos = q * (double)(v.size() - 1);
     const size_t lo = (size_t)pos;
-    const size_t hi = std::min(lo + 1, v.size() - 1);
-    return v[lo] + (v[hi] - v[lo]) * (pos - (double)lo);
+    std::nth_element(v.begin(), v.begin() + lo, v.end());
+    const double a = v[lo];
+    const double b = lo + 1 < v.size() ? *std::min_element(v.begin() + lo + 1, v.end()) : a;
+    return a + (b - a) * (pos - (double)lo);
 }
 
 } // namespace
 
 struct sf_server {
     sf_server_config cfg{};
+    // SF_SERVE_FAIL_BATCH=n (read at creation): the n-th batch (1-based) fails without a launch.
+    // A fault injected from the host for tests of failure handling; 0 (unset) is off.
+    uint64_t fail_batch = 0;
     uint64_t store_bases = 0, store_words = 0;
     uint32_t *d_seq = nullptr; // [store words | slot read areas]
     uint64_t cap_words = 0, cap_seg = 0;
     std::vector<Slot> slots;
     std::mutex mu;
+    // Held while a finished batch's results are copied into the callers' arrays, and by
+    // sf_server_cancel while it marks a request in a batch as cancelled, so that no copy into a
+    // cancelled request's outputs happens after the cancel returns.  Order: copy_mu before mu.
+    std::mutex copy_mu;
     std::condition_variable cv_sched, cv_done;
     std::deque<Req *> pending;
+    // Recycled requests (pool_mu).  Filled at creation, so that the first submissions allocate
+    // nothing either.
+    std::mutex pool_mu;
+    std::vector<Req *> pool;
+    static constexpr size_t kPoolPrefill = 1024, kPoolMax = 1u << 16;
     std::unordered_map<uint64_t, Req *> reqs;
     uint64_t next_ticket = 1, launches = 0;
     bool stop = false;
     std::thread th;
     // stats
     std::vector<double> lat_ms;
-    uint64_t reads_done = 0, reads_failed = 0, batches = 0, segments = 0, batch_reads = 0;
+    uint64_t reads_done = 0, reads_failed = 0, batches = 0, segments = 0, batch_reads = 0, max_batch_seg = 0;
     uint64_t cells = 0;
     Clock::time_point t_first, t_last;
     bool any = false;
-    // SF_SERVE_TRACE set: the longest scheduler iteration, launch and gap between submissions
-    // (with when they ended, from creation), reported on stderr when the server is destroyed.
+    // SF_SERVE_TRACE set: the longest scheduler iteration (busy time: an idle wait for requests is
+    // not counted), launch and gap between submissions (with when they ended, from creation),
+    // reported on stderr at each stats reset and when the server is destroyed.  Guarded by
+    // trace_mu (the scheduler and submitters write them, sf_server_get_stats reads them).
     bool trace = std::getenv("SF_SERVE_TRACE") != nullptr;
+    std::mutex trace_mu;
     Clock::time_point t_created = Clock::now(), t_last_submit{};
     double max_iter_ms = 0, max_iter_at = 0, max_launch_ms = 0, max_launch_at = 0, max_submit_gap_ms = 0,
            max_submit_gap_at = 0, max_build_ms = 0, max_lock_ms = 0;
+
+    Req *take_req() {
+        {
+            std::lock_guard<std::mutex> lk(pool_mu);
+            if (!pool.empty()) {
+                Req *r = pool.back();
+                pool.pop_back();
+                return r;
+            }
+        }
+        return new Req();
+    }
+    void give_req(Req *r) {
+        r->reset();
+        {
+            std::lock_guard<std::mutex> lk(pool_mu);
+            if (pool.size() < kPoolMax) {
+                pool.push_back(r);
+                return;
+            }
+        }
+        delete r;
+    }
+    void trace_max(double &field, double &at, double ms, Clock::time_point when) {
+        std::lock_guard<std::mutex> lk(trace_mu);
+        if (ms > field) field = ms, at = std::chrono::duration<double>(when - t_created).count();
+    }
 
     void loop();
     void launch(Slot &s);
@@ -198,16 +257,18 @@ struct sf_server {
     void publish(Slot &s);
 };
 
-// Results of a finished batch into the callers' arrays.  Runs without the lock: a request is
-// not freed before it is marked done (sf_server_wait), and its fields do not change after
-// submission.
+// Results of a finished batch into the callers' arrays.  Runs without the server's lock: a
+// request is not freed before it is marked done (sf_server_wait), and its fields do not change
+// after submission; copy_mu keeps a cancel from overlapping the copy.
 void sf_server::copy_out(Slot &s, int status) {
     const uint32_t ns = s.nseg;
     const uint32_t *rd = s.h_res, *re = s.h_res + ns, *rc = s.h_res + 3 * (uint64_t)ns;
     const int32_t *rs = reinterpret_cast<const int32_t *>(s.h_res + 2 * (uint64_t)ns);
     if (status == SF_OK) {
+        std::lock_guard<std::mutex> lk(copy_mu);
         for (size_t q = 0; q < s.reqs.size(); ++q) {
             Req *r = s.reqs[q];
+            if (r->cancelled) continue;
             const uint32_t k0 = s.seg0[q], n = (uint32_t)r->rlen.size();
             if (r->dist) std::memcpy(r->dist, rd + k0, n * 4);
             if (r->end) std::memcpy(r->end, re + k0, n * 4);
@@ -223,18 +284,22 @@ void sf_server::copy_out(Slot &s, int status) {
 void sf_server::publish(Slot &s) {
     const auto now = Clock::now();
     for (Req *r : s.reqs) {
-        if (s.status == SF_OK) {
+        if (r->cancelled) {
+            r->status = SF_ECANCELED;
+        } else if (s.status == SF_OK) {
             ++reads_done;
             lat_ms.push_back(std::chrono::duration<double, std::milli>(now - r->t_submit).count());
+            r->status = SF_OK;
         } else {
             ++reads_failed;
+            r->status = s.status;
         }
-        r->status = s.status;
         r->t_done = now;
         r->done = true;
     }
     ++batches;
     segments += s.nseg;
+    max_batch_seg = std::max<uint64_t>(max_batch_seg, s.nseg);
     batch_reads += s.reqs.size();
     cells += s.cells;
     t_last = now;
@@ -340,7 +405,7 @@ void sf_server::launch(Slot &s) {
     }
     uint32_t *dist = s.d_res, *end = s.d_res + seg, *col = s.d_res + 3 * (uint64_t)seg;
     int32_t *score = reinterpret_cast<int32_t *>(s.d_res + 2 * (uint64_t)seg);
-    const bool inject = cfg.fail_batch && s.launch_no == cfg.fail_batch;
+    const bool inject = fail_batch && s.launch_no == fail_batch;
     if (!inject) {
         MyersArgs ma{};
         ma.pv = pv;
@@ -427,9 +492,7 @@ void sf_server::loop() {
     for (;;) {
         if (trace) {
             const auto now = Clock::now();
-            const double ms = std::chrono::duration<double, std::milli>(now - t_iter).count();
-            if (ms > max_iter_ms)
-                max_iter_ms = ms, max_iter_at = std::chrono::duration<double>(now - t_created).count();
+            trace_max(max_iter_ms, max_iter_at, std::chrono::duration<double, std::milli>(now - t_iter).count(), now);
             t_iter = now;
         }
         // Batches that ended: results out to the callers (no lock).
@@ -469,6 +532,7 @@ void sf_server::loop() {
                     break;
                 words += r->words.size();
                 seg += r->rlen.size();
+                r->queued = false;
                 s.reqs.push_back(r);
                 pending.pop_front();
             }
@@ -481,8 +545,12 @@ void sf_server::loop() {
         if (stop && pending.empty() && !busy) break;
         if (fresh.empty()) {
             if (!ended) {
-                if (busy) cv_sched.wait_for(lk, std::chrono::microseconds(10));
-                else cv_sched.wait(lk, [&] { return stop || !pending.empty(); });
+                if (busy) {
+                    cv_sched.wait_for(lk, std::chrono::microseconds(10));
+                } else {
+                    cv_sched.wait(lk, [&] { return stop || !pending.empty(); });
+                    if (trace) t_iter = Clock::now(); // idle, not an iteration's work
+                }
             }
             continue;
         }
@@ -492,9 +560,7 @@ void sf_server::loop() {
             launch(*s);
             if (trace) {
                 const auto t1 = Clock::now();
-                const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-                if (ms > max_launch_ms)
-                    max_launch_ms = ms, max_launch_at = std::chrono::duration<double>(t1 - t_created).count();
+                trace_max(max_launch_ms, max_launch_at, std::chrono::duration<double, std::milli>(t1 - t0).count(), t1);
             }
         }
     }
@@ -525,6 +591,7 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
     if (share < (1u << 16)) return sf_runtime_fail(SF_EINVAL, "hbm budget too small for the slots");
     auto *sv = new sf_server();
     sv->cfg = *cfg;
+    if (const char *e = std::getenv("SF_SERVE_FAIL_BATCH")) sv->fail_batch = std::strtoull(e, nullptr, 10);
     sv->store_bases = store_bases;
     sv->store_words = (store_bases + 15) / 16;
     sv->cap_words = share / 2 / 4;
@@ -554,6 +621,17 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
             !host_alloc(&s.h_res, 4 * c))
             return bail("slot pinned buffers");
     }
+    // The request pool, each request sized for a long read (2,048 words: 32 kb) of 16 segments,
+    // its memory touched now rather than during the first submissions.
+    sv->pool.reserve(sf_server::kPoolMax);
+    for (size_t q = 0; q < sf_server::kPoolPrefill; ++q) {
+        Req *r = new Req();
+        r->words.resize(2048);
+        for (auto *v : {&r->roff, &r->rlen, &r->tlen}) v->resize(16);
+        r->toff.resize(16);
+        r->reset();
+        sv->pool.push_back(r);
+    }
     sv->th = std::thread([sv] { sv->loop(); });
     *out = sv;
     return SF_OK;
@@ -565,7 +643,7 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
         return sf_runtime_fail(SF_EINVAL, "null segment arrays");
     if (rq->read_len && !rq->read) return sf_runtime_fail(SF_EINVAL, "read is null");
     const auto t_enter = Clock::now();
-    auto *r = new Req();
+    Req *r = sv->take_req();
     r->roff.assign(rq->seg_read_off, rq->seg_read_off + rq->nseg);
     r->rlen.assign(rq->seg_read_len, rq->seg_read_len + rq->nseg);
     r->toff.assign(rq->seg_tx_off, rq->seg_tx_off + rq->nseg);
@@ -580,7 +658,7 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
                            scores_in_range(sv->cfg.sc.mat, sv->cfg.sc.mis, sv->cfg.sc.open, sv->cfg.sc.ext, m, n,
                                            aff_slots(K));
         if (bad || !shape) {
-            delete r;
+            sv->give_req(r);
             return sf_runtime_fail(bad ? SF_EINVAL : SF_ERANGE, bad ? "segment outside the read or the store"
                                                                     : "segment beyond this library's shape classes");
         }
@@ -594,7 +672,7 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
         r->words[w] = x;
     }
     if (r->words.size() > sv->cap_words || r->rlen.size() > sv->cap_seg) {
-        delete r;
+        sv->give_req(r);
         return sf_runtime_fail(SF_ENOMEM, "request exceeds one slot's share of the HBM budget");
     }
     r->dist = rq->dist;
@@ -606,14 +684,9 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
         std::lock_guard<std::mutex> lk(sv->mu);
         if (sv->trace) {
             const auto t_locked = Clock::now();
+            std::lock_guard<std::mutex> tl(sv->trace_mu);
             sv->max_build_ms = std::max(sv->max_build_ms, std::chrono::duration<double, std::milli>(r->t_submit - t_enter).count());
             sv->max_lock_ms = std::max(sv->max_lock_ms, std::chrono::duration<double, std::milli>(t_locked - r->t_submit).count());
-        }
-        if (sv->stop) {
-            delete r;
-            return sf_runtime_fail(SF_EINVAL, "server is shutting down");
-        }
-        if (sv->trace) {
             if (sv->t_last_submit != Clock::time_point{}) {
                 const double ms = std::chrono::duration<double, std::milli>(r->t_submit - sv->t_last_submit).count();
                 if (ms > sv->max_submit_gap_ms)
@@ -622,7 +695,12 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
             }
             sv->t_last_submit = r->t_submit;
         }
+        if (sv->stop) {
+            sv->give_req(r);
+            return sf_runtime_fail(SF_EINVAL, "server is shutting down");
+        }
         r->ticket = sv->next_ticket++;
+        r->queued = true;
         sv->reqs[r->ticket] = r;
         sv->pending.push_back(r);
         if (!sv->any) {
@@ -642,19 +720,48 @@ int sf_server_wait(sf_server *sv, uint64_t ticket, double timeout_s) {
     if (it == sv->reqs.end()) return sf_runtime_fail(SF_EINVAL, "unknown ticket");
     Req *r = it->second;
     auto pred = [&] { return r->done; };
+    ++r->waiters;
+    bool done = true;
     if (timeout_s < 0) sv->cv_done.wait(lk, pred);
-    else if (!sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
-        return sf_runtime_fail(SF_ERANGE, "timed out");
+    else done = sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
+    --r->waiters;
+    if (!done) return sf_runtime_fail(SF_ETIMEDOUT, "timed out; the request is still queued or running");
     const int st = r->status;
-    sv->reqs.erase(it);
-    delete r;
+    if (r->waiters == 0) { // the last waiter retires the ticket
+        sv->reqs.erase(ticket);
+        lk.unlock();
+        sv->give_req(r);
+    }
+    if (st == SF_ECANCELED) return sf_runtime_fail(st, "the request was cancelled");
     if (st != SF_OK) return sf_runtime_fail(st, "the request's batch failed");
+    return SF_OK;
+}
+
+int sf_server_cancel(sf_server *sv, uint64_t ticket) {
+    if (!sv) return sf_runtime_fail(SF_EINVAL, "server is null");
+    std::lock_guard<std::mutex> ck(sv->copy_mu); // no batch's results are being copied out meanwhile
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return sf_runtime_fail(SF_EINVAL, "unknown ticket");
+    Req *r = it->second;
+    if (r->done) return r->status == SF_OK ? SF_OK : sf_runtime_fail(r->status, "the request completed before");
+    if (r->queued) { // not in a batch yet: it completes now
+        sv->pending.erase(std::find(sv->pending.begin(), sv->pending.end(), r));
+        r->queued = false;
+        r->status = SF_ECANCELED;
+        r->t_done = Clock::now();
+        r->done = true;
+        sv->cv_done.notify_all();
+    } else { // in a batch: it finishes on the device, and its results are dropped
+        r->cancelled = true;
+    }
     return SF_OK;
 }
 
 int sf_server_get_stats(sf_server *sv, sf_server_stats *out, int reset) {
     if (!sv || !out) return sf_runtime_fail(SF_EINVAL, "null argument");
-    std::lock_guard<std::mutex> lk(sv->mu);
+    std::vector<double> lat; // copied under the lock, ranked outside it
+    std::unique_lock<std::mutex> lk(sv->mu);
     std::memset(out, 0, sizeof *out);
     out->reads_done = sv->reads_done;
     out->reads_failed = sv->reads_failed;
@@ -662,24 +769,31 @@ int sf_server_get_stats(sf_server *sv, sf_server_stats *out, int reset) {
     out->segments = sv->segments;
     out->cells = sv->cells;
     out->mean_reads_per_batch = sv->batches ? (double)sv->batch_reads / (double)sv->batches : 0;
-    out->p50_ms = pct(sv->lat_ms, 0.50);
-    out->p99_ms = pct(sv->lat_ms, 0.99);
-    out->max_ms = sv->lat_ms.empty() ? 0 : *std::max_element(sv->lat_ms.begin(), sv->lat_ms.end());
+    lat = sv->lat_ms;
     out->span_s = sv->any ? std::chrono::duration<double>(sv->t_last - sv->t_first).count() : 0;
     out->cap_segments_per_batch = sv->cap_seg;
     out->cap_read_bases_per_batch = sv->cap_words * 16;
+    out->max_segments_per_batch = sv->max_batch_seg;
     if (reset) {
         sv->lat_ms.clear();
         sv->reads_done = sv->reads_failed = sv->batches = sv->segments = sv->batch_reads = sv->cells = 0;
+        sv->max_batch_seg = 0;
         sv->any = false;
+    }
+    lk.unlock();
+    out->p50_ms = pct(lat, 0.50);
+    out->p99_ms = pct(lat, 0.99);
+    out->max_ms = lat.empty() ? 0 : *std::max_element(lat.begin(), lat.end());
+    if (reset) {
         if (sv->trace) {
+            std::lock_guard<std::mutex> tl(sv->trace_mu);
             if (sv->max_iter_ms > 0)
                 std::fprintf(stderr, "sf_server: longest scheduler iteration %.3f ms (ended %.3f s), launch %.3f ms "
                                      "(%.3f s), gap between submissions %.3f ms (%.3f s), request build %.3f ms, "
                                      "lock wait %.3f ms\n",
                              sv->max_iter_ms, sv->max_iter_at, sv->max_launch_ms, sv->max_launch_at,
                              sv->max_submit_gap_ms, sv->max_submit_gap_at, sv->max_build_ms, sv->max_lock_ms);
-            sv->max_iter_ms = sv->max_launch_ms = sv->max_submit_gap_ms = 0; // (the scheduler's own, read racily)
+            sv->max_iter_ms = sv->max_launch_ms = sv->max_submit_gap_ms = 0;
             sv->max_build_ms = sv->max_lock_ms = 0;
             sv->t_last_submit = Clock::time_point{};
         }
@@ -702,6 +816,7 @@ void sf_server_destroy(sf_server *sv) {
                      sv->max_iter_ms, sv->max_iter_at, sv->max_launch_ms, sv->max_launch_at, sv->max_submit_gap_ms,
                      sv->max_submit_gap_at);
     for (auto &kv : sv->reqs) delete kv.second;
+    for (Req *r : sv->pool) delete r;
     for (Slot &s : sv->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         (void)hipFree(s.d_stage), (void)hipFree(s.d_res);
@@ -722,18 +837,31 @@ int sf_server_replay(sf_server *sv, const sf_read_request *rq, const double *t_a
     if (!sv || (n && (!rq || !t_arrive))) return sf_runtime_fail(SF_EINVAL, "null argument");
     std::vector<uint64_t> tickets(n, 0);
     std::vector<int> sub(n, SF_OK);
+    // The waits run beside the submissions (each request, once completed, goes back to the pool
+    // for a later submission to reuse), on the calling thread.
+    std::mutex m;
+    std::condition_variable cv;
+    uint64_t submitted = 0;
     const auto t0 = Clock::now();
     std::thread submitter([&] {
         for (uint64_t i = 0; i < n; ++i) {
             const auto due = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(t_arrive[i]));
             std::this_thread::sleep_until(due);
             sub[i] = sf_server_submit(sv, &rq[i], &tickets[i]);
+            {
+                std::lock_guard<std::mutex> lk(m);
+                submitted = i + 1;
+            }
+            cv.notify_one();
         }
     });
-    submitter.join();
     int status = SF_OK;
     uint64_t nref = 0;
     for (uint64_t i = 0; i < n; ++i) {
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return submitted > i; });
+        }
         if (sub[i] != SF_OK) {
             ++nref; // refused at admission: counted, not fatal
             continue;
@@ -741,6 +869,7 @@ int sf_server_replay(sf_server *sv, const sf_read_request *rq, const double *t_a
         const int st = sf_server_wait(sv, tickets[i], -1.0);
         if (st != SF_OK) status = st;
     }
+    submitter.join();
     if (refused) *refused = nref;
     const auto t1 = Clock::now();
     if (wall_s) *wall_s = std::chrono::duration<double>(t1 - t0).count() - (n ? t_arrive[0] : 0.0);

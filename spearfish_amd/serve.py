@@ -131,11 +131,11 @@ def requests(reads: Reads, res: Results, idx=None):
 class Server:
     def __init__(self, store: Store, hbm_budget_bytes: int = 256 << 20, slots: int = 4, max_reads_per_batch: int = 0,
                  myers_mode: Mode = Mode.Infix, affine_mode: Mode = Mode.Global, sc: Affine = Affine.ONT,
-                 band: int = 64, fail_batch: int = 0):
+                 band: int = 64):
         self.lib = N.load()
         self.store = store
         cfg = N.ServerConfig(hbm_budget_bytes, slots, max_reads_per_batch, int(myers_mode), int(affine_mode), sc.c(),
-                             band, fail_batch)
+                             band)
         h = C.c_void_p()
         N.check(self.lib.sf_server_create(store.words.ctypes.data, store.bases, C.byref(cfg), C.byref(h)))
         self.h = h
@@ -162,6 +162,9 @@ class Server:
 
     def wait(self, ticket: int, timeout: float = -1.0) -> int:
         return self.lib.sf_server_wait(self.h, ticket, timeout)
+
+    def cancel(self, ticket: int) -> int:
+        return self.lib.sf_server_cancel(self.h, ticket)
 
     def stats(self, reset: bool = False) -> N.ServerStats:
         s = N.ServerStats()

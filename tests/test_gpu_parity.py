@@ -26,7 +26,7 @@ def test_lane_moves():
     import ctypes as C
     from spearfish_amd import _native as N
     out = (C.c_int32 * 320)()
-    N.check(N.load().sf_selftest_lanes(C.cast(out, C.c_void_p)))
+    N.check_diag(N.load_diag().sf_selftest_lanes(C.cast(out, C.c_void_p)))
     v = list(out)
     assert v[0:64] == [l + 1 for l in range(63)] + [-1], "wave_shl:1"
     assert v[64:128] == [-1] + list(range(63)), "wave_shr:1"
@@ -425,9 +425,9 @@ def test_issue_probe_runs_every_kind():
     # The VALU issue-rate probe behind the rooflines: every kind runs and takes measurable time;
     # an unknown kind is refused.
     import ctypes as C
-    lib = A.N.load()
+    lib = A.N.load_diag()
     ms = C.c_float()
     for kind in range(4):
-        A.N.check(lib.sf_probe_issue(kind, 200, 64, C.byref(ms)))
+        A.N.check_diag(lib.sf_probe_issue(kind, 200, 64, C.byref(ms)))
         assert ms.value > 0
     assert lib.sf_probe_issue(4, 200, 64, C.byref(ms)) != 0

@@ -70,11 +70,13 @@ def test_admission_refuses_requests_over_the_budget(workload):
         assert s.submit_status(r) == N.SF_ENOMEM
 
 
-def test_injected_batch_failure_fails_only_its_reads(workload):
+def test_injected_batch_failure_fails_only_its_reads(workload, monkeypatch):
+    # SF_SERVE_FAIL_BATCH (read when a server is created): the n-th batch fails without a launch.
     store, reads = workload
     res = S.Results(reads)
     reqs = S.requests(reads, res, idx=range(6))
-    with S.Server(store, slots=1, max_reads_per_batch=1, band=BAND, fail_batch=3) as s:
+    monkeypatch.setenv("SF_SERVE_FAIL_BATCH", "3")
+    with S.Server(store, slots=1, max_reads_per_batch=1, band=BAND) as s:
         status = []
         for i in range(6):
             status.append(s.wait(s.submit(reqs[i])))
@@ -167,7 +169,7 @@ def test_large_batches_equal_reads_alone():
         s.stats(reset=True)
         s.replay(S.requests(reads, mix), np.zeros(n))
         st = s.stats()
-        assert st.reads_done == n and st.mean_reads_per_batch * 8 > 2048, st.mean_reads_per_batch
+        assert st.reads_done == n and st.max_segments_per_batch > 2048, st.max_segments_per_batch
     for a in ("dist", "end", "score", "score_col"):
         assert np.array_equal(getattr(alone, a), getattr(mix, a)), a
     codes = store_codes(store)
@@ -182,3 +184,68 @@ def test_large_batches_equal_reads_alone():
             assert (int(mix.dist[q]), int(mix.end[q])) == (d, e)
             dp = O.affine_dp(p, t, O.GLOBAL, O.ONT, BAND)
             assert (int(mix.score[q]), int(mix.score_col[q])) == (dp.score, dp.score_col)
+
+
+def test_timed_out_wait_then_wait_again_and_cancel(workload):
+    # One slot, one read a batch, 120 reads offered at once: the last ones are still queued right
+    # after submission.  A wait that times out leaves its request queued and its ticket valid (the
+    # next wait returns the result); a cancelled request completes with SF_ECANCELED and its
+    # ticket is retired by that wait.
+    store, reads = workload
+    n = 120
+    alone, res = S.Results(reads), S.Results(reads)
+    ra, rr = S.requests(reads, alone, idx=range(n)), S.requests(reads, res, idx=range(n))
+    with S.Server(store, slots=1, max_reads_per_batch=1, band=BAND) as s:
+        for i in range(n):
+            assert s.wait(s.submit(ra[i])) == N.SF_OK
+        tickets = [s.submit(rr[i]) for i in range(n)]
+        assert s.wait(tickets[-2], 0.0) == N.SF_ETIMEDOUT
+        assert s.wait(tickets[-1], 0.0) == N.SF_ETIMEDOUT
+        assert s.cancel(tickets[-1]) == N.SF_OK
+        assert s.wait(tickets[-1]) == N.SF_ECANCELED
+        assert s.wait(tickets[-1]) == N.SF_EINVAL  # retired
+        assert s.cancel(tickets[-1]) == N.SF_EINVAL
+        for t in tickets[:-1]:
+            assert s.wait(t) == N.SF_OK
+    last = reads.seg0[n - 1]
+    for a in ("dist", "end", "score", "score_col"):
+        assert np.array_equal(getattr(alone, a)[:last], getattr(res, a)[:last]), a
+    # The cancelled read was still queued: nothing was written to its outputs.
+    assert (res.dist[last:last + reads.nseg[n - 1]] == 0xFFFFFFFF).all()
+
+
+def test_cancel_many_and_concurrent_waiters(workload):
+    # One slot, one read a batch, 200 reads offered at once: the last read waits behind 199
+    # batches.  Two threads wait on its ticket (both get its status, then the ticket is retired)
+    # while the main thread cancels the reads before it in the second half.
+    import threading
+    store, reads = workload
+    n = 200
+    alone, res = S.Results(reads), S.Results(reads)
+    ra, rr = S.requests(reads, alone, idx=range(n)), S.requests(reads, res, idx=range(n))
+    with S.Server(store, slots=1, max_reads_per_batch=1, band=BAND) as s:
+        for i in range(n):
+            assert s.wait(s.submit(ra[i])) == N.SF_OK
+        s.stats(reset=True)
+        tickets = [s.submit(rr[i]) for i in range(n)]
+        assert s.wait(tickets[-1], 0.0) == N.SF_ETIMEDOUT
+        got = []
+        th = [threading.Thread(target=lambda: got.append(s.wait(tickets[-1]))) for _ in range(2)]
+        for t in th:
+            t.start()
+        cancelled = [i for i in range(n // 2, n - 1) if s.cancel(tickets[i]) == N.SF_OK]
+        for t in th:
+            t.join()
+        assert got == [N.SF_OK, N.SF_OK]
+        assert s.wait(tickets[-1]) == N.SF_EINVAL
+        assert len(cancelled) > 0
+        for i in range(n - 1):
+            st = s.wait(tickets[i])
+            assert st == (N.SF_ECANCELED if i in cancelled else N.SF_OK), (i, st)
+        assert s.stats().reads_done == n - len(cancelled)
+    for i in range(n):
+        if i in cancelled:
+            continue
+        a, b = int(reads.seg0[i]), int(reads.seg0[i]) + int(reads.nseg[i])
+        for f in ("dist", "end", "score", "score_col"):
+            assert np.array_equal(getattr(alone, f)[a:b], getattr(res, f)[a:b]), (i, f)
