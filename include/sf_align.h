@@ -100,6 +100,16 @@ int sf_init(int device);
  * same distance (0 for Prefix and Global).  end/start have no reference counterpart. */
 int sf_myers(const sf_pairs *pairs, int mode, uint32_t k, uint32_t *dist, uint32_t *end, uint32_t *start);
 
+/* No counterpart in the reference: sf_myers for callers that cannot allocate (harnesses that forbid
+ * device allocation, tight serving loops).  The same arguments and results as sf_myers, computed
+ * by the same kernels, with no hipMalloc, hipFree, stream creation or host heap allocation: the
+ * device memory it needs lives in one fixed 32 MiB device array per GPU, held from the first call
+ * until the process ends.  A call needs about 1.25 B per base of the two buffers (0.25 B if packed)
+ * plus 72 B per pair; SF_ERANGE if that exceeds 32 MiB.  Calls on one GPU share the array and are
+ * serialised by the library (a call waits for the one before it); each call leaves its data in the
+ * array and the next call overwrites it. */
+int sf_myers_noalloc(const sf_pairs *pairs, int mode, uint32_t k, uint32_t *dist, uint32_t *end, uint32_t *start);
+
 /* Replaces align.rs:699-701 `affine` and align.rs:854-861 `affine_sorted` (equal to separate
  * calls, align.rs:2210-2213): score[i] of pair i in `mode` with band[i] (SF_BAND_EXACT for
  * exact; band == NULL: all exact), SF_NEG where the pass stops (align.rs:2543-2556).
