@@ -347,6 +347,7 @@ def bench_uneven(args, steps=None, warmup=None):
     out["affine_gcups"] = round(tm.affine_cells / (tm.affine_ms * 1e-3) / 1e9, 1)
     out["kernel_ms_per_step"] = {"myers": round(tm.myers_ms / steps, 2), "affine": round(tm.affine_ms / steps, 2)}
     out["affine_launches_per_step"] = tm.affine_launches // steps
+    out["schedules"] = uneven_schedules(b, bands, steps)
     if not args.no_cpu:
         dist, _, _ = b.myers_results(0)
         res = b.affine_results()
@@ -363,6 +364,55 @@ def bench_uneven(args, steps=None, warmup=None):
         # The target: at least 20x the all-host-cores CPU path on this workload.
         out["gpu_vs_cpu"] = {"ratio": round(out["value"] / out["cpu_baseline"]["value"], 1), "target": 20.0}
     b.close()
+    return out
+
+
+def uneven_schedules(b, bands, steps):
+    """The same step under each schedule, ms a step (wall time over `steps` steps after a warm-up):
+    launches of one workgroup a block or persistent launches over work queues, each queued eagerly
+    or replayed from a captured graph.  Every schedule's results are checked against the first."""
+    from spearfish_amd import _native as N
+    from spearfish_amd import align as A
+    lib = N.load()
+
+    def step():
+        b.myers(A.Mode.Infix, locate=1)
+        b.affine(A.Mode.Global, A.Affine.ONT, bands=bands)
+
+    def res():
+        d, e, _ = b.myers_results(1)
+        r = b.affine_results()
+        return d, e, r["score"], r["score_col"]
+
+    out, ref = {}, None
+    try:
+        for pers in (0, 1, 3):
+            N.check(lib.sf_set_persistent(pers))
+            name = {0: "per_block", 1: "persistent_myers", 3: "persistent"}[pers]
+            step()
+            b.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                step()
+            b.sync()
+            out[f"{name}_eager_ms"] = round((time.perf_counter() - t0) / steps * 1e3, 2)
+            r = res()
+            ref = ref or r
+            ok = all(np.array_equal(x, y) for x, y in zip(r, ref))
+            b.capture_begin()
+            step()
+            b.capture_end()
+            b.replay(1)
+            b.sync()
+            t0 = time.perf_counter()
+            b.replay(steps)
+            b.sync()
+            out[f"{name}_graph_ms"] = round((time.perf_counter() - t0) / steps * 1e3, 2)
+            ok = ok and all(np.array_equal(x, y) for x, y in zip(res(), ref))
+            out[f"{name}_bit_exact"] = ok
+            log(f"uneven schedules: {out}")
+    finally:
+        N.check(lib.sf_set_persistent(0))
     return out
 
 

@@ -80,6 +80,8 @@ typedef struct {
     uint64_t myers_cells;  /* sum of m * n over the pairs run (the GCUPS numerator) */
     uint64_t affine_cells; /* DP cells as the reference's DP_WORK counts them (align.rs:2573-2578) */
     uint64_t affine_launches;
+    double graph_ms;        /* replays of the captured step (sf_batch_replay); the cells above count them */
+    uint64_t graph_replays;
 } sf_timing;
 
 /* ---- library ---- */
@@ -152,6 +154,16 @@ int sf_batch_sync(sf_batch *b);
 int sf_batch_myers_results(sf_batch *b, uint32_t *dist, uint32_t *end, uint32_t *start);
 int sf_batch_affine_results(sf_batch *b, int32_t *score, uint32_t *score_col, uint32_t *tstart, uint32_t *cigar,
                             uint64_t cigar_cap, uint64_t *cigar_off);
+/* Graph capture (no reference counterpart): the runs queued between sf_batch_capture_begin and
+ * sf_batch_capture_end -- sf_batch_myers (any mode, k, locate) and score-only sf_batch_affine whose
+ * launch plan exists (the same mode, bands and scoring ran on this batch before) -- are recorded
+ * as one HIP graph instead of running; sf_batch_replay launches that graph `times` times on the
+ * batch's main stream, asynchronously, with the results of running those calls eagerly.  Calls that
+ * wait for the device, copy results or build a plan fail during a capture with SF_EINVAL, and so
+ * does the capture's end.  A new capture replaces the batch's graph. */
+int sf_batch_capture_begin(sf_batch *b);
+int sf_batch_capture_end(sf_batch *b);
+int sf_batch_replay(sf_batch *b, uint32_t times);
 /* Device time of the runs since the last sf_batch_reset_timing (implies sync). */
 int sf_batch_timing(sf_batch *b, sf_timing *t);
 void sf_batch_reset_timing(sf_batch *b);
@@ -289,6 +301,14 @@ int sf_server_replay(sf_server *s, const sf_read_request *req, const double *t_a
  * `slots` is 0, run with 64 lanes per pair.  Default 512 (or SF_AFFINE_LP16_MAX from the
  * environment).  Applies to affine runs started afterwards in this process. */
 int sf_set_affine_lp16_max(uint64_t slots);
+
+/* Tuning (no reference counterpart; results never depend on it): makes the launches that run several
+ * shape classes together (a length-skewed batch's Myers runs: bit 0 of `on`; its score runs: bit 1)
+ * persistent: as many workgroups as the GPU holds at once, each taking the next block of pairs from
+ * a device-side work queue (the classes' bins in the launch's order, longest chains first) until
+ * none is left; a clear bit launches one workgroup per block.  Default 0 (or SF_PERSISTENT from the
+ * environment).  Applies to runs started (or captured) afterwards in this process. */
+int sf_set_persistent(int on);
 
 /* Tuning (no reference counterpart; results never depend on it): lanes per pair of affine runs.
  * policy 0 (default): per pair, by the launch planner's estimate (score-only runs may use 8 lanes

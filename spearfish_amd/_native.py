@@ -77,6 +77,8 @@ class Timing(C.Structure):
         ("myers_cells", C.c_uint64),
         ("affine_cells", C.c_uint64),
         ("affine_launches", C.c_uint64),
+        ("graph_ms", C.c_double),
+        ("graph_replays", C.c_uint64),
     ]
 
 
@@ -148,6 +150,10 @@ SIGNATURES = {
     "sf_batch_affine_results": ([_vp, _vp, _vp, _vp, _vp, _u64, _vp], _int),
     "sf_batch_timing": ([_vp, C.POINTER(Timing)], _int),
     "sf_batch_reset_timing": ([_vp], None),
+    "sf_batch_capture_begin": ([_vp], _int),
+    "sf_batch_capture_end": ([_vp], _int),
+    "sf_batch_replay": ([_vp, _u32], _int),
+    "sf_set_persistent": ([_int], _int),
     "sf_server_create": ([_vp, _u64, C.POINTER(ServerConfig), C.POINTER(_vp)], _int),
     "sf_server_submit": ([_vp, C.POINTER(ReadRequest), C.POINTER(_u64)], _int),
     "sf_server_wait": ([_vp, _u64, C.c_double], _int),

@@ -215,6 +215,17 @@ class Batch:
         N.check(self.lib.sf_batch_affine_results(self.h, None, None, None, _p(cig), len(cig), _p(off)))
         return {"score": score, "score_col": col, "tstart": tstart, "cigar": cig[:int(off[-1])], "cigar_off": off}
 
+    def capture_begin(self):
+        """Record the following myers() and score-only affine() calls as one HIP graph."""
+        N.check(self.lib.sf_batch_capture_begin(self.h))
+
+    def capture_end(self):
+        N.check(self.lib.sf_batch_capture_end(self.h))
+
+    def replay(self, times: int = 1):
+        """Launch the captured step `times` times (asynchronous; results as after those calls)."""
+        N.check(self.lib.sf_batch_replay(self.h, times))
+
     def timing(self) -> N.Timing:
         t = N.Timing()
         N.check(self.lib.sf_batch_timing(self.h, C.byref(t)))

@@ -6,41 +6,13 @@
 // waves to fill the chip: its time is its longest pair's chain of rows.  Launched one class after
 // another such classes add up their chains; here each workgroup finds its class (its segment of
 // the launch) and runs that class's kernel body, so they all run side by side.
-#include "affine_pk.hpp"
-#include "affine_wave.hpp"
+#include "mix_blocks.hpp"
 
 namespace {
 
+
 __global__ __launch_bounds__(256) void affine_mix_kernel(AffineArgs a, MixArgs mx) {
-    uint32_t s = 0;
-    while (s + 1 < mx.nseg && blockIdx.x >= mx.blk0[s + 1]) ++s;
-    AffineArgs x = a;
-    x.order += mx.off[s];
-    x.desc += mx.off[s];
-    x.count = mx.count[s];
-    const uint32_t blk = blockIdx.x - mx.blk0[s];
-    switch (mx.cls[s]) {
-#define W(KK)                                                                                    \
-    case KK: affine_wave_body<64, KK, false>(x, blk); break;
-        W(1) W(2) W(3) W(4) W(5) W(6) W(7) W(8) W(9) W(10) W(11) W(12) W(13) W(14) W(15) W(16) W(18) W(20)
-        W(24) W(28) W(32)
-#undef W
-#define P(KK)                                                                                    \
-    case 64100 + 2 * KK: affine_pk16_body<64, KK, false>(x, blk); break;                          \
-    case 64200 + 2 * KK: affine_pk16_body<64, KK, true>(x, blk); break;
-        P(1) P(2) P(3) P(4) P(5) P(6) P(7) P(8) P(9) P(10) P(12) P(14) P(16)
-#undef P
-#define P8(KK)                                                                                   \
-    case 8200 + 2 * KK: affine_pk16_body<8, KK, true>(x, blk); break;
-#define P16(KK)                                                                                  \
-    case 16100 + 2 * KK: affine_pk16_body<16, KK, false>(x, blk); break;                          \
-    case 16200 + 2 * KK: affine_pk16_body<16, KK, true>(x, blk); break;
-        P8(14) P8(15) P8(16) P8(17) P8(18) P16(14) P16(15) P16(16)
-#undef P8
-#undef P16
-    default:
-        break; // the host launches only the classes above (sf_affine_mix_group)
-    }
+    affine_mix_block(a, mx, blockIdx.x);
 }
 
 } // namespace
@@ -88,6 +60,7 @@ hipError_t sf_launch_affine_mix_group(const AffineArgs &a, const MixArgs &mx, in
     if ((group != 1 && group != 2) || !mix_segments_ok(mx, group)) return hipErrorInvalidValue;
     if (mx.blocks == 0) return hipSuccess;
     if (group == 1) return sf_launch_affine_mix1(a, mx, s);
+    if (mx.queue) return sf_launch_affine_mix_persistent(a, mx, s);
     hipLaunchKernelGGL(affine_mix_kernel, dim3(mx.blocks), dim3(256), 0, s, a, mx);
     return hipGetLastError();
 }
@@ -99,6 +72,7 @@ hipError_t sf_launch_affine_mix(const AffineArgs &a, const MixArgs &mx, hipStrea
             (uint64_t)mx.blk0[i] + ((uint64_t)mx.count[i] * 64 + 255) / 256 != (i + 1 < mx.nseg ? mx.blk0[i + 1] : mx.blocks))
             return hipErrorInvalidValue;
     if (mx.blocks == 0) return hipSuccess;
+    if (mx.queue) return sf_launch_affine_mix_persistent(a, mx, s);
     hipLaunchKernelGGL(affine_mix_kernel, dim3(mx.blocks), dim3(256), 0, s, a, mx);
     return hipGetLastError();
 }

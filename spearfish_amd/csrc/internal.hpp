@@ -63,12 +63,45 @@ struct AffineArgs {
 // One launch over several 64-lane score classes: segment i holds count[i] pairs of class cls[i]
 // at launch-order offset off[i] and starts at workgroup blk0[i] (4 pairs a workgroup); blocks is
 // the launch's total.
+//
+// queue (null: one workgroup a block index, blocks workgroups): two device words, a cursor and a
+// count of workgroups done, both zero before the launch.  The launch is then persistent: as many
+// workgroups as the chip holds at once, each taking the next block index from the cursor (segments
+// in the launch's order: the longest chains first) until none is left; the last workgroup to finish
+// sets both words back to zero, so the next launch (or graph replay) finds them ready.
 struct MixArgs {
     static constexpr uint32_t kMax = 48;
     uint32_t nseg, blocks;
     int32_t cls[kMax];
     uint32_t off[kMax], count[kMax], blk0[kMax];
+    uint32_t *queue;
 };
+
+// The block indices of a mixed launch, as MixArgs::queue describes:
+//     for (uint32_t b = mix_next(q, true); b < mx.blocks; b = mix_next(q, false)) body(b);
+//     mix_done(q);
+// Without a queue that is the workgroup's own index, once.  With one, each workgroup takes indices
+// from the cursor until they run out; the bodies hold no workgroup barrier, so lanes and waves that
+// end a block early simply wait at the next index's barrier.
+__device__ __forceinline__ uint32_t mix_next(uint32_t *queue, bool first) {
+    if (!queue) return first ? blockIdx.x : 0xffffffffu;
+    __shared__ uint32_t next;
+    if (threadIdx.x == 0) next = atomicAdd(queue, 1u);
+    __syncthreads();
+    // (The same in every lane: readfirstlane lets the compiler keep it, and all it indexes, scalar.)
+    const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)next);
+    __syncthreads(); // every thread has read `next` before thread 0 takes the next index
+    return b;
+}
+
+// Every workgroup has taken its last index before it counts itself done, so the last one to count
+// can reset the cursor: no index is taken after it.
+__device__ __forceinline__ void mix_done(uint32_t *queue) {
+    if (queue && threadIdx.x == 0 && atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+        atomicExch(&queue[0], 0u);
+        atomicExch(&queue[1], 0u);
+    }
+}
 
 // A pass with per-row bookkeeping: align.rs:2446-2666 run_core with clip (X-drop, best cell) and
 // cut tracking, or (local) align.rs:783-852 local_band_core.  Band geometry comes in the pair
@@ -346,6 +379,18 @@ hipError_t sf_launch_desc(uint64_t n, const uint32_t *order, const sf::PairsView
                           sf::PairDesc *desc, hipStream_t s);
 hipError_t sf_launch_compact(const uint32_t *order, uint32_t count, const uint32_t *runs, const uint64_t *runs_off,
                              const uint32_t *nruns, const uint64_t *out_off, uint32_t *out, hipStream_t s);
+
+// Workgroups of 256 threads of `kernel` the device holds at once (its occupancy x compute units),
+// at most `blocks`: the grid of a persistent mixed launch (cached per kernel).
+uint32_t sf_persistent_grid(const void *kernel, uint32_t blocks);
+// Which mixed launches run persistent (MixArgs::queue; bit 0 Myers runs, bit 1 score runs):
+// sf_set_persistent / SF_PERSISTENT.
+int sf_persistent_policy();
+// The persistent forms of the mixed launches (mix_persistent_*.hip; mx.queue set, segments checked
+// by the callers above).
+hipError_t sf_launch_myers_mix_persistent(const sf::MyersArgs &a, const sf::MixArgs &mx, int group, hipStream_t s);
+hipError_t sf_launch_affine_mix_persistent(const sf::AffineArgs &a, const sf::MixArgs &mx, hipStream_t s);
+hipError_t sf_launch_affine_mix1_persistent(const sf::AffineArgs &a, const sf::MixArgs &mx, hipStream_t s);
 
 // Supported shape classes.
 int sf_myers_maxb_for(uint32_t nblocks);     // 0 if unsupported

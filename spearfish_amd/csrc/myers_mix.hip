@@ -9,44 +9,14 @@
 // footprint (at most 108, 140 and 248 VGPRs: 4, 3 and 2 waves a SIMD), one kernel each, and a
 // run takes at most three launches.  Segments begin on workgroup boundaries, so a wave never
 // holds two classes.
-#include "myers_body.hpp"
+#include "mix_blocks.hpp"
 
 namespace {
 
+
 template <int GROUP>
 __global__ __launch_bounds__(256) void myers_mix_kernel(MyersArgs a, MixArgs mx) {
-    uint32_t s = 0;
-    while (s + 1 < mx.nseg && blockIdx.x >= mx.blk0[s + 1]) ++s;
-    MyersArgs x = a;
-    x.order += mx.off[s];
-    x.desc += mx.off[s];
-    x.count = mx.count[s];
-    const uint32_t blk = blockIdx.x - mx.blk0[s];
-#define L(B)                                                                                     \
-    case B: myers_lane_body<B>(x, blk); break;
-#define P(GG, RR)                                                                                \
-    case GG * 1000 + RR: myers_pipe_body<GG, RR>(x, blk); break;
-    if constexpr (GROUP == 1) {
-        switch (mx.cls[s]) {
-            L(1) L(2) L(4) L(8) P(16, 1) P(16, 2) P(16, 3) P(64, 2) P(64, 3)
-        default:
-            break; // the host launches only this group's classes (sf_myers_mix_group)
-        }
-    } else if constexpr (GROUP == 2) {
-        switch (mx.cls[s]) {
-            L(12) P(16, 4) P(16, 5) P(16, 6) P(32, 5) P(32, 7) P(64, 4) P(64, 5) P(64, 6)
-        default:
-            break;
-        }
-    } else {
-        switch (mx.cls[s]) {
-            L(16) L(20) L(24)
-        default:
-            break;
-        }
-    }
-#undef L
-#undef P
+    myers_mix_block<GROUP>(a, mx, blockIdx.x);
 }
 
 } // namespace
@@ -79,6 +49,7 @@ hipError_t sf_launch_myers_mix(const MyersArgs &a, const MixArgs &mx, int group,
                 (i + 1 < mx.nseg ? mx.blk0[i + 1] : mx.blocks))
             return hipErrorInvalidValue;
     if (mx.blocks == 0) return hipSuccess;
+    if (mx.queue) return sf_launch_myers_mix_persistent(a, mx, group, s);
     switch (group) {
     case 1: hipLaunchKernelGGL(myers_mix_kernel<1>, dim3(mx.blocks), dim3(256), 0, s, a, mx); break;
     case 2: hipLaunchKernelGGL(myers_mix_kernel<2>, dim3(mx.blocks), dim3(256), 0, s, a, mx); break;

@@ -63,6 +63,9 @@ int ensure_device() {
     return SF_OK;
 }
 
+// -1: not read yet (SF_PERSISTENT from the environment, default off).
+std::atomic<int> g_persistent{-1};
+
 unsigned host_threads() {
     unsigned t = std::thread::hardware_concurrency();
     if (const char *e = std::getenv("SF_HOST_THREADS")) t = (unsigned)std::atoi(e);
@@ -266,6 +269,18 @@ struct sf_batch {
     // marked by joins[t]).  A Myers run leaves them open so that a following independent run
     // (affine scores) starts beside its last bins; every other call joins them first (join_aux).
     uint32_t aux_open = 0;
+    // Work queues of persistent mixed launches (MixArgs::queue): kQueues pairs of words, zero
+    // between launches.  Slots [0, 8) serve a Myers run's launches, [8, 16) a score run's grouped
+    // launches, 16 its mixed 64-lane launch: launches that may run at the same time never share one.
+    static constexpr int kQueues = 17;
+    uint32_t *d_queues = nullptr;
+    // Graph capture (sf_batch_capture_begin/end) and the captured step, replayed by sf_batch_replay.
+    bool capturing = false;
+    int capture_error = SF_OK;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t graph_myers_cells = 0, graph_affine_cells = 0, cap_myers_cells = 0, cap_affine_cells = 0;
+    uint64_t graph_replays = 0;
     hipEvent_t dp_done[2] = {nullptr, nullptr}, walk_done[2] = {nullptr, nullptr}, walks_end = nullptr;
     bool cigar_valid = false;
     // timing
@@ -276,7 +291,7 @@ struct sf_batch {
     };
     std::vector<Ev> pending;
     std::vector<hipEvent_t> pool;
-    double ms[4] = {0, 0, 0, 0};
+    double ms[5] = {0, 0, 0, 0, 0}; // Myers, start pass, affine, walk, graph replays
     uint64_t myers_cells = 0, affine_cells = 0, affine_launches = 0;
     uint64_t myers_cells_per_run = 0;
 
@@ -319,7 +334,9 @@ struct sf_batch {
         dfree(d_dp), dfree(d_best), dfree(d_pinv), dfree(d_col);
         dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
         dfree(d_scan);
-        dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0);
+        dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0), dfree(d_queues);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
         for (hipStream_t x : aux)
             if (x) (void)hipStreamSynchronize(x);
         for (hipEvent_t e : {dp_done[0], dp_done[1], walk_done[0], walk_done[1], walks_end, fork, joins[0], joins[1]})
@@ -391,7 +408,12 @@ void pack_into(uint32_t *dst, const void *buf, int packed, uint64_t off, uint32_
     }
 }
 
+// (While a step is being captured into a graph nothing is timed: the replays are.)
 int timed_begin(sf_batch *b, int cat, hipEvent_t *a, hipStream_t s = nullptr) {
+    if (b->capturing) {
+        *a = nullptr;
+        return SF_OK;
+    }
     *a = b->ev();
     HIP_TRY(hipEventRecord(*a, s ? s : b->stream));
     (void)cat;
@@ -399,6 +421,7 @@ int timed_begin(sf_batch *b, int cat, hipEvent_t *a, hipStream_t s = nullptr) {
 }
 
 int timed_end(sf_batch *b, int cat, hipEvent_t a, hipStream_t s = nullptr) {
+    if (b->capturing) return SF_OK;
     hipEvent_t e = b->ev();
     HIP_TRY(hipEventRecord(e, s ? s : b->stream));
     b->pending.push_back({cat, a, e, {}});
@@ -416,6 +439,7 @@ int join_aux(sf_batch *b) {
 // timed_end for a run that left extra streams open: an end on the main stream and one on each open
 // stream, the interval running to the latest (no stream waits for another).
 int timed_end_open(sf_batch *b, int cat, hipEvent_t a) {
+    if (b->capturing) return SF_OK;
     if (const int st = timed_end(b, cat, a)) return st;
     sf_batch::Ev &ev = b->pending.back();
     for (int t = 0; t < sf_batch::kAux; ++t)
@@ -451,12 +475,23 @@ using MixFn = std::function<hipError_t(const std::vector<size_t> &, hipStream_t)
 // The extra streams (and their fork and join events), made on first use.
 int ensure_aux(sf_batch *b) {
     if (b->fork) return SF_OK;
+    if (b->capturing) return fail(SF_EINVAL, "the batch's streams are made by a run before the capture");
+    HIP_TRY(hipMalloc((void **)&b->d_queues, sf_batch::kQueues * 2 * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(b->d_queues, 0, sf_batch::kQueues * 2 * sizeof(uint32_t)));
     HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     for (int i = 0; i < sf_batch::kAux; ++i) {
         HIP_TRY(hipStreamCreateWithFlags(&b->aux[i], hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming));
     }
     return SF_OK;
+}
+
+// The queue of a persistent mixed launch (slot `q` of the batch's queues: Myers below 8), or null
+// for a launch of one workgroup a block index (the policy's bit for the run's kind clear, or no free
+// slot).
+uint32_t *queue_for(sf_batch *b, int q, int end) {
+    const int bit = q < 8 ? 1 : 2;
+    return ((sf_persistent_policy() & bit) && b->d_queues && q < end) ? b->d_queues + 2 * q : nullptr;
 }
 
 template <class F>
@@ -527,7 +562,8 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch, uint
 // stream.  A single bin is one plain launch.
 template <class One, class Mix>
 int run_grouped(sf_batch *b, const std::vector<sf_batch::Bin> &bins, One one, int (*group_of)(int),
-                uint32_t (*blocks_for)(int, uint32_t), Mix mix, uint64_t *launches, bool leave_open = false) {
+                uint32_t (*blocks_for)(int, uint32_t), Mix mix, uint64_t *launches, bool leave_open = false,
+                int queue0 = 0) {
     if (bins.size() <= 1) {
         for (const auto &bin : bins) HIP_TRY(one(bin, b->stream));
         if (launches) *launches += bins.size();
@@ -585,6 +621,7 @@ int run_grouped(sf_batch *b, const std::vector<sf_batch::Bin> &bins, One one, in
     hipStream_t ss[S] = {b->stream, b->aux[0], b->aux[1]};
     double load[S] = {0, 0, 0};
     bool used[S] = {true, false, false};
+    int queue = queue0;
     HIP_TRY(hipEventRecord(b->fork, b->stream));
     for (const Unit &u : units) {
         int s = 0;
@@ -609,6 +646,7 @@ int run_grouped(sf_batch *b, const std::vector<sf_batch::Bin> &bins, One one, in
                 mx.blocks += blocks_for(bin.cls, bin.end - bin.begin);
                 ++mx.nseg;
             }
+            mx.queue = queue_for(b, queue++, queue0 + 8);
             HIP_TRY(mix(mx, u.group, ss[s]));
         }
         load[s] += u.est;
@@ -642,7 +680,7 @@ int run_myers(sf_batch *b, const MyersArgs &a) {
         x.desc = b->d_mdesc;
         return sf_launch_myers_mix(x, mx, group, s);
     };
-    return run_grouped(b, b->mbins, one, sf_myers_mix_group, sf_myers_blocks_for, mix, nullptr, true);
+    return run_grouped(b, b->mbins, one, sf_myers_mix_group, sf_myers_blocks_for, mix, nullptr, true, 0);
 }
 
 // Estimated VALU wave-instructions a row of one wave costs in shape class `cls`, shared by its
@@ -771,6 +809,15 @@ int quiesce(sf_batch *b) {
     return SF_OK;
 }
 
+// A call that cannot be part of a captured step (it waits for the device, copies between host and
+// device or builds a launch plan) fails while the batch is capturing, and so does the capture.
+int refuse_in_capture(sf_batch *b, const char *what) {
+    if (!b->capturing) return SF_OK;
+    b->capture_error = SF_EINVAL;
+    return fail(SF_EINVAL, "%s cannot be captured (captured steps hold Myers runs and score-only affine runs "
+                           "whose launch plans exist: run the step once before capturing it)", what);
+}
+
 int build_plan(sf_batch *b, sf_batch::AffPlan &pl, const std::vector<uint64_t> &ba, const std::vector<int> &kcls,
                uint64_t key, uint64_t cells) {
     const uint64_t n = b->n;
@@ -803,6 +850,40 @@ int build_plan(sf_batch *b, sf_batch::AffPlan &pl, const std::vector<uint64_t> &
 }
 
 } // namespace
+
+int sf_persistent_policy() {
+    int v = g_persistent.load();
+    if (v < 0) {
+        const char *e = std::getenv("SF_PERSISTENT");
+        v = e ? std::max(0, std::min(3, std::atoi(e))) : 0;
+        int expected = -1;
+        if (!g_persistent.compare_exchange_strong(expected, v)) v = expected;
+    }
+    return v;
+}
+
+uint32_t sf_persistent_grid(const void *kernel, uint32_t blocks) {
+    static std::mutex mu;
+    static std::map<const void *, uint32_t> cap;
+    uint32_t c = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cap.find(kernel);
+        if (it != cap.end()) c = it->second;
+    }
+    if (!c) {
+        int per_cu = 0, dev = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        c = (uint32_t)per_cu * (uint32_t)cus;
+        std::lock_guard<std::mutex> lk(mu);
+        cap[kernel] = c;
+    }
+    return std::min(c, blocks);
+}
 
 // Hooks for the server (serve.hip).
 int sf_runtime_fail(int code, const char *msg) { return fail(code, "%s", msg); }
@@ -845,7 +926,16 @@ int sf_init(int device) {
 
 uint64_t sf_batch_size(const sf_batch *b) { return b ? b->n : 0; }
 
-void sf_batch_destroy(sf_batch *b) { delete b; }
+void sf_batch_destroy(sf_batch *b) {
+    if (b && b->capturing) { // the capture is abandoned
+        hipGraph_t g = nullptr;
+        (void)join_aux(b);
+        (void)hipStreamEndCapture(b->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        b->capturing = false;
+    }
+    delete b;
+}
 
 int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
     if (!out) return fail(SF_EINVAL, "out is null");
@@ -1016,7 +1106,7 @@ int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
     a.reverse = 0;
     if ((st = run_myers(b, a))) return st;
     if ((st = timed_end_open(b, 0, e0))) return st;
-    b->myers_cells += b->myers_cells_per_run;
+    (b->capturing ? b->cap_myers_cells : b->myers_cells) += b->myers_cells_per_run;
     b->located_ready = locate == 2;
     if (locate == 2 && mode == SF_INFIX) {
         if ((st = join_aux(b))) return st; // the start pass reads the forward pass's results
@@ -1038,6 +1128,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         return fail(SF_ERANGE, "scores above 65535 are outside this library's bound");
     int st;
     if ((st = ensure_device())) return st;
+    if (cigar && (st = refuse_in_capture(b, "an affine run with traceback"))) return st;
     // A score run needs nothing from a Myers run: it starts beside that run's last bins.  The
     // traceback path (scratch, walks, uploads) starts from a joined state.
     if (cigar && (st = join_aux(b))) return st;
@@ -1051,6 +1142,8 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         for (uint64_t i = 0; i < n; ++i) key = (key ^ band[i]) * 1099511628211ull;
     else
         key ^= 0x9e3779b97f4a7c15ull;
+    if ((!pl.valid || pl.key != key) && (st = refuse_in_capture(b, "an affine run that needs a new launch plan")))
+        return st;
     if (!pl.valid || pl.key != key) {
         std::vector<uint64_t> ba(n);
         std::vector<int> kcls(n);
@@ -1101,6 +1194,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
                 mx.blocks += sf_affine_blocks_for(bin.cls, mx.count[t]);
             }
             mx.nseg = (uint32_t)ids.size();
+            mx.queue = queue_for(b, 16, 17);
             AffineArgs x = a;
             x.order = pl.d_order;
             x.desc = pl.d_desc;
@@ -1116,13 +1210,13 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
                 return sf_launch_affine_mix_group(x, mx, group, s);
             };
             if ((st = run_grouped(b, pl.bins, one, sf_affine_mix_group, sf_affine_blocks_for, grouped,
-                                  &b->affine_launches)))
+                                  &b->affine_launches, false, 8)))
                 return st;
         } else if ((st = run_bins(b, pl.bins, one, &b->affine_launches, mixed, sf_affine_mixable))) {
             return st;
         }
         if ((st = timed_end(b, 2, e0))) return st;
-        b->affine_cells += pl.cells;
+        (b->capturing ? b->cap_affine_cells : b->affine_cells) += pl.cells;
         b->cigar_valid = false;
         return SF_OK;
     }
@@ -1271,6 +1365,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
 int sf_batch_use_located(sf_batch *b, int on) {
     if (!b) return fail(SF_EINVAL, "batch is null");
     int st;
+    if ((st = refuse_in_capture(b, "sf_batch_use_located"))) return st;
     if ((st = ensure_device())) return st;
     if ((st = join_aux(b))) return st; // reads the Myers results
     const uint64_t n = b->n;
@@ -1301,12 +1396,79 @@ int sf_set_affine_lp8(int policy) {
     return SF_OK;
 }
 
+int sf_set_persistent(int on) {
+    if (on < 0 || on > 3) return fail(SF_EINVAL, "on must be 0..3 (bit 0: Myers runs, bit 1: score runs)");
+    g_persistent.store(on);
+    return SF_OK;
+}
+
+int sf_batch_capture_begin(sf_batch *b) {
+    if (!b) return fail(SF_EINVAL, "batch is null");
+    if (b->capturing) return fail(SF_EINVAL, "the batch is capturing already");
+    int st;
+    if ((st = ensure_device()) || (st = ensure_aux(b))) return st;
+    // Buffers a captured run writes are made now (no allocation while capturing).
+    if (!b->d_score && ((st = dalloc(&b->d_score, b->n)) || (st = dalloc(&b->d_col, b->n)))) return st;
+    if ((st = quiesce(b))) return st; // nothing queued before the capture is part of it
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+    b->capturing = true;
+    b->capture_error = SF_OK;
+    b->cap_myers_cells = b->cap_affine_cells = 0;
+    return SF_OK;
+}
+
+int sf_batch_capture_end(sf_batch *b) {
+    if (!b) return fail(SF_EINVAL, "batch is null");
+    if (!b->capturing) return fail(SF_EINVAL, "the batch is not capturing");
+    const int joined = join_aux(b); // every stream the captured runs forked joins the main stream
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    b->capturing = false;
+    if (joined || b->capture_error || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        if (joined) return joined;
+        if (b->capture_error) return fail(b->capture_error, "a call during the capture could not be captured");
+        return fail(SF_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(SF_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    }
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph = g;
+    b->graph_exec = x;
+    b->graph_myers_cells = b->cap_myers_cells;
+    b->graph_affine_cells = b->cap_affine_cells;
+    return SF_OK;
+}
+
+int sf_batch_replay(sf_batch *b, uint32_t times) {
+    if (!b) return fail(SF_EINVAL, "batch is null");
+    if (b->capturing) return fail(SF_EINVAL, "the batch is capturing");
+    if (!b->graph_exec) return fail(SF_EINVAL, "no captured step on this batch");
+    int st;
+    if ((st = ensure_device()) || (st = join_aux(b))) return st;
+    hipEvent_t e0;
+    if ((st = timed_begin(b, 4, &e0))) return st;
+    for (uint32_t t = 0; t < times; ++t) HIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
+    if ((st = timed_end(b, 4, e0))) return st;
+    b->myers_cells += b->graph_myers_cells * times;
+    b->affine_cells += b->graph_affine_cells * times;
+    b->graph_replays += times;
+    return SF_OK;
+}
+
 int sf_batch_sync(sf_batch *b) {
     if (!b) return fail(SF_EINVAL, "batch is null");
+    if (const int st = refuse_in_capture(b, "sf_batch_sync")) return st;
     return quiesce(b);
 }
 
 int sf_batch_myers_results(sf_batch *b, uint32_t *dist, uint32_t *end, uint32_t *start) {
+    if (b && b->capturing) return refuse_in_capture(b, "reading results");
     int st;
     if ((st = sf_batch_sync(b))) return st;
     const uint64_t n = b->n;
@@ -1356,15 +1518,17 @@ int sf_batch_timing(sf_batch *b, sf_timing *t) {
     t->myers_cells = b->myers_cells;
     t->affine_cells = b->affine_cells;
     t->affine_launches = b->affine_launches;
+    t->graph_ms = b->ms[4];
+    t->graph_replays = b->graph_replays;
     return SF_OK;
 }
 
 void sf_batch_reset_timing(sf_batch *b) {
-    if (!b) return;
+    if (!b || refuse_in_capture(b, "sf_batch_reset_timing")) return;
     (void)quiesce(b);
     b->collect();
     for (double &x : b->ms) x = 0;
-    b->myers_cells = b->affine_cells = b->affine_launches = 0;
+    b->myers_cells = b->affine_cells = b->affine_launches = b->graph_replays = 0;
 }
 
 // ---- one-shot calls ----
@@ -1373,6 +1537,7 @@ void sf_batch_reset_timing(sf_batch *b) {
 
 int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t *band, const uint64_t *geom,
                        int has_clip, int32_t xdrop, int clip_cell, const uint64_t *cut) {
+    if (b && b->capturing) return refuse_in_capture(b, "sf_batch_affine_dp");
     if (!b) return fail(SF_EINVAL, "batch is null");
     if (!sc) return fail(SF_EINVAL, "scoring is null");
     if (mode < SF_INFIX || mode > SF_GLOBAL) return fail(SF_EINVAL, "mode %d", mode);
@@ -1450,6 +1615,7 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
 }
 
 int sf_batch_dp_results(sf_batch *b, sf_dp *out) {
+    if (b && b->capturing) return refuse_in_capture(b, "reading results");
     int st;
     if ((st = sf_batch_sync(b))) return st;
     if (!out) return fail(SF_EINVAL, "out is null");
@@ -1461,6 +1627,7 @@ int sf_batch_dp_results(sf_batch *b, sf_dp *out) {
 
 int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, const int64_t *dhi, int32_t bonus_start,
                         int32_t bonus_end) {
+    if (b && b->capturing) return refuse_in_capture(b, "sf_batch_local_band");
     if (!b) return fail(SF_EINVAL, "batch is null");
     if (!sc || !dlo || !dhi) return fail(SF_EINVAL, "scoring and diagonals are required");
     if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0)
@@ -1533,6 +1700,7 @@ int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, c
 }
 
 int sf_batch_local_results(sf_batch *b, int32_t *best) {
+    if (b && b->capturing) return refuse_in_capture(b, "reading results");
     int st;
     if ((st = sf_batch_sync(b))) return st;
     if (!best) return fail(SF_EINVAL, "best is null");
