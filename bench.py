@@ -94,6 +94,7 @@ def parse():
     ap.add_argument("--workload", default="uniform", choices=["uniform", "uneven", "long", "serve", "probe"])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-extra", action="store_true", help="the uniform workload only")
+    ap.add_argument("--tune-out", default=None, help="write the autotuner's measurements here (uneven)")
     return ap.parse_args()
 
 
@@ -347,7 +348,7 @@ def bench_uneven(args, steps=None, warmup=None):
     out["affine_gcups"] = round(tm.affine_cells / (tm.affine_ms * 1e-3) / 1e9, 1)
     out["kernel_ms_per_step"] = {"myers": round(tm.myers_ms / steps, 2), "affine": round(tm.affine_ms / steps, 2)}
     out["affine_launches_per_step"] = tm.affine_launches // steps
-    out["schedules"] = uneven_schedules(b, bands, steps)
+    out["schedules"] = uneven_schedules(b, bands, steps, args.tune_out)
     if not args.no_cpu:
         dist, _, _ = b.myers_results(0)
         res = b.affine_results()
@@ -367,7 +368,7 @@ def bench_uneven(args, steps=None, warmup=None):
     return out
 
 
-def uneven_schedules(b, bands, steps):
+def uneven_schedules(b, bands, steps, tune_out=None):
     """The same step under each schedule, ms a step (wall time over `steps` steps after a warm-up):
     launches of one workgroup a block or persistent launches over work queues, each queued eagerly
     or replayed from a captured graph.  Every schedule's results are checked against the first."""
@@ -411,8 +412,38 @@ def uneven_schedules(b, bands, steps):
             ok = ok and all(np.array_equal(x, y) for x, y in zip(res(), ref))
             out[f"{name}_bit_exact"] = ok
             log(f"uneven schedules: {out}")
+        # The autotuner: measure the candidate classes of this batch's pairs, then the same step
+        # planned with the measurements.
+        N.check(lib.sf_set_persistent(0))
+        N.check(lib.sf_tune_reset())
+        t0 = time.perf_counter()
+        b.autotune(A.Mode.Global, A.Affine.ONT, bands=bands)
+        out["autotune_s"] = round(time.perf_counter() - t0, 2)
+        out["autotune_classes"] = int(lib.sf_tune_classes())
+        step()
+        b.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        b.sync()
+        out["tuned_eager_ms"] = round((time.perf_counter() - t0) / steps * 1e3, 2)
+        out["tuned_bit_exact"] = all(np.array_equal(x, y) for x, y in zip(res(), ref))
+        b.capture_begin()
+        step()
+        b.capture_end()
+        b.replay(1)
+        b.sync()
+        t0 = time.perf_counter()
+        b.replay(steps)
+        b.sync()
+        out["tuned_graph_ms"] = round((time.perf_counter() - t0) / steps * 1e3, 2)
+        out["tuned_bit_exact"] = out["tuned_bit_exact"] and all(np.array_equal(x, y) for x, y in zip(res(), ref))
+        if tune_out:
+            N.check(lib.sf_tune_save(tune_out.encode()))
+        log(f"uneven schedules: {out}")
     finally:
         N.check(lib.sf_set_persistent(0))
+        N.check(lib.sf_tune_reset())
     return out
 
 

@@ -302,6 +302,25 @@ int sf_server_replay(sf_server *s, const sf_read_request *req, const double *t_a
  * environment).  Applies to affine runs started afterwards in this process. */
 int sf_set_affine_lp16_max(uint64_t slots);
 
+/* Autotuner (no reference counterpart; results never depend on it).  The launch planner picks each
+ * pair's shape -- lanes per pair (8, 16, 32 or 64), slots per lane (the class edge its band falls
+ * under, or the next one up), 16-bit packed or 32-bit -- from two costs per shape class: ns per
+ * pair-row with the chip full, and ns per row of one wave alone.  Untuned they are estimates from
+ * instruction counts.  sf_batch_autotune times, on this device, the score kernel of every class a
+ * score run of `b` with this mode, scoring and bands could pick (those not measured yet in this
+ * process), then times b's score run planned with the estimates and with the measurements (a lone
+ * wave's chain scaled by 1, 1.5, 2 or 3 for the load it runs under) and keeps the fastest setting
+ * for later score runs' plans, of every batch in the process.  It runs b's score run 20 times (the
+ * results are those of any run).  sf_tune_save / sf_tune_load write and read the measurements and
+ * the setting as text (per device model; lines of another device are skipped); SF_TUNE_FILE in the
+ * environment names a file read on first use.  sf_tune_reset forgets them; sf_tune_classes counts
+ * the classes measured. */
+int sf_batch_autotune(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t *band);
+int sf_tune_save(const char *path);
+int sf_tune_load(const char *path);
+int sf_tune_reset(void);
+uint64_t sf_tune_classes(void);
+
 /* Tuning (no reference counterpart; results never depend on it): makes the launches that run several
  * shape classes together (a length-skewed batch's Myers runs: bit 0 of `on`; its score runs: bit 1)
  * persistent: as many workgroups as the GPU holds at once, each taking the next block of pairs from

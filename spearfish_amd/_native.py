@@ -154,6 +154,11 @@ SIGNATURES = {
     "sf_batch_capture_end": ([_vp], _int),
     "sf_batch_replay": ([_vp, _u32], _int),
     "sf_set_persistent": ([_int], _int),
+    "sf_batch_autotune": ([_vp, _int, C.POINTER(Scoring), _vp], _int),
+    "sf_tune_save": ([C.c_char_p], _int),
+    "sf_tune_load": ([C.c_char_p], _int),
+    "sf_tune_reset": ([], _int),
+    "sf_tune_classes": ([], _u64),
     "sf_server_create": ([_vp, _u64, C.POINTER(ServerConfig), C.POINTER(_vp)], _int),
     "sf_server_submit": ([_vp, C.POINTER(ReadRequest), C.POINTER(_u64)], _int),
     "sf_server_wait": ([_vp, _u64, C.c_double], _int),

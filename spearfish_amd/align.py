@@ -215,6 +215,14 @@ class Batch:
         N.check(self.lib.sf_batch_affine_results(self.h, None, None, None, _p(cig), len(cig), _p(off)))
         return {"score": score, "score_col": col, "tstart": tstart, "cigar": cig[:int(off[-1])], "cigar_off": off}
 
+    def autotune(self, mode: Mode, sc: Affine = Affine.ONT, band=None, bands=None):
+        """Measure on the device the score kernels' shape classes this batch's pairs could take
+        (sf_batch_autotune); later score runs plan with the measurements."""
+        if bands is None:
+            bands = _bands(band, self.n)
+        scc = sc.c()
+        N.check(self.lib.sf_batch_autotune(self.h, int(mode), C.byref(scc), bands.ctypes.data))
+
     def capture_begin(self):
         """Record the following myers() and score-only affine() calls as one HIP graph."""
         N.check(self.lib.sf_batch_capture_begin(self.h))

@@ -21,6 +21,12 @@
 
 using namespace sf;
 
+// The autotuner's table (tune.hip).
+bool sf_tune_cost(int cls, double *tput_ns, double *chain_ns);
+int sf_tune_measure(const std::vector<int> &classes, int mode, const sf_scoring *sc);
+void sf_tune_set(bool use, double chain_factor);
+uint64_t sf_tune_generation();
+
 // ------------------------------------------------------------------------------------------
 // errors
 // ------------------------------------------------------------------------------------------
@@ -224,8 +230,9 @@ struct sf_batch {
     struct Bin {
         int cls;
         uint32_t begin, end;
-        // Estimated time in VALU wave-instructions: the longest single wave's chain, and the
-        // whole bin's work spread over the chip's 1024 SIMDs.  Only orders the launches.
+        // Estimated time: the longest single wave's chain, and the whole bin's work spread over
+        // the chip (affine bins: ns, measured or estimated per class; Myers bins: block steps).
+        // Orders the launches and places them on streams.
         double chain = 0, spread = 0;
         double est() const { return chain > spread ? chain : spread; }
     };
@@ -692,6 +699,25 @@ double aff_row_cost(int cls) {
     return 3.5 * k + (aff_lp(cls) == 64 ? 30.0 : (aff_lp(cls) == 32 ? 27.0 : 24.0));
 }
 
+// The planner's two costs of a class, in ns: a pair-row's share of the chip's time when the class
+// fills it (throughput), and one row of a wave running alone (chain).  Measured by the autotuner
+// where its table holds the class (tune.hip), else estimated: aff_row_cost wave-instructions at one
+// per SIMD every 4 cycles (2.4 GHz), spread over the chip's 1,024 SIMDs for the throughput, and
+// 1.5 times that interval for a lone wave's dependent instructions.  The autotuner times the score
+// kernels, so only score-only plans (`tuned`) take its measurements: a class code in a traceback
+// plan or a pass names another kernel of the same shape.
+constexpr double kNsPerIssue = 4.0 / 2.4;
+double aff_tput_ns(int cls, bool tuned) {
+    double t, c;
+    if (tuned && sf_tune_cost(cls, &t, &c)) return t;
+    return aff_row_cost(cls) * aff_lp(cls) / 64.0 * kNsPerIssue / 1024.0;
+}
+double aff_chain_ns(int cls, bool tuned) {
+    double t, c;
+    if (tuned && sf_tune_cost(cls, &t, &c)) return c;
+    return 1.5 * aff_row_cost(cls) * kNsPerIssue;
+}
+
 // Per pair, the shape an affine run uses.  A shape's cost per pair-row is its wave-row cost over
 // its pairs a wave; its chain, the time one pair's wave needs alone, is m wave-rows at about 1.5
 // times the issue interval (dependent VALU, DPP and waits); the run's spread is every pair's rows
@@ -704,65 +730,89 @@ double aff_row_cost(int cls) {
 // as a whole where that adds little to the run (its 64-lane spread within 5% of the run's): its
 // pairs then run at a quarter to an eighth of the chain, and score-only runs launch all such
 // 64-lane classes together (run_bins).  Policy 1 keeps every pair in its fewest-lanes shape.
+// A pair's candidate shapes: 8, 16, 32 and 64 lanes (0 where not allowed), each at the smallest
+// class that holds its band and at the next class edge up (a tile of more slots a lane: chosen only
+// where the autotuner measured it cheaper).  Index kC64 is the 64-lane shape at its smallest class.
+constexpr int kCands = 7, kC64 = 5;
+using Cands = std::array<int, kCands>;
+Cands pair_candidates(uint32_t m, uint64_t below, uint64_t width, const sf_scoring *sc, bool cigar, int lp8,
+                      uint64_t lp16max) {
+    auto pk = [&](int c, int lanes) {
+        if (!c) return 0;
+        const int p = sf_affine_pk16_class(aff_k(c), sc->mat, sc->mis, sc->open, sc->ext, m, (uint32_t)below, cigar, lanes);
+        return p ? p : c;
+    };
+    int c64 = pk(sf_affine_class(width, false), 64);
+    int c64n = c64 ? pk(sf_affine_class(aff_slots(sf_affine_class(width, false)) + 1, false), 64) : 0;
+    int c16 = 0, c16n = 0;
+    if (width <= lp16max && width <= 16 * 32) {
+        const int w16 = sf_affine_class(width, true);
+        c16 = pk(w16, 16);
+        const uint64_t nx = aff_slots(w16) + 1;
+        if (nx <= lp16max && nx <= 16 * 32) c16n = pk(sf_affine_class(nx, true), 16);
+    }
+    int c8 = 0, c8n = 0;
+    if (!cigar && lp8 >= 0) {
+        c8 = sf_affine_pk8_class(width, sc->mat, sc->mis, sc->open, sc->ext, m, (uint32_t)below);
+        if (c8) c8n = sf_affine_pk8_class(aff_slots(c8) + 1, sc->mat, sc->mis, sc->open, sc->ext, m, (uint32_t)below);
+    }
+    // 32 lanes: traceback runs of bands over 512 slots (2 * ceil(width / 64) slots a lane).
+    const int c32 = (cigar && width > 16 * 32 && width <= 32 * 32)
+                        ? sf_affine_pk16_class(2 * (int)((width + 63) / 64), sc->mat, sc->mis, sc->open, sc->ext, m,
+                                               (uint32_t)below, true, 32)
+                        : 0;
+    // (A next-edge class the kernels lack, or outside the 16-bit range, comes back as 0 or as the
+    // 32-bit class: drop it where it is not a different shape of the same kind.)
+    if (cigar || c64n == c64 || (c64n && aff_packed(c64n) != aff_packed(c64))) c64n = 0;
+    if (cigar || c16n == c16 || (c16n && aff_packed(c16n) != aff_packed(c16))) c16n = 0;
+    if (c8n == c8) c8n = 0;
+    return Cands{c8, c16, c32, c8n, c16n, c64, c64n};
+}
+
 void choose_shapes(const sf_batch *b, const sf_scoring *sc, const std::vector<uint64_t> &ba, bool cigar,
                    std::vector<int> &kcls) {
     const uint64_t n = b->n;
     const int lp8 = sf_affine_lp8_policy();
     const uint64_t lp16max = sf_affine_lp16_max();
-    std::vector<std::array<int, 4>> cand(n); // 8, 16, 32 and 64 lanes a pair; 0 where not allowed
-    auto pair_row = [](int c) { return aff_row_cost(c) * aff_lp(c) / 64.0; };
-    auto chain = [&](uint64_t i, int c) { return 1.5 * b->plen[i] * aff_row_cost(c); };
+    std::vector<Cands> cand(n);
+    const bool tuned = !cigar;
+    auto tput = [&](int c) { return aff_tput_ns(c, tuned); };
+    auto chain = [&](uint64_t i, int c) { return b->plen[i] * aff_chain_ns(c, tuned); };
     parallel_for(n, [&](uint64_t lo, uint64_t hi) {
         for (uint64_t i = lo; i < hi; ++i) {
             const uint32_t m = b->plen[i], below = (uint32_t)ba[i];
             const uint64_t width = below + (ba[i] >> 32) + 1;
-            int c64 = sf_affine_class(width, false);
-            if (int p = sf_affine_pk16_class(aff_k(c64), sc->mat, sc->mis, sc->open, sc->ext, m, below, cigar, 64))
-                c64 = p;
-            int c16 = 0;
-            if (width <= lp16max && width <= 16 * 32) {
-                c16 = sf_affine_class(width, true);
-                if (int p = sf_affine_pk16_class(aff_k(c16), sc->mat, sc->mis, sc->open, sc->ext, m, below, cigar, 16))
-                    c16 = p;
-            }
-            const int c8 =
-                (!cigar && lp8 >= 0) ? sf_affine_pk8_class(width, sc->mat, sc->mis, sc->open, sc->ext, m, below) : 0;
-            // 32 lanes: traceback runs of bands over 512 slots (2 * ceil(width / 64) slots a lane).
-            const int c32 = (cigar && width > 16 * 32 && width <= 32 * 32)
-                                ? sf_affine_pk16_class(2 * (int)((width + 63) / 64), sc->mat, sc->mis, sc->open, sc->ext,
-                                                       m, below, true, 32)
-                                : 0;
-            cand[i] = {c8, c16, c32, c64};
+            cand[i] = pair_candidates(m, below, width, sc, cigar, lp8, lp16max);
             int pick = 0;
             for (int c : cand[i])
-                if (c && (!pick || (lp8 <= 0 && pair_row(c) < pair_row(pick)))) pick = c;
+                if (c && (!pick || (lp8 <= 0 && tput(c) < tput(pick)))) pick = c;
             kcls[i] = pick;
         }
     });
     if (lp8 > 0) return;
     double spread = 0;
-    for (uint64_t i = 0; i < n; ++i) spread += b->plen[i] * pair_row(kcls[i]) / 1024.0;
+    for (uint64_t i = 0; i < n; ++i) spread += b->plen[i] * tput(kcls[i]);
     parallel_for(n, [&](uint64_t lo, uint64_t hi) {
         for (uint64_t i = lo; i < hi; ++i) {
             if (chain(i, kcls[i]) <= spread) continue;
             int pick = 0;
             for (int c : cand[i])
-                if (c && chain(i, c) <= spread && (!pick || pair_row(c) < pair_row(pick))) pick = c;
-            kcls[i] = pick ? pick : cand[i][3];
+                if (c && chain(i, c) <= spread && (!pick || tput(c) < tput(pick))) pick = c;
+            kcls[i] = pick ? pick : cand[i][kC64];
         }
     });
     // Per class (codes are below 65536): its chain, its spread, and its spread on 64 lanes.
     std::vector<double> cchain(65536, 0.0), cspread(65536, 0.0), cspread64(65536, 0.0);
     for (uint64_t i = 0; i < n; ++i) {
         const int k = kcls[i];
-        cchain[k] = std::max(cchain[k], b->plen[i] * aff_row_cost(k));
-        cspread[k] += b->plen[i] * pair_row(k) / 1024.0;
+        cchain[k] = std::max(cchain[k], chain(i, k));
+        cspread[k] += b->plen[i] * tput(k);
         // (The move costs a class 4 to 8 times its spread: only where that stays small.)
-        if (aff_lp(k) != 64) cspread64[k] += b->plen[i] * pair_row(cand[i][3]) / 1024.0;
+        if (aff_lp(k) != 64) cspread64[k] += b->plen[i] * tput(cand[i][kC64]);
     }
     for (uint64_t i = 0; i < n; ++i) {
         const int k = kcls[i];
-        if (aff_lp(k) != 64 && cchain[k] > cspread[k] && cspread64[k] <= 0.05 * spread) kcls[i] = cand[i][3];
+        if (aff_lp(k) != 64 && cchain[k] > cspread[k] && cspread64[k] <= 0.05 * spread) kcls[i] = cand[i][kC64];
     }
 }
 
@@ -819,7 +869,7 @@ int refuse_in_capture(sf_batch *b, const char *what) {
 }
 
 int build_plan(sf_batch *b, sf_batch::AffPlan &pl, const std::vector<uint64_t> &ba, const std::vector<int> &kcls,
-               uint64_t key, uint64_t cells) {
+               uint64_t key, uint64_t cells, bool tuned = false) {
     const uint64_t n = b->n;
     int st;
     if (pl.d_ba && (st = quiesce(b))) return st; // the plan being replaced may still be in use
@@ -829,10 +879,10 @@ int build_plan(sf_batch *b, sf_batch::AffPlan &pl, const std::vector<uint64_t> &
         uint64_t e = q;
         while (e < n && kcls[order[e]] == kcls[order[q]]) ++e;
         sf_batch::Bin bin{kcls[order[q]], (uint32_t)q, (uint32_t)e};
-        const double per_row = aff_row_cost(bin.cls), ppw = 64.0 / aff_lp(bin.cls);
+        const double tput = aff_tput_ns(bin.cls, tuned), lat = aff_chain_ns(bin.cls, tuned);
         for (uint64_t x = q; x < e; ++x) {
-            bin.chain = std::max(bin.chain, b->plen[order[x]] * per_row);
-            bin.spread += b->plen[order[x]] * per_row / ppw / 1024.0;
+            bin.chain = std::max(bin.chain, b->plen[order[x]] * lat);
+            bin.spread += b->plen[order[x]] * tput;
         }
         pl.bins.push_back(bin);
         q = e;
@@ -1136,7 +1186,8 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     // Shape classes, cached across runs with the same mode, bands, scoring and tuning.
     sf_batch::AffPlan &pl = b->aplan[cigar ? 1 : 0];
     uint64_t key = (1469598103934665603ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8) ^
-                    ((uint64_t)(sf_affine_lp8_policy() + 1) << 4)) * 1099511628211ull;
+                    ((uint64_t)(sf_affine_lp8_policy() + 1) << 4) ^ (cigar ? 0 : sf_tune_generation() << 24)) *
+                   1099511628211ull;
     for (int32_t v : {sc->mat, sc->mis, sc->open, sc->ext}) key = (key ^ (uint64_t)(uint32_t)v) * 1099511628211ull;
     if (band)
         for (uint64_t i = 0; i < n; ++i) key = (key ^ band[i]) * 1099511628211ull;
@@ -1164,7 +1215,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
             // Values stay far above NEG over all slots of the class.
             if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, b->plen[i], b->tlen[i], aff_slots(kcls[i])))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
-        if ((st = build_plan(b, pl, ba, kcls, key, cells))) return st;
+        if ((st = build_plan(b, pl, ba, kcls, key, cells, !cigar))) return st;
     }
     if (!b->d_score && ((st = dalloc(&b->d_score, n)) || (st = dalloc(&b->d_col, n)))) return st;
     AffineArgs a{};
@@ -1394,6 +1445,78 @@ int sf_set_affine_lp8(int policy) {
     if (policy < -1 || policy > 1) return fail(SF_EINVAL, "policy must be -1, 0 or 1");
     sf_affine_set_lp8_policy(policy);
     return SF_OK;
+}
+
+int sf_batch_autotune(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t *band) {
+    if (!b) return fail(SF_EINVAL, "batch is null");
+    if (!sc) return fail(SF_EINVAL, "scoring is null");
+    if (mode < SF_INFIX || mode > SF_GLOBAL) return fail(SF_EINVAL, "mode %d", mode);
+    if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0 || sc->mat > 65535 || sc->mis > 65535 ||
+        sc->open > 65535 || sc->ext > 65535)
+        return fail(SF_EINVAL, "scores must be in 0..65535");
+    int st;
+    if ((st = refuse_in_capture(b, "sf_batch_autotune")) || (st = ensure_device()) || (st = quiesce(b))) return st;
+    const uint64_t lp16max = sf_affine_lp16_max();
+    std::vector<char> seen(65536, 0);
+    std::vector<int> classes;
+    for (uint64_t i = 0; i < b->n; ++i) {
+        uint64_t below, above;
+        sf::band_geom(b->plen[i], b->tlen[i], mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
+        const uint64_t width = below + above + 1;
+        if (sf_affine_class(width, false) == 0) continue; // sf_batch_affine refuses it
+        for (int c : pair_candidates(b->plen[i], below, width, sc, false, 0, lp16max))
+            if (c && !seen[c]) {
+                seen[c] = 1;
+                classes.push_back(c);
+            }
+    }
+    if ((st = sf_tune_measure(classes, mode, sc))) return st;
+    // The setting: the batch's score run timed under the estimates and under the measurements with
+    // the chain scaled by 1, 1.5, 2 and 3 (median of three runs each, after one that plans it).
+    struct Setting {
+        bool use;
+        double factor;
+    };
+    const Setting settings[] = {{false, 1.0}, {true, 1.0}, {true, 1.5}, {true, 2.0}, {true, 3.0}};
+    // (The timing runs are not the batch's runs: its timing totals are restored afterwards.)
+    b->collect();
+    double ms_saved[5];
+    std::copy(b->ms, b->ms + 5, ms_saved);
+    const uint64_t cells_saved = b->affine_cells, launches_saved = b->affine_launches;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best = 1e30;
+    Setting pick = settings[0];
+    for (const Setting &x : settings) {
+        sf_tune_set(x.use, x.factor);
+        double t[3];
+        for (int r = -1; r < 3 && !st; ++r) {
+            if ((st = sf_batch_affine(b, mode, sc, band, 0)) || (st = quiesce(b))) break;
+            if (r < 0) continue;
+            float ms = 0;
+            if (hipEventRecord(e0, b->stream) != hipSuccess || (st = sf_batch_affine(b, mode, sc, band, 0)) ||
+                hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                if (!st) st = fail(SF_EDEVICE, "autotune: timing the score run");
+                break;
+            }
+            t[r] = ms;
+        }
+        if (st) break;
+        std::sort(t, t + 3);
+        if (t[1] < best) best = t[1], pick = x;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    sf_tune_set(pick.use, pick.factor);
+    b->aplan[0].valid = false; // the next score run plans with the setting kept
+    if (const int q = quiesce(b)) return st ? st : q;
+    b->collect();
+    std::copy(ms_saved, ms_saved + 5, b->ms);
+    b->affine_cells = cells_saved;
+    b->affine_launches = launches_saved;
+    return st;
 }
 
 int sf_set_persistent(int on) {
