@@ -102,6 +102,18 @@ int sf_init(int device);
  * same distance (0 for Prefix and Global).  end/start have no reference counterpart. */
 int sf_myers(const sf_pairs *pairs, int mode, uint32_t k, uint32_t *dist, uint32_t *end, uint32_t *start);
 
+/* Replaces crates/map/src/lib.rs:851-911 `Mapper::banded` (equal to its definition `sequential`,
+ * lib.rs:3016-3034) for many reads at once.  Group g -- pairs [group_off[g], group_off[g + 1]) --
+ * is one read (every pair's pattern) against its texts in the caller's order; group_off has
+ * ngroups + 1 entries, from 0 to pairs->n.  out[i] is the value the reference gives text i: in its
+ * group, in order, with k = (best so far) + cap (saturating, best starting at u32::MAX), the edit
+ * distance in `mode` where it is <= k, else k + 1, best falling to it; a text equal to an earlier
+ * one of its group takes that one's value re-capped at the current k.  `distinct` != 0: the
+ * caller promises no text repeats within a group (no comparison is made).  Texts compare by codes
+ * masked & 3 (the results are those of comparing bytes, see the implementation). */
+int sf_myers_banded(const sf_pairs *pairs, const uint64_t *group_off, uint64_t ngroups, int mode, uint32_t cap,
+                    int distinct, uint32_t *out);
+
 /* No counterpart in the reference: sf_myers for callers that cannot allocate (harnesses that forbid
  * device allocation, tight serving loops).  The same arguments and results as sf_myers, computed
  * by the same kernels, with no hipMalloc, hipFree, stream creation or host heap allocation: the

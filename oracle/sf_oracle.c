@@ -258,6 +258,94 @@ void sfo_batch_distances(const uint8_t *pat, uint64_t m, const uint8_t *const *t
     free(peq);
 }
 
+/* ---- the caller's re-capping (crates/map/src/lib.rs:851-911 `Mapper::banded`, and the test
+ * definition `sequential`, lib.rs:3016-3034) ---- */
+
+static int same_text(const uint8_t *a, uint64_t na, const uint8_t *b, uint64_t nb) {
+    return na == nb && (na == 0 || memcmp(a, b, na) == 0);
+}
+
+static uint32_t sat_add_u32(uint32_t a, uint32_t b) { return a > UINT32_MAX - b ? UINT32_MAX : a + b; }
+
+/* lib.rs:851-911: texts deduplicated in first-occurrence order (`distinct`: no repeats, no
+ * hashing), distinct texts in batches of eight (batch_distances, lib.rs:891-897) at the threshold
+ * k = best + cap in force at each batch's first text, each text's value re-capped in order at its
+ * own threshold (lib.rs:898-904), a repeat taking its first occurrence's value re-capped
+ * (lib.rs:884-889).  The repeat lookup is a linear scan here (the reference hashes; the same
+ * first-occurrence map). */
+void sfo_banded(const uint8_t *read, uint64_t m, const uint8_t *const *texts, const uint64_t *lens, uint64_t nt,
+                int mode, uint32_t cap, int distinct, uint32_t *out) {
+    uint64_t *id = (uint64_t *)malloc((nt ? nt : 1) * sizeof(uint64_t));
+    uint64_t *at = (uint64_t *)malloc((nt ? nt : 1) * sizeof(uint64_t)); /* distinct u -> first text index */
+    uint64_t nu = 0;
+    for (uint64_t i = 0; i < nt; i++) {
+        uint64_t u = nu;
+        if (!distinct)
+            for (uint64_t v = 0; v < nu; v++)
+                if (same_text(texts[at[v]], lens[at[v]], texts[i], lens[i])) {
+                    u = v;
+                    break;
+                }
+        if (u == nu) at[nu++] = i;
+        id[i] = u;
+    }
+    uint32_t *raw = (uint32_t *)malloc((nu ? nu : 1) * sizeof(uint32_t));
+    uint32_t *val = (uint32_t *)calloc(nu ? nu : 1, sizeof(uint32_t));
+    const uint8_t **bt = (const uint8_t **)malloc(8 * sizeof(uint8_t *));
+    uint64_t bl[8];
+    uint64_t nraw = 0;
+    uint32_t best = UINT32_MAX;
+    for (uint64_t i = 0; i < nt; i++) {
+        const uint64_t u = id[i];
+        const uint32_t k = sat_add_u32(best, cap);
+        if (at[u] != i) {
+            const uint32_t d = val[u];
+            out[i] = d <= k ? d : sat_add_u32(k, 1);
+            continue;
+        }
+        if (u == nraw) {
+            const uint64_t c = nu - u < 8 ? nu - u : 8;
+            for (uint64_t q = 0; q < c; q++) {
+                bt[q] = texts[at[u + q]];
+                bl[q] = lens[at[u + q]];
+            }
+            sfo_batch_distances(read, m, bt, bl, c, mode, k, raw + u);
+            nraw += c;
+        }
+        const uint32_t d = (raw[u] != SFO_NONE && raw[u] <= k) ? raw[u] : sat_add_u32(k, 1);
+        if (d < best) best = d;
+        val[u] = d;
+        out[i] = d;
+    }
+    free(id), free(at), free(raw), free(val), free(bt);
+}
+
+/* lib.rs:3016-3034 `sequential`: one distance_with per distinct text in order at k = best + cap
+ * (None -> k + 1), a repeat its first occurrence's value re-capped at the current k. */
+void sfo_sequential(const uint8_t *read, uint64_t m, const uint8_t *const *texts, const uint64_t *lens, uint64_t nt,
+                    int mode, uint32_t cap, uint32_t *out) {
+    uint32_t best = UINT32_MAX;
+    for (uint64_t i = 0; i < nt; i++) {
+        const uint32_t k = sat_add_u32(best, cap);
+        uint64_t first = i;
+        for (uint64_t j = 0; j < i; j++)
+            if (same_text(texts[j], lens[j], texts[i], lens[i])) {
+                first = j;
+                break;
+            }
+        if (first != i) {
+            const uint32_t d = out[first]; /* the first occurrence's value, as it was computed */
+            /* (out[first] is the first occurrence's own result: lib.rs stores `seen[t] = d`) */
+            out[i] = d <= k ? d : sat_add_u32(k, 1);
+            continue;
+        }
+        const uint32_t r = sfo_distance(read, m, texts[i], lens[i], mode, k);
+        const uint32_t d = r == SFO_NONE ? sat_add_u32(k, 1) : r;
+        if (d < best) best = d;
+        out[i] = d;
+    }
+}
+
 /* D[m][j] for j = 0..n by a full bit-parallel pass (no cut-off): the last row of the
  * (mode-dependent) edit-distance matrix.  Own helper for sfo_locate. */
 static void myers_lastrow(const uint8_t *p, uint64_t m, const uint8_t *t, uint64_t n, int mode, int32_t *row) {

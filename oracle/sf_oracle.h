@@ -60,6 +60,13 @@ uint32_t sfo_distance(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64
 void sfo_batch_distances(const uint8_t *pat, uint64_t m, const uint8_t *const *txts,
                          const uint64_t *lens, uint64_t nt, int mode, uint32_t k, uint32_t *out);
 
+/* crates/map/src/lib.rs:851-911 `Mapper::banded` (batches of eight distinct texts, re-capped in
+ * order) and lib.rs:3016-3034 `sequential` (its test definition): one u32 per text. */
+void sfo_banded(const uint8_t *read, uint64_t m, const uint8_t *const *texts, const uint64_t *lens, uint64_t nt,
+                int mode, uint32_t cap, int distinct, uint32_t *out);
+void sfo_sequential(const uint8_t *read, uint64_t m, const uint8_t *const *texts, const uint64_t *lens, uint64_t nt,
+                    int mode, uint32_t cap, uint32_t *out);
+
 /* Own definition (no reference counterpart): distance as sfo_distance, `end` = smallest j with
  * D[m][j] == distance (exclusive text end; n for Global), `start` = end - L where L is the
  * smallest prefix of reverse(text[..end]) whose Prefix-mode distance to reverse(pattern) is the

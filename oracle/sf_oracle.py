@@ -65,6 +65,10 @@ def lib():
         L.sfo_batch_distances.argtypes = [u8p, u64, C.POINTER(C.c_char_p), C.POINTER(u64), u64, C.c_int, u32,
                                           C.POINTER(u32)]
         L.sfo_locate.argtypes = [u8p, u64, u8p, u64, C.c_int, u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+        L.sfo_banded.argtypes = [u8p, u64, C.POINTER(C.c_char_p), C.POINTER(u64), u64, C.c_int, u32, C.c_int,
+                                 C.POINTER(u32)]
+        L.sfo_sequential.argtypes = [u8p, u64, C.POINTER(C.c_char_p), C.POINTER(u64), u64, C.c_int, u32,
+                                     C.POINTER(u32)]
         L.sfo_affine_dp.argtypes = [u8p, u64, u8p, u64, C.c_int, sc, u64, u64, C.c_int, i32, C.c_int, u64,
                                     C.POINTER(Dp)]
         L.sfo_affine.argtypes = [u8p, u64, u8p, u64, C.c_int, sc, u64]
@@ -145,6 +149,30 @@ def batch_distances(p, texts, mode, k=NONE):
     out = (C.c_uint32 * max(n, 1))()
     lib().sfo_batch_distances(p, len(p), arr, lens, n, mode, k, out)
     return [None if out[i] == NONE else out[i] for i in range(n)]
+
+
+def _texts(texts):
+    ts = [_b(t) for t in texts]
+    n = len(ts)
+    return ts, n, (C.c_char_p * max(n, 1))(*ts), (C.c_uint64 * max(n, 1))(*[len(t) for t in ts])
+
+
+def banded(read, texts, mode, cap, distinct=False):
+    """crates/map/src/lib.rs:851-911 `Mapper::banded`: one u32 per text."""
+    r = _b(read)
+    ts, n, arr, lens = _texts(texts)
+    out = (C.c_uint32 * max(n, 1))()
+    lib().sfo_banded(r, len(r), arr, lens, n, mode, cap, int(distinct), out)
+    return [out[i] for i in range(n)]
+
+
+def sequential(read, texts, mode, cap):
+    """crates/map/src/lib.rs:3016-3034 `sequential` (the definition `banded` is tested against)."""
+    r = _b(read)
+    ts, n, arr, lens = _texts(texts)
+    out = (C.c_uint32 * max(n, 1))()
+    lib().sfo_sequential(r, len(r), arr, lens, n, mode, cap, out)
+    return [out[i] for i in range(n)]
 
 
 def locate(p, t, mode, k=NONE):

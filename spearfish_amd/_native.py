@@ -137,6 +137,7 @@ SIGNATURES = {
     "sf_init": ([_int], _int),
     "sf_myers": ([C.POINTER(Pairs), _int, _u32, _vp, _vp, _vp], _int),
     "sf_myers_noalloc": ([C.POINTER(Pairs), _int, _u32, _vp, _vp, _vp], _int),
+    "sf_myers_banded": ([C.POINTER(Pairs), _vp, _u64, _int, _u32, _int, _vp], _int),
     "sf_affine": ([C.POINTER(Pairs), _int, C.POINTER(Scoring), _vp, _vp, _vp], _int),
     "sf_affine_cigar": ([C.POINTER(Pairs), _int, C.POINTER(Scoring), _vp, _vp, _vp, _vp, _vp, _u64, _vp], _int),
     "sf_batch_create": ([C.POINTER(Pairs), C.POINTER(_vp)], _int),

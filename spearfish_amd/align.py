@@ -390,6 +390,27 @@ def batch_distances(pat: Pattern, texts: Sequence, mode: Mode, k: int) -> list:
     return [None if x == U32_MAX else int(x) for x in d]
 
 
+def banded_batch(pairs: PairSet, group_off, mode: Mode, cap: int, distinct: bool = False) -> np.ndarray:
+    """crates/map/src/lib.rs:851-911 `Mapper::banded` for many reads: group g is pairs
+    [group_off[g], group_off[g + 1]), one read against its texts in order; one u32 per pair."""
+    lib = N.load()
+    n = len(pairs)
+    go = np.ascontiguousarray(group_off, dtype=np.uint64)
+    out = np.empty(max(n, 1), np.uint32)
+    cp = pairs.c()
+    N.check(lib.sf_myers_banded(C.byref(cp), go.ctypes.data, len(go) - 1, int(mode), cap, int(distinct),
+                                out.ctypes.data))
+    return out[:n]
+
+
+def banded(read, texts: Sequence, mode: Mode, cap: int, distinct: bool = False) -> list:
+    """crates/map/src/lib.rs:851-911 `Mapper::banded`: one read against its texts."""
+    if not len(texts):
+        return []
+    ps = PairSet.from_seqs([read] * len(texts), texts)
+    return [int(x) for x in banded_batch(ps, [0, len(texts)], mode, cap, distinct)]
+
+
 def infix_distance(pattern, text) -> int:
     """align.rs:655-658."""
     return distance(pattern, text, Mode.Infix, U32_MAX)

@@ -14,6 +14,7 @@
 #include <numeric>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/sf_align.h"
@@ -1931,6 +1932,88 @@ int sf_local_band_ends(const sf_pairs *pairs, const sf_scoring *sc, const int64_
     if (!st) st = sf_batch_local_results(b, best);
     sf_batch_destroy(b);
     return st;
+}
+
+int sf_myers_banded(const sf_pairs *pairs, const uint64_t *group_off, uint64_t ngroups, int mode, uint32_t cap,
+                    int distinct, uint32_t *out) {
+    int st = check_pairs(pairs);
+    if (st) return st;
+    if (!group_off || (pairs->n && !out)) return fail(SF_EINVAL, "null argument");
+    if (mode < SF_INFIX || mode > SF_GLOBAL) return fail(SF_EINVAL, "mode %d", mode);
+    const uint64_t n = pairs->n;
+    if (group_off[0] != 0 || group_off[ngroups] != n) return fail(SF_EINVAL, "groups must cover the pairs");
+    for (uint64_t g = 0; g < ngroups; ++g)
+        if (group_off[g + 1] < group_off[g]) return fail(SF_EINVAL, "group offsets must not decrease");
+    if (n == 0) return SF_OK;
+    // Distinct texts of each group, in first-occurrence order (lib.rs:862-879).  Texts compare by
+    // their codes masked & 3: the reference compares bytes, but texts whose bytes differ only
+    // above bit 1 have the same distance, and a repeat and a distinct text of equal distance take
+    // the same value in order (k + 1 > best either way), so the results are the same.
+    std::vector<uint64_t> first(n); // pair -> the pair holding its group's first occurrence
+    auto hash = [&](uint64_t i) {
+        uint64_t h = 1469598103934665603ull ^ pairs->txt_len[i];
+        for (uint32_t q = 0; q < pairs->txt_len[i]; ++q)
+            h = (h ^ code_at(pairs->txt, pairs->packed, pairs->txt_off[i] + q)) * 1099511628211ull;
+        return h;
+    };
+    auto same = [&](uint64_t i, uint64_t j) {
+        if (pairs->txt_len[i] != pairs->txt_len[j]) return false;
+        for (uint32_t q = 0; q < pairs->txt_len[i]; ++q)
+            if (code_at(pairs->txt, pairs->packed, pairs->txt_off[i] + q) !=
+                code_at(pairs->txt, pairs->packed, pairs->txt_off[j] + q))
+                return false;
+        return true;
+    };
+    std::vector<uint64_t> uniq;
+    for (uint64_t g = 0; g < ngroups; ++g) {
+        std::unordered_multimap<uint64_t, uint64_t> seen;
+        for (uint64_t i = group_off[g]; i < group_off[g + 1]; ++i) {
+            first[i] = i;
+            if (!distinct) {
+                const uint64_t h = hash(i);
+                auto r = seen.equal_range(h);
+                for (auto it = r.first; it != r.second; ++it)
+                    if (same(it->second, i)) {
+                        first[i] = it->second;
+                        break;
+                    }
+                if (first[i] == i) seen.emplace(h, i);
+            }
+            if (first[i] == i) uniq.push_back(i);
+        }
+    }
+    // The distinct texts' exact distances on the device (distance_with is exact wherever it is
+    // within k, so these give every threshold's answer).
+    std::vector<uint64_t> poff(uniq.size()), toff(uniq.size());
+    std::vector<uint32_t> plen(uniq.size()), tlen(uniq.size()), d(uniq.size());
+    for (size_t q = 0; q < uniq.size(); ++q) {
+        const uint64_t i = uniq[q];
+        poff[q] = pairs->pat_off[i], plen[q] = pairs->pat_len[i];
+        toff[q] = pairs->txt_off[i], tlen[q] = pairs->txt_len[i];
+    }
+    sf_pairs sub = *pairs;
+    sub.pat_off = poff.data(), sub.pat_len = plen.data(), sub.txt_off = toff.data(), sub.txt_len = tlen.data();
+    sub.n = uniq.size();
+    if ((st = sf_myers(&sub, mode, UINT32_MAX, d.data(), nullptr, nullptr))) return st;
+    std::vector<uint32_t> exact(n, 0);
+    for (size_t q = 0; q < uniq.size(); ++q) exact[uniq[q]] = d[q];
+    // The sequential values (lib.rs:3016-3034, equal to lib.rs:880-910 by its own argument).
+    auto sat = [](uint32_t a, uint32_t b) { return a > UINT32_MAX - b ? UINT32_MAX : a + b; };
+    for (uint64_t g = 0; g < ngroups; ++g) {
+        uint32_t best = UINT32_MAX;
+        for (uint64_t i = group_off[g]; i < group_off[g + 1]; ++i) {
+            const uint32_t k = sat(best, cap);
+            if (first[i] != i) {
+                const uint32_t v = out[first[i]];
+                out[i] = v <= k ? v : sat(k, 1);
+                continue;
+            }
+            const uint32_t v = exact[i] <= k ? exact[i] : sat(k, 1);
+            best = std::min(best, v);
+            out[i] = v;
+        }
+    }
+    return SF_OK;
 }
 
 int sf_myers(const sf_pairs *pairs, int mode, uint32_t k, uint32_t *dist, uint32_t *end, uint32_t *start) {
