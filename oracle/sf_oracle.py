@@ -57,7 +57,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        _lib = declare(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def declare(L):
+    """Sets the ctypes signatures of the oracle's functions on a loaded copy of the library."""
+    if True:
         u8p, u64, i32, u32 = C.c_char_p, C.c_uint64, C.c_int32, C.c_uint32
         sc = C.POINTER(Scoring)
         L.sfo_distance.argtypes = [u8p, u64, u8p, u64, C.c_int, u32]
@@ -103,8 +109,7 @@ def lib():
         L.sfo_batch_affine_avx512.restype = C.c_int
         L.sfo_avx512_available.argtypes = []
         L.sfo_avx512_available.restype = C.c_int
-        _lib = L
-    return _lib
+    return L
 
 
 def _b(s) -> bytes:
