@@ -45,6 +45,14 @@ struct MyersArgs {
     uint32_t *dist;
     uint32_t *end;
     uint32_t *start;
+    // Patterns longer than the largest class (kMyersStriped): launch `stripe` runs blocks
+    // [stripe * 1536, (stripe + 1) * 1536) of each pair's pattern; a column's horizontal delta into
+    // the stripe's first block comes from hcol_in (the stripe above's), the delta out of its last
+    // block goes to hcol_out, one byte a column (bit 0: +1, bit 1: -1) at launch slot * hcol_stride.
+    uint32_t stripe;
+    const uint8_t *hcol_in;
+    uint8_t *hcol_out;
+    uint64_t hcol_stride;
 };
 
 struct AffineArgs {
@@ -129,6 +137,19 @@ struct PassArgs {
     int32_t bonus0, bonus1;  // local: end bonuses for the read's start and end
     const uint32_t *pinv;    // local: bit (base offset) set where a pattern code is >= 4 (null: none)
     int32_t *best;           // per pair (local passes)
+};
+
+// A wide-band pass (affine_wide.hip): the pass arguments (run_core bookkeeping into p.out, or local
+// passes into p.best), or with p.out and p.best null a score run into score / score_col; rows:
+// scratch of 4 * stride int32 per pair of the launch (H and F of the previous and the current row,
+// each at least the widest band plus one slot).
+struct WideArgs {
+    PassArgs p;
+    int local;
+    int32_t *score;
+    uint32_t *score_col;
+    int32_t *rows;
+    uint64_t stride;
 };
 
 struct WalkArgs {
@@ -309,9 +330,13 @@ __device__ __forceinline__ int32_t wave_excl_max(int32_t v, int32_t ident) {
 } // namespace sf
 
 // Affine shape classes: K slots per lane and LP lanes per pair, coded K for LP = 64 and
-// 1000 * LP + K otherwise, plus 100 for the 16-bit packed kernel (scores only).
-constexpr int aff_lp(int cls) { return cls >= 1000 ? cls / 1000 : 64; }
-constexpr int aff_k(int cls) { return cls >= 1000 ? cls % 100 : cls; }
+// 1000 * LP + K otherwise, plus 100 for the 16-bit packed kernel (scores only).  kAffWide is the
+// kernel for bands over 2,048 cells a row (affine_wide.hip: one pair per 256-thread workgroup, the
+// rows in scratch, any width up to kAffWideMax), counted as 256 lanes of 8 slots.
+constexpr int kAffWide = 60000;
+constexpr uint64_t kAffWideMax = 1ull << 26;
+constexpr int aff_lp(int cls) { return cls == kAffWide ? 256 : (cls >= 1000 ? cls / 1000 : 64); }
+constexpr int aff_k(int cls) { return cls == kAffWide ? 8 : (cls >= 1000 ? cls % 100 : cls); }
 constexpr bool aff_packed(int cls) { return cls >= 1000 && cls % 1000 >= 100; }
 constexpr uint64_t aff_slots(int cls) { return (uint64_t)aff_lp(cls) * (uint64_t)aff_k(cls); }
 // Traceback words per row of one pair: 4 bits per slot, a lane's slots in ceil(4K / 32) words.
@@ -372,6 +397,7 @@ int sf_myers_mix_group(int cls);
 uint32_t sf_myers_blocks_for(int cls, uint32_t count);
 hipError_t sf_launch_myers_mix(const sf::MyersArgs &a, const sf::MixArgs &mx, int group, hipStream_t s);
 hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
+hipError_t sf_launch_affine_wide(const sf::WideArgs &a, hipStream_t s);
 // cls: a 32-bit affine class (16 or 64 lanes); local selects local_band_core.
 hipError_t sf_launch_pass(const sf::PassArgs &a, int cls, bool local, hipStream_t s);
 // desc[w] of pair order[w], w < n (ba may be null: no band).
@@ -394,6 +420,12 @@ hipError_t sf_launch_affine_mix1_persistent(const sf::AffineArgs &a, const sf::M
 
 // Supported shape classes.
 int sf_myers_maxb_for(uint32_t nblocks);     // 0 if unsupported
+// Patterns over 1,536 blocks (98,304 bases): the 64-lane, 24-block pipeline run stripe by stripe
+// (myers_pipe_body<64, 24, true>), up to kMyersStripedMax blocks.
+constexpr int kMyersStriped = 99000;
+constexpr uint32_t kMyersStripedBlocks = 64 * 24;
+constexpr uint32_t kMyersStripedMax = 1u << 24;
+hipError_t sf_launch_myers_striped(const sf::MyersArgs &a, hipStream_t s);
 // The 64-lane pipeline class with the shorter chain that holds a 32-lane class's patterns.
 int sf_myers_wide_class(int cls);
 int sf_affine_k_for(uint64_t width);         // 0 if unsupported

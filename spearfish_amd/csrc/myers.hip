@@ -32,6 +32,11 @@ __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
     myers_pipe_body<G, R>(a, blockIdx.x);
 }
 
+// One stripe of 1,536 blocks of patterns longer than the largest class.
+__global__ __launch_bounds__(256) void myers_striped_kernel(MyersArgs a) {
+    myers_pipe_body<64, 24, true>(a, blockIdx.x);
+}
+
 } // namespace
 
 // Shape classes: lane kernel MAXB (1..24; more planes would spill), then pipelined classes
@@ -90,6 +95,13 @@ hipError_t sf_launch_myers(const MyersArgs &a, int maxb, hipStream_t s) {
     return hipGetLastError();
 }
 
+
+hipError_t sf_launch_myers_striped(const MyersArgs &a, hipStream_t s) {
+    if (a.count == 0) return hipSuccess;
+    const uint64_t threads = (uint64_t)a.count * 64;
+    hipLaunchKernelGGL(myers_striped_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 
 // Loads this file's code object now (see sf_load_code_objects).
 hipError_t sf_load_code_myers() {

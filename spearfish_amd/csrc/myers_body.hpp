@@ -207,7 +207,12 @@ __device__ __forceinline__ void myers_lane_body(const MyersArgs &a, uint32_t blk
 // last block is the last slot of lane G - 1 (positions below off = G*R - nb hold no rows and pass
 // the deltas through), so one slot per lane needs the per-pair last-row bit.  Columns past a
 // text's end (in its last step) run on unspecified codes and are left out of the score.
-template <int G, int R>
+//
+// ST: one stripe of a pattern longer than G * R blocks (MyersArgs::stripe): the pipeline holds the
+// stripe's blocks, lane 0 takes each column's delta from the stripe above (hcol_in; the first
+// stripe takes `top`), the last lane writes its deltas out (hcol_out) unless the stripe holds the
+// pattern's last block, which keeps the score as above.
+template <int G, int R, bool ST = false>
 __device__ __forceinline__ void myers_pipe_body(const MyersArgs &a, uint32_t blk) {
     // 16 lanes: one DPP row each; 32 and 64: the wave shift (with 32, lane 32 -- the second pair's
     // lane 0 -- keeps its own deltas, as lane 0 does).
@@ -233,17 +238,32 @@ __device__ __forceinline__ void myers_pipe_body(const MyersArgs &a, uint32_t blk
         if (a.dist[pi] == kNone) work = false;
         else n = a.end[pi];
     }
+    // Stripes: this launch's blocks [b0, b0 + nb) of the pair's pattern (all of them without ST).
+    bool st_last = true;
+    int b0 = 0;
+    const uint8_t *hcin = nullptr;
+    uint8_t *hcout = nullptr;
+    if constexpr (ST) {
+        const int nb_all = work ? (int)((m + 63) / 64) : 0;
+        const int nst = (nb_all + G * R - 1) / (G * R);
+        st_last = (int)a.stripe == nst - 1;
+        if ((int)a.stripe >= nst) work = false; // (its stripes are done)
+        b0 = (int)a.stripe * G * R;
+        if (a.stripe > 0) hcin = a.hcol_in + (uint64_t)grp * a.hcol_stride;
+        if (!st_last) hcout = a.hcol_out + (uint64_t)grp * a.hcol_stride;
+    }
     if (!work) n = 0;
-    const int nb = work ? (int)((m + 63) / 64) : 0;
+    int nb = work ? (int)((m + 63) / 64) : 0;
+    if constexpr (ST) nb = !work ? 0 : (!st_last ? G * R : nb - b0);
     const int off = G * R - nb; // >= 0: the launcher's class holds every block
     const unsigned last_bit = work ? (m - 1) & 63u : 63u;
-    const unsigned bitR = l == G - 1 ? last_bit : 63u; // row of the delta out of slot R - 1
+    const unsigned bitR = (l == G - 1 && st_last) ? last_bit : 63u; // row of the delta out of slot R - 1
     Blk K[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         uint64_t p1 = 0, p2 = 0;
         const int b = l * R + r - off;
-        if (work && b >= 0) load_planes(P, m, b, rev, p1, p2);
+        if (work && b >= 0) load_planes(P, m, ST ? b0 + b : b, rev, p1, p2);
         K[r] = Blk{(uint32_t)p1, (uint32_t)(p1 >> 32), (uint32_t)p2, (uint32_t)(p2 >> 32), ~0u, ~0u, 0u, 0u};
     }
     // Steps: the wave runs until its longest text has passed every lane.  From step G on, up to
@@ -291,6 +311,17 @@ __device__ __forceinline__ void myers_pipe_body(const MyersArgs &a, uint32_t blk
             nin[c] = (G == 32 && l == 0) ? nin[c] : un;
         }
         const int32_t j0 = CB * (t - l);
+        if constexpr (ST) {
+            // Lane 0: the step's four columns' deltas from the stripe above (one aligned word).
+            if (hcin && l == 0 && j0 < (int32_t)n) {
+                const uint32_t v = *reinterpret_cast<const uint32_t *>(hcin + j0);
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    hin[c] = (v >> (8 * c)) & 1u;
+                    nin[c] = (v >> (8 * c + 1)) & 1u;
+                }
+            }
+        }
         if (!GD || (work && j0 >= 0 && j0 < (int32_t)n)) {
             // Match masks: all-ones where the column's code bit is 0 (block_step's M1/M2).
             const uint32_t nc = ~codes;
@@ -310,6 +341,15 @@ __device__ __forceinline__ void myers_pipe_body(const MyersArgs &a, uint32_t blk
                         if (r < R - 1) block_step<true>(K[r], M1[c], M2[c], xp[c], xn[c], 63u);
                         else block_step<false>(K[r], M1[c], M2[c], xp[c], xn[c], bitR);
                     }
+                }
+            }
+            if constexpr (ST) {
+                // The deltas out of the stripe's last block, for the stripe below.
+                if (hcout && l == G - 1 && j0 >= 0 && j0 < (int32_t)n) {
+                    uint32_t v = 0;
+#pragma unroll
+                    for (int c = 0; c < CB; ++c) v |= (xp[c] | (xn[c] << 1)) << (8 * c);
+                    *reinterpret_cast<uint32_t *>(hcout + j0) = v; // (the row holds n rounded up to 4)
                 }
             }
             // The score row (lane G - 1's last slot): the pattern's last row.
@@ -348,6 +388,9 @@ __device__ __forceinline__ void myers_pipe_body(const MyersArgs &a, uint32_t blk
         return;
     }
     if (!work || l != G - 1) return;
+    if constexpr (ST) {
+        if (!st_last) return;
+    }
     if (bestj) bestj -= CB * l;
     if (!rev) {
         if (mode == kGlobal) {

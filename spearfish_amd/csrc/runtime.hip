@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -242,6 +243,11 @@ struct sf_batch {
     PairDesc *d_mdesc = nullptr; // pairs in Myers launch order
     uint32_t *d_dist = nullptr, *d_end = nullptr, *d_start = nullptr;
     std::string myers_unsupported;
+    // Patterns over 1,536 blocks (kMyersStriped): two rows of column deltas per such pair, and the
+    // most stripes any of them has.
+    uint8_t *d_hcol = nullptr;
+    uint64_t hcol_stride = 0, hcol_pairs = 0;
+    uint32_t stripes_max = 0;
     // affine
     // Affine launch plans, cached per (mode, bands, shape tuning): one for score-only runs (which
     // may use the 16-bit kernel), one for runs with traceback.
@@ -254,6 +260,7 @@ struct sf_batch {
         uint32_t *d_order = nullptr;
         PairDesc *d_desc = nullptr;  // pairs in launch order
         uint64_t cells = 0;
+        uint64_t wide_stride = 0;    // int32 a scratch row of the wide-band pairs (kAffWide), 0 if none
     };
     AffPlan aplan[4]; // scores only, with traceback, passes with bookkeeping, local bands
     DpOut *d_dp = nullptr;      // per pair: the last sf_batch_affine_dp run
@@ -265,6 +272,8 @@ struct sf_batch {
     uint64_t *d_runs_off = nullptr, *d_out_off = nullptr, *d_tboff = nullptr;
     uint64_t *d_scan = nullptr; // tile sums of the CIGAR offset scan
     uint32_t *d_tb[2] = {nullptr, nullptr}; // traceback scratch, alternating between chunks
+    int32_t *d_wide = nullptr; // rows of the wide-band passes (kAffWide)
+    uint64_t wide_words = 0;
     uint64_t tb_words = 0;
     hipStream_t wstream = nullptr;           // traceback walks, overlapping the next chunk's DP
     // Extra streams on which independent bins run beside the main stream's.
@@ -342,7 +351,7 @@ struct sf_batch {
         dfree(d_dp), dfree(d_best), dfree(d_pinv), dfree(d_col);
         dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
         dfree(d_scan);
-        dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0), dfree(d_queues);
+        dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0), dfree(d_queues), dfree(d_wide), dfree(d_hcol);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (hipStream_t x : aux)
@@ -680,7 +689,18 @@ int run_myers(sf_batch *b, const MyersArgs &a) {
         x.order = b->d_morder + bin.begin;
         x.desc = b->d_mdesc + bin.begin;
         x.count = bin.end - bin.begin;
-        return sf_launch_myers(x, bin.cls, s);
+        if (bin.cls != kMyersStriped) return sf_launch_myers(x, bin.cls, s);
+        // The stripes in order, on one stream: stripe t reads the deltas stripe t - 1 wrote.
+        const uint64_t half = b->hcol_pairs * b->hcol_stride;
+        for (uint32_t t = 0; t < b->stripes_max; ++t) {
+            MyersArgs y = x;
+            y.stripe = t;
+            y.hcol_in = b->d_hcol + (t % 2) * half;
+            y.hcol_out = b->d_hcol + ((t + 1) % 2) * half;
+            y.hcol_stride = b->hcol_stride;
+            if (const hipError_t e = sf_launch_myers_striped(y, s)) return e;
+        }
+        return hipSuccess;
     };
     auto mix = [&](const MixArgs &mx, int group, hipStream_t s) {
         MyersArgs x = a;
@@ -695,6 +715,7 @@ int run_myers(sf_batch *b, const MyersArgs &a) {
 // 64 / LP pairs: about 9 a slot and 15 besides for the 32-bit kernels; 3.5 a slot (two slots a
 // register) and 24 besides for the 16-bit packed ones, 27 on 32 lanes, 30 with the 64-lane scan.
 double aff_row_cost(int cls) {
+    if (cls == kAffWide) return 30.0 * 8 + 60.0; // (a 256-thread workgroup: about 30 a slot, 8 a thread)
     const double k = aff_k(cls);
     if (!aff_packed(cls)) return 9.0 * k + 15.0;
     return 3.5 * k + (aff_lp(cls) == 64 ? 30.0 : (aff_lp(cls) == 32 ? 27.0 : 24.0));
@@ -738,6 +759,7 @@ constexpr int kCands = 7, kC64 = 5;
 using Cands = std::array<int, kCands>;
 Cands pair_candidates(uint32_t m, uint64_t below, uint64_t width, const sf_scoring *sc, bool cigar, int lp8,
                       uint64_t lp16max) {
+    if (width > 64 * 32) return Cands{0, 0, 0, 0, 0, cigar ? 0 : kAffWide, 0}; // wider than any lane shape
     auto pk = [&](int c, int lanes) {
         if (!c) return 0;
         const int p = sf_affine_pk16_class(aff_k(c), sc->mat, sc->mis, sc->open, sc->ext, m, (uint32_t)below, cigar, lanes);
@@ -888,6 +910,13 @@ int build_plan(sf_batch *b, sf_batch::AffPlan &pl, const std::vector<uint64_t> &
         pl.bins.push_back(bin);
         q = e;
     }
+    pl.wide_stride = 0;
+    for (uint64_t i = 0; i < n; ++i)
+        if (kcls[i] == kAffWide) { // below and above as signed (a local pass's below is -dlo)
+            const int64_t w = (int64_t)(int32_t)(uint32_t)ba[i] + (int64_t)(int32_t)(uint32_t)(ba[i] >> 32) + 1;
+            pl.wide_stride = std::max<uint64_t>(pl.wide_stride, (uint64_t)std::max<int64_t>(w, 0) + 1);
+        }
+    pl.wide_stride = (pl.wide_stride + 63) & ~63ull;
     if (!pl.d_ba && ((st = dalloc(&pl.d_ba, n)) || (st = dalloc(&pl.d_order, n)) || (st = dalloc(&pl.d_desc, n))))
         return st;
     HIP_TRY(hipMemcpy(pl.d_ba, ba.data(), n * 8, hipMemcpyHostToDevice));
@@ -898,6 +927,46 @@ int build_plan(sf_batch *b, sf_batch::AffPlan &pl, const std::vector<uint64_t> &
     pl.valid = true;
     pl.cells = cells;
     return SF_OK;
+}
+
+// Wide-band pairs (kAffWide) run as launches of as many pairs as 1 GiB of row scratch holds.
+constexpr uint64_t kWideScratchWords = (1ull << 30) / 4;
+uint64_t wide_pairs_per_launch(const sf_batch::AffPlan &pl) {
+    return pl.wide_stride ? std::max<uint64_t>(1, kWideScratchWords / (4 * pl.wide_stride)) : 0;
+}
+
+// Scratch for a plan's wide-band bins, made before a run queues anything (waits for the batch's
+// work if the scratch grows).
+int ensure_wide_scratch(sf_batch *b, const sf_batch::AffPlan &pl) {
+    if (!pl.wide_stride) return SF_OK;
+    uint64_t count = 0;
+    for (const auto &bin : pl.bins)
+        if (bin.cls == kAffWide) count = std::max<uint64_t>(count, bin.end - bin.begin);
+    const uint64_t words = std::min(count, wide_pairs_per_launch(pl)) * 4 * pl.wide_stride;
+    if (words <= b->wide_words) return SF_OK;
+    int st;
+    if ((st = refuse_in_capture(b, "a run whose wide-band scratch must grow")) || (st = quiesce(b))) return st;
+    dfree(b->d_wide);
+    b->wide_words = 0;
+    if ((st = dalloc(&b->d_wide, words))) return st;
+    b->wide_words = words;
+    return SF_OK;
+}
+
+// A wide-band bin's launches, one after another on stream s (they share the scratch).
+hipError_t launch_wide(sf_batch *b, const sf_batch::AffPlan &pl, const sf_batch::Bin &bin, WideArgs x,
+                       hipStream_t s) {
+    const uint64_t per = wide_pairs_per_launch(pl);
+    for (uint64_t q = bin.begin; q < bin.end; q += per) {
+        WideArgs y = x;
+        y.p.order = pl.d_order + q;
+        y.p.desc = pl.d_desc + q;
+        y.p.count = (uint32_t)std::min<uint64_t>(per, bin.end - q);
+        y.rows = b->d_wide;
+        y.stride = pl.wide_stride;
+        if (const hipError_t e = sf_launch_affine_wide(y, s)) return e;
+    }
+    return hipSuccess;
 }
 
 } // namespace
@@ -1040,23 +1109,36 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
     for (uint64_t i = 0; i < n; ++i)
         mkey[i] = ((uint64_t)std::min<uint32_t>((b->plen[i] + 63) / 64, 1u << 17) << 30) | b->tlen[i];
     std::vector<uint32_t> order = radix_order(mkey);
+    // A pattern past the largest class runs striped (kMyersStriped), up to kMyersStripedMax blocks.
+    auto myers_class = [&](uint32_t m) {
+        const uint32_t nb = (m + 63) / 64;
+        const int c = sf_myers_maxb_for(nb);
+        return (c == 0 && nb <= kMyersStripedMax) ? kMyersStriped : c;
+    };
     for (uint64_t q = 0; q < n;) {
-        int cls = sf_myers_maxb_for((b->plen[order[q]] + 63) / 64);
+        int cls = myers_class(b->plen[order[q]]);
         uint64_t e = q;
-        while (e < n && sf_myers_maxb_for((b->plen[order[e]] + 63) / 64) == cls) ++e;
+        while (e < n && myers_class(b->plen[order[e]]) == cls) ++e;
         if (cls == 0 && b->myers_unsupported.empty()) {
             char msg[160];
-            snprintf(msg, sizeof msg, "pattern of %u bases: Myers supports up to 98304", b->plen[order[q]]);
+            snprintf(msg, sizeof msg, "pattern of %u bases: Myers supports up to %llu", b->plen[order[q]],
+                     (unsigned long long)kMyersStripedMax * 64);
             b->myers_unsupported = msg;
         }
         sf_batch::Bin bin{cls, (uint32_t)q, (uint32_t)e};
         // Per wave: lane classes take 64 pairs of `cls` blocks a column; pipelined ones take
-        // 64 / G pairs, R blocks a lane a column, G - 1 columns of skew.
+        // 64 / G pairs, R blocks a lane a column, G - 1 columns of skew; a striped pattern runs
+        // its stripes (64 x 24 blocks) one after another.
         auto estimate = [&](sf_batch::Bin &x) {
-            const double G = x.cls >= 16000 ? x.cls / 1000 : 1, R = x.cls >= 16000 ? x.cls % 1000 : x.cls;
+            const bool st = x.cls == kMyersStriped;
+            const double G = st ? 64 : (x.cls >= 16000 ? x.cls / 1000 : 1),
+                         R = st ? 24 : (x.cls >= 16000 ? x.cls % 1000 : x.cls);
             x.chain = x.spread = 0;
             for (uint64_t y = x.begin; y < x.end; ++y) {
-                const double cols = (double)b->tlen[order[y]] + (G - 1);
+                const double stripes = st ? (double)((b->plen[order[y]] + 63) / 64 + kMyersStripedBlocks - 1) /
+                                                kMyersStripedBlocks
+                                          : 1.0;
+                const double cols = ((double)b->tlen[order[y]] + (G - 1)) * (st ? std::floor(stripes) : 1.0);
                 x.chain = std::max(x.chain, cols * R);
                 x.spread += cols * R * G / 64.0 / 1024.0;
             }
@@ -1077,6 +1159,16 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
         q = e;
     }
     for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];
+    // Striped patterns: two rows of a byte per text column (the stripes' deltas, ping-pong), each
+    // row rounded to whole words.
+    for (uint64_t i = 0; i < n; ++i)
+        if (myers_class(b->plen[i]) == kMyersStriped) {
+            ++b->hcol_pairs;
+            b->hcol_stride = std::max<uint64_t>(b->hcol_stride, ((uint64_t)b->tlen[i] + 8) & ~3ull);
+            b->stripes_max = std::max<uint32_t>(
+                b->stripes_max, ((b->plen[i] + 63) / 64 + kMyersStripedBlocks - 1) / kMyersStripedBlocks);
+        }
+    if (b->hcol_pairs && (st = dalloc(&b->d_hcol, 2 * b->hcol_pairs * b->hcol_stride))) return bail(st);
     hipError_t e1 = hipMemcpy(b->d_seq, seq, seq_words * 4, hipMemcpyHostToDevice);
     // The kernels address bases: word offsets * 16.
     std::vector<uint64_t> pbase(n), tbase(n);
@@ -1205,20 +1297,25 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
             const uint32_t m = b->plen[i], t = b->tlen[i];
             sf::band_geom(m, t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
             const uint64_t width = below + above + 1;
-            if (sf_affine_class(width, false) == 0)
-                return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this library handles up to 2048",
+            if (sf_affine_class(width, false) == 0 && (cigar || width > kAffWideMax))
+                return fail(SF_ERANGE,
+                            cigar ? "pair %llu: band of %llu cells per row; traceback handles up to 2048"
+                                  : "pair %llu: band of %llu cells per row; this library handles up to 2^26",
                             (unsigned long long)i, (unsigned long long)width);
             ba[i] = below | (above << 32);
             cells += sf::band_cells(m, t, below, above); // DP_WORK (align.rs:2573-2578)
         }
         choose_shapes(b, sc, ba, cigar, kcls);
-        for (uint64_t i = 0; i < n; ++i)
-            // Values stay far above NEG over all slots of the class.
-            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, b->plen[i], b->tlen[i], aff_slots(kcls[i])))
+        for (uint64_t i = 0; i < n; ++i) {
+            // Values stay far above NEG over all slots of the class (of the band, for the wide kernel).
+            const uint64_t slots = kcls[i] == kAffWide ? (uint32_t)ba[i] + (ba[i] >> 32) + 1 : aff_slots(kcls[i]);
+            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, b->plen[i], b->tlen[i], slots))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
+        }
         if ((st = build_plan(b, pl, ba, kcls, key, cells, !cigar))) return st;
     }
     if (!b->d_score && ((st = dalloc(&b->d_score, n)) || (st = dalloc(&b->d_col, n)))) return st;
+    if ((st = ensure_wide_scratch(b, pl))) return st;
     AffineArgs a{};
     a.pv = view(b);
     a.mode = mode;
@@ -1229,6 +1326,15 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     if (!cigar) {
         if ((st = timed_begin(b, 2, &e0))) return st;
         auto one = [&](const sf_batch::Bin &bin, hipStream_t s) {
+            if (bin.cls == kAffWide) {
+                WideArgs y{};
+                y.p.pv = a.pv;
+                y.p.mode = mode;
+                y.p.mat = a.mat, y.p.mis = a.mis, y.p.open = a.open, y.p.ext = a.ext;
+                y.score = a.score;
+                y.score_col = a.score_col;
+                return launch_wide(b, pl, bin, y, s);
+            }
             AffineArgs x = a;
             x.order = pl.d_order + bin.begin;
             x.desc = pl.d_desc + bin.begin;
@@ -1690,10 +1796,12 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
             sf::band_geom_for(m, t, geom ? geom[i] : t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
             const uint64_t width = below + above + 1;
             kcls[i] = sf_affine_pass_class(width, width <= sf_affine_lp16_max());
+            if (kcls[i] == 0 && width <= kAffWideMax) kcls[i] = kAffWide;
             if (kcls[i] == 0)
-                return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this library handles up to 2048",
+                return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this library handles up to 2^26",
                             (unsigned long long)i, (unsigned long long)width);
-            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t, aff_slots(kcls[i])))
+            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t,
+                                     kcls[i] == kAffWide ? width : aff_slots(kcls[i])))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
             ba[i] = below | (above << 32);
             cells += sf::band_cells(m, t, below, above);
@@ -1701,6 +1809,7 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
         if ((st = build_plan(b, pl, ba, kcls, key, cells))) return st;
     }
     if (!b->d_dp && (st = dalloc(&b->d_dp, n))) return st;
+    if ((st = ensure_wide_scratch(b, pl))) return st;
     uint64_t *d_cut = nullptr;
     if (cut) {
         if ((st = dalloc(&d_cut, n))) return st;
@@ -1721,6 +1830,11 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
     hipEvent_t e0;
     if ((st = timed_begin(b, 2, &e0))) return st;
     st = run_bins(b, pl.bins, [&](const sf_batch::Bin &bin, hipStream_t s) {
+        if (bin.cls == kAffWide) {
+            WideArgs y{};
+            y.p = a;
+            return launch_wide(b, pl, bin, y, s);
+        }
         PassArgs x = a;
         x.order = pl.d_order + bin.begin;
         x.desc = pl.d_desc + bin.begin;
@@ -1783,10 +1897,12 @@ int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, c
                 return fail(SF_ERANGE, "pair %llu: diagonals beyond +-2^30", (unsigned long long)i);
             const uint64_t width = (uint64_t)(hi - lo + 1);
             kcls[i] = sf_affine_pass_class(width, width <= sf_affine_lp16_max());
+            if (kcls[i] == 0 && width <= kAffWideMax) kcls[i] = kAffWide;
             if (kcls[i] == 0)
-                return fail(SF_ERANGE, "pair %llu: band of %llu diagonals; this library handles up to 2048",
+                return fail(SF_ERANGE, "pair %llu: band of %llu diagonals; this library handles up to 2^26",
                             (unsigned long long)i, (unsigned long long)width);
-            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t, aff_slots(kcls[i])))
+            if (!sf::scores_in_range(sc->mat, sc->mis, sc->open, sc->ext, m, t,
+                                     kcls[i] == kAffWide ? width : aff_slots(kcls[i])))
                 return fail(SF_ERANGE, "pair %llu: scores could reach NEG", (unsigned long long)i);
             const uint32_t below = (uint32_t)(int32_t)(-lo), above = (uint32_t)(int32_t)(dhi[i] < lo ? lo - 1 : dhi[i]);
             ba[i] = (uint64_t)below | ((uint64_t)above << 32);
@@ -1799,6 +1915,7 @@ int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, c
         if ((st = build_plan(b, pl, ba, kcls, key, cells))) return st;
     }
     if (!b->d_best && (st = dalloc(&b->d_best, n))) return st;
+    if ((st = ensure_wide_scratch(b, pl))) return st;
     PassArgs a{};
     a.pv = view(b);
     a.mode = SF_PREFIX;
@@ -1810,6 +1927,12 @@ int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, c
     hipEvent_t e0;
     if ((st = timed_begin(b, 2, &e0))) return st;
     if ((st = run_bins(b, pl.bins, [&](const sf_batch::Bin &bin, hipStream_t s) {
+             if (bin.cls == kAffWide) {
+                 WideArgs y{};
+                 y.p = a;
+                 y.local = 1;
+                 return launch_wide(b, pl, bin, y, s);
+             }
              PassArgs x = a;
              x.order = pl.d_order + bin.begin;
              x.desc = pl.d_desc + bin.begin;
