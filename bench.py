@@ -95,6 +95,8 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-extra", action="store_true", help="the uniform workload only")
     ap.add_argument("--tune-out", default=None, help="write the autotuner's measurements here (uneven)")
+    ap.add_argument("--no-schedules", action="store_true",
+                    help="uneven: skip the schedule comparison and the autotuner (for profiled runs)")
     return ap.parse_args()
 
 
@@ -348,7 +350,8 @@ def bench_uneven(args, steps=None, warmup=None):
     out["affine_gcups"] = round(tm.affine_cells / (tm.affine_ms * 1e-3) / 1e9, 1)
     out["kernel_ms_per_step"] = {"myers": round(tm.myers_ms / steps, 2), "affine": round(tm.affine_ms / steps, 2)}
     out["affine_launches_per_step"] = tm.affine_launches // steps
-    out["schedules"] = uneven_schedules(b, bands, steps, args.tune_out)
+    if not args.no_schedules:
+        out["schedules"] = uneven_schedules(b, bands, steps, args.tune_out)
     if not args.no_cpu:
         dist, _, _ = b.myers_results(0)
         res = b.affine_results()
