@@ -239,6 +239,55 @@ int sf_affine_extend_cut_bonus(const sf_pairs *pairs, const sf_scoring *sc, cons
 int sf_local_band_ends(const sf_pairs *pairs, const sf_scoring *sc, const int64_t *dlo, const int64_t *dhi,
                        int32_t bonus_start, int32_t bonus_end, int32_t *best);
 
+/* ---- the tie-break's segment alignments over a device-resident transcript store ---- */
+
+/* The index's packed transcript sequence (crates/index/src/lib.rs:45-151): `bases` bases in u64
+ * words, base i at bits 2*(i % 32) of words[i / 32], copied to the device once.  Candidate texts
+ * are ranges of it. */
+typedef struct sf_store sf_store;
+int sf_store_create(const uint64_t *words, uint64_t bases, sf_store **out);
+void sf_store_destroy(sf_store *s);
+/* crates/index/src/lib.rs:135-137 `PackedRange::sequence_eq` for ranges of equal length (lib.rs:104-114
+ * `packed_eq`, 32 bases a compare), on the device: eq[i] = bases [a_off[i], +len[i]) equal bases
+ * [b_off[i], +len[i]). */
+int sf_store_codes_eq(sf_store *s, const uint64_t *a_off, const uint64_t *b_off, const uint32_t *len, uint64_t n,
+                      uint8_t *eq);
+
+/* One read segment of crates/map/src/lib.rs:1307-1492 (`seg`): read bases [read_off, +read_len)
+ * of `reads` (one code a byte) against candidates cand0 .. cand0 + ncand - 1. */
+#define SF_SEG_REVERSE 1          /* `rev`: a left read end, read and texts aligned reversed */
+#define SF_SEG_EXTEND 2           /* Prefix with `clip_ends`: align.rs:903-949 (else align.rs:854-861) */
+#define SF_SEG_DIAGONAL 4         /* `end_diagonal` */
+#define SF_SEG_TRACK_CUT 8        /* `flank_min > 0` */
+#define SF_SEG_BONUS_WITHIN_CUT 16 /* `end_bonus_cut` */
+typedef struct {
+    uint64_t read_off;
+    uint32_t read_len;
+    uint32_t cand0, ncand;
+    int32_t mode;    /* SF_INFIX / SF_PREFIX / SF_GLOBAL (SF_PREFIX with SF_SEG_EXTEND) */
+    uint32_t flags;  /* SF_SEG_* */
+    uint64_t window; /* `tb.window(read length)`; the band is laid out for the larger of it and the
+                        segment's longest text (lib.rs:1446-1451) */
+    uint64_t cut;    /* `cut`, UINT64_MAX for None */
+} sf_tb_segment;
+typedef struct {
+    sf_scoring sc;     /* `tb.affine` */
+    uint64_t band;     /* `tb.band` */
+    int32_t end_bonus; /* `tb.end_bonus` */
+    int32_t xdrop;     /* `tb.xdrop` */
+} sf_tb_params;
+/* Per candidate c, the scores the reference's `seg` gives it with `tb.affine` set and
+ * `affine_within` unset (a caller with `affine_within` runs sf_myers_banded first and passes the
+ * texts within reach): out[c] = (whole, cut, text bases) of align.rs:903-949 for an SF_SEG_EXTEND
+ * segment, else (x, x, 0) with x of align.rs:854-861 (lib.rs:1442-1445).  Candidate c's text is
+ * store bases [cand_off[c], +cand_len[c]).  Equal texts within a segment are found on the device
+ * (packed equality) and aligned once; rep (nullable) gets each candidate's first equal candidate,
+ * counted from its segment's cand0 (lib.rs:1337-1345 `rep0`).  Every candidate must belong to
+ * exactly one segment. */
+int sf_tiebreak_segments(sf_store *st, const uint8_t *reads, uint64_t reads_len, const sf_tb_segment *segs,
+                         uint64_t nseg, const uint64_t *cand_off, const uint32_t *cand_len, uint64_t ncand,
+                         const sf_tb_params *p, sf_extend_cut *out, uint32_t *rep);
+
 /* ---- serving: continuous batching over a device-resident transcript store ----
  * No reference counterpart as an object: it serves the calls the tie-break makes per read
  * (crates/map/src/lib.rs:1307-1492, `seg`), many reads at a time.  Each request is a read and

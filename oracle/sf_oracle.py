@@ -327,3 +327,75 @@ OPS = {0: "M", 1: "I", 2: "D", 7: "=", 8: "X"}
 
 def cigar_str(cig) -> str:
     return "".join(f"{c >> 4}{OPS[c & 15]}" for c in cig)
+
+
+# ---- the index's packed store and the tie-break's segment call ----
+
+def pack_u64(codes) -> np.ndarray:
+    """crates/index/src/lib.rs packing: base i at bits 2 (i % 32) of u64 word i / 32."""
+    c = np.asarray(bytearray(_b(codes)), np.uint64) & 3
+    w = np.zeros((len(c) + 31) // 32, np.uint64)
+    for k in range(32):
+        part = c[k::32]
+        w[: len(part)] |= part << np.uint64(2 * k)
+    return w
+
+
+def decode_u64(words, lo, hi) -> bytes:
+    """crates/index/src/lib.rs:64-90 `decode_into`: bases [lo, hi) as one code a byte."""
+    return bytes(int(words[i // 32]) >> (2 * (i % 32)) & 3 for i in range(lo, hi))
+
+
+def word_at(words, i) -> int:
+    """crates/index/src/lib.rs:92-100 `word_at`: 32 bases from base i, words past the end read as 0."""
+    q, s = i // 32, 2 * (i % 32)
+    get = lambda k: int(words[k]) if k < len(words) else 0
+    lo = get(q) >> s
+    return lo if s == 0 else (lo | (get(q + 1) << (64 - s))) & (2**64 - 1)
+
+
+def packed_eq(a, alo, b, blo, length) -> bool:
+    """crates/index/src/lib.rs:104-114 `packed_eq`."""
+    k = 0
+    while k + 32 <= length:
+        if word_at(a, alo + k) != word_at(b, blo + k):
+            return False
+        k += 32
+    r = length - k
+    return r == 0 or ((word_at(a, alo + k) ^ word_at(b, blo + k)) & ((1 << (2 * r)) - 1)) == 0
+
+
+SEG_REVERSE, SEG_EXTEND, SEG_DIAGONAL, SEG_TRACK_CUT, SEG_BONUS_WITHIN_CUT = 1, 2, 4, 8, 16
+
+
+def tiebreak_segment(words, read, cands, mode, flags, window, cut, sc, band, end_bonus, xdrop):
+    """crates/map/src/lib.rs:1307-1492 `seg` with `tb.affine` set and `affine_within` unset, as
+    per-candidate results (the reference's deltas are their negations): cands = [(store offset,
+    length)]; returns ([(whole, cut, text bases)], rep) where rep[c] is the first candidate with an
+    equal text (lib.rs:1337-1345, by `sequence_eq`, lib.rs:135-137).  Distinct texts are decoded,
+    reversed with SEG_REVERSE together with the read (lib.rs:1347-1365), and scored one at a time:
+    align.rs:903-949 for SEG_EXTEND with the window the larger of `window` and the longest text
+    (lib.rs:1446-1463), else align.rs:854-861 as (x, x, 0) (lib.rs:1464-1469).  Scoring sorted
+    texts together equals scoring them one at a time (align.rs:2210-2213)."""
+    rev = bool(flags & SEG_REVERSE)
+    rep, distinct = [], []
+    for c, (off, n) in enumerate(cands):
+        r = next((d for d in distinct if cands[d][1] == n and packed_eq(words, cands[d][0], words, off, n)), c)
+        if r == c:
+            distinct.append(c)
+        rep.append(r)
+    r_ = _b(read)[::-1] if rev else _b(read)
+    texts = {d: (decode_u64(words, cands[d][0], cands[d][0] + cands[d][1])[::-1] if rev
+                 else decode_u64(words, cands[d][0], cands[d][0] + cands[d][1])) for d in distinct}
+    res = {}
+    if flags & SEG_EXTEND:
+        w = max([window] + [len(t) for t in texts.values()])
+        for d, t in texts.items():
+            res[d] = affine_extend_cut_bonus(r_, t, sc, band, end_bonus, xdrop, bool(flags & SEG_DIAGONAL), w,
+                                             None if cut == UMAX else cut, bool(flags & SEG_TRACK_CUT),
+                                             bool(flags & SEG_BONUS_WITHIN_CUT))
+    else:
+        for d, t in texts.items():
+            x = affine(r_, t, mode, sc, band)
+            res[d] = (x, x, 0)
+    return [tuple(res[r]) for r in rep], rep

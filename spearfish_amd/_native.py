@@ -128,6 +128,27 @@ class ServerStats(C.Structure):
     ]
 
 
+class TbSegment(C.Structure):
+    """sf_tb_segment: one read segment of the tie-break and its candidates."""
+    _fields_ = [
+        ("read_off", C.c_uint64),
+        ("read_len", C.c_uint32),
+        ("cand0", C.c_uint32),
+        ("ncand", C.c_uint32),
+        ("mode", C.c_int32),
+        ("flags", C.c_uint32),
+        ("window", C.c_uint64),
+        ("cut", C.c_uint64),
+    ]
+
+
+class TbParams(C.Structure):
+    _fields_ = [("sc", Scoring), ("band", C.c_uint64), ("end_bonus", C.c_int32), ("xdrop", C.c_int32)]
+
+
+SEG_REVERSE, SEG_EXTEND, SEG_DIAGONAL, SEG_TRACK_CUT, SEG_BONUS_WITHIN_CUT = 1, 2, 4, 8, 16
+
+
 # Every symbol include/sf_align.h declares, with its ctypes signature.
 _vp, _u32, _u64, _i32, _int = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_int
 SIGNATURES = {
@@ -177,6 +198,10 @@ SIGNATURES = {
     "sf_affine_extend_window": ([_vp, _vp, _vp, _vp, C.c_int32, C.c_int32, _vp], _int),
     "sf_affine_extend_cut_bonus": ([_vp, _vp, _vp, C.c_int32, C.c_int32, _int, _vp, _vp, _int, _int, _vp], _int),
     "sf_local_band_ends": ([_vp, _vp, _vp, _vp, C.c_int32, C.c_int32, _vp], _int),
+    "sf_store_create": ([_vp, _u64, C.POINTER(_vp)], _int),
+    "sf_store_destroy": ([_vp], None),
+    "sf_store_codes_eq": ([_vp, _vp, _vp, _vp, _u64, _vp], _int),
+    "sf_tiebreak_segments": ([_vp, _vp, _u64, _vp, _u64, _vp, _vp, _u64, C.POINTER(TbParams), _vp, _vp], _int),
     "sf_set_affine_lp16_max": ([C.c_uint64], _int),
     "sf_set_affine_lp8": ([C.c_int], _int),
 }

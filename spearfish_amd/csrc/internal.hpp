@@ -418,6 +418,13 @@ hipError_t sf_launch_myers_mix_persistent(const sf::MyersArgs &a, const sf::MixA
 hipError_t sf_launch_affine_mix_persistent(const sf::AffineArgs &a, const sf::MixArgs &mx, hipStream_t s);
 hipError_t sf_launch_affine_mix1_persistent(const sf::AffineArgs &a, const sf::MixArgs &mx, hipStream_t s);
 
+// A batch (sf_batch, include/sf_align.h) over sequences already on the device in the 2-bit word
+// layout: pair i's pattern at base pbase[i] of `seq`, its text at tbase[i].  The batch does not own
+// `seq`: it must stay allocated and unchanged until the batch is destroyed.  Lengths are under 2^30.
+struct sf_batch;
+int sf_batch_create_resident(const uint32_t *seq, uint64_t n, const uint32_t *plen, const uint32_t *tlen,
+                             const uint64_t *pbase, const uint64_t *tbase, sf_batch **out);
+
 // Supported shape classes.
 int sf_myers_maxb_for(uint32_t nblocks);     // 0 if unsupported
 // Patterns over 1,536 blocks (98,304 bases): the 64-lane, 24-block pipeline run stripe by stripe

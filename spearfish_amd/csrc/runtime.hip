@@ -221,6 +221,7 @@ struct sf_batch {
     uint64_t n = 0;
     std::vector<uint32_t> plen, tlen;
     uint32_t *d_seq = nullptr;
+    bool seq_borrowed = false; // d_seq belongs to the caller (sf_batch_create_resident)
     uint64_t *d_poff = nullptr, *d_toff = nullptr;
     uint32_t *d_plen = nullptr, *d_tlen = nullptr;
     // The texts as made, while the working texts may be restricted to located windows.
@@ -344,6 +345,7 @@ struct sf_batch {
         if (stream) (void)hipStreamSynchronize(stream);
         collect();
         for (auto e : pool) (void)hipEventDestroy(e);
+        if (seq_borrowed) d_seq = nullptr;
         dfree(d_seq), dfree(d_poff), dfree(d_toff), dfree(d_plen), dfree(d_tlen), dfree(d_morder);
         dfree(d_dist), dfree(d_end), dfree(d_start), dfree(aplan[0].d_ba), dfree(aplan[0].d_order), dfree(aplan[1].d_ba), dfree(aplan[1].d_order), dfree(d_score);
         dfree(aplan[0].d_desc), dfree(aplan[1].d_desc), dfree(d_mdesc);
@@ -969,6 +971,102 @@ hipError_t launch_wide(sf_batch *b, const sf_batch::AffPlan &pl, const sf_batch:
     return hipSuccess;
 }
 
+// The rest of a batch's set-up once b->n, plen and tlen are set and its sequences are on the
+// device (d_seq): pair i's pattern at base pbase[i], its text at tbase[i].  Streams, per-pair
+// arrays, Myers bins and launch records.  On failure the caller deletes b.
+int batch_init(sf_batch *b, const std::vector<uint64_t> &pbase, const std::vector<uint64_t> &tbase) {
+    const uint64_t n = b->n;
+    int st;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(SF_EDEVICE, "hipStreamCreate failed");
+    if ((st = dalloc(&b->d_poff, n)) || (st = dalloc(&b->d_toff, n)) ||
+        (st = dalloc(&b->d_plen, n)) || (st = dalloc(&b->d_tlen, n)) || (st = dalloc(&b->d_dist, n)) ||
+        (st = dalloc(&b->d_end, n)) || (st = dalloc(&b->d_start, n)) || (st = dalloc(&b->d_morder, n)) ||
+        (st = dalloc(&b->d_mdesc, n)) ||
+        (st = dalloc(&b->d_toff0, n)) || (st = dalloc(&b->d_tlen0, n)))
+        return st;
+    // Myers bins: by blocks per pattern (the lane kernel's register class), then text length.
+    // Key: blocks (clamped to 2^17: patterns past 98,304 bases have no class anyway) over 30 bits
+    // of text length, 47 bits.
+    std::vector<uint64_t> mkey(n);
+    for (uint64_t i = 0; i < n; ++i)
+        mkey[i] = ((uint64_t)std::min<uint32_t>((b->plen[i] + 63) / 64, 1u << 17) << 30) | b->tlen[i];
+    std::vector<uint32_t> order = radix_order(mkey);
+    // A pattern past the largest class runs striped (kMyersStriped), up to kMyersStripedMax blocks.
+    auto myers_class = [&](uint32_t m) {
+        const uint32_t nb = (m + 63) / 64;
+        const int c = sf_myers_maxb_for(nb);
+        return (c == 0 && nb <= kMyersStripedMax) ? kMyersStriped : c;
+    };
+    for (uint64_t q = 0; q < n;) {
+        int cls = myers_class(b->plen[order[q]]);
+        uint64_t e = q;
+        while (e < n && myers_class(b->plen[order[e]]) == cls) ++e;
+        if (cls == 0 && b->myers_unsupported.empty()) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "pattern of %u bases: Myers supports up to %llu", b->plen[order[q]],
+                     (unsigned long long)kMyersStripedMax * 64);
+            b->myers_unsupported = msg;
+        }
+        sf_batch::Bin bin{cls, (uint32_t)q, (uint32_t)e};
+        // Per wave: lane classes take 64 pairs of `cls` blocks a column; pipelined ones take
+        // 64 / G pairs, R blocks a lane a column, G - 1 columns of skew; a striped pattern runs
+        // its stripes (64 x 24 blocks) one after another.
+        auto estimate = [&](sf_batch::Bin &x) {
+            const bool st = x.cls == kMyersStriped;
+            const double G = st ? 64 : (x.cls >= 16000 ? x.cls / 1000 : 1),
+                         R = st ? 24 : (x.cls >= 16000 ? x.cls % 1000 : x.cls);
+            x.chain = x.spread = 0;
+            for (uint64_t y = x.begin; y < x.end; ++y) {
+                const double stripes = st ? (double)((b->plen[order[y]] + 63) / 64 + kMyersStripedBlocks - 1) /
+                                                kMyersStripedBlocks
+                                          : 1.0;
+                const double cols = ((double)b->tlen[order[y]] + (G - 1)) * (st ? std::floor(stripes) : 1.0);
+                x.chain = std::max(x.chain, cols * R);
+                x.spread += cols * R * G / 64.0 / 1024.0;
+            }
+        };
+        estimate(bin);
+        // A 32-lane bin that cannot fill the chip is bound by its chain: 64 lanes halve it.  It
+        // then joins a neighbouring bin of that class (one launch, not two in a row).
+        if (bin.chain > bin.spread && sf_myers_wide_class(cls) != cls) {
+            bin.cls = sf_myers_wide_class(cls);
+            estimate(bin);
+        }
+        if (!b->mbins.empty() && b->mbins.back().cls == bin.cls && b->mbins.back().end == bin.begin) {
+            b->mbins.back().end = bin.end;
+            estimate(b->mbins.back());
+        } else {
+            b->mbins.push_back(bin);
+        }
+        q = e;
+    }
+    for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];
+    // Striped patterns: two rows of a byte per text column (the stripes' deltas, ping-pong), each
+    // row rounded to whole words.
+    for (uint64_t i = 0; i < n; ++i)
+        if (myers_class(b->plen[i]) == kMyersStriped) {
+            ++b->hcol_pairs;
+            b->hcol_stride = std::max<uint64_t>(b->hcol_stride, ((uint64_t)b->tlen[i] + 8) & ~3ull);
+            b->stripes_max = std::max<uint32_t>(
+                b->stripes_max, ((b->plen[i] + 63) / 64 + kMyersStripedBlocks - 1) / kMyersStripedBlocks);
+        }
+    if (b->hcol_pairs && (st = dalloc(&b->d_hcol, 2 * b->hcol_pairs * b->hcol_stride))) return st;
+    hipError_t e2 = hipMemcpy(b->d_poff, pbase.data(), n * 8, hipMemcpyHostToDevice);
+    hipError_t e3 = hipMemcpy(b->d_toff, tbase.data(), n * 8, hipMemcpyHostToDevice);
+    hipError_t e4 = hipMemcpy(b->d_plen, b->plen.data(), n * 4, hipMemcpyHostToDevice);
+    hipError_t e5 = hipMemcpy(b->d_tlen, b->tlen.data(), n * 4, hipMemcpyHostToDevice);
+    hipError_t e6 = hipMemcpy(b->d_morder, order.data(), n * 4, hipMemcpyHostToDevice);
+    hipError_t e7 = hipMemcpy(b->d_toff0, tbase.data(), n * 8, hipMemcpyHostToDevice);
+    hipError_t e8 = hipMemcpy(b->d_tlen0, b->tlen.data(), n * 4, hipMemcpyHostToDevice);
+    if (e2 || e3 || e4 || e5 || e6 || e7 || e8) return fail(SF_EDEVICE, "upload failed");
+    if (sf_launch_desc(n, b->d_morder, view(b), nullptr, b->d_mdesc, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return fail(SF_EDEVICE, "launch records failed");
+    b->tlen0 = b->tlen;
+    return SF_OK;
+}
+
 } // namespace
 
 int sf_persistent_policy() {
@@ -1094,100 +1192,16 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
         delete b;
         return s;
     };
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(SF_EDEVICE, "hipStreamCreate failed"));
-    if ((st = dalloc(&b->d_seq, seq_words)) || (st = dalloc(&b->d_poff, n)) || (st = dalloc(&b->d_toff, n)) ||
-        (st = dalloc(&b->d_plen, n)) || (st = dalloc(&b->d_tlen, n)) || (st = dalloc(&b->d_dist, n)) ||
-        (st = dalloc(&b->d_end, n)) || (st = dalloc(&b->d_start, n)) || (st = dalloc(&b->d_morder, n)) ||
-        (st = dalloc(&b->d_mdesc, n)) ||
-        (st = dalloc(&b->d_toff0, n)) || (st = dalloc(&b->d_tlen0, n)))
-        return bail(st);
-    // Myers bins: by blocks per pattern (the lane kernel's register class), then text length.
-    // Key: blocks (clamped to 2^17: patterns past 98,304 bases have no class anyway) over 30 bits
-    // of text length, 47 bits.
-    std::vector<uint64_t> mkey(n);
-    for (uint64_t i = 0; i < n; ++i)
-        mkey[i] = ((uint64_t)std::min<uint32_t>((b->plen[i] + 63) / 64, 1u << 17) << 30) | b->tlen[i];
-    std::vector<uint32_t> order = radix_order(mkey);
-    // A pattern past the largest class runs striped (kMyersStriped), up to kMyersStripedMax blocks.
-    auto myers_class = [&](uint32_t m) {
-        const uint32_t nb = (m + 63) / 64;
-        const int c = sf_myers_maxb_for(nb);
-        return (c == 0 && nb <= kMyersStripedMax) ? kMyersStriped : c;
-    };
-    for (uint64_t q = 0; q < n;) {
-        int cls = myers_class(b->plen[order[q]]);
-        uint64_t e = q;
-        while (e < n && myers_class(b->plen[order[e]]) == cls) ++e;
-        if (cls == 0 && b->myers_unsupported.empty()) {
-            char msg[160];
-            snprintf(msg, sizeof msg, "pattern of %u bases: Myers supports up to %llu", b->plen[order[q]],
-                     (unsigned long long)kMyersStripedMax * 64);
-            b->myers_unsupported = msg;
-        }
-        sf_batch::Bin bin{cls, (uint32_t)q, (uint32_t)e};
-        // Per wave: lane classes take 64 pairs of `cls` blocks a column; pipelined ones take
-        // 64 / G pairs, R blocks a lane a column, G - 1 columns of skew; a striped pattern runs
-        // its stripes (64 x 24 blocks) one after another.
-        auto estimate = [&](sf_batch::Bin &x) {
-            const bool st = x.cls == kMyersStriped;
-            const double G = st ? 64 : (x.cls >= 16000 ? x.cls / 1000 : 1),
-                         R = st ? 24 : (x.cls >= 16000 ? x.cls % 1000 : x.cls);
-            x.chain = x.spread = 0;
-            for (uint64_t y = x.begin; y < x.end; ++y) {
-                const double stripes = st ? (double)((b->plen[order[y]] + 63) / 64 + kMyersStripedBlocks - 1) /
-                                                kMyersStripedBlocks
-                                          : 1.0;
-                const double cols = ((double)b->tlen[order[y]] + (G - 1)) * (st ? std::floor(stripes) : 1.0);
-                x.chain = std::max(x.chain, cols * R);
-                x.spread += cols * R * G / 64.0 / 1024.0;
-            }
-        };
-        estimate(bin);
-        // A 32-lane bin that cannot fill the chip is bound by its chain: 64 lanes halve it.  It
-        // then joins a neighbouring bin of that class (one launch, not two in a row).
-        if (bin.chain > bin.spread && sf_myers_wide_class(cls) != cls) {
-            bin.cls = sf_myers_wide_class(cls);
-            estimate(bin);
-        }
-        if (!b->mbins.empty() && b->mbins.back().cls == bin.cls && b->mbins.back().end == bin.begin) {
-            b->mbins.back().end = bin.end;
-            estimate(b->mbins.back());
-        } else {
-            b->mbins.push_back(bin);
-        }
-        q = e;
-    }
-    for (uint64_t i = 0; i < n; ++i) b->myers_cells_per_run += (uint64_t)b->plen[i] * b->tlen[i];
-    // Striped patterns: two rows of a byte per text column (the stripes' deltas, ping-pong), each
-    // row rounded to whole words.
-    for (uint64_t i = 0; i < n; ++i)
-        if (myers_class(b->plen[i]) == kMyersStriped) {
-            ++b->hcol_pairs;
-            b->hcol_stride = std::max<uint64_t>(b->hcol_stride, ((uint64_t)b->tlen[i] + 8) & ~3ull);
-            b->stripes_max = std::max<uint32_t>(
-                b->stripes_max, ((b->plen[i] + 63) / 64 + kMyersStripedBlocks - 1) / kMyersStripedBlocks);
-        }
-    if (b->hcol_pairs && (st = dalloc(&b->d_hcol, 2 * b->hcol_pairs * b->hcol_stride))) return bail(st);
-    hipError_t e1 = hipMemcpy(b->d_seq, seq, seq_words * 4, hipMemcpyHostToDevice);
+    if ((st = dalloc(&b->d_seq, seq_words))) return bail(st);
+    if (hipMemcpy(b->d_seq, seq, seq_words * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(SF_EDEVICE, "upload failed"));
     // The kernels address bases: word offsets * 16.
     std::vector<uint64_t> pbase(n), tbase(n);
     for (uint64_t i = 0; i < n; ++i) {
         pbase[i] = poff[i] * 16;
         tbase[i] = toff[i] * 16;
     }
-    hipError_t e2 = hipMemcpy(b->d_poff, pbase.data(), n * 8, hipMemcpyHostToDevice);
-    hipError_t e3 = hipMemcpy(b->d_toff, tbase.data(), n * 8, hipMemcpyHostToDevice);
-    hipError_t e4 = hipMemcpy(b->d_plen, b->plen.data(), n * 4, hipMemcpyHostToDevice);
-    hipError_t e5 = hipMemcpy(b->d_tlen, b->tlen.data(), n * 4, hipMemcpyHostToDevice);
-    hipError_t e6 = hipMemcpy(b->d_morder, order.data(), n * 4, hipMemcpyHostToDevice);
-    hipError_t e7 = hipMemcpy(b->d_toff0, tbase.data(), n * 8, hipMemcpyHostToDevice);
-    hipError_t e8 = hipMemcpy(b->d_tlen0, b->tlen.data(), n * 4, hipMemcpyHostToDevice);
-    if (e1 || e2 || e3 || e4 || e5 || e6 || e7 || e8) return bail(fail(SF_EDEVICE, "upload failed"));
-    if (sf_launch_desc(n, b->d_morder, view(b), nullptr, b->d_mdesc, b->stream) != hipSuccess ||
-        hipStreamSynchronize(b->stream) != hipSuccess)
-        return bail(fail(SF_EDEVICE, "launch records failed"));
-    b->tlen0 = b->tlen;
+    if ((st = batch_init(b, pbase, tbase))) return bail(st);
     // Pattern codes of 4 or more (unpacked input only) match nothing in local passes
     // (align.rs:731-734); the other functions mask codes with & 3 as the reference does.
     if (!pairs->packed) {
@@ -1228,6 +1242,7 @@ int sf_batch_create(const sf_pairs *pairs, sf_batch **out) {
     *out = b;
     return SF_OK;
 }
+
 
 int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
     if (!b) return fail(SF_EINVAL, "batch is null");
@@ -2182,3 +2197,22 @@ int sf_affine_cigar(const sf_pairs *pairs, int mode, const sf_scoring *sc, const
 }
 
 } // extern "C"
+
+// A batch over sequences already on the device (declared in internal.hpp).
+int sf_batch_create_resident(const uint32_t *seq, uint64_t n, const uint32_t *plen, const uint32_t *tlen,
+                             const uint64_t *pbase, const uint64_t *tbase, sf_batch **out) {
+    *out = nullptr;
+    if (int st = ensure_device()) return st;
+    auto *b = new sf_batch();
+    b->n = n;
+    b->plen.assign(plen, plen + n);
+    b->tlen.assign(tlen, tlen + n);
+    b->d_seq = const_cast<uint32_t *>(seq);
+    b->seq_borrowed = true;
+    if (int st = batch_init(b, std::vector<uint64_t>(pbase, pbase + n), std::vector<uint64_t>(tbase, tbase + n))) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return SF_OK;
+}
