@@ -13,11 +13,11 @@ value = DP cells of a step (sum of m*n for Myers plus the affine band cells, cou
 reference's DP_WORK does) / wall time per step, in GCUPS.
 
     python bench.py [--steps K] [--warmup W] [--pairs N] [--no-cpu] [--no-extra]
-    python bench.py --workload uneven|long|serve      # one of the other workloads alone
+    python bench.py --workload uneven|long|serve|tiebreak   # one of the other workloads alone
 
 Prints one JSON line.  By default the line also carries the other workloads under
 "other_workloads" (about a minute more): the uneven batch with its CPU baseline and the ratio to
-it, the long-segment batch, and the serving loop.
+it, the long-segment batch, the serving loop, and the tie-break's call mix.
 
 Other workloads (same JSON shape, their own config):
   uneven  5e5 pairs, targets LogNormal(median 800 bp, sigma 0.8) clipped to 100..20,000, 5-15%
@@ -27,6 +27,9 @@ Other workloads (same JSON shape, their own config):
   serve   20,000-transcript synthetic transcriptome, reads of median 1.2 kb at 8% error with 1-16
           isoform segments, Poisson arrivals at 5k, 20k and 50k reads/s; reads/s and p50/p99
           latency with continuous batching next to one read per launch.
+  tiebreak  the tie-break's calls (read ends and interior stretches against each isoform's text)
+          of 20,000 reads of the serving transcriptome, all in one call over the device-resident
+          packed store; candidate alignments/s, next to one call per segment and the CPU port.
 """
 from __future__ import annotations
 
@@ -91,7 +94,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pairs", type=int, default=0, help="CPU baseline sample size (0: auto)")
     ap.add_argument("--per-pair", type=int, default=256, help="pairs for the per-pair-launch comparison")
-    ap.add_argument("--workload", default="uniform", choices=["uniform", "uneven", "long", "serve", "probe"])
+    ap.add_argument("--workload", default="uniform", choices=["uniform", "uneven", "long", "serve", "tiebreak", "probe"])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-extra", action="store_true", help="the uniform workload only")
     ap.add_argument("--tune-out", default=None, help="write the autotuner's measurements here (uneven)")
@@ -550,6 +553,98 @@ def bench_serve(args):
     return out
 
 
+def bench_tiebreak(args, steps=None, warmup=None):
+    """The tie-break's call mix: every `seg` call (crates/map/src/lib.rs:1307-1492) of 20,000 reads
+    of the serving workload's transcriptome -- left ends, interior stretches and right ends against
+    each isoform's text, windows and cuts as `segmented` makes them -- in one sf_tiebreak_segments
+    call a step over the device-resident packed store (ONT scoring, the reference's tie-break
+    defaults: band 16, X-drop 50, end bonus 5, bonus within the cut)."""
+    from spearfish_amd import serve as S
+    from spearfish_amd import tiebreak as T
+    steps = steps or args.steps
+    warmup = args.warmup if warmup is None else warmup
+    n_reads = 20000
+    store, reads = S.generate_serve_workload(n_reads, seed=args.seed)
+    tb = T.TieBreak()
+    segs, coff, clen = T.calls_from_isoform_segments(store.tx_off, store.tx_len, reads, tb)
+    words = store.words if len(store.words) % 2 == 0 else np.append(store.words, np.uint32(0))
+    words64 = words.view(np.uint64)  # the index's u64 words: the same bytes as 16-base u32 words
+    ps = T.PackedStore(words64, store.bases)
+    log(f"tiebreak: {n_reads} reads, {len(segs)} calls, {len(coff)} candidates")
+    # The calls' inputs made resident once (reads and candidates uploaded, distinct texts found,
+    # passes planned); a step runs every call's passes and reads the results back.
+    t0 = time.perf_counter()
+    batch = T.TbBatch(ps, reads.codes, segs, coff, clen)
+    setup_s = time.perf_counter() - t0
+    for w in range(warmup):
+        batch.run(tb)
+        batch.results()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run(tb)
+        out, rep = batch.results()
+    wall = time.perf_counter() - t0
+    batch.close()
+    distinct = int((rep == np.arange(len(rep)) - np.repeat(segs["cand0"], segs["ncand"])).sum())
+    log(f"tiebreak: timed {steps} steps in {wall:.3f}s")
+    rate = len(coff) * steps / wall
+    # The one-shot call (host buffers in and out: uploads, dedup and planning every time).
+    T.segments_arrays(ps, reads.codes, segs, coff, clen, tb)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one, _ = T.segments_arrays(ps, reads.codes, segs, coff, clen, tb)
+    wall1 = time.perf_counter() - t0
+    assert np.array_equal(one, out)
+    res = common(args, round(rate, 1), "candidate alignments/s", round(wall / steps * 1e3, 2), "int32",
+                 {"workload": "tiebreak-20k-reads-serve-transcriptome", "reads": n_reads, "calls": len(segs),
+                  "candidates": len(coff), "distinct_texts": distinct,
+                  "per_call": "seg: dedup by packed equality, left ends reversed, Prefix ends "
+                              "affine_extend_sorted_cut_bonus, interior affine Global; ONT, band 16, xdrop 50, "
+                              "end bonus 5"})
+    res["steps"], res["warmup"] = steps, warmup
+    res["calls_per_s"] = round(len(segs) * steps / wall, 1)
+    res["reads_per_s"] = round(n_reads * steps / wall, 1)
+    res["setup_ms"] = round(setup_s * 1e3, 2)
+    res["one_shot"] = {"ms_per_call": round(wall1 / steps * 1e3, 2),
+                       "candidate_alignments_per_s": round(len(coff) * steps / wall1, 1),
+                       "note": "sf_tiebreak_segments from host buffers: reads and candidates uploaded, "
+                               "distinct texts found and passes planned in every call"}
+    # The same calls one `seg` call per library call, as the reference makes them.
+    k1 = min(len(segs), 400)
+    t0 = time.perf_counter()
+    for i in range(k1):
+        a, n = int(segs["cand0"][i]), int(segs["ncand"][i])
+        one = segs[i:i + 1].copy()
+        one["cand0"] = 0
+        T.segments_arrays(ps, reads.codes, one, coff[a:a + n], clen[a:a + n], tb)
+    w1 = time.perf_counter() - t0
+    res["one_call_per_seg"] = {"calls": k1, "calls_per_s": round(k1 / w1, 1),
+                               "candidate_alignments_per_s": round(int(segs["ncand"][:k1].sum()) / w1, 1)}
+    if not args.no_cpu:
+        from oracle import sf_oracle as O
+        threads = host_threads()
+        simd = O.avx512_available()
+        kr = 4000  # reads in the sample: about 10 s of host work
+        ks = int(np.searchsorted(segs["read_off"], reads.read_off[kr])) if kr < n_reads else len(segs)
+        kc = int(segs["cand0"][ks]) if ks < len(segs) else len(coff)
+        sc = (tb.affine.mat, tb.affine.mis, tb.affine.open, tb.affine.ext)
+        t0 = time.perf_counter()
+        cpu, crep = O.tiebreak_batch(words64, store.bases, reads.codes, segs[:ks], coff[:kc], clen[:kc], sc, tb.band,
+                                     tb.end_bonus, tb.xdrop, threads=threads, avx512=simd)
+        cs = time.perf_counter() - t0
+        exact = bool(np.array_equal(cpu[:, 0], out["whole"][:kc]) and np.array_equal(cpu[:, 1], out["cut"][:kc])
+                     and np.array_equal(cpu[:, 2], out["cols"][:kc]) and np.array_equal(crep, rep[:kc]))
+        res["cpu_baseline"] = {"value": round(kc / cs, 1), "unit": "candidate alignments/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"the calls of the first {kr} reads ({ks} calls, {kc} candidates), {cs:.1f}s: "
+                                         f"C restatement of seg with "
+                                         + ("the x86 rows (32-bit AVX-512 rows for the X-drop passes, 16-bit "
+                                            "where narrow holds for the others)" if simd else "scalar rows"),
+                               "sample_bit_exact_vs_gpu": exact}
+        res["gpu_vs_cpu"] = round(res["value"] / res["cpu_baseline"]["value"], 1)
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -562,7 +657,8 @@ def main():
                           "nominal_one_wave_instruction_per_4_cycles": round(VALU_ISSUE_PEAK_T, 1)}))
         return
     if args.workload != "uniform":
-        print(json.dumps({"uneven": bench_uneven, "long": bench_long, "serve": bench_serve}[args.workload](args)))
+        print(json.dumps({"uneven": bench_uneven, "long": bench_long, "serve": bench_serve,
+                          "tiebreak": bench_tiebreak}[args.workload](args)))
         return
     out = bench_uniform(args)
     if not args.no_extra:
@@ -571,7 +667,8 @@ def main():
         extra = {}
         for name, fn in (("uneven", lambda: bench_uneven(args, steps=5, warmup=1)),
                          ("long", lambda: bench_long(args, steps=2, warmup=1)),
-                         ("serve", lambda: bench_serve(args))):
+                         ("serve", lambda: bench_serve(args)),
+                         ("tiebreak", lambda: bench_tiebreak(args, steps=5, warmup=1))):
             t0 = time.perf_counter()
             try:
                 r = fn()

@@ -287,6 +287,18 @@ typedef struct {
 int sf_tiebreak_segments(sf_store *st, const uint8_t *reads, uint64_t reads_len, const sf_tb_segment *segs,
                          uint64_t nseg, const uint64_t *cand_off, const uint32_t *cand_len, uint64_t ncand,
                          const sf_tb_params *p, sf_extend_cut *out, uint32_t *rep);
+/* sf_tiebreak_segments in steps, for callers that keep a call's inputs on the device: create
+ * uploads the reads and candidates, finds the distinct texts and plans the passes (it copies the
+ * store's words into the batch, which may outlive the store); run queues the passes with `p` and
+ * returns; results waits and fills out (and rep, nullable) as sf_tiebreak_segments does.  Runs may
+ * be repeated (with other parameters too); results reads the last. */
+typedef struct sf_tb_batch sf_tb_batch;
+int sf_tb_batch_create(sf_store *st, const uint8_t *reads, uint64_t reads_len, const sf_tb_segment *segs,
+                       uint64_t nseg, const uint64_t *cand_off, const uint32_t *cand_len, uint64_t ncand,
+                       sf_tb_batch **out);
+int sf_tb_batch_run(sf_tb_batch *b, const sf_tb_params *p);
+int sf_tb_batch_results(sf_tb_batch *b, sf_extend_cut *out, uint32_t *rep);
+void sf_tb_batch_destroy(sf_tb_batch *b);
 
 /* ---- serving: continuous batching over a device-resident transcript store ----
  * No reference counterpart as an object: it serves the calls the tie-break makes per read

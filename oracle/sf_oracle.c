@@ -868,13 +868,12 @@ void sfo_affine_extend_window(const uint8_t *pat, uint64_t m, const uint8_t *txt
 }
 
 /* align.rs:903-949, one text. */
-void sfo_affine_extend_cut_bonus(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_t n, const sfo_scoring *sc,
-                                 uint64_t band, int32_t end_bonus, int32_t xdrop, int diagonal, uint64_t window,
-                                 uint64_t cut, int track_cut, int bonus_within_cut, int64_t out3[3]) {
+static void extend_cut_bonus(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_t n, const sfo_scoring *sc,
+                             uint64_t band, int32_t end_bonus, int32_t xdrop, int diagonal, uint64_t window,
+                             uint64_t cut, int track_cut, int bonus_within_cut, int64_t out3[3], row_fn row) {
     uint64_t geom = diagonal ? (m < n ? m : n) : (n > window ? n : window);
     sfo_dp d;
-    run_core(pat, m, txt, n, SFO_PREFIX, sc, band, geom, 1, xdrop, track_cut && cut != SFO_UMAX, cut, &d, NULL,
-             row_scalar);
+    run_core(pat, m, txt, n, SFO_PREFIX, sc, band, geom, 1, xdrop, track_cut && cut != SFO_UMAX, cut, &d, NULL, row);
     int32_t end;
     if (bonus_within_cut && cut != SFO_UMAX) end = max32(d.score, sat_add_i(d.cut_score, end_bonus));
     else end = sat_add_i(d.score, end_bonus);
@@ -884,6 +883,13 @@ void sfo_affine_extend_cut_bonus(const uint8_t *pat, uint64_t m, const uint8_t *
     out3[0] = whole;
     out3[1] = cutv;
     out3[2] = (int64_t)cols;
+}
+
+void sfo_affine_extend_cut_bonus(const uint8_t *pat, uint64_t m, const uint8_t *txt, uint64_t n, const sfo_scoring *sc,
+                                 uint64_t band, int32_t end_bonus, int32_t xdrop, int diagonal, uint64_t window,
+                                 uint64_t cut, int track_cut, int bonus_within_cut, int64_t out3[3]) {
+    extend_cut_bonus(pat, m, txt, n, sc, band, end_bonus, xdrop, diagonal, window, cut, track_cut, bonus_within_cut,
+                     out3, row_scalar);
 }
 
 /* align.rs:783-852 `local_band_core::<ScalarRows>`. */
@@ -1366,4 +1372,147 @@ int sfo_batch_affine_avx512(const uint8_t *pat, const uint64_t *poff, const uint
 #else
     return -1;
 #endif
+}
+
+/* ---- the tie-break's segment calls (crates/map/src/lib.rs:1307-1492 `seg`), many at once ---- */
+
+/* crates/index/src/lib.rs:92-100 `word_at` on u64 words (base i at bits 2 (i % 32) of word i / 32). */
+static uint64_t word_at64(const uint64_t *w, uint64_t nwords, uint64_t i) {
+    uint64_t q = i >> 5;
+    unsigned s = 2 * (unsigned)(i & 31);
+    uint64_t lo = q < nwords ? w[q] >> s : 0;
+    if (s == 0) return lo;
+    return lo | (q + 1 < nwords ? w[q + 1] << (64 - s) : 0);
+}
+
+/* crates/index/src/lib.rs:104-114 `packed_eq`. */
+int sfo_packed_eq(const uint64_t *w, uint64_t nwords, uint64_t a, uint64_t b, uint32_t len) {
+    uint32_t k = 0;
+    for (; k + 32 <= len; k += 32)
+        if (word_at64(w, nwords, a + k) != word_at64(w, nwords, b + k)) return 0;
+    uint32_t r = len - k;
+    return r == 0 || ((word_at64(w, nwords, a + k) ^ word_at64(w, nwords, b + k)) & ((1ull << (2 * r)) - 1)) == 0;
+}
+
+typedef struct {
+    const uint64_t *words;
+    uint64_t nwords;
+    const uint8_t *reads;
+    const sfo_tb_segment *segs;
+    uint64_t nseg;
+    const uint64_t *cand_off;
+    const uint32_t *cand_len;
+    const sfo_scoring *sc;
+    uint64_t band;
+    int32_t end_bonus, xdrop;
+    int64_t *out;
+    uint32_t *rep;
+    row_fn row;
+    int narrow;
+    atomic_uint_fast64_t next;
+} tb_job_t;
+
+static void tb_segment(tb_job_t *jb, const sfo_tb_segment *g, uint8_t **buf, uint64_t *cap) {
+    const int rev = (g->flags & SFO_SEG_REVERSE) != 0;
+    /* lib.rs:1337-1345: each candidate's first equal candidate (packed comparison). */
+    uint64_t longest = 0, need = g->read_len;
+    for (uint32_t c = g->cand0; c < g->cand0 + g->ncand; c++) {
+        uint32_t r = c;
+        for (uint32_t d = g->cand0; d < c; d++)
+            if (jb->rep[d] == d - g->cand0 && jb->cand_len[d] == jb->cand_len[c] &&
+                sfo_packed_eq(jb->words, jb->nwords, jb->cand_off[d], jb->cand_off[c], jb->cand_len[c])) {
+                r = d;
+                break;
+            }
+        jb->rep[c] = r - g->cand0;
+        if (jb->cand_len[c] > longest) longest = jb->cand_len[c];
+        if (r == c && jb->cand_len[c] > need) need = jb->cand_len[c];
+    }
+    if (2 * need > *cap) {
+        free(*buf);
+        *cap = 2 * need + 64;
+        *buf = (uint8_t *)malloc(*cap);
+    }
+    /* lib.rs:1347-1365: the read slice and each distinct text decoded, reversed for a left end. */
+    uint8_t *rd = *buf, *tx = *buf + need;
+    for (uint32_t k = 0; k < g->read_len; k++)
+        rd[k] = jb->reads[g->read_off + (rev ? g->read_len - 1 - k : k)];
+    const uint64_t window = g->window > longest ? g->window : longest; /* lib.rs:1446-1451 */
+    for (uint32_t c = g->cand0; c < g->cand0 + g->ncand; c++) {
+        int64_t *o = jb->out + 3 * (uint64_t)c;
+        if (jb->rep[c] != c - g->cand0) {
+            const int64_t *q = jb->out + 3 * (uint64_t)(g->cand0 + jb->rep[c]);
+            o[0] = q[0], o[1] = q[1], o[2] = q[2];
+            continue;
+        }
+        const uint64_t n = jb->cand_len[c], off = jb->cand_off[c];
+        for (uint64_t k = 0; k < n; k++) {
+            uint64_t b = off + (rev ? n - 1 - k : k);
+            tx[k] = (uint8_t)((jb->words[b >> 5] >> (2 * (b & 31))) & 3);
+        }
+        const uint64_t m = g->read_len;
+        if (g->flags & SFO_SEG_EXTEND) {
+            /* align.rs:903-949 for one text. */
+            extend_cut_bonus(rd, m, tx, n, jb->sc, jb->band, jb->end_bonus, jb->xdrop,
+                             (g->flags & SFO_SEG_DIAGONAL) != 0, window, g->cut, (g->flags & SFO_SEG_TRACK_CUT) != 0,
+                             (g->flags & SFO_SEG_BONUS_WITHIN_CUT) != 0, o, jb->row);
+        } else {
+            /* align.rs:854-861, mapped to (x, x, 0) (lib.rs:1464-1469). */
+            int32_t x;
+#if defined(__x86_64__)
+            if (jb->narrow && narrow16(m, g->mode, jb->sc, jb->band, n)) {
+                uint64_t col;
+                run16_core(rd, m, tx, n, g->mode, jb->sc, jb->band, &x, &col);
+            } else
+#endif
+            {
+                sfo_dp d;
+                run_core(rd, m, tx, n, g->mode, jb->sc, jb->band, n, 0, 0, 0, SFO_UMAX, &d, NULL, jb->row);
+                x = d.score;
+            }
+            o[0] = o[1] = x;
+            o[2] = 0;
+        }
+    }
+}
+
+static void *tb_worker(void *arg) {
+    tb_job_t *jb = (tb_job_t *)arg;
+    uint8_t *buf = NULL;
+    uint64_t cap = 0;
+    for (;;) {
+        uint64_t s0 = atomic_fetch_add(&jb->next, 4);
+        if (s0 >= jb->nseg) break;
+        uint64_t s1 = s0 + 4 < jb->nseg ? s0 + 4 : jb->nseg;
+        for (uint64_t s = s0; s < s1; s++) tb_segment(jb, &jb->segs[s], &buf, &cap);
+    }
+    free(buf);
+    return NULL;
+}
+
+int sfo_tiebreak_segments(const uint64_t *words, uint64_t bases, const uint8_t *reads, const sfo_tb_segment *segs,
+                          uint64_t nseg, const uint64_t *cand_off, const uint32_t *cand_len, const sfo_scoring *sc,
+                          uint64_t band, int32_t end_bonus, int32_t xdrop, int64_t *out, uint32_t *rep, int threads,
+                          int avx512) {
+    tb_job_t jb;
+    memset(&jb, 0, sizeof jb);
+    jb.words = words, jb.nwords = (bases + 31) / 32, jb.reads = reads, jb.segs = segs, jb.nseg = nseg;
+    jb.cand_off = cand_off, jb.cand_len = cand_len, jb.sc = sc, jb.band = band, jb.end_bonus = end_bonus;
+    jb.xdrop = xdrop, jb.out = out, jb.rep = rep, jb.row = row_scalar;
+#if defined(__x86_64__)
+    if (avx512) {
+        if (!sfo_avx512_available()) return -1;
+        jb.row = row_avx512;
+        jb.narrow = 1;
+    }
+#else
+    if (avx512) return -1;
+#endif
+    atomic_init(&jb.next, 0);
+    if (threads < 1) threads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+    for (int q = 0; q < threads; q++) pthread_create(&th[q], NULL, tb_worker, &jb);
+    for (int q = 0; q < threads; q++) pthread_join(th[q], NULL);
+    free(th);
+    return 0;
 }

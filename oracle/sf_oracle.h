@@ -157,6 +157,32 @@ int sfo_batch_affine_avx512(const uint8_t *pat, const uint64_t *poff, const uint
                             uint64_t npairs, int mode, const sfo_scoring *sc, const uint64_t *band,
                             int32_t *score, uint64_t *score_col, int threads);
 
+/* crates/index/src/lib.rs:104-114 `packed_eq` on u64 words (base i at bits 2 (i % 32) of word i / 32). */
+int sfo_packed_eq(const uint64_t *words, uint64_t nwords, uint64_t a, uint64_t b, uint32_t len);
+
+/* crates/map/src/lib.rs:1307-1492 `seg` (with `tb.affine` set, `affine_within` unset) over many
+ * segments: segment s aligns reads[read_off, +read_len) against candidates cand0 .. cand0+ncand-1,
+ * each a range of the packed store `words`; per candidate out[3c..3c+2] = (whole, cut, text bases)
+ * and rep[c] = its first equal candidate counted from cand0.  Flags as sf_align.h's SF_SEG_*.
+ * Threads over segments; avx512 runs the reference's x86 rows (32-bit AVX-512 rows for the
+ * extension passes, 16-bit rows where `narrow` holds for the others); -1 if the host lacks them. */
+#define SFO_SEG_REVERSE 1
+#define SFO_SEG_EXTEND 2
+#define SFO_SEG_DIAGONAL 4
+#define SFO_SEG_TRACK_CUT 8
+#define SFO_SEG_BONUS_WITHIN_CUT 16
+typedef struct {
+    uint64_t read_off;
+    uint32_t read_len, cand0, ncand;
+    int32_t mode;
+    uint32_t flags;
+    uint64_t window, cut;
+} sfo_tb_segment;
+int sfo_tiebreak_segments(const uint64_t *words, uint64_t bases, const uint8_t *reads, const sfo_tb_segment *segs,
+                          uint64_t nseg, const uint64_t *cand_off, const uint32_t *cand_len, const sfo_scoring *sc,
+                          uint64_t band, int32_t end_bonus, int32_t xdrop, int64_t *out, uint32_t *rep, int threads,
+                          int avx512);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,10 @@ def declare(L):
         L.sfo_batch_affine_avx512.restype = C.c_int
         L.sfo_avx512_available.argtypes = []
         L.sfo_avx512_available.restype = C.c_int
+        L.sfo_packed_eq.argtypes = [vp, u64, u64, u64, u32]
+        L.sfo_packed_eq.restype = C.c_int
+        L.sfo_tiebreak_segments.argtypes = [vp, u64, vp, vp, u64, vp, vp, sc, u64, i32, i32, vp, vp, C.c_int, C.c_int]
+        L.sfo_tiebreak_segments.restype = C.c_int
     return L
 
 
@@ -399,3 +403,28 @@ def tiebreak_segment(words, read, cands, mode, flags, window, cut, sc, band, end
             x = affine(r_, t, mode, sc, band)
             res[d] = (x, x, 0)
     return [tuple(res[r]) for r in rep], rep
+
+
+# The C form of `tiebreak_segment` over many segments (the CPU baseline of the tie-break call mix).
+TB_SEGMENT = np.dtype([("read_off", np.uint64), ("read_len", np.uint32), ("cand0", np.uint32), ("ncand", np.uint32),
+                       ("mode", np.int32), ("flags", np.uint32), ("window", np.uint64), ("cut", np.uint64)],
+                      align=True)
+
+
+def tiebreak_batch(words, bases, reads, segs, cand_off, cand_len, sc, band, end_bonus, xdrop, threads=1,
+                   avx512=False):
+    """sfo_tiebreak_segments: segs a TB_SEGMENT array; returns (out (n, 3) int64, rep uint32)."""
+    w = np.ascontiguousarray(words, np.uint64)
+    r = np.frombuffer(_b(reads), np.uint8) if len(reads) else np.zeros(1, np.uint8)
+    sg = np.ascontiguousarray(segs, TB_SEGMENT)
+    co = np.ascontiguousarray(cand_off, np.uint64)
+    cl = np.ascontiguousarray(cand_len, np.uint32)
+    n = len(co)
+    out = np.zeros((max(n, 1), 3), np.int64)
+    rep = np.zeros(max(n, 1), np.uint32)
+    st = lib().sfo_tiebreak_segments(w.ctypes.data, bases, r.ctypes.data, sg.ctypes.data, len(sg), co.ctypes.data,
+                                     cl.ctypes.data, C.byref(_sc(sc)), _band(band), end_bonus, xdrop,
+                                     out.ctypes.data, rep.ctypes.data, threads, int(avx512))
+    if st != 0:
+        raise RuntimeError("this host has no AVX-512 F/BW/VL")
+    return out[:n], rep[:n]

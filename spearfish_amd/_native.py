@@ -202,6 +202,10 @@ SIGNATURES = {
     "sf_store_destroy": ([_vp], None),
     "sf_store_codes_eq": ([_vp, _vp, _vp, _vp, _u64, _vp], _int),
     "sf_tiebreak_segments": ([_vp, _vp, _u64, _vp, _u64, _vp, _vp, _u64, C.POINTER(TbParams), _vp, _vp], _int),
+    "sf_tb_batch_create": ([_vp, _vp, _u64, _vp, _u64, _vp, _vp, _u64, C.POINTER(_vp)], _int),
+    "sf_tb_batch_run": ([_vp, C.POINTER(TbParams)], _int),
+    "sf_tb_batch_results": ([_vp, _vp, _vp], _int),
+    "sf_tb_batch_destroy": ([_vp], None),
     "sf_set_affine_lp16_max": ([C.c_uint64], _int),
     "sf_set_affine_lp8": ([C.c_int], _int),
 }

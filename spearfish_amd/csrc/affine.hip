@@ -17,12 +17,14 @@ namespace {
 
 constexpr int kRowRor1 = 0x121, kRowRor2 = 0x122, kRowRor4 = 0x124, kRowRor8 = 0x128;
 
+// (A rotation has no invalid source lane, so bound_ctrl changes nothing; set, it lets the max take
+// the DPP operand directly.)
 template <int LP>
 __device__ __forceinline__ int32_t group_max(int32_t x) {
-    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor8, 0xf, 0xf, false));
-    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor4, 0xf, 0xf, false));
-    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor2, 0xf, 0xf, false));
-    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor1, 0xf, 0xf, false));
+    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor8, 0xf, 0xf, true));
+    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor4, 0xf, 0xf, true));
+    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor2, 0xf, 0xf, true));
+    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor1, 0xf, 0xf, true));
     if constexpr (LP == 64) {
         x = imax(x, __shfl_xor(x, 16));
         x = imax(x, __shfl_xor(x, 32));
@@ -32,10 +34,10 @@ __device__ __forceinline__ int32_t group_max(int32_t x) {
 
 template <int LP>
 __device__ __forceinline__ int32_t group_min(int32_t x) {
-    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor8, 0xf, 0xf, false));
-    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor4, 0xf, 0xf, false));
-    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor2, 0xf, 0xf, false));
-    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor1, 0xf, 0xf, false));
+    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor8, 0xf, 0xf, true));
+    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor4, 0xf, 0xf, true));
+    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor2, 0xf, 0xf, true));
+    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor1, 0xf, 0xf, true));
     if constexpr (LP == 64) {
         x = imin(x, __shfl_xor(x, 16));
         x = imin(x, __shfl_xor(x, 32));
@@ -75,6 +77,10 @@ __global__ __launch_bounds__(256) void affine_pass_kernel(PassArgs a) {
     const int32_t s0 = l * K;
 
     uint32_t V[K], Fb[K], Ub[K], dltv[K];
+    // A slot's value is V - (row bias) - ext * s; slots past the band's width (cells outside it,
+    // computed like band cells) also lose 2^30, which puts them below every band cell (scores stay
+    // far from NEG), so a row's maximum needs no mask there.
+    int32_t cpen[K];
     {
         const int32_t lo0 = below < 0 ? -below : 0; // row 0 covers [max(0, dlo), min(n, dhi)] (local)
         const int32_t hi0 = above < n ? above : n;
@@ -88,6 +94,7 @@ __global__ __launch_bounds__(256) void affine_pass_kernel(PassArgs a) {
             Fb[q] = (uint32_t)kNeg + kOff;
             Ub[q] = V[q] + dlt;
             dltv[q] = s >= Wd ? dlt - (1u << 30) : dlt;
+            cpen[q] = ext * s + (s >= Wd ? (1 << 30) : 0);
         }
     }
     // Rows past n + below have no band cell (the band ran off the text).
@@ -161,14 +168,21 @@ __global__ __launch_bounds__(256) void affine_pass_kernel(PassArgs a) {
                 Fb[q] = FQ[q];
                 Ub[q] = V[q] + dltv[q];
             }
-            // The row's maximum over its band cells (align.rs:2599-2628 `row_max`).
+            // The row's maximum over its band cells (align.rs:2599-2628 `row_max`): masked only in
+            // rows whose band starts left of column 0 or ends past column n.
             int32_t hq[K];
             int32_t lm = INT32_MIN;
 #pragma unroll
-            for (int q = 0; q < K; ++q) {
-                const int32_t s = s0 + q;
-                hq[q] = (int32_t)(V[q] - rowb) - ext * s;
-                if (s >= slo && s <= shi) lm = imax(lm, hq[q]);
+            for (int q = 0; q < K; ++q) hq[q] = (int32_t)(V[q] - rowb) - cpen[q];
+            if (slo > 0 || shi < Wd - 1) {
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const int32_t s = s0 + q;
+                    if (s >= slo && s <= shi) lm = imax(lm, hq[q]);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < K; ++q) lm = imax(lm, hq[q]);
             }
             const int32_t rmax = group_max<LP>(lm);
             if constexpr (LOCAL) {

@@ -116,6 +116,34 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
+// Streams are recycled across batches of this process's device: making one costs about 0.4 ms,
+// and every one-shot call makes a batch.  A stream comes back with no work queued (its batch waited
+// for it); beyond 64 idle ones they are destroyed.
+std::mutex g_stream_mu;
+std::vector<hipStream_t> g_streams;
+hipError_t take_stream(hipStream_t *s) {
+    {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        if (!g_streams.empty()) {
+            *s = g_streams.back();
+            g_streams.pop_back();
+            return hipSuccess;
+        }
+    }
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+void give_stream(hipStream_t s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        if (g_streams.size() < 64) {
+            g_streams.push_back(s);
+            return;
+        }
+    }
+    (void)hipStreamDestroy(s);
+}
+
 // Words of one traceback scratch buffer (two are used, alternately): SF_TB_BUDGET_MB, else a
 // quarter of the device's memory, at most 64 GiB.  Larger chunks run more walks at once, and the
 // walk (one dependent chain per pair) needs that concurrency.
@@ -265,6 +293,8 @@ struct sf_batch {
     };
     AffPlan aplan[4]; // scores only, with traceback, passes with bookkeeping, local bands
     DpOut *d_dp = nullptr;      // per pair: the last sf_batch_affine_dp run
+    uint64_t *d_cut = nullptr;  // the cuts of the last sf_batch_affine_dp run that had them
+    uint64_t cut_key = 0;       // their hash (0: none uploaded)
     int32_t *d_best = nullptr;  // per pair: the last sf_batch_local_band run
     uint32_t *d_pinv = nullptr; // bit (base offset) set where a pattern code is >= 4; null if none
     uint64_t pinv_words = 0;
@@ -350,7 +380,7 @@ struct sf_batch {
         dfree(d_dist), dfree(d_end), dfree(d_start), dfree(aplan[0].d_ba), dfree(aplan[0].d_order), dfree(aplan[1].d_ba), dfree(aplan[1].d_order), dfree(d_score);
         dfree(aplan[0].d_desc), dfree(aplan[1].d_desc), dfree(d_mdesc);
         for (int q = 2; q < 4; ++q) dfree(aplan[q].d_ba), dfree(aplan[q].d_order), dfree(aplan[q].d_desc);
-        dfree(d_dp), dfree(d_best), dfree(d_pinv), dfree(d_col);
+        dfree(d_dp), dfree(d_best), dfree(d_pinv), dfree(d_col), dfree(d_cut);
         dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
         dfree(d_scan);
         dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0), dfree(d_queues), dfree(d_wide), dfree(d_hcol);
@@ -360,10 +390,9 @@ struct sf_batch {
             if (x) (void)hipStreamSynchronize(x);
         for (hipEvent_t e : {dp_done[0], dp_done[1], walk_done[0], walk_done[1], walks_end, fork, joins[0], joins[1]})
             if (e) (void)hipEventDestroy(e);
-        for (hipStream_t x : aux)
-            if (x) (void)hipStreamDestroy(x);
-        if (wstream) (void)hipStreamDestroy(wstream);
-        if (stream) (void)hipStreamDestroy(stream);
+        for (hipStream_t x : aux) give_stream(x);
+        give_stream(wstream);
+        give_stream(stream);
     }
 };
 
@@ -499,7 +528,7 @@ int ensure_aux(sf_batch *b) {
     HIP_TRY(hipMemset(b->d_queues, 0, sf_batch::kQueues * 2 * sizeof(uint32_t)));
     HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     for (int i = 0; i < sf_batch::kAux; ++i) {
-        HIP_TRY(hipStreamCreateWithFlags(&b->aux[i], hipStreamNonBlocking));
+        HIP_TRY(take_stream(&b->aux[i]));
         HIP_TRY(hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming));
     }
     return SF_OK;
@@ -794,6 +823,19 @@ Cands pair_candidates(uint32_t m, uint64_t below, uint64_t width, const sf_scori
     return Cands{c8, c16, c32, c8n, c16n, c64, c64n};
 }
 
+// The planner's costs of the classes one plan considers, each read once (the autotuner's lookups
+// lock; the pair loops run on all host threads).  Class codes are below 65536.
+struct ClassCosts {
+    std::vector<uint16_t> slot = std::vector<uint16_t>(65536, 0xFFFF);
+    std::vector<double> tput, chain;
+    void add(int cls, bool tuned) {
+        if (cls <= 0 || cls >= 65536 || slot[cls] != 0xFFFF) return;
+        slot[cls] = (uint16_t)tput.size();
+        tput.push_back(aff_tput_ns(cls, tuned));
+        chain.push_back(aff_chain_ns(cls, tuned));
+    }
+};
+
 void choose_shapes(const sf_batch *b, const sf_scoring *sc, const std::vector<uint64_t> &ba, bool cigar,
                    std::vector<int> &kcls) {
     const uint64_t n = b->n;
@@ -801,13 +843,24 @@ void choose_shapes(const sf_batch *b, const sf_scoring *sc, const std::vector<ui
     const uint64_t lp16max = sf_affine_lp16_max();
     std::vector<Cands> cand(n);
     const bool tuned = !cigar;
-    auto tput = [&](int c) { return aff_tput_ns(c, tuned); };
-    auto chain = [&](uint64_t i, int c) { return b->plen[i] * aff_chain_ns(c, tuned); };
     parallel_for(n, [&](uint64_t lo, uint64_t hi) {
         for (uint64_t i = lo; i < hi; ++i) {
             const uint32_t m = b->plen[i], below = (uint32_t)ba[i];
             const uint64_t width = below + (ba[i] >> 32) + 1;
             cand[i] = pair_candidates(m, below, width, sc, cigar, lp8, lp16max);
+        }
+    });
+    ClassCosts cc;
+    for (uint64_t i = 0; i < n; ++i)
+        for (int c : cand[i]) cc.add(c, tuned);
+    auto slot = [&](int c) { return (c > 0 && c < 65536) ? cc.slot[c] : (uint16_t)0xFFFF; };
+    auto tput = [&](int c) { const uint16_t q = slot(c); return q != 0xFFFF ? cc.tput[q] : aff_tput_ns(c, tuned); };
+    auto chain = [&](uint64_t i, int c) {
+        const uint16_t q = slot(c);
+        return b->plen[i] * (q != 0xFFFF ? cc.chain[q] : aff_chain_ns(c, tuned));
+    };
+    parallel_for(n, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t i = lo; i < hi; ++i) {
             int pick = 0;
             for (int c : cand[i])
                 if (c && (!pick || (lp8 <= 0 && tput(c) < tput(pick)))) pick = c;
@@ -826,18 +879,22 @@ void choose_shapes(const sf_batch *b, const sf_scoring *sc, const std::vector<ui
             kcls[i] = pick ? pick : cand[i][kC64];
         }
     });
-    // Per class (codes are below 65536): its chain, its spread, and its spread on 64 lanes.
-    std::vector<double> cchain(65536, 0.0), cspread(65536, 0.0), cspread64(65536, 0.0);
+    // Per class: its chain, its spread, and its spread on 64 lanes.
+    const size_t nc = cc.tput.size();
+    std::vector<double> cchain(nc, 0.0), cspread(nc, 0.0), cspread64(nc, 0.0);
     for (uint64_t i = 0; i < n; ++i) {
         const int k = kcls[i];
-        cchain[k] = std::max(cchain[k], chain(i, k));
-        cspread[k] += b->plen[i] * tput(k);
+        const uint16_t q = slot(k);
+        if (q == 0xFFFF) continue;
+        cchain[q] = std::max(cchain[q], chain(i, k));
+        cspread[q] += b->plen[i] * tput(k);
         // (The move costs a class 4 to 8 times its spread: only where that stays small.)
-        if (aff_lp(k) != 64) cspread64[k] += b->plen[i] * tput(cand[i][kC64]);
+        if (aff_lp(k) != 64) cspread64[q] += b->plen[i] * tput(cand[i][kC64]);
     }
     for (uint64_t i = 0; i < n; ++i) {
         const int k = kcls[i];
-        if (aff_lp(k) != 64 && cchain[k] > cspread[k] && cspread64[k] <= 0.05 * spread) kcls[i] = cand[i][kC64];
+        const uint16_t q = slot(k);
+        if (q != 0xFFFF && aff_lp(k) != 64 && cchain[q] > cspread[q] && cspread64[q] <= 0.05 * spread) kcls[i] = cand[i][kC64];
     }
 }
 
@@ -977,8 +1034,7 @@ hipError_t launch_wide(sf_batch *b, const sf_batch::AffPlan &pl, const sf_batch:
 int batch_init(sf_batch *b, const std::vector<uint64_t> &pbase, const std::vector<uint64_t> &tbase) {
     const uint64_t n = b->n;
     int st;
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
-        return fail(SF_EDEVICE, "hipStreamCreate failed");
+    if (take_stream(&b->stream) != hipSuccess) return fail(SF_EDEVICE, "hipStreamCreate failed");
     if ((st = dalloc(&b->d_poff, n)) || (st = dalloc(&b->d_toff, n)) ||
         (st = dalloc(&b->d_plen, n)) || (st = dalloc(&b->d_tlen, n)) || (st = dalloc(&b->d_dist, n)) ||
         (st = dalloc(&b->d_end, n)) || (st = dalloc(&b->d_start, n)) || (st = dalloc(&b->d_morder, n)) ||
@@ -1464,7 +1520,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         if ((st = dalloc(&b->d_tb[1], b->tb_words))) return st;
     }
     if (!b->wstream) {
-        HIP_TRY(hipStreamCreateWithFlags(&b->wstream, hipStreamNonBlocking));
+        HIP_TRY(take_stream(&b->wstream));
         for (hipEvent_t *e : {&b->dp_done[0], &b->dp_done[1], &b->walk_done[0], &b->walk_done[1], &b->walks_end})
             HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
@@ -1796,6 +1852,8 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
     const uint64_t n = b->n;
     sf_batch::AffPlan &pl = b->aplan[2];
     uint64_t key = (0x51ed270b27f1a3c5ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8)) * 1099511628211ull;
+    // (The scoring is in the key: the plan's range check depends on it.)
+    for (int32_t v : {sc->mat, sc->mis, sc->open, sc->ext}) key = (key ^ (uint64_t)(uint32_t)v) * 1099511628211ull;
     for (uint64_t i = 0; i < n; ++i) {
         key = (key ^ (band ? band[i] : SF_BAND_EXACT)) * 1099511628211ull;
         key = (key ^ (geom ? geom[i] : 0x9e3779b97f4a7c15ull)) * 1099511628211ull;
@@ -1825,14 +1883,21 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
     }
     if (!b->d_dp && (st = dalloc(&b->d_dp, n))) return st;
     if ((st = ensure_wide_scratch(b, pl))) return st;
-    uint64_t *d_cut = nullptr;
+    // The cuts stay on the device for the next run with the same ones (an upload waits for the
+    // batch's work, which may still read the previous ones).
     if (cut) {
-        if ((st = dalloc(&d_cut, n))) return st;
-        if (hipMemcpy(d_cut, cut, n * 8, hipMemcpyHostToDevice) != hipSuccess) {
-            dfree(d_cut);
-            return fail(SF_EDEVICE, "upload failed");
+        uint64_t ck = 0x243f6a8885a308d3ull;
+        for (uint64_t i = 0; i < n; ++i) ck = (ck ^ cut[i]) * 1099511628211ull;
+        ck |= 1;
+        if (ck != b->cut_key) {
+            if ((st = quiesce(b))) return st;
+            b->cut_key = 0;
+            if (!b->d_cut && (st = dalloc(&b->d_cut, n))) return st;
+            HIP_TRY(hipMemcpy(b->d_cut, cut, n * 8, hipMemcpyHostToDevice));
+            b->cut_key = ck;
         }
     }
+    uint64_t *d_cut = cut ? b->d_cut : nullptr;
     PassArgs a{};
     a.pv = view(b);
     a.mode = mode;
@@ -1857,10 +1922,6 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
         return sf_launch_pass(x, bin.cls, false, s);
     });
     if (!st) st = timed_end(b, 2, e0);
-    if (d_cut) {
-        (void)hipStreamSynchronize(b->stream);
-        dfree(d_cut);
-    }
     if (st) return st;
     b->affine_cells += pl.cells;
     b->affine_launches += pl.bins.size();

@@ -244,16 +244,47 @@ int sf_store_codes_eq(sf_store *s, const uint64_t *a_off, const uint64_t *b_off,
     return e ? dev("sf_store_codes_eq", e) : SF_OK;
 }
 
-int sf_tiebreak_segments(sf_store *st, const uint8_t *reads, uint64_t reads_len, const sf_tb_segment *segs,
-                         uint64_t nseg, const uint64_t *cand_off, const uint32_t *cand_len, uint64_t ncand,
-                         const sf_tb_params *p, sf_extend_cut *out, uint32_t *rep_out) {
-    if (!st || !p || (nseg && !segs) || (ncand && (!cand_off || !cand_len || !out)) || (reads_len && !reads))
-        return sf_runtime_fail(SF_EINVAL, "null argument");
+} // extern "C"
+
+// A tie-break call prepared on the device: the reads packed into a sequence buffer beside the store
+// (and its reversed copy), each segment's distinct texts found, and the distinct (segment, text)
+// pairs as resident batches over that buffer, one for the extensions and one per mode for the
+// sorted scores.
+struct TbGroup {
+    std::vector<uint64_t> cand, pbase, tbase, geom, cut;
+    std::vector<uint32_t> plen, tlen, seg;
+    sf_batch *b = nullptr;
+};
+
+struct sf_tb_batch {
+    sf_store *st = nullptr;
+    uint32_t *own_seq = nullptr; // [store | reversed store | reads] of an sf_tb_batch (null: the store's)
+    std::vector<sf_tb_segment> segs;
+    std::vector<uint32_t> rep, first;
+    uint64_t ncand = 0;
+    TbGroup ext, srt[3];
+    sf_tb_params p{};
+    bool ran = false;
+    ~sf_tb_batch() {
+        for (TbGroup *g : {&ext, &srt[0], &srt[1], &srt[2]})
+            if (g->b) sf_batch_destroy(g->b);
+        if (own_seq) (void)hipFree(own_seq);
+    }
+};
+
+namespace {
+
+// Checks a call's arguments (every candidate in exactly one segment, every range inside its buffer)
+// and gives each candidate's segment and its segment's first candidate, and each segment's longest
+// text.
+int tb_check(const sf_store *st, uint64_t reads_len, const sf_tb_segment *segs, uint64_t nseg,
+             const uint64_t *cand_off, const uint32_t *cand_len, uint64_t ncand, std::vector<uint32_t> &seg_of,
+             std::vector<uint32_t> &first, std::vector<uint64_t> &longest) {
     if (ncand >= UINT32_MAX) return sf_runtime_fail(SF_ERANGE, "2^32 - 1 candidates or more");
     if (reads_len >= (1ull << 40)) return sf_runtime_fail(SF_ERANGE, "2^40 read bases or more");
-    // Each candidate's segment, checked; the longest text of each segment.
-    std::vector<uint32_t> seg_of(ncand, UINT32_MAX), first(ncand, 0);
-    std::vector<uint64_t> longest(nseg, 0);
+    seg_of.assign(ncand, UINT32_MAX);
+    first.assign(ncand, 0);
+    longest.assign(nseg, 0);
     for (uint64_t s = 0; s < nseg; ++s) {
         const sf_tb_segment &g = segs[s];
         if (g.read_off > reads_len || g.read_len > reads_len - g.read_off)
@@ -277,18 +308,30 @@ int sf_tiebreak_segments(sf_store *st, const uint8_t *reads, uint64_t reads_len,
         if (cand_len[c] >= (1u << 30)) return sf_runtime_fail(SF_ERANGE, "a text of 2^30 bases or more");
         longest[seg_of[c]] = std::max<uint64_t>(longest[seg_of[c]], cand_len[c]);
     }
+    return SF_OK;
+}
+
+// Prepares w over `seq` (the store's buffer or a copy of its two store parts, with room for
+// 2 ceil(reads_len / 16) words of reads after them), the store's mutex held: the reads packed
+// there forward and reversed, the candidates deduplicated per segment (with the store's scratch),
+// the distinct pairs grouped and their batches made.
+int tb_prepare(sf_store *st, uint32_t *seq, const uint8_t *reads, uint64_t reads_len, const sf_tb_segment *segs,
+               uint64_t nseg, const uint64_t *cand_off, const uint32_t *cand_len, uint64_t ncand, sf_tb_batch &w) {
+    std::vector<uint32_t> seg_of;
+    std::vector<uint64_t> longest;
+    if (const int q = tb_check(st, reads_len, segs, nseg, cand_off, cand_len, ncand, seg_of, w.first, longest))
+        return q;
+    w.segs.assign(segs, segs + nseg);
+    w.ncand = ncand;
+    w.rep.assign(ncand, 0);
     if (ncand == 0) return SF_OK;
-    std::lock_guard<std::mutex> lk(st->mu);
-    // Reads to the device, packed there forward and reversed; candidates deduplicated there.
     const uint64_t rwords = (reads_len + 15) / 16;
-    if (const int q = st->reserve_reads(2 * rwords + 1)) return q;
     if (!reserve(st->d_bytes, st->bytes_cap, reads_len) || !reserve(st->d_off, st->off_cap, ncand) ||
         !reserve(st->d_len, st->len_cap, ncand) || !reserve(st->d_first, st->first_cap, ncand) ||
         !reserve(st->d_rep, st->rep_cap, ncand))
         return sf_runtime_fail(SF_ENOMEM, "tie-break scratch");
-    uint32_t *d_rfwd = st->d_seq + 4 * st->nwords, *d_rrev = d_rfwd + rwords;
+    uint32_t *d_rfwd = seq + 4 * st->nwords, *d_rrev = d_rfwd + rwords;
     const uint64_t rfwd_base = st->reads_base(), rrev_base = rfwd_base + 16 * rwords;
-    std::vector<uint32_t> rep(ncand);
     {
         hipError_t e = reads_len ? hipMemcpyAsync(st->d_bytes, reads, reads_len, hipMemcpyHostToDevice, st->stream)
                                  : hipSuccess;
@@ -299,97 +342,171 @@ int sf_tiebreak_segments(sf_store *st, const uint8_t *reads, uint64_t reads_len,
         }
         if (!e) e = hipMemcpyAsync(st->d_off, cand_off, ncand * 8, hipMemcpyHostToDevice, st->stream);
         if (!e) e = hipMemcpyAsync(st->d_len, cand_len, ncand * 4, hipMemcpyHostToDevice, st->stream);
-        if (!e) e = hipMemcpyAsync(st->d_first, first.data(), ncand * 4, hipMemcpyHostToDevice, st->stream);
+        if (!e) e = hipMemcpyAsync(st->d_first, w.first.data(), ncand * 4, hipMemcpyHostToDevice, st->stream);
         if (!e) {
             hipLaunchKernelGGL(dedup_kernel, dim3(grid(ncand)), dim3(256), 0, st->stream, st->words64(), st->nwords,
                                st->d_off, st->d_len, st->d_first, ncand, st->d_rep);
             e = hipGetLastError();
         }
-        if (!e) e = hipMemcpyAsync(rep.data(), st->d_rep, ncand * 4, hipMemcpyDeviceToHost, st->stream);
+        if (!e) e = hipMemcpyAsync(w.rep.data(), st->d_rep, ncand * 4, hipMemcpyDeviceToHost, st->stream);
         if (!e) e = hipStreamSynchronize(st->stream);
         if (e) return dev("tie-break dedup", e);
     }
-    // The distinct (segment, text) pairs as ranges of the store's buffer: the read slice and the
-    // text, both reversed for a left read end (lib.rs:1347-1365).
-    struct Group {
-        std::vector<uint64_t> cand, pbase, tbase;
-        std::vector<uint32_t> plen, tlen;
-    } ext, srt[3];
+    // The distinct (segment, text) pairs as ranges of the buffer: the read slice and the text, both
+    // reversed for a left read end (lib.rs:1347-1365).  An extension's band is laid out for
+    // min(m, n) with `diagonal`, else for the larger of the segment's window and its longest text
+    // (align.rs:916, lib.rs:1446-1451).
     for (uint64_t c = 0; c < ncand; ++c) {
-        if (rep[c] != c) continue;
+        if (w.rep[c] != c) continue;
         const sf_tb_segment &g = segs[seg_of[c]];
         const bool rev = g.flags & SF_SEG_REVERSE;
-        Group &q = (g.flags & SF_SEG_EXTEND) ? ext : srt[g.mode];
+        TbGroup &q = (g.flags & SF_SEG_EXTEND) ? w.ext : w.srt[g.mode];
         q.cand.push_back(c);
+        q.seg.push_back(seg_of[c]);
         q.pbase.push_back(rev ? rrev_base + (reads_len - g.read_off - g.read_len) : rfwd_base + g.read_off);
         q.tbase.push_back(rev ? st->rev_base() + (st->bases - cand_off[c] - cand_len[c]) : cand_off[c]);
         q.plen.push_back(g.read_len);
         q.tlen.push_back(cand_len[c]);
-    }
-    std::vector<sf_extend_cut> res(ncand);
-    auto run = [&](Group &q, auto &&body) {
-        if (q.cand.empty()) return (int)SF_OK;
-        sf_batch *b = nullptr;
-        int s = sf_batch_create_resident(st->d_seq, q.cand.size(), q.plen.data(), q.tlen.data(), q.pbase.data(),
-                                         q.tbase.data(), &b);
-        if (!s) s = body(b);
-        if (b) sf_batch_destroy(b);
-        return s;
-    };
-    // Extensions: one pass with clipping over every distinct extension text, the band laid out per
-    // pair (align.rs:916: min(m, n) with `diagonal`, else max(n, window), the window being the
-    // larger of the segment's and its longest text, lib.rs:1446-1451), the best cell tracked (it
-    // changes no score), then align.rs:919-948 per pair with its segment's flags.
-    int s = run(ext, [&](sf_batch *b) {
-        const size_t n = ext.cand.size();
-        std::vector<uint64_t> geom(n), cut(n), band(n, p->band);
-        for (size_t i = 0; i < n; ++i) {
-            const sf_tb_segment &g = segs[seg_of[ext.cand[i]]];
-            geom[i] = (g.flags & SF_SEG_DIAGONAL) ? std::min<uint64_t>(ext.plen[i], ext.tlen[i])
-                                                  : std::max<uint64_t>(longest[seg_of[ext.cand[i]]], g.window);
-            cut[i] = g.cut;
+        if (g.flags & SF_SEG_EXTEND) {
+            q.geom.push_back((g.flags & SF_SEG_DIAGONAL) ? std::min<uint64_t>(g.read_len, cand_len[c])
+                                                         : std::max<uint64_t>(longest[seg_of[c]], g.window));
+            q.cut.push_back(g.cut);
         }
+    }
+    for (TbGroup *q : {&w.ext, &w.srt[0], &w.srt[1], &w.srt[2]})
+        if (!q->cand.empty())
+            if (const int r = sf_batch_create_resident(seq, q->cand.size(), q->plen.data(), q->tlen.data(),
+                                                       q->pbase.data(), q->tbase.data(), &q->b))
+                return r;
+    return SF_OK;
+}
+
+// Queues the passes of every group (their batches' streams run side by side).
+int tb_run(sf_tb_batch &w, const sf_tb_params *p) {
+    w.p = *p;
+    if (w.ext.b) {
+        std::vector<uint64_t> band(w.ext.cand.size(), p->band);
+        // align.rs:924: the best cell is tracked only with track_cut and a cut.  One pass serves
+        // every pair: it tracks the cell if any pair needs it (tracking changes no score; the results
+        // drop the cell where it is not kept).
+        bool cell = false;
+        for (size_t i = 0; i < w.ext.cand.size() && !cell; ++i)
+            cell = (w.segs[w.ext.seg[i]].flags & SF_SEG_TRACK_CUT) && w.ext.cut[i] != UINT64_MAX;
+        if (const int r = sf_batch_affine_dp(w.ext.b, SF_PREFIX, &p->sc, band.data(), w.ext.geom.data(), 1, p->xdrop,
+                                             cell ? 1 : 0, w.ext.cut.data()))
+            return r;
+    }
+    for (int mode = SF_INFIX; mode <= SF_GLOBAL; ++mode)
+        if (w.srt[mode].b) {
+            std::vector<uint64_t> band(w.srt[mode].cand.size(), p->band);
+            if (const int r = sf_batch_affine(w.srt[mode].b, mode, &p->sc, band.data(), 0)) return r;
+        }
+    w.ran = true;
+    return SF_OK;
+}
+
+// Waits for the passes and gives every candidate its distinct text's result: align.rs:919-948 per
+// extension pair with its segment's flags, (x, x, 0) for a sorted score (lib.rs:1442-1445).
+int tb_results(sf_tb_batch &w, sf_extend_cut *out, uint32_t *rep_out) {
+    if (!w.ran) return sf_runtime_fail(SF_EINVAL, "no run on this tie-break batch");
+    std::vector<sf_extend_cut> res(w.ncand);
+    const sf_tb_params &p = w.p;
+    if (w.ext.b) {
+        const size_t n = w.ext.cand.size();
         std::vector<sf_dp> d(n);
-        int q = sf_batch_affine_dp(b, SF_PREFIX, &p->sc, band.data(), geom.data(), 1, p->xdrop, 1, cut.data());
-        if (!q) q = sf_batch_dp_results(b, d.data());
-        if (q) return q;
+        if (const int r = sf_batch_dp_results(w.ext.b, d.data())) return r;
         for (size_t i = 0; i < n; ++i) {
-            const sf_tb_segment &g = segs[seg_of[ext.cand[i]]];
+            const sf_tb_segment &g = w.segs[w.ext.seg[i]];
             const bool has_cut = g.cut != UINT64_MAX, track = (g.flags & SF_SEG_TRACK_CUT) && has_cut;
             sf_dp x = d[i];
-            if (!track) x.clip_i = x.clip_j = 0; // align.rs:924: the best cell is kept only with track_cut and a cut
+            if (!track) x.clip_i = x.clip_j = 0;
             const int32_t end = (g.flags & SF_SEG_BONUS_WITHIN_CUT) && has_cut
-                                    ? std::max(x.score, sat_add(x.cut_score, p->end_bonus))
-                                    : sat_add(x.score, p->end_bonus);
+                                    ? std::max(x.score, sat_add(x.cut_score, p.end_bonus))
+                                    : sat_add(x.score, p.end_bonus);
             const int32_t whole = std::max(x.clip_best, end);
             const uint32_t cols = x.clip_best >= end ? x.clip_j : x.score_col;
-            const int32_t cutv = has_cut ? std::max(x.cut_best, sat_add(x.cut_score, p->end_bonus)) : whole;
-            res[ext.cand[i]] = sf_extend_cut{whole, cutv, cols};
+            const int32_t cutv = has_cut ? std::max(x.cut_best, sat_add(x.cut_score, p.end_bonus)) : whole;
+            res[w.ext.cand[i]] = sf_extend_cut{whole, cutv, cols};
         }
-        return (int)SF_OK;
-    });
-    if (s) return s;
-    // Sorted scores (align.rs:854-861, the results of separate calls): one pass per mode, (x, x, 0)
-    // as the reference maps them (lib.rs:1442-1445).
-    for (int mode = SF_INFIX; mode <= SF_GLOBAL; ++mode) {
-        Group &q = srt[mode];
-        s = run(q, [&](sf_batch *b) {
-            const size_t n = q.cand.size();
-            std::vector<uint64_t> band(n, p->band);
-            std::vector<int32_t> sc(n);
-            int r = sf_batch_affine(b, mode, &p->sc, band.data(), 0);
-            if (!r) r = sf_batch_affine_results(b, sc.data(), nullptr, nullptr, nullptr, 0, nullptr);
-            if (r) return r;
-            for (size_t i = 0; i < n; ++i) res[q.cand[i]] = sf_extend_cut{sc[i], sc[i], 0u};
-            return (int)SF_OK;
-        });
-        if (s) return s;
     }
-    for (uint64_t c = 0; c < ncand; ++c) {
-        out[c] = res[rep[c]];
-        if (rep_out) rep_out[c] = rep[c] - first[c];
+    for (int mode = SF_INFIX; mode <= SF_GLOBAL; ++mode) {
+        TbGroup &q = w.srt[mode];
+        if (!q.b) continue;
+        std::vector<int32_t> sc(q.cand.size());
+        if (const int r = sf_batch_affine_results(q.b, sc.data(), nullptr, nullptr, nullptr, 0, nullptr)) return r;
+        for (size_t i = 0; i < q.cand.size(); ++i) res[q.cand[i]] = sf_extend_cut{sc[i], sc[i], 0u};
+    }
+    for (uint64_t c = 0; c < w.ncand; ++c) {
+        if (out) out[c] = res[w.rep[c]];
+        if (rep_out) rep_out[c] = w.rep[c] - w.first[c];
     }
     return SF_OK;
 }
+
+} // namespace
+
+extern "C" {
+
+int sf_tiebreak_segments(sf_store *st, const uint8_t *reads, uint64_t reads_len, const sf_tb_segment *segs,
+                         uint64_t nseg, const uint64_t *cand_off, const uint32_t *cand_len, uint64_t ncand,
+                         const sf_tb_params *p, sf_extend_cut *out, uint32_t *rep_out) {
+    if (!st || !p || (nseg && !segs) || (ncand && (!cand_off || !cand_len || !out)) || (reads_len && !reads))
+        return sf_runtime_fail(SF_EINVAL, "null argument");
+    // On the store's own buffer: its reads area is the call's, so calls on one store take turns.
+    std::lock_guard<std::mutex> lk(st->mu);
+    if (const int q = st->reserve_reads(2 * ((reads_len + 15) / 16) + 1)) return q;
+    sf_tb_batch w;
+    w.st = st;
+    int r = tb_prepare(st, st->d_seq, reads, reads_len, segs, nseg, cand_off, cand_len, ncand, w);
+    if (!r && ncand) r = tb_run(w, p);
+    if (!r && ncand) r = tb_results(w, out, rep_out);
+    return r;
+}
+
+int sf_tb_batch_create(sf_store *st, const uint8_t *reads, uint64_t reads_len, const sf_tb_segment *segs,
+                       uint64_t nseg, const uint64_t *cand_off, const uint32_t *cand_len, uint64_t ncand,
+                       sf_tb_batch **out) {
+    if (!out) return sf_runtime_fail(SF_EINVAL, "out is null");
+    *out = nullptr;
+    if (!st || (nseg && !segs) || (ncand && (!cand_off || !cand_len)) || (reads_len && !reads))
+        return sf_runtime_fail(SF_EINVAL, "null argument");
+    if (reads_len >= (1ull << 40)) return sf_runtime_fail(SF_ERANGE, "2^40 read bases or more");
+    auto *w = new sf_tb_batch();
+    w->st = st;
+    // Its own buffer: the store's two parts copied (device to device) and room for the reads.
+    const uint64_t words = 4 * st->nwords + 2 * ((reads_len + 15) / 16) + 1;
+    if (hipMalloc((void **)&w->own_seq, words * 4) != hipSuccess) {
+        delete w;
+        return sf_runtime_fail(SF_ENOMEM, "tie-break batch buffer");
+    }
+    std::lock_guard<std::mutex> lk(st->mu);
+    hipError_t e = st->nwords ? hipMemcpyAsync(w->own_seq, st->d_seq, 4 * st->nwords * 4, hipMemcpyDeviceToDevice,
+                                               st->stream)
+                              : hipSuccess;
+    if (e) {
+        delete w;
+        return dev("tie-break batch buffer", e);
+    }
+    if (const int r = tb_prepare(st, w->own_seq, reads, reads_len, segs, nseg, cand_off, cand_len, ncand, *w)) {
+        (void)hipStreamSynchronize(st->stream);
+        delete w;
+        return r;
+    }
+    *out = w;
+    return SF_OK;
+}
+
+int sf_tb_batch_run(sf_tb_batch *b, const sf_tb_params *p) {
+    if (!b || !p) return sf_runtime_fail(SF_EINVAL, "null argument");
+    return b->ncand ? tb_run(*b, p) : (b->ran = true, (int)SF_OK);
+}
+
+int sf_tb_batch_results(sf_tb_batch *b, sf_extend_cut *out, uint32_t *rep) {
+    if (!b) return sf_runtime_fail(SF_EINVAL, "batch is null");
+    if (b->ncand && !out) return sf_runtime_fail(SF_EINVAL, "out is null");
+    return tb_results(*b, out, rep);
+}
+
+void sf_tb_batch_destroy(sf_tb_batch *b) { delete b; }
 
 } // extern "C"

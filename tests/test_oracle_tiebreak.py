@@ -153,3 +153,29 @@ def test_extension_band_is_laid_out_for_the_longest_text():
             wrong += FM.extend_cut_bonus(read, t, O.ONT, 2, 5, 50, False, max(window, len(t)), None, False,
                                          False) != want
     assert wrong > 0
+
+
+def test_c_batch_equals_the_restatement():
+    # oracle/sf_oracle.c `sfo_tiebreak_segments` (the CPU baseline's driver: threads over segments,
+    # scalar or the reference's x86 rows) against `tiebreak_segment` above, on the isoform cases.
+    from tb_cases import TBS, flags_of, make_segments, make_store, oracle
+
+    for name, tb in TBS.items():
+        rng = random.Random(200 + list(TBS).index(name))
+        codes, genes = make_store(rng, genes=12)
+        words = O.pack_u64(codes)
+        reads, segs = make_segments(rng, codes, genes, tb)
+        want, want_rep = oracle(words, reads, segs, tb)
+        arr = np.zeros(len(segs), O.TB_SEGMENT)
+        off, ln = [], []
+        for i, s in enumerate(segs):
+            arr[i] = (s.read_off, s.read_len, len(off), len(s.cands), int(s.mode), flags_of(s, tb),
+                      tb.window(s.read_len), O.UMAX if s.cut is None else s.cut)
+            off += [c[0] for c in s.cands]
+            ln += [c[1] for c in s.cands]
+        sc = (tb.affine.mat, tb.affine.mis, tb.affine.open, tb.affine.ext)
+        for avx in ([False, True] if O.avx512_available() else [False]):
+            out, rep = O.tiebreak_batch(words, len(codes), reads, arr, off, ln, sc, tb.band, tb.end_bonus, tb.xdrop,
+                                        threads=4, avx512=avx)
+            assert [tuple(x) for x in out.tolist()] == want, (name, avx)
+            assert rep.tolist() == want_rep
