@@ -63,6 +63,10 @@ VALU_INSTS_PER_CELL = {"score": 1.9077e10 / 2.46731e11, "traceback": 5.6398e10 /
 # (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.  Score: the
 # step's affine phase; traceback: the average of the DP's four launches a step.
 TRAFFIC_BYTES = {"score": 722_270_235, "traceback": 49_738_133_876}
+# Issued VALU wave-instructions of one extension pass of the tie-break workload (its four
+# affine_pass_kernel launches), from a rocprofv3 --pmc SQ_INSTS_VALU run of this script
+# (profiles/tiebreak_20k_reads_passes.md).
+TB_EXT_VALU_INSTS = 1.5311e9
 
 
 # Issue-rate probe (sf_probe_issue): waves of 32 independent VALU instructions of one encoding.
@@ -579,11 +583,13 @@ def bench_tiebreak(args, steps=None, warmup=None):
     for w in range(warmup):
         batch.run(tb)
         batch.results()
+    batch.timing(reset=True)
     t0 = time.perf_counter()
     for _ in range(steps):
         batch.run(tb)
         out, rep = batch.results()
     wall = time.perf_counter() - t0
+    tm = batch.timing()
     batch.close()
     distinct = int((rep == np.arange(len(rep)) - np.repeat(segs["cand0"], segs["ncand"])).sum())
     log(f"tiebreak: timed {steps} steps in {wall:.3f}s")
@@ -602,6 +608,17 @@ def bench_tiebreak(args, steps=None, warmup=None):
                               "affine_extend_sorted_cut_bonus, interior affine Global; ONT, band 16, xdrop 50, "
                               "end bonus 5"})
     res["steps"], res["warmup"] = steps, warmup
+    # The dominant phase: the extension pass (affine_pass_kernel launches side by side, HIP events
+    # around the phase), bound by VALU issue.
+    ext_s = tm.ext_ms / steps * 1e-3
+    achieved = TB_EXT_VALU_INSTS * 64 / ext_s / 1e12
+    res["roofline"] = {"bound": "valu", "kernel": "affine_pass_kernel<16, 5..8> (the extension pass)",
+                       "achieved": round(achieved, 3), "peak": round(VALU_ISSUE_PEAK_T, 1),
+                       "unit": "T VALU lane-instructions/s issued (SQ_INSTS_VALU of a pass / pass time)",
+                       "frac": round(achieved / VALU_ISSUE_PEAK_T, 4), "traffic": None,
+                       "avg_pass_ms": round(tm.ext_ms / steps, 3), "sorted_ms": round(tm.sorted_ms / steps, 3),
+                       "ext_cells_full_band": int(tm.ext_cells // steps),
+                       "ext_gcups_full_band": round(tm.ext_cells / steps / ext_s / 1e9, 1)}
     res["calls_per_s"] = round(len(segs) * steps / wall, 1)
     res["reads_per_s"] = round(n_reads * steps / wall, 1)
     res["setup_ms"] = round(setup_s * 1e3, 2)

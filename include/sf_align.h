@@ -299,6 +299,14 @@ int sf_tb_batch_create(sf_store *st, const uint8_t *reads, uint64_t reads_len, c
 int sf_tb_batch_run(sf_tb_batch *b, const sf_tb_params *p);
 int sf_tb_batch_results(sf_tb_batch *b, sf_extend_cut *out, uint32_t *rep);
 void sf_tb_batch_destroy(sf_tb_batch *b);
+/* Device time of the runs since the last reset (HIP events around each pass): the extension pass
+ * (all its launches) and the sorted-score passes (summed over modes), with their cells counted over
+ * the whole band (an upper bound of DP_WORK where an X-drop stops a pass early). */
+typedef struct {
+    double ext_ms, sorted_ms;
+    uint64_t ext_cells, sorted_cells;
+} sf_tb_timing;
+int sf_tb_batch_timing(sf_tb_batch *b, sf_tb_timing *t, int reset);
 
 /* ---- serving: continuous batching over a device-resident transcript store ----
  * No reference counterpart as an object: it serves the calls the tie-break makes per read

@@ -146,6 +146,11 @@ class TbParams(C.Structure):
     _fields_ = [("sc", Scoring), ("band", C.c_uint64), ("end_bonus", C.c_int32), ("xdrop", C.c_int32)]
 
 
+class TbTiming(C.Structure):
+    _fields_ = [("ext_ms", C.c_double), ("sorted_ms", C.c_double), ("ext_cells", C.c_uint64),
+                ("sorted_cells", C.c_uint64)]
+
+
 SEG_REVERSE, SEG_EXTEND, SEG_DIAGONAL, SEG_TRACK_CUT, SEG_BONUS_WITHIN_CUT = 1, 2, 4, 8, 16
 
 
@@ -206,6 +211,7 @@ SIGNATURES = {
     "sf_tb_batch_run": ([_vp, C.POINTER(TbParams)], _int),
     "sf_tb_batch_results": ([_vp, _vp, _vp], _int),
     "sf_tb_batch_destroy": ([_vp], None),
+    "sf_tb_batch_timing": ([_vp, C.POINTER(TbTiming), _int], _int),
     "sf_set_affine_lp16_max": ([C.c_uint64], _int),
     "sf_set_affine_lp8": ([C.c_int], _int),
 }

@@ -544,7 +544,7 @@ uint32_t *queue_for(sf_batch *b, int q, int end) {
 
 template <class F>
 int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch, uint64_t *launches = nullptr,
-             const MixFn &mix = nullptr, bool (*mixable)(int) = nullptr) {
+             const MixFn &mix = nullptr, bool (*mixable)(int) = nullptr, bool spread_big = false) {
     constexpr int S = 1 + sf_batch::kAux;
     if (bins.size() <= 1) {
         for (auto &bin : bins) HIP_TRY(launch(bin, b->stream));
@@ -575,6 +575,23 @@ int run_bins(sf_batch *b, const std::vector<sf_batch::Bin> &bins, F launch, uint
     double load[S] = {0, 0, 0};
     bool used[S] = {true, false, false};
     HIP_TRY(hipEventRecord(b->fork, b->stream));
+    if (spread_big) {
+        // Big bins too go to the least-loaded stream, the longest first, so that one bin's tail
+        // overlaps the next one's start (passes whose classes each fill the chip a few times over).
+        std::stable_sort(big.begin(), big.end(), [&](size_t x, size_t y) { return bins[x].est() > bins[y].est(); });
+        for (size_t i : big) {
+            int s = 0;
+            for (int t = 1; t < S; ++t)
+                if (load[t] < load[s]) s = t;
+            if (!used[s]) {
+                HIP_TRY(hipStreamWaitEvent(ss[s], b->fork, 0));
+                used[s] = true;
+            }
+            HIP_TRY(launch(bins[i], ss[s]));
+            load[s] += bins[i].est();
+        }
+        big.clear();
+    }
     for (size_t i : big) {
         HIP_TRY(launch(bins[i], b->stream));
         load[0] += bins[i].est();
@@ -1920,7 +1937,7 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
         x.desc = pl.d_desc + bin.begin;
         x.count = bin.end - bin.begin;
         return sf_launch_pass(x, bin.cls, false, s);
-    });
+    }, nullptr, nullptr, nullptr, true);
     if (!st) st = timed_end(b, 2, e0);
     if (st) return st;
     b->affine_cells += pl.cells;

@@ -507,6 +507,25 @@ int sf_tb_batch_results(sf_tb_batch *b, sf_extend_cut *out, uint32_t *rep) {
     return tb_results(*b, out, rep);
 }
 
+int sf_tb_batch_timing(sf_tb_batch *b, sf_tb_timing *t, int reset) {
+    if (!b || !t) return sf_runtime_fail(SF_EINVAL, "null argument");
+    *t = sf_tb_timing{};
+    if (b->ext.b) {
+        sf_timing x;
+        if (const int r = sf_batch_timing(b->ext.b, &x)) return r;
+        t->ext_ms = x.affine_ms, t->ext_cells = x.affine_cells;
+        if (reset) sf_batch_reset_timing(b->ext.b);
+    }
+    for (TbGroup &q : b->srt)
+        if (q.b) {
+            sf_timing x;
+            if (const int r = sf_batch_timing(q.b, &x)) return r;
+            t->sorted_ms += x.affine_ms, t->sorted_cells += x.affine_cells;
+            if (reset) sf_batch_reset_timing(q.b);
+        }
+    return SF_OK;
+}
+
 void sf_tb_batch_destroy(sf_tb_batch *b) { delete b; }
 
 } // extern "C"

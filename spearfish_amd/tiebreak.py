@@ -161,6 +161,11 @@ class TbBatch:
         N.check(self._lib.sf_tb_batch_results(self._h, out.ctypes.data, rep.ctypes.data))
         return out[: self.n], rep[: self.n]
 
+    def timing(self, reset: bool = False) -> N.TbTiming:
+        t = N.TbTiming()
+        N.check(self._lib.sf_tb_batch_timing(self._h, C.byref(t), int(reset)))
+        return t
+
     def close(self):
         if getattr(self, "_h", None):
             self._lib.sf_tb_batch_destroy(self._h)
