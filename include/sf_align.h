@@ -40,7 +40,7 @@ extern "C" {
 #define SF_ERANGE 5  /* an input exceeds this library's limits, or an output buffer is too small */
 /* Limits (beyond them SF_ERANGE): sequences under 2^30 bases; Myers patterns up to 2^24 blocks of
  * 64 bases (patterns over 98,304 bases run as stripes); affine bands up to 2^26 cells a row (over
- * 2,048 on the wide-band kernel; tracebacks up to 2,048); scoring values 0..65,535 and scores that
+ * 2,048 on the wide-band kernel; tracebacks up to 2,048); scoring values 0..4,194,303 and scores that
  * stay far from NEG (|score| under about 2^28); the serving loop's shape classes (patterns up to
  * 98,304 bases, bands up to 2,048 cells). */
 #define SF_ETIMEDOUT 6 /* sf_server_wait: the timeout passed before the request completed */

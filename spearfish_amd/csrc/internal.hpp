@@ -14,6 +14,9 @@
 namespace sf {
 
 constexpr int32_t kNeg = INT32_MIN / 4;       // align.rs:693
+// Largest scoring value (mat, mis, open, ext): the 32-bit kernels take the mismatch step
+// -(mat + mis) as a signed 24-bit multiplier; per pair, sf::scores_in_range bounds the scores.
+constexpr int32_t kScoreMax = (1 << 22) - 1;
 constexpr int32_t kScanIdent = INT32_MIN / 2; // below every value a cell or gap can take
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr int kInfix = 0, kPrefix = 1, kGlobal = 2;

@@ -1355,8 +1355,8 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
     if (mode < SF_INFIX || mode > SF_GLOBAL) return fail(SF_EINVAL, "mode %d", mode);
     if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0)
         return fail(SF_EINVAL, "scores must be non-negative (align.rs:2511 requires open, ext >= 0)");
-    if (sc->mat > 65535 || sc->mis > 65535 || sc->open > 65535 || sc->ext > 65535)
-        return fail(SF_ERANGE, "scores above 65535 are outside this library's bound");
+    if (sc->mat > kScoreMax || sc->mis > kScoreMax || sc->open > kScoreMax || sc->ext > kScoreMax)
+        return fail(SF_ERANGE, "scoring values above %d are outside this library's bound", kScoreMax);
     int st;
     if ((st = ensure_device())) return st;
     if (cigar && (st = refuse_in_capture(b, "an affine run with traceback"))) return st;
@@ -1646,9 +1646,9 @@ int sf_batch_autotune(sf_batch *b, int mode, const sf_scoring *sc, const uint64_
     if (!b) return fail(SF_EINVAL, "batch is null");
     if (!sc) return fail(SF_EINVAL, "scoring is null");
     if (mode < SF_INFIX || mode > SF_GLOBAL) return fail(SF_EINVAL, "mode %d", mode);
-    if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0 || sc->mat > 65535 || sc->mis > 65535 ||
-        sc->open > 65535 || sc->ext > 65535)
-        return fail(SF_EINVAL, "scores must be in 0..65535");
+    if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0) return fail(SF_EINVAL, "scores must be non-negative");
+    if (sc->mat > kScoreMax || sc->mis > kScoreMax || sc->open > kScoreMax || sc->ext > kScoreMax)
+        return fail(SF_ERANGE, "scoring values above %d are outside this library's bound", kScoreMax);
     int st;
     if ((st = refuse_in_capture(b, "sf_batch_autotune")) || (st = ensure_device()) || (st = quiesce(b))) return st;
     const uint64_t lp16max = sf_affine_lp16_max();
@@ -1861,8 +1861,8 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
     if (mode < SF_INFIX || mode > SF_GLOBAL) return fail(SF_EINVAL, "mode %d", mode);
     if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0)
         return fail(SF_EINVAL, "scores must be non-negative (align.rs:2511 requires open, ext >= 0)");
-    if (sc->mat > 65535 || sc->mis > 65535 || sc->open > 65535 || sc->ext > 65535)
-        return fail(SF_ERANGE, "scores above 65535 are outside this library's bound");
+    if (sc->mat > kScoreMax || sc->mis > kScoreMax || sc->open > kScoreMax || sc->ext > kScoreMax)
+        return fail(SF_ERANGE, "scoring values above %d are outside this library's bound", kScoreMax);
     int st;
     if ((st = ensure_device())) return st;
     if ((st = join_aux(b))) return st;
@@ -1963,8 +1963,8 @@ int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, c
     if (!sc || !dlo || !dhi) return fail(SF_EINVAL, "scoring and diagonals are required");
     if (sc->open < 0 || sc->ext < 0 || sc->mat < 0 || sc->mis < 0)
         return fail(SF_EINVAL, "scores must be non-negative (align.rs:2511 requires open, ext >= 0)");
-    if (sc->mat > 65535 || sc->mis > 65535 || sc->open > 65535 || sc->ext > 65535)
-        return fail(SF_ERANGE, "scores above 65535 are outside this library's bound");
+    if (sc->mat > kScoreMax || sc->mis > kScoreMax || sc->open > kScoreMax || sc->ext > kScoreMax)
+        return fail(SF_ERANGE, "scoring values above %d are outside this library's bound", kScoreMax);
     if (bonus_start < -(1 << 24) || bonus_start > (1 << 24) || bonus_end < -(1 << 24) || bonus_end > (1 << 24))
         return fail(SF_ERANGE, "end bonuses beyond +-2^24 are outside this library's bound");
     int st;

@@ -582,9 +582,10 @@ int sf_server_create(const uint32_t *store_words, uint64_t store_bases, const sf
     if (cfg->myers_mode < SF_INFIX || cfg->myers_mode > SF_GLOBAL || cfg->affine_mode < SF_INFIX ||
         cfg->affine_mode > SF_GLOBAL)
         return sf_runtime_fail(SF_EINVAL, "mode");
-    if (cfg->sc.mat < 0 || cfg->sc.mis < 0 || cfg->sc.open < 0 || cfg->sc.ext < 0 || cfg->sc.mat > 65535 ||
-        cfg->sc.mis > 65535 || cfg->sc.open > 65535 || cfg->sc.ext > 65535)
-        return sf_runtime_fail(SF_EINVAL, "scores must be in 0..65535");
+    if (cfg->sc.mat < 0 || cfg->sc.mis < 0 || cfg->sc.open < 0 || cfg->sc.ext < 0)
+        return sf_runtime_fail(SF_EINVAL, "scores must be non-negative");
+    if (cfg->sc.mat > kScoreMax || cfg->sc.mis > kScoreMax || cfg->sc.open > kScoreMax || cfg->sc.ext > kScoreMax)
+        return sf_runtime_fail(SF_ERANGE, "scoring values above 4,194,303 are outside this library's bound");
     // Per slot: half the share for read bases (4 B per 16 bases), half for segments (128 B each:
     // offsets, lengths, band, orders, two launch records and results).
     const uint64_t share = cfg->hbm_budget_bytes / cfg->slots;

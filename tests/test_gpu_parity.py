@@ -431,3 +431,43 @@ def test_issue_probe_runs_every_kind():
         A.N.check_diag(lib.sf_probe_issue(kind, 200, 64, C.byref(ms)))
         assert ms.value > 0
     assert lib.sf_probe_issue(4, 200, 64, C.byref(ms)) != 0
+
+
+# Scoring values past the 16-bit range (the reference takes any i32 values, `align.rs:660-691`):
+# the 32-bit kernels take them up to 2^22 - 1 while the pair's scores stay in range.
+LARGE = [A.Affine(300_000, 700_000, 1_000_000, 200_000), A.Affine(4_194_303, 1, 3, 2), A.Affine(7, 4_000_000, 0, 1)]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("sc", LARGE, ids=["mixed", "mat-max", "mis-max"])
+def test_affine_large_scoring_values(mode, sc):
+    rng = random.Random(77 + int(mode))
+    # Lengths within the library's range bound, (mat + mis + ext)(m + n + 2) < 2^28.
+    nmax = min(45, ((1 << 28) // (sc.mat + sc.mis + sc.ext) - 2) // 2 - 6)
+    P, T, B = [], [], []
+    for _ in range(120):
+        n = rng.randrange(0, nmax)
+        t = [rng.randrange(4) for _ in range(n)]
+        p = [x if rng.random() > 0.1 else rng.randrange(4) for x in t][: rng.randrange(0, n + 1) if n else 0]
+        if rng.random() < 0.3:
+            p = p + [rng.randrange(4) for _ in range(rng.randrange(0, 6))]
+        P.append(p)
+        T.append(t)
+        B.append(rng.choice([0, 2, 8, A.USIZE_MAX]))
+    ps = A.PairSet.from_seqs(P, T)
+    check_affine(ps, mode, sc, np.array(B, np.uint64))
+    # The passes with bookkeeping (X-drop, best cell, cut) with the same values.
+    for i in range(0, len(P), 7):
+        sct = (sc.mat, sc.mis, sc.open, sc.ext)
+        for clip, cut in (((1_500_000, True), None), ((0, False), 5), (None, 3)):
+            want = O.affine_dp(P[i], T[i], int(mode), sct, None if B[i] == A.USIZE_MAX else int(B[i]), None, clip, cut)
+            got = A.affine_dp(P[i], T[i], mode, sc, int(B[i]), clip, cut=A.USIZE_MAX if cut is None else cut)
+            assert tuple(got) == (want.score, want.score_col, want.clip_best, want.clip_i, want.clip_j,
+                                  want.text_read, want.last_row, want.cut_best, want.cut_score), (i, clip, cut)
+
+
+def test_scoring_values_past_the_bound_are_refused():
+    ps = A.PairSet.from_seqs([[0, 1]], [[0, 1]])
+    with pytest.raises(A.N.NativeError) as e:
+        A.affine_batch(ps, A.Mode.Global, A.Affine(1 << 22, 1, 1, 1), 4)
+    assert e.value.status == A.N.SF_ERANGE
