@@ -57,7 +57,7 @@ AFFINE_TB_OPS_PER_CELL = 11 + 4
 # Issued VALU wave-instructions per DP cell on the uniform workload, from rocprofv3 --pmc
 # SQ_INSTS_VALU runs of this script (profiles/valu_issue_uniform_1e6x1000_band128.md): the
 # score step's affine phase (its 8-lane packed bins and the LP64 bin) and the traceback DP.
-VALU_INSTS_PER_CELL = {"score": 1.9077e10 / 2.46731e11, "traceback": 5.6398e10 / 2.46731e11}
+VALU_INSTS_PER_CELL = {"score": 1.7930e10 / 2.46731e11, "traceback": 5.4447e10 / 2.46731e11}
 # HBM bytes per launch of the dominant kernels on the uniform workload, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the same counters on a 1 GiB copy
 # (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.  Score: the

@@ -326,6 +326,10 @@ typedef struct {
     int32_t affine_mode;
     sf_scoring sc;
     uint64_t band;
+    /* 0: a free slot takes whatever is waiting at once.  Otherwise a free slot waits until the
+     * waiting requests fill its share (or max_reads_per_batch), or until the oldest of them has
+     * waited this many microseconds: fuller batches for a bounded extra latency. */
+    uint32_t batch_wait_us;
 } sf_server_config;
 
 typedef struct {

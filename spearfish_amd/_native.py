@@ -91,6 +91,7 @@ class ServerConfig(C.Structure):
         ("affine_mode", C.c_int32),
         ("sc", Scoring),
         ("band", C.c_uint64),
+        ("batch_wait_us", C.c_uint32),
     ]
 
 

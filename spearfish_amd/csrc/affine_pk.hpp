@@ -107,6 +107,40 @@ __device__ __forceinline__ void pk16_result(const AffineArgs &a, const u16x2 (&V
     }
 }
 
+// The text fields of a 16-row block whose register 0 low slot reads text position `pos` in the
+// block's first row: field f holds the codes at positions pos + f (low half) and pos + K2 + f
+// (high half) as byte selectors (bytes 0 and 2; 0x0c selects a zero byte), so that one byte
+// permute against a row's one-hot pattern code gives a register's two match bits.  Row r of the
+// block reads field q + r in register q.
+template <int K2>
+__device__ __forceinline__ void text_fields(const SeqRef &T, int32_t pos, uint32_t (&W)[K2 + 15]) {
+    constexpr int NF = K2 + 15;
+    constexpr int NWP = (NF + 7) / 8; // paired text words holding them
+    uint32_t wp[NWP];
+#pragma unroll
+    for (int k = 0; k < NWP; ++k)
+        wp[k] = (codes16(T, pos + 8 * k) & 0xFFFFu) | (codes16(T, pos + K2 + 8 * k) << 16);
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+        W[f] = ((wp[f / 8] >> (2 * (f % 8))) & 0x00030003u) | 0x0c000c00u;
+}
+
+// The fields of the next block (16 positions on): field f takes field f + 16, and the top 16
+// fields come from `lo` and `hi`, the 16 codes at the next block's positions pos + K2 - 1 and
+// pos + 2 K2 - 1 (loaded a block ahead, so the loads' latency hides behind the rows).
+template <int K2>
+__device__ __forceinline__ void text_fields_next(uint32_t lo, uint32_t hi, uint32_t (&W)[K2 + 15]) {
+    constexpr int NF = K2 + 15;
+#pragma unroll
+    for (int f = 0; f < NF - 16; ++f) W[f] = W[f + 16];
+    // Paired words as text_fields builds them: 8 low-half codes under 8 high-half codes.
+    const uint32_t wa = __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+    const uint32_t wb = __builtin_amdgcn_perm(hi, lo, 0x07060302u);
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        W[NF - 16 + k] = (((k < 8 ? wa : wb) >> (2 * (k % 8))) & 0x00030003u) | 0x0c000c00u;
+}
+
 // LP lanes per pair: 16 (four pairs a wave) or 8 (eight pairs a wave: the per-row work that every
 // lane repeats -- scan, neighbour move, pattern code -- is shared by twice the slots).  M3: every
 // value is below 0x7C00 (checked on the host), so H takes its three terms in one max.
@@ -118,7 +152,6 @@ __device__ __forceinline__ void affine_pk16_body(const AffineArgs &a, uint32_t b
     constexpr int S = 2 * K2;                     // slots per lane
     constexpr int BLK = 16;                       // rows per block: one pattern word
     constexpr int NF = K2 + BLK - 1;              // text fields of a block: q + r, q < K2, r < 16
-    constexpr int NWP = (NF + 7) / 8;             // paired text words holding them
     const int lane = threadIdx.x & 63;
     const int l = lane & (LP - 1);
     const uint32_t w = (blk * 256u + threadIdx.x) / LP;
@@ -172,67 +205,76 @@ __device__ __forceinline__ void affine_pk16_body(const AffineArgs &a, uint32_t b
     }
     rows = __builtin_amdgcn_readfirstlane(rows);
     rows_all = __builtin_amdgcn_readfirstlane(rows_all);
-    for (int32_t i0 = 1; i0 <= rows; i0 += BLK) {
-        const uint32_t pw = codes16(P, i0 - 1);
-        const int32_t pos = i0 - below + s0 - 1; // text position of register 0's low slot in row i0
-        uint32_t wp[NWP];
+    // The text window slides 16 positions a block (register 0's low slot reads position
+    // i0 - below + s0 - 1 in row i0): the first block loads every field, each later one only
+    // its top 16.  Positions outside the text give unspecified codes, here as in a fresh load:
+    // they only reach cells right of column n or left of column 0, whose values never flow back
+    // into the band (see the bias note above).
+    uint32_t W[NF];
+    text_fields<K2>(T, -below + s0, W);
+    uint32_t pw = 0; // the block's 16 pattern codes
+    // One row, r static after unrolling (so every field index is a constant).
+    auto row = [&](const int r) {
+        const uint32_t onehot = 1u << (__builtin_amdgcn_ubfe(pw, 2 * r, 2) << 3);
+        // Register K2 - 1: low half reads slot s0 + K2 (register 0, high half), high half
+        // reads slot s0 + 2 K2 (the next lane's register 0, low half).
+        // (With 8 lanes a pair the row shift hands a group's top lane the next group's
+        // value: that lane takes the floor.)
+        const uint32_t u0 = as1(U[0]);
+        uint32_t un = dpp0<SHL>(u0);
+        if (LP == 8) un = l == LP - 1 ? 0u : un;
+        const u16x2 fK = as2(__builtin_amdgcn_alignbit(un, u0, 16));
+        u16x2 X[K2], G[K2];
 #pragma unroll
-        for (int k = 0; k < NWP; ++k)
-            wp[k] = (codes16(T, pos + 8 * k) & 0xFFFFu) | (codes16(T, pos + K2 + 8 * k) << 16);
-        // Field f of the block: the text codes of both halves' slots, as byte selectors (bytes 0
-        // and 2; 0x0c selects a zero byte), so one byte permute against the row's one-hot pattern
-        // code gives a register's two match bits.
-        uint32_t W[NF];
-#pragma unroll
-        for (int f = 0; f < NF; ++f)
-            W[f] = ((wp[f / 8] >> (2 * (f % 8))) & 0x00030003u) | 0x0c000c00u;
-        // One row, r static after unrolling (so every field index is a constant).
-        auto row = [&](const int r) {
-            const uint32_t onehot = 1u << (__builtin_amdgcn_ubfe(pw, 2 * r, 2) << 3);
-            // Register K2 - 1: low half reads slot s0 + K2 (register 0, high half), high half
-            // reads slot s0 + 2 K2 (the next lane's register 0, low half).
-            // (With 8 lanes a pair the row shift hands a group's top lane the next group's
-            // value: that lane takes the floor.)
-            const uint32_t u0 = as1(U[0]);
-            uint32_t un = dpp0<SHL>(u0);
-            if (LP == 8) un = l == LP - 1 ? 0u : un;
-            const u16x2 fK = as2(__builtin_amdgcn_alignbit(un, u0, 16));
-            u16x2 X[K2], G[K2];
-#pragma unroll
-            for (int q = 0; q < K2; ++q) {
-                const uint32_t mt = __builtin_amdgcn_perm(0u, onehot, W[q + r]); // match bits
-                const u16x2 d = V[q] + as2(mt) * vdm;
-                const u16x2 f = q < K2 - 1 ? U[q + 1] : fK; // F of this row
-                X[q] = pmax(d, f);
-                G[q] = psub(X[q], dl[q]);
-                U[q] = pmax(G[q], f); // in place: register q - 1 has read U[q] already
-            }
-            // Horizontal gaps: Pre[q] = max(G[0 .. q-1]), the lane's own prefix, is built while
-            // the group scan runs; then every register takes max(X, Pre, gaps entering the lane)
-            // with no chain after the scan.
-            u16x2 Pre[K2];
-            Pre[1 % K2] = G[0];
-#pragma unroll
-            for (int q = 2; q < K2; ++q) Pre[q] = pmax(Pre[q - 1], G[q - 1]);
-            const u16x2 tot = K2 > 1 ? pmax(Pre[K2 - 1], G[K2 - 1]) : G[0];
-            const uint32_t tt = as1(tot);
-            const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
-            const uint32_t C = group_excl_umax<LP>(tlo > thi ? tlo : thi);
-            const u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16)); // gap values entering register 0
-            V[0] = pmax(X[0], E);
-#pragma unroll
-            for (int q = 1; q < K2; ++q) V[q] = M3 ? pmax3(X[q], Pre[q], E) : pmax(X[q], pmax(Pre[q], E));
-        };
-        if (i0 + BLK - 1 <= rows_all) {
-            // Every pair of the wave has all 16 rows of this block: no per-row guard, so the
-            // rows' values stay in fixed registers.
-#pragma unroll
-            for (int r = 0; r < BLK; ++r) row(r);
-        } else {
-#pragma unroll
-            for (int r = 0; r < BLK; ++r)
-                if (i0 + r <= last) row(r); // this pair's rows are not done
+        for (int q = 0; q < K2; ++q) {
+            const uint32_t mt = __builtin_amdgcn_perm(0u, onehot, W[q + r]); // match bits
+            const u16x2 d = V[q] + as2(mt) * vdm;
+            const u16x2 f = q < K2 - 1 ? U[q + 1] : fK; // F of this row
+            X[q] = pmax(d, f);
+            G[q] = psub(X[q], dl[q]);
+            U[q] = pmax(G[q], f); // in place: register q - 1 has read U[q] already
         }
+        // Horizontal gaps: Pre[q] = max(G[0 .. q-1]), the lane's own prefix, is built while
+        // the group scan runs; then every register takes max(X, Pre, gaps entering the lane)
+        // with no chain after the scan.
+        u16x2 Pre[K2];
+        Pre[1 % K2] = G[0];
+#pragma unroll
+        for (int q = 2; q < K2; ++q) Pre[q] = pmax(Pre[q - 1], G[q - 1]);
+        const u16x2 tot = K2 > 1 ? pmax(Pre[K2 - 1], G[K2 - 1]) : G[0];
+        const uint32_t tt = as1(tot);
+        const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
+        const uint32_t C = group_excl_umax<LP>(tlo > thi ? tlo : thi);
+        const u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16)); // gap values entering register 0
+        V[0] = pmax(X[0], E);
+#pragma unroll
+        for (int q = 1; q < K2; ++q) V[q] = M3 ? pmax3(X[q], Pre[q], E) : pmax(X[q], pmax(Pre[q], E));
+    };
+    // A block's pattern codes, and the next block's new text codes (loaded a block ahead).
+    auto block_loads = [&](int32_t i0, uint32_t &nlo, uint32_t &nhi) {
+        pw = codes16(P, i0 - 1);
+        const int32_t posn = i0 + BLK - below + s0 - 1;
+        nlo = codes16(T, posn + K2 - 1);
+        nhi = codes16(T, posn + 2 * K2 - 1);
+    };
+    int32_t i0 = 1;
+    // Blocks whose 16 rows every pair of the wave has: no per-row guard, one loop body, so the
+    // rows' values stay in fixed registers across blocks.
+    for (; i0 + BLK - 1 <= rows_all; i0 += BLK) {
+        uint32_t nlo, nhi;
+        block_loads(i0, nlo, nhi);
+#pragma unroll
+        for (int r = 0; r < BLK; ++r) row(r);
+        text_fields_next<K2>(nlo, nhi, W);
+    }
+    // The wave's last blocks: each pair stops at its own last row.
+    for (; i0 <= rows; i0 += BLK) {
+        uint32_t nlo, nhi;
+        block_loads(i0, nlo, nhi);
+#pragma unroll
+        for (int r = 0; r < BLK; ++r)
+            if (i0 + r <= last) row(r);
+        text_fields_next<K2>(nlo, nhi, W);
     }
     pk16_result<LP, K2>(a, V, valid, pi, lane, l, m, n, below, Wd, last, off);
 }
@@ -275,7 +317,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K2 == 9 ? 3
     constexpr int S = 2 * K2;
     constexpr int BLK = 16;
     constexpr int NF = K2 + BLK - 1;
-    constexpr int NWP = (NF + 7) / 8;
     constexpr int NWL = kTbWords<S>;   // words per lane and row
     constexpr int NACC = (K2 + 3) / 4; // nibble accumulators, four registers each
     const int lane = threadIdx.x & 63;
@@ -336,111 +377,105 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K2 == 9 ? 3
     __shared__ uint32_t tbs[4 * NWL * 64 * 17];
     uint32_t *stage = tbs + ((threadIdx.x >> 6) * NWL * 64 + lane) * 17;
     uint32_t *tbp = a.tb + (valid ? a.tb_off[w] : 0) + l * 16;
-    for (int32_t i0 = 1; i0 <= rows; i0 += BLK) {
-        const uint32_t pw = codes16(P, i0 - 1);
-        const int32_t pos = i0 - below + s0 - 1;
-        uint32_t wp[NWP];
+    // The text window slides 16 positions a block, as in the score kernel.
+    uint32_t W[NF];
+    text_fields<K2>(T, -below + s0, W);
+    uint32_t pw = 0; // the block's 16 pattern codes
+    auto row = [&](const int r) {
+        const uint32_t onehot = 1u << (__builtin_amdgcn_ubfe(pw, 2 * r, 2) << 3);
+        const uint32_t un = from_up<LP>(as1(Ub[0]));
+        const uint32_t fn = from_up<LP>(as1(Fb[0]));
+        const u16x2 uuK = as2(__builtin_amdgcn_alignbit(un, as1(Ub[0]), 16));
+        const u16x2 fuK = as2(__builtin_amdgcn_alignbit(fn, as1(Fb[0]), 16));
+        u16x2 X[K2], G[K2], D[K2], BF[K2];
 #pragma unroll
-        for (int k = 0; k < NWP; ++k)
-            wp[k] = (codes16(T, pos + 8 * k) & 0xFFFFu) | (codes16(T, pos + K2 + 8 * k) << 16);
-        uint32_t W[NF];
-#pragma unroll
-        for (int f = 0; f < NF; ++f)
-            W[f] = ((wp[f / 8] >> (2 * (f % 8))) & 0x00030003u) | 0x0c000c00u;
-        auto row = [&](const int r) {
-            const uint32_t onehot = 1u << (__builtin_amdgcn_ubfe(pw, 2 * r, 2) << 3);
-            const uint32_t un = from_up<LP>(as1(Ub[0]));
-            const uint32_t fn = from_up<LP>(as1(Fb[0]));
-            const u16x2 uuK = as2(__builtin_amdgcn_alignbit(un, as1(Ub[0]), 16));
-            const u16x2 fuK = as2(__builtin_amdgcn_alignbit(fn, as1(Fb[0]), 16));
-            u16x2 X[K2], G[K2], D[K2], BF[K2];
-#pragma unroll
-            for (int q = 0; q < K2; ++q) {
-                const u16x2 uu = q < K2 - 1 ? Ub[q + 1] : uuK;
-                const u16x2 fu = q < K2 - 1 ? Fb[q + 1] : fuK;
-                const uint32_t mt = __builtin_amdgcn_perm(0u, onehot, W[q + r]);
-                D[q] = V[q] + as2(mt) * vdm;
-                BF[q] = __builtin_elementwise_min(psub(fu, uu), one); // fx: F extends the gap above
-                const u16x2 f = pmax(uu, fu);
-                Fb[q] = f; // in place: register q - 1 has read Fb[q] already
-                X[q] = pmax(D[q], f);
-                G[q] = psub(X[q], vopen);
-            }
-            u16x2 Pre[K2];
-            Pre[1 % K2] = G[0];
-#pragma unroll
-            for (int q = 2; q < K2; ++q) Pre[q] = pmax(Pre[q - 1], G[q - 1]);
-            const u16x2 tot = K2 > 1 ? pmax(Pre[K2 - 1], G[K2 - 1]) : G[0];
-            const uint32_t tt = as1(tot);
-            const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
-            const uint32_t C = group_excl_umax<LP>(tlo > thi ? tlo : thi);
-            const u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16));
-            // The last register first: its new Ub holds the left neighbours of register 0 (slot
-            // s0 - 1, the previous lane's last slot, in the low half; slot s0 + K2 - 1 in the
-            // high half).  Then each register's new value and its four bits in one pass, so that
-            // D, X and the gap term of a register die with it (fewer live registers, more waves).
-            const u16x2 EQL = K2 > 1 ? pmax(Pre[K2 - 1], E) : E; // the horizontal gap entering
-            V[K2 - 1] = pmax(X[K2 - 1], EQL);
-            Ub[K2 - 1] = psub(V[K2 - 1], dl[K2 - 1]);
-            const uint32_t ulp = from_down<LP>(as1(Ub[K2 - 1]));
-            const u16x2 ul0 = as2(__builtin_amdgcn_alignbit(as1(Ub[K2 - 1]), ulp, 16));
-            u16x2 acc[NACC];
-#pragma unroll
-            for (int g = 0; g < NACC; ++g) acc[g] = as2(0u);
-#pragma unroll
-            for (int q = 0; q < K2; ++q) {
-                const u16x2 ul = q ? Ub[q - 1] : ul0;
-                u16x2 EQq = EQL;
-                if (q < K2 - 1) {
-                    EQq = q ? pmax(Pre[q], E) : E;
-                    V[q] = pmax(X[q], EQq);
-                    Ub[q] = psub(V[q], dl[q]);
-                }
-                const u16x2 nd = __builtin_elementwise_min(psub(V[q], D[q]), one);
-                const u16x2 ne = __builtin_elementwise_min(psub(V[q], X[q]), one);
-                const u16x2 ex = __builtin_elementwise_min(psub(EQq, ul), one);
-                u16x2 A = acc[q / 4];
-                A = A * two + nd;
-                A = A * two + ne;
-                A = A * two + BF[q];
-                A = A * two + ex;
-                acc[q / 4] = A;
-            }
-            // The row's nibbles in slot order -- the low halves' registers, then the high
-            // halves' -- cut into words of 8 slots, the first slot highest; the last word holds
-            // the remaining slots in its low bits.
-            uint64_t buf = 0;
-            int nb = 0, wd = 0;
-            auto push = [&](uint32_t v, int bits) {
-                buf = (buf << bits) | v;
-                nb += bits;
-                if (nb >= 32) {
-                    stage[wd * 64 * 17 + r] = (uint32_t)(buf >> (nb - 32));
-                    ++wd;
-                    nb -= 32;
-                    buf &= (1ull << nb) - 1;
-                }
-            };
-#pragma unroll
-            for (int g = 0; g < NACC; ++g) {
-                const int cnt = K2 - 4 * g < 4 ? K2 - 4 * g : 4;
-                push(as1(acc[g]) & 0xFFFFu, 4 * cnt);
-            }
-#pragma unroll
-            for (int g = 0; g < NACC; ++g) {
-                const int cnt = K2 - 4 * g < 4 ? K2 - 4 * g : 4;
-                push(as1(acc[g]) >> 16, 4 * cnt);
-            }
-            if (nb > 0) stage[wd * 64 * 17 + r] = (uint32_t)buf;
-        };
-        if (i0 + BLK - 1 <= rows_all) {
-#pragma unroll
-            for (int r = 0; r < BLK; ++r) row(r);
-        } else {
-#pragma unroll
-            for (int r = 0; r < BLK; ++r)
-                if (i0 + r <= last) row(r);
+        for (int q = 0; q < K2; ++q) {
+            const u16x2 uu = q < K2 - 1 ? Ub[q + 1] : uuK;
+            const u16x2 fu = q < K2 - 1 ? Fb[q + 1] : fuK;
+            const uint32_t mt = __builtin_amdgcn_perm(0u, onehot, W[q + r]);
+            D[q] = V[q] + as2(mt) * vdm;
+            BF[q] = __builtin_elementwise_min(psub(fu, uu), one); // fx: F extends the gap above
+            const u16x2 f = pmax(uu, fu);
+            Fb[q] = f; // in place: register q - 1 has read Fb[q] already
+            X[q] = pmax(D[q], f);
+            G[q] = psub(X[q], vopen);
         }
+        u16x2 Pre[K2];
+        Pre[1 % K2] = G[0];
+#pragma unroll
+        for (int q = 2; q < K2; ++q) Pre[q] = pmax(Pre[q - 1], G[q - 1]);
+        const u16x2 tot = K2 > 1 ? pmax(Pre[K2 - 1], G[K2 - 1]) : G[0];
+        const uint32_t tt = as1(tot);
+        const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
+        const uint32_t C = group_excl_umax<LP>(tlo > thi ? tlo : thi);
+        const u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16));
+        // The last register first: its new Ub holds the left neighbours of register 0 (slot
+        // s0 - 1, the previous lane's last slot, in the low half; slot s0 + K2 - 1 in the
+        // high half).  Then each register's new value and its four bits in one pass, so that
+        // D, X and the gap term of a register die with it (fewer live registers, more waves).
+        const u16x2 EQL = K2 > 1 ? pmax(Pre[K2 - 1], E) : E; // the horizontal gap entering
+        V[K2 - 1] = pmax(X[K2 - 1], EQL);
+        Ub[K2 - 1] = psub(V[K2 - 1], dl[K2 - 1]);
+        const uint32_t ulp = from_down<LP>(as1(Ub[K2 - 1]));
+        const u16x2 ul0 = as2(__builtin_amdgcn_alignbit(as1(Ub[K2 - 1]), ulp, 16));
+        u16x2 acc[NACC];
+#pragma unroll
+        for (int g = 0; g < NACC; ++g) acc[g] = as2(0u);
+#pragma unroll
+        for (int q = 0; q < K2; ++q) {
+            const u16x2 ul = q ? Ub[q - 1] : ul0;
+            u16x2 EQq = EQL;
+            if (q < K2 - 1) {
+                EQq = q ? pmax(Pre[q], E) : E;
+                V[q] = pmax(X[q], EQq);
+                Ub[q] = psub(V[q], dl[q]);
+            }
+            const u16x2 nd = __builtin_elementwise_min(psub(V[q], D[q]), one);
+            const u16x2 ne = __builtin_elementwise_min(psub(V[q], X[q]), one);
+            const u16x2 ex = __builtin_elementwise_min(psub(EQq, ul), one);
+            u16x2 A = acc[q / 4];
+            A = A * two + nd;
+            A = A * two + ne;
+            A = A * two + BF[q];
+            A = A * two + ex;
+            acc[q / 4] = A;
+        }
+        // The row's nibbles in slot order -- the low halves' registers, then the high
+        // halves' -- cut into words of 8 slots, the first slot highest; the last word holds
+        // the remaining slots in its low bits.
+        uint64_t buf = 0;
+        int nb = 0, wd = 0;
+        auto push = [&](uint32_t v, int bits) {
+            buf = (buf << bits) | v;
+            nb += bits;
+            if (nb >= 32) {
+                stage[wd * 64 * 17 + r] = (uint32_t)(buf >> (nb - 32));
+                ++wd;
+                nb -= 32;
+                buf &= (1ull << nb) - 1;
+            }
+        };
+#pragma unroll
+        for (int g = 0; g < NACC; ++g) {
+            const int cnt = K2 - 4 * g < 4 ? K2 - 4 * g : 4;
+            push(as1(acc[g]) & 0xFFFFu, 4 * cnt);
+        }
+#pragma unroll
+        for (int g = 0; g < NACC; ++g) {
+            const int cnt = K2 - 4 * g < 4 ? K2 - 4 * g : 4;
+            push(as1(acc[g]) >> 16, 4 * cnt);
+        }
+        if (nb > 0) stage[wd * 64 * 17 + r] = (uint32_t)buf;
+    };
+    // A block's pattern codes, and the next block's new text codes (loaded a block ahead).
+    auto block_loads = [&](int32_t i0, uint32_t &nlo, uint32_t &nhi) {
+        pw = codes16(P, i0 - 1);
+        const int32_t posn = i0 + BLK - below + s0 - 1;
+        nlo = codes16(T, posn + K2 - 1);
+        nhi = codes16(T, posn + 2 * K2 - 1);
+    };
+    // The block's staged words out: each lane-word's 16 rows as one 64-byte line.
+    auto block_store = [&](int32_t i0) {
         if (valid && i0 <= last) {
             uint32_t *blk = tbp + (uint32_t)((i0 - 1) >> 4) * (NWL * LP * 16);
 #pragma unroll
@@ -452,6 +487,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K2 == 9 ? 3
                     dst[k] = make_uint4(src[4 * k], src[4 * k + 1], src[4 * k + 2], src[4 * k + 3]);
             }
         }
+    };
+    int32_t i0 = 1;
+    // Blocks whose 16 rows every pair of the wave has (one loop body, no per-row guard), then the
+    // wave's last blocks, where each pair stops at its own last row.
+    for (; i0 + BLK - 1 <= rows_all; i0 += BLK) {
+        uint32_t nlo, nhi;
+        block_loads(i0, nlo, nhi);
+#pragma unroll
+        for (int r = 0; r < BLK; ++r) row(r);
+        block_store(i0);
+        text_fields_next<K2>(nlo, nhi, W);
+    }
+    for (; i0 <= rows; i0 += BLK) {
+        uint32_t nlo, nhi;
+        block_loads(i0, nlo, nhi);
+#pragma unroll
+        for (int r = 0; r < BLK; ++r)
+            if (i0 + r <= last) row(r);
+        block_store(i0);
+        text_fields_next<K2>(nlo, nhi, W);
     }
     pk16_result<LP, K2>(a, V, valid, pi, lane, l, m, n, below, Wd, last, off);
 }

@@ -199,6 +199,7 @@ struct sf_server {
     std::mutex copy_mu;
     std::condition_variable cv_sched, cv_done;
     std::deque<Req *> pending;
+    uint64_t pend_words = 0, pend_seg = 0; // the waiting requests' read words and segments
     // Recycled requests (pool_mu).  Filled at creation, so that the first submissions allocate
     // nothing either.
     std::mutex pool_mu;
@@ -521,9 +522,19 @@ void sf_server::loop() {
             cv_done.notify_all();
         }
         // Free slots take the waiting requests, oldest first, while they fit the slot's share.
+        // With a batch wait, they hold off until the waiting requests fill a share or the oldest
+        // has waited that long.
         fresh.clear();
+        bool hold = false;
+        Clock::time_point hold_until{};
+        if (cfg.batch_wait_us && !stop && !pending.empty()) {
+            const bool fills = pend_words >= cap_words || pend_seg >= cap_seg ||
+                               (cfg.max_reads_per_batch && pending.size() >= cfg.max_reads_per_batch);
+            hold_until = pending.front()->t_submit + std::chrono::microseconds(cfg.batch_wait_us);
+            hold = !fills && Clock::now() < hold_until;
+        }
         for (Slot &s : slots) {
-            if (s.busy || pending.empty()) continue;
+            if (s.busy || pending.empty() || hold) continue;
             uint64_t words = 0, seg = 0;
             while (!pending.empty()) {
                 Req *r = pending.front();
@@ -535,6 +546,8 @@ void sf_server::loop() {
                 r->queued = false;
                 s.reqs.push_back(r);
                 pending.pop_front();
+                pend_words -= r->words.size();
+                pend_seg -= r->rlen.size();
             }
             s.busy = true;
             s.launch_no = ++launches;
@@ -547,6 +560,8 @@ void sf_server::loop() {
             if (!ended) {
                 if (busy) {
                     cv_sched.wait_for(lk, std::chrono::microseconds(10));
+                } else if (hold) {
+                    cv_sched.wait_until(lk, hold_until); // (a submission wakes it to look again)
                 } else {
                     cv_sched.wait(lk, [&] { return stop || !pending.empty(); });
                     if (trace) t_iter = Clock::now(); // idle, not an iteration's work
@@ -704,6 +719,8 @@ int sf_server_submit(sf_server *sv, const sf_read_request *rq, uint64_t *ticket)
         r->queued = true;
         sv->reqs[r->ticket] = r;
         sv->pending.push_back(r);
+        sv->pend_words += r->words.size();
+        sv->pend_seg += r->rlen.size();
         if (!sv->any) {
             sv->any = true;
             sv->t_first = r->t_submit;
@@ -748,6 +765,8 @@ int sf_server_cancel(sf_server *sv, uint64_t ticket) {
     if (r->done) return r->status == SF_OK ? SF_OK : sf_runtime_fail(r->status, "the request completed before");
     if (r->queued) { // not in a batch yet: it completes now
         sv->pending.erase(std::find(sv->pending.begin(), sv->pending.end(), r));
+        sv->pend_words -= r->words.size();
+        sv->pend_seg -= r->rlen.size();
         r->queued = false;
         r->status = SF_ECANCELED;
         r->t_done = Clock::now();

@@ -131,11 +131,11 @@ def requests(reads: Reads, res: Results, idx=None):
 class Server:
     def __init__(self, store: Store, hbm_budget_bytes: int = 256 << 20, slots: int = 4, max_reads_per_batch: int = 0,
                  myers_mode: Mode = Mode.Infix, affine_mode: Mode = Mode.Global, sc: Affine = Affine.ONT,
-                 band: int = 64):
+                 band: int = 64, batch_wait_us: int = 0):
         self.lib = N.load()
         self.store = store
         cfg = N.ServerConfig(hbm_budget_bytes, slots, max_reads_per_batch, int(myers_mode), int(affine_mode), sc.c(),
-                             band)
+                             band, batch_wait_us)
         h = C.c_void_p()
         N.check(self.lib.sf_server_create(store.words.ctypes.data, store.bases, C.byref(cfg), C.byref(h)))
         self.h = h

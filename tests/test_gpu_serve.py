@@ -156,9 +156,10 @@ def test_myers_every_block_count_alone_and_together(workload, mode):
 
 
 def test_large_batches_equal_reads_alone():
-    # Every read offered at once to one slot: batches grow past the small-batch limit (2,048
-    # segments: one launch per class, pair records built on the device), and each read's results
-    # equal those it gets served alone (pair records written on the host, one combined launch).
+    # Every read offered at once to one slot that waits to fill its batch (up to 0.5 s after the
+    # first read): batches grow past the small-batch limit (2,048 segments: one launch per class,
+    # pair records built on the device), and each read's results equal those it gets served alone
+    # (pair records written on the host, one combined launch).
     store, reads = S.generate_serve_workload(4000, seed=11)
     n = len(reads)
     alone, mix = S.Results(reads), S.Results(reads)
@@ -166,7 +167,7 @@ def test_large_batches_equal_reads_alone():
         ra = S.requests(reads, alone)
         for i in range(n):
             assert s.wait(s.submit(ra[i])) == N.SF_OK
-        s.stats(reset=True)
+    with S.Server(store, slots=1, band=BAND, batch_wait_us=500_000) as s:
         s.replay(S.requests(reads, mix), np.zeros(n))
         st = s.stats()
         assert st.reads_done == n and st.max_segments_per_batch > 2048, st.max_segments_per_batch
