@@ -100,6 +100,9 @@ def parse():
     ap.add_argument("--per-pair", type=int, default=256, help="pairs for the per-pair-launch comparison")
     ap.add_argument("--workload", default="uniform", choices=["uniform", "uneven", "long", "serve", "tiebreak", "probe"])
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-replays", type=int, default=0,
+                    help="serve: replay the 50k reads/s row this many more times in the same process "
+                         "and report each replay's p99")
     ap.add_argument("--no-extra", action="store_true", help="the uniform workload only")
     ap.add_argument("--tune-out", default=None, help="write the autotuner's measurements here (uneven)")
     ap.add_argument("--no-schedules", action="store_true",
@@ -141,6 +144,40 @@ def cpu_baseline(pairs, bands, gpu, threads):
     exact = bool(np.array_equal(d, gpu["dist"][:n]) and np.array_equal(s, gpu["score"][:n]))
     return {"cells": cells + acells, "seconds": t2 - t0, "myers_s": t1 - t0, "affine_s": t2 - t1,
             "bit_exact": exact, "pairs": n, "rows": "avx512 (i16 where narrow holds, else i32)" if simd else "scalar"}
+
+
+def ukkonen_rows(batch, steps=3):
+    """The workload's Myers pass with a threshold (the reference's capped calls: Mapper::banded
+    aligns later texts at k = best + cap, crates/map/src/lib.rs:851-911) and the reversed pass that
+    finds `start`, over Ukkonen's band and over every block (sf_set_myers_ukk): device ms a pass,
+    and whether both give the same results."""
+    from spearfish_amd import _native as N
+    from spearfish_amd import align as A
+    lib = N.load()
+    rows = []
+    for k, locate in ((A.U32_MAX, 2), (100, 1), (30, 1)):
+        row = {"k": None if k == A.U32_MAX else k,
+               "pass": "infix distance+end" + ("+start" if locate == 2 else "")}
+        res = []
+        for on in (0, 1):
+            N.check(lib.sf_set_myers_ukk(on))
+            batch.myers(A.Mode.Infix, k, locate)
+            batch.sync()
+            batch.reset_timing()
+            for _ in range(steps):
+                batch.myers(A.Mode.Infix, k, locate)
+            batch.sync()
+            tm = batch.timing()
+            key = "band" if on else "every_block"
+            row[f"{key}_ms"] = round(tm.myers_ms / steps, 2)
+            if locate == 2:
+                row[f"{key}_start_ms"] = round(tm.locate_ms / steps, 2)
+            res.append(batch.myers_results(locate))
+        row["bit_exact"] = all(np.array_equal(x, y) for x, y in zip(res[0], res[1]) if x is not None)
+        log(f"ukkonen: {row}")
+        rows.append(row)
+    N.check(lib.sf_set_myers_ukk(1))
+    return rows
 
 
 def per_pair_launch_gcups(pairs, bands, k):
@@ -282,6 +319,7 @@ def bench_uniform(args):
         times.append(time.perf_counter() - t0)
     out["ms_per_step_median"] = round(sorted(times)[len(times) // 2] * 1e3, 2)
     out["steps_for_median"] = len(times)
+    out["ukkonen"] = ukkonen_rows(batch)
     # End to end from host buffers, never `value`: 2-bit packing and upload, the launch plans, both
     # kernels, and the distances, ends, scores and columns copied back.
     batch.close()
@@ -547,6 +585,21 @@ def bench_serve(args):
            "gcups": round(st.cells / wall / 1e9, 1)}
     log(str(sat))
     rows.append(sat)
+    # The top rate replayed again and again in one process (one server, a warm-up first): the
+    # spread of p99 across replays, which a rare host pause would show as an outlier.
+    replays = []
+    if args.serve_replays > 0:
+        rate = max(rates)
+        n = int(rate * dur)
+        with S.Server(store, slots=4, band=64) as s:
+            s.replay(reqs, S.poisson_arrivals(2000, rate, seed=1))
+            for k in range(args.serve_replays):
+                s.stats(reset=True)
+                wall = s.replay(reqs, S.poisson_arrivals(n, rate, seed=100 + k))
+                st = s.stats()
+                replays.append({"reads_per_s": round(st.reads_done / wall, 1), "p50_ms": round(st.p50_ms, 3),
+                                "p99_ms": round(st.p99_ms, 3), "max_ms": round(st.max_ms, 3)})
+                log(f"replay {k + 1}/{args.serve_replays}: {replays[-1]}")
     top = [r for r in rows if r["mode"] == "continuous batching" and r["rate"] == max(rates)][-1]
     out = common(args, top["reads_per_s"], "reads/s", None, "int32",
                  {"workload": "serve-20k-transcripts-poisson-5k-20k-50k", "band": 64,
@@ -554,6 +607,8 @@ def bench_serve(args):
                   "server": "4 slots, 256 MiB HBM budget"})
     out["steps"], out["warmup"] = None, None
     out["serve"] = rows
+    if replays:
+        out["replays_at_top_rate"] = replays
     return out
 
 

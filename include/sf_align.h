@@ -419,6 +419,13 @@ uint64_t sf_tune_classes(void);
  * environment).  Applies to runs started (or captured) afterwards in this process. */
 int sf_set_persistent(int on);
 
+/* Tuning (results never depend on it): Myers passes of patterns of 2-24 blocks (65-1,536 bases)
+ * with a threshold below the pattern (k + 1 <= 64 * (blocks - 1)), and the reversed pass that finds
+ * `start`, compute only the blocks of Ukkonen's band (align.rs:113-149), over the union of a
+ * wave's 64 pairs' bands as align.rs:194-350 runs eight texts; 0 computes every block.  Default 1
+ * (or SF_MYERS_UKK from the environment).  Applies to passes started afterwards in this process. */
+int sf_set_myers_ukk(int on);
+
 /* Tuning (no reference counterpart; results never depend on it): lanes per pair of affine runs.
  * policy 0 (default): per pair, by the launch planner's estimate (score-only runs may use 8 lanes
  * per pair, eight pairs per wave, where the 16-bit range holds); 1: every pair in its fewest-lanes

@@ -44,6 +44,7 @@ struct MyersArgs {
     uint32_t count;
     int mode;
     uint32_t k;
+    const uint32_t *kvec; // a threshold per pair (by pair index) in place of k, or null
     int reverse; // 0: forward pass (dist, end); 1: reversed Prefix pass for `start`
     uint32_t *dist;
     uint32_t *end;
@@ -56,6 +57,7 @@ struct MyersArgs {
     const uint8_t *hcol_in;
     uint8_t *hcol_out;
     uint64_t hcol_stride;
+    __device__ uint32_t thresh(uint32_t pi) const { return kvec ? kvec[pi] : k; }
 };
 
 struct AffineArgs {
@@ -436,6 +438,10 @@ constexpr int kMyersStriped = 99000;
 constexpr uint32_t kMyersStripedBlocks = 64 * 24;
 constexpr uint32_t kMyersStripedMax = 1u << 24;
 hipError_t sf_launch_myers_striped(const sf::MyersArgs &a, hipStream_t s);
+// Lane classes (2..24 blocks) over the wave's Ukkonen band (myers_lane_ukk_kernel): for a finite k
+// below the patterns' lengths, and for the reversed pass, whose threshold is the distance.
+bool sf_myers_ukk_class(int cls);
+hipError_t sf_launch_myers_ukk(const sf::MyersArgs &a, int maxb, hipStream_t s);
 // The 64-lane pipeline class with the shorter chain that holds a 32-lane class's patterns.
 int sf_myers_wide_class(int cls);
 int sf_affine_k_for(uint64_t width);         // 0 if unsupported

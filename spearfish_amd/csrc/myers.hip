@@ -27,6 +27,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MAXB == 16 
     myers_lane_body<MAXB>(a, blockIdx.x);
 }
 
+// The lane kernel over the wave's Ukkonen band, for passes whose threshold is below the patterns'
+// lengths (a finite k, and the reversed pass for `start`, whose threshold is the distance).
+template <int MAXB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MAXB >= 12 && MAXB <= 16 ? 2 : 1))) void myers_lane_ukk_kernel(MyersArgs a) {
+    myers_lane_body<MAXB, true>(a, blockIdx.x);
+}
+
 template <int G, int R>
 __global__ __launch_bounds__(256) void myers_pipe_kernel(MyersArgs a) {
     myers_pipe_body<G, R>(a, blockIdx.x);
@@ -95,6 +102,22 @@ hipError_t sf_launch_myers(const MyersArgs &a, int maxb, hipStream_t s) {
     return hipGetLastError();
 }
 
+
+bool sf_myers_ukk_class(int cls) { return cls >= 2 && cls <= 24; }
+
+hipError_t sf_launch_myers_ukk(const MyersArgs &a, int maxb, hipStream_t s) {
+    if (a.count == 0) return hipSuccess;
+    dim3 grid((a.count + 255) / 256), block(256);
+    switch (maxb) {
+#define CASE(B) \
+    case B: hipLaunchKernelGGL(myers_lane_ukk_kernel<B>, grid, block, 0, s, a); break;
+        CASE(2) CASE(4) CASE(8) CASE(12) CASE(16) CASE(20) CASE(24)
+#undef CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t sf_launch_myers_striped(const MyersArgs &a, hipStream_t s) {
     if (a.count == 0) return hipSuccess;

@@ -46,6 +46,8 @@ __device__ __forceinline__ void load_planes(const SeqRef &P, uint32_t m, int b, 
     p2 = (uint64_t)lo2 | ((uint64_t)hi2 << 32);
 }
 
+__device__ __forceinline__ int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
+
 // Three-input bitwise ops (v_bitop3_b32): the immediate is the truth table, bit 4a + 2b + c, so a
 // function's table is the function applied to the three columns below.
 template <uint32_t TT>
@@ -98,7 +100,174 @@ __device__ __forceinline__ void block_step(Blk &k, uint32_t M1, uint32_t M2, uin
     hn = om;
 }
 
+// block_step<true> (the delta out of row 63 into hp/hn) that also ORs the horizontal deltas of the
+// rows selected by (ml, mh) into (ap, an): with the lane's last-row bit as the mask in its last block
+// and zero elsewhere, ap/an end nonzero exactly where that row's delta out is +1 / -1.
+__device__ __forceinline__ void block_step_cap(Blk &k, uint32_t M1, uint32_t M2, uint32_t &hp, uint32_t &hn,
+                                               uint32_t ml, uint32_t mh, uint32_t &ap, uint32_t &an) {
+    const uint32_t eql = lop3<kXorAnd>(k.p1l, M1, k.p2l ^ M2);
+    const uint32_t eqh = lop3<kXorAnd>(k.p1h, M1, k.p2h ^ M2);
+    const uint32_t xvl = eql | k.vml, xvh = eqh | k.vmh;
+    const uint32_t enl = eql | hn, enh = eqh;
+    const uint32_t al = enl & k.vpl, ah = enh & k.vph;
+    const uint32_t suml = al + k.vpl;
+    const uint32_t sumh = ah + k.vph + (suml < al ? 1u : 0u);
+    const uint32_t xhl = lop3<kXorOr>(suml, k.vpl, enl), xhh = lop3<kXorOr>(sumh, k.vph, enh);
+    const uint32_t hpl = lop3<kOrNor>(k.vml, xhl, k.vpl), hph = lop3<kOrNor>(k.vmh, xhh, k.vph);
+    const uint32_t hnl = k.vpl & xhl, hnh = k.vph & xhh;
+    ap |= (hpl & ml) | (hph & mh);
+    an |= (hnl & ml) | (hnh & mh);
+    const uint32_t spl = (hpl << 1) | hp, sph = __builtin_amdgcn_alignbit(hph, hpl, 31);
+    const uint32_t sml = (hnl << 1) | hn, smh = __builtin_amdgcn_alignbit(hnh, hnl, 31);
+    k.vpl = lop3<kOrNor>(sml, xvl, spl);
+    k.vph = lop3<kOrNor>(smh, xvh, sph);
+    k.vml = spl & xvl;
+    k.vmh = sph & xvh;
+    hp = hph >> 31;
+    hn = hnh >> 31;
+}
+
+// The columns of a lane pass over the wave's band of Ukkonen's cut-off (align.rs:113-149), in the
+// add-only form: the wave computes blocks 0..L, and after a column it adds block L + 1 when some
+// live lane holding that block has its band's bottom row (row 64 (L + 1)) at or below its
+// threshold kk.  A cell scoring <= kk has its up-left neighbour at <= kk (D[i][j] >= D[i-1][j-1]),
+// so every such cell of the next column lies in the band, and a block added after column j has no
+// cell <= kk in column j: its untouched start state (every vertical delta +1, rows counting up from
+// the row above) is an upper bound there, as the reference's added blocks are (align.rs:129-131).
+// Computed cells are therefore exact where they score <= kk and upper bounds elsewhere: the
+// distance when <= kk, its first end column, and above kk otherwise, exactly as the full column
+// gives them.  Blocks are never dropped (a wider band than the reference's, still exact), so no
+// block's state is ever reset, and the band only grows: each width L runs as its own loop
+// (ukk_phase<L>, its L + 1 block steps unrolled, no per-block test) until the band grows.
+//
+// Lanes of a class hold LB + 1 .. MAXB blocks (ukk_lb): blocks below LB step as in every lane's
+// interior; from LB on, each lane also takes the deltas out of its own last row (`hlast`), and
+// blocks past a lane's last run on its zero bit-planes, ignored (rows below a pattern never reach
+// it).  `sc` is the score at the bottom row of block min(L, lastb).
 template <int MAXB>
+constexpr int ukk_lb() { return MAXB <= 2 ? MAXB - 1 : (MAXB <= 4 ? 2 : (MAXB <= 8 ? 4 : MAXB - 4)); }
+
+struct UkkLane {
+    int nbw, lastb;        // blocks held (0 without work), the pattern's last block
+    unsigned last_bit;
+    uint32_t bl, bh;       // the last row's bit in the low and high word of its block
+    int32_t k;             // the threshold
+    int32_t sc, best, gres;
+    uint32_t bestj, n, top;
+    bool rev, global;
+    SeqRef T;
+    int nb_hi;             // wave-uniform: the most blocks a lane holds
+    int32_t nmax;          // wave-uniform: the longest text
+};
+
+template <int MAXB, int L>
+__device__ __forceinline__ int32_t ukk_phase(UkkLane &u, Blk (&K)[MAXB], int32_t j, uint32_t &tw) {
+    constexpr int LB = ukk_lb<MAXB>();
+    for (; j < u.nmax; ++j) {
+        const bool live = j < (int32_t)u.n; // (a lane past its text runs on clamped codes; its results are final)
+        if ((j & 15) == 0) tw = u.rev ? reverse2(codes16(u.T, (int64_t)u.n - 16 - (int64_t)j)) : codes16(u.T, j);
+        const uint32_t c = tw & 3;
+        tw >>= 2;
+        const uint32_t M1 = (c & 1) ? 0u : ~0u;
+        const uint32_t M2 = (c & 2) ? 0u : ~0u;
+        uint32_t hp = u.top, hn = 0, ap = 0, an = 0;
+#pragma unroll
+        for (int b = 0; b <= L; ++b) {
+            if (b < LB) {
+                block_step<true>(K[b], M1, M2, hp, hn, 63u);
+            } else {
+                // Bit 63 on to the next block, and the deltas out of the lane's own last row.
+                const bool own = b == u.lastb;
+                block_step_cap(K[b], M1, M2, hp, hn, own ? u.bl : 0u, own ? u.bh : 0u, ap, an);
+            }
+        }
+        // Out of block min(L, lastb): the lane's own last row, or the band's bottom row.
+        u.sc += u.lastb <= L ? (int32_t)(ap != 0) - (int32_t)(an != 0) : (int32_t)hp - (int32_t)hn;
+        int32_t nl = L;
+        if constexpr (L + 1 < MAXB) {
+            if (L + 1 < u.nb_hi) {
+                const bool holds = L + 1 < u.nbw;
+                if (__ballot(live && holds && u.sc <= u.k)) {
+                    nl = L + 1;
+                    if (holds) u.sc += L + 1 == u.lastb ? (int32_t)u.last_bit + 1 : 64;
+                }
+            }
+        }
+        if (live && nl >= u.lastb) { // the pattern's last row is in the lane's band: sc = D[m][j + 1]
+            if (!u.global) {
+                if (u.sc < u.best) {
+                    u.best = u.sc;
+                    u.bestj = j + 1;
+                }
+            } else if (j + 1 == (int32_t)u.n) {
+                u.gres = u.sc;
+            }
+        }
+        if (nl != L) return j + 1;
+    }
+    return j;
+}
+
+// The phases from width L on, entered at width L0 (the band only grows, one block at a time).
+template <int MAXB, int L = 0>
+__device__ __forceinline__ void ukk_phases(UkkLane &u, Blk (&K)[MAXB], int32_t &j, uint32_t &tw, int L0) {
+    if constexpr (L < MAXB) {
+        if (L >= L0 && j < u.nmax) j = ukk_phase<MAXB, L>(u, K, j, tw);
+        ukk_phases<MAXB, L + 1>(u, K, j, tw, L0);
+    }
+}
+
+// Returns false (nothing done) for a wave holding a pattern below its class's block counts.
+template <int MAXB>
+__device__ __forceinline__ bool myers_lane_ukk_columns(Blk (&K)[MAXB], uint32_t m, uint32_t n, int nb, int lastb,
+                                                       unsigned last_bit, bool work, bool rev, int mode, uint32_t top,
+                                                       const SeqRef &T, uint32_t kk, int32_t &best, uint32_t &bestj) {
+    if (__ballot(work && lastb < ukk_lb<MAXB>())) return false;
+    UkkLane u;
+    u.nbw = work ? nb : 0;
+    u.lastb = work ? lastb : MAXB; // (no work: holds nothing, never in a band)
+    u.last_bit = last_bit;
+    u.bl = last_bit < 32 ? 1u << last_bit : 0u;
+    u.bh = last_bit < 32 ? 0u : 1u << (last_bit - 32);
+    u.k = (int32_t)imin64((int64_t)kk, (int64_t)INT32_MAX - 64);
+    u.best = best;
+    u.bestj = bestj;
+    u.gres = (mode == kGlobal && n == 0) ? (int32_t)m : INT32_MAX;
+    u.n = n;
+    u.top = top;
+    u.rev = rev;
+    u.global = mode == kGlobal;
+    u.T = T;
+    // Column 0: blocks 0 .. ceil((kk + 1) / 64) - 1 (align.rs:110), the wave's widest.
+    int L0 = work ? (int)imin64((int64_t)nb, ((int64_t)kk + 64) / 64) - 1 : -1;
+    int nb_hi = u.nbw, nmax = (int)n;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        L0 = imax(L0, __shfl_xor(L0, off));
+        nb_hi = imax(nb_hi, __shfl_xor(nb_hi, off));
+        nmax = imax(nmax, __shfl_xor(nmax, off));
+    }
+    // Wave-uniform; the columns run to the longest text, each lane live while inside its own (as
+    // align.rs:237-238 runs eight texts).
+    L0 = __builtin_amdgcn_readfirstlane(L0);
+    u.nb_hi = __builtin_amdgcn_readfirstlane(nb_hi);
+    u.nmax = __builtin_amdgcn_readfirstlane(nmax);
+    u.sc = !work ? 0 : (L0 >= lastb ? (int32_t)m : 64 * (L0 + 1));
+    uint32_t tw = 0;
+    int32_t j = 0;
+    if (L0 >= 0) ukk_phases<MAXB>(u, K, j, tw, L0);
+    best = u.best;
+    bestj = u.bestj;
+    if (mode == kGlobal) {
+        best = u.gres;
+        bestj = n;
+    }
+    return true;
+}
+
+// UKK: the column runs only the blocks of the wave's Ukkonen band (myers_lane_ukk_columns), for
+// passes with a threshold below the patterns' lengths.
+template <int MAXB, bool UKK = false>
 __device__ __forceinline__ void myers_lane_body(const MyersArgs &a, uint32_t blk) {
     const uint32_t t = blk * 256u + threadIdx.x;
     const bool valid = t < a.count;
@@ -122,7 +291,7 @@ __device__ __forceinline__ void myers_lane_body(const MyersArgs &a, uint32_t blk
     if (valid && !rev && m == 0) {
         // align.rs:96-99: the empty pattern is n (Global) or 0 away.
         const uint32_t d = mode == kGlobal ? tlen : 0;
-        const bool ok = d <= a.k;
+        const bool ok = d <= a.thresh(pi);
         a.dist[pi] = ok ? d : kNone;
         if (a.end) a.end[pi] = ok ? (mode == kGlobal ? tlen : 0) : kNone;
         if (a.start) a.start[pi] = ok ? 0 : kNone;
@@ -153,6 +322,14 @@ __device__ __forceinline__ void myers_lane_body(const MyersArgs &a, uint32_t blk
     int32_t best = (int32_t)m;
     uint32_t bestj = 0;
     uint32_t tw = 0;
+    bool ukk_done = false;
+    if constexpr (UKK) {
+        // The threshold: the call's k clamped to m + n (align.rs:100), or for the reversed pass
+        // the pair's distance (the reversed Prefix pass from the end column reaches exactly it).
+        const uint32_t kk = !work ? 0u : (rev ? a.dist[pi] : (uint32_t)imin64((int64_t)a.thresh(pi), (int64_t)m + n));
+        ukk_done = myers_lane_ukk_columns<MAXB>(K, m, n, nb, lastb, last_bit, work, rev, mode, top, T, kk, best, bestj);
+    }
+    if (!ukk_done)
     for (uint32_t j = 0; j < n; ++j) {
         if ((j & 15) == 0) {
             tw = rev ? reverse2(codes16(T, (int64_t)n - 16 - (int64_t)j)) : codes16(T, j);
@@ -183,11 +360,11 @@ __device__ __forceinline__ void myers_lane_body(const MyersArgs &a, uint32_t blk
     }
     if (!work) return;
     if (!rev) {
-        if (mode == kGlobal) {
+        if (mode == kGlobal && !ukk_done) {
             best = score;
             bestj = n;
         }
-        const bool ok = (uint32_t)best <= a.k;
+        const bool ok = (uint32_t)best <= a.thresh(pi);
         a.dist[pi] = ok ? (uint32_t)best : kNone;
         if (a.end) a.end[pi] = ok ? bestj : kNone;
         if (a.start && (mode != kInfix || !ok)) a.start[pi] = ok ? 0 : kNone;
@@ -379,7 +556,7 @@ __device__ __forceinline__ void myers_pipe_body(const MyersArgs &a, uint32_t blk
     for (; t < steps; ++t) step(fetch4(t), t, guarded);
     if (valid && !rev && m == 0) {
         const uint32_t d = mode == kGlobal ? tlen : 0;
-        const bool ok = d <= a.k;
+        const bool ok = d <= a.thresh(pi);
         if (l == 0) {
             a.dist[pi] = ok ? d : kNone;
             if (a.end) a.end[pi] = ok ? (mode == kGlobal ? tlen : 0) : kNone;
@@ -397,7 +574,7 @@ __device__ __forceinline__ void myers_pipe_body(const MyersArgs &a, uint32_t blk
             best = score;
             bestj = n;
         }
-        const bool ok = (uint32_t)best <= a.k;
+        const bool ok = (uint32_t)best <= a.thresh(pi);
         a.dist[pi] = ok ? (uint32_t)best : kNone;
         if (a.end) a.end[pi] = ok ? bestj : kNone;
         if (a.start && (mode != kInfix || !ok)) a.start[pi] = ok ? 0 : kNone;

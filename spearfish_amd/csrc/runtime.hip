@@ -73,6 +73,7 @@ int ensure_device() {
 
 // -1: not read yet (SF_PERSISTENT from the environment, default off).
 std::atomic<int> g_persistent{-1};
+std::atomic<int> g_myers_ukk{-1}; // -1: not read from the environment yet
 
 unsigned host_threads() {
     unsigned t = std::thread::hardware_concurrency();
@@ -730,6 +731,27 @@ int run_grouped(sf_batch *b, const std::vector<sf_batch::Bin> &bins, One one, in
     return SF_OK;
 }
 
+int myers_ukk_policy() {
+    int v = g_myers_ukk.load();
+    if (v < 0) {
+        const char *e = std::getenv("SF_MYERS_UKK");
+        v = e ? (std::atoi(e) != 0) : 1;
+        int expected = -1;
+        if (!g_myers_ukk.compare_exchange_strong(expected, v)) v = expected;
+    }
+    return v;
+}
+
+// Whether a lane class runs over the wave's Ukkonen band: the reversed pass (its threshold is each
+// pair's distance) and passes whose k leaves at least one of the class's blocks out at the start.
+bool myers_use_ukk(const MyersArgs &a, int cls) {
+    if (!sf_myers_ukk_class(cls) || !myers_ukk_policy()) return false;
+    return a.reverse != 0 || a.kvec || (uint64_t)a.k + 1 <= 64ull * (uint64_t)(cls - 1);
+}
+
+// The mix groups for passes whose lane classes run over the Ukkonen band: those launch alone.
+int myers_group_ukk(int cls) { return sf_myers_ukk_class(cls) ? 0 : sf_myers_mix_group(cls); }
+
 // Every Myers bin, grouped (myers_mix.hip).
 int run_myers(sf_batch *b, const MyersArgs &a) {
     auto one = [&](const sf_batch::Bin &bin, hipStream_t s) {
@@ -737,6 +759,7 @@ int run_myers(sf_batch *b, const MyersArgs &a) {
         x.order = b->d_morder + bin.begin;
         x.desc = b->d_mdesc + bin.begin;
         x.count = bin.end - bin.begin;
+        if (myers_use_ukk(a, bin.cls)) return sf_launch_myers_ukk(x, bin.cls, s);
         if (bin.cls != kMyersStriped) return sf_launch_myers(x, bin.cls, s);
         // The stripes in order, on one stream: stripe t reads the deltas stripe t - 1 wrote.
         const uint64_t half = b->hcol_pairs * b->hcol_stride;
@@ -756,7 +779,11 @@ int run_myers(sf_batch *b, const MyersArgs &a) {
         x.desc = b->d_mdesc;
         return sf_launch_myers_mix(x, mx, group, s);
     };
-    return run_grouped(b, b->mbins, one, sf_myers_mix_group, sf_myers_blocks_for, mix, nullptr, true, 0);
+    // With the Ukkonen band the lane classes launch alone (a threshold that skips no block at the
+    // start of the smallest lane class leaves the plain grouping).
+    const bool ukk = myers_use_ukk(a, 24);
+    return run_grouped(b, b->mbins, one, ukk ? myers_group_ukk : sf_myers_mix_group, sf_myers_blocks_for, mix,
+                       nullptr, true, 0);
 }
 
 // Estimated VALU wave-instructions a row of one wave costs in shape class `cls`, shared by its
@@ -1714,6 +1741,12 @@ int sf_batch_autotune(sf_batch *b, int mode, const sf_scoring *sc, const uint64_
     return st;
 }
 
+int sf_set_myers_ukk(int on) {
+    if (on < 0 || on > 1) return fail(SF_EINVAL, "on must be 0 or 1");
+    g_myers_ukk.store(on);
+    return SF_OK;
+}
+
 int sf_set_persistent(int on) {
     if (on < 0 || on > 3) return fail(SF_EINVAL, "on must be 0..3 (bit 0: Myers runs, bit 1: score runs)");
     g_persistent.store(on);
@@ -2150,6 +2183,33 @@ int sf_local_band_ends(const sf_pairs *pairs, const sf_scoring *sc, const int64_
     return st;
 }
 
+// sf_myers with a threshold per pair (distances only): the pairs' lane classes run Ukkonen's band.
+static int myers_thresholds_run(sf_batch *b, int mode, const uint32_t *k, uint32_t *dk, uint32_t *dist) {
+    if (!b->myers_unsupported.empty()) return fail(SF_ERANGE, "%s", b->myers_unsupported.c_str());
+    int st;
+    if ((st = ensure_device()) || (st = join_aux(b))) return st;
+    HIP_TRY(hipMemcpyAsync(dk, k, b->n * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    MyersArgs a{};
+    a.pv = view(b);
+    a.mode = mode;
+    a.k = UINT32_MAX;
+    a.kvec = dk;
+    a.dist = b->d_dist;
+    if ((st = run_myers(b, a))) return st;
+    return sf_batch_myers_results(b, dist, nullptr, nullptr);
+}
+
+static int myers_thresholds(const sf_pairs *pairs, int mode, const uint32_t *k, uint32_t *dist) {
+    sf_batch *b;
+    int st = sf_batch_create(pairs, &b);
+    if (st) return st;
+    uint32_t *dk = nullptr;
+    if (!(st = dalloc(&dk, pairs->n))) st = myers_thresholds_run(b, mode, k, dk, dist);
+    sf_batch_destroy(b); // (waits for the batch's work)
+    if (dk) (void)hipFree(dk);
+    return st;
+}
+
 int sf_myers_banded(const sf_pairs *pairs, const uint64_t *group_off, uint64_t ngroups, int mode, uint32_t cap,
                     int distinct, uint32_t *out) {
     int st = check_pairs(pairs);
@@ -2180,10 +2240,12 @@ int sf_myers_banded(const sf_pairs *pairs, const uint64_t *group_off, uint64_t n
                 return false;
         return true;
     };
-    std::vector<uint64_t> uniq;
+    std::vector<uint64_t> uniq, group_head(n); // group_head: the pair holding its group's first text
+    auto first_of_group = [&](uint64_t i) { return group_head[i] == i; };
     for (uint64_t g = 0; g < ngroups; ++g) {
         std::unordered_multimap<uint64_t, uint64_t> seen;
         for (uint64_t i = group_off[g]; i < group_off[g + 1]; ++i) {
+            group_head[i] = group_off[g];
             first[i] = i;
             if (!distinct) {
                 const uint64_t h = hash(i);
@@ -2198,23 +2260,39 @@ int sf_myers_banded(const sf_pairs *pairs, const uint64_t *group_off, uint64_t n
             if (first[i] == i) uniq.push_back(i);
         }
     }
-    // The distinct texts' exact distances on the device (distance_with is exact wherever it is
-    // within k, so these give every threshold's answer).
-    std::vector<uint64_t> poff(uniq.size()), toff(uniq.size());
-    std::vector<uint32_t> plen(uniq.size()), tlen(uniq.size()), d(uniq.size());
-    for (size_t q = 0; q < uniq.size(); ++q) {
-        const uint64_t i = uniq[q];
-        poff[q] = pairs->pat_off[i], plen[q] = pairs->pat_len[i];
-        toff[q] = pairs->txt_off[i], tlen[q] = pairs->txt_len[i];
-    }
-    sf_pairs sub = *pairs;
-    sub.pat_off = poff.data(), sub.pat_len = plen.data(), sub.txt_off = toff.data(), sub.txt_len = tlen.data();
-    sub.n = uniq.size();
-    if ((st = sf_myers(&sub, mode, UINT32_MAX, d.data(), nullptr, nullptr))) return st;
-    std::vector<uint32_t> exact(n, 0);
-    for (size_t q = 0; q < uniq.size(); ++q) exact[uniq[q]] = d[q];
-    // The sequential values (lib.rs:3016-3034, equal to lib.rs:880-910 by its own argument).
+    // The distinct texts' distances on the device, in two passes.  A group's first text runs
+    // unbounded (its threshold is u32::MAX + cap, saturated); every later one at that distance +
+    // cap, which no later threshold of the group exceeds (the best only falls).  distance_with is
+    // exact wherever it is within its k, so each result is exact or known to lie above every
+    // threshold the text meets in order, and the second pass computes only Ukkonen's bands.
     auto sat = [](uint32_t a, uint32_t b) { return a > UINT32_MAX - b ? UINT32_MAX : a + b; };
+    std::vector<uint32_t> exact(n, 0);
+    auto pass = [&](const std::vector<uint64_t> &ids, const std::vector<uint32_t> *kv) {
+        if (ids.empty()) return SF_OK;
+        std::vector<uint64_t> poff(ids.size()), toff(ids.size());
+        std::vector<uint32_t> plen(ids.size()), tlen(ids.size()), d(ids.size());
+        for (size_t q = 0; q < ids.size(); ++q) {
+            const uint64_t i = ids[q];
+            poff[q] = pairs->pat_off[i], plen[q] = pairs->pat_len[i];
+            toff[q] = pairs->txt_off[i], tlen[q] = pairs->txt_len[i];
+        }
+        sf_pairs sub = *pairs;
+        sub.pat_off = poff.data(), sub.pat_len = plen.data(), sub.txt_off = toff.data(), sub.txt_len = tlen.data();
+        sub.n = ids.size();
+        int r = kv ? myers_thresholds(&sub, mode, kv->data(), d.data()) : sf_myers(&sub, mode, UINT32_MAX, d.data(),
+                                                                                    nullptr, nullptr);
+        if (r) return r;
+        for (size_t q = 0; q < ids.size(); ++q) exact[ids[q]] = d[q];
+        return SF_OK;
+    };
+    std::vector<uint64_t> heads, rest;
+    std::vector<uint32_t> kv;
+    for (uint64_t i : uniq) (first_of_group(i) ? heads : rest).push_back(i);
+    if ((st = pass(heads, nullptr))) return st;
+    for (uint64_t i : rest) kv.push_back(sat(exact[group_head[i]], cap));
+    if ((st = pass(rest, &kv))) return st;
+    // The sequential values (lib.rs:3016-3034, equal to lib.rs:880-910 by its own argument).  A text
+    // above its pass's threshold (SF_NONE) is above every threshold it meets here.
     for (uint64_t g = 0; g < ngroups; ++g) {
         uint32_t best = UINT32_MAX;
         for (uint64_t i = group_off[g]; i < group_off[g + 1]; ++i) {
