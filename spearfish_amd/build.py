@@ -27,7 +27,7 @@ HIP_FLAGS = [
 HIP_SOURCES = ["affine_mix1.hip", "mix_persistent_affine1.hip", "serve_mix.hip", "affine_pk.hip", "affine_mix.hip",
                "mix_persistent_affine.hip", "myers_mix.hip", "mix_persistent_myers.hip", "noalloc.hip",
                "affine_pk64.hip", "affine.hip", "myers.hip", "runtime.hip", "serve.hip", "tune.hip",
-               "affine_wide.hip", "tiebreak.hip"]
+               "affine_wide.hip", "tiebreak.hip", "myers_start.hip"]
 DIAG_SOURCES = ["probe.hip"]
 ALIGN_LIB = os.path.join(HERE, "libsf_align.so")
 DIAG_LIB = os.path.join(HERE, "libsf_diag.so")

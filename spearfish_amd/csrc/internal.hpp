@@ -60,6 +60,21 @@ struct MyersArgs {
     __device__ uint32_t thresh(uint32_t pi) const { return kvec ? kvec[pi] : k; }
 };
 
+// The start pass of patterns over 1,536 bases, stripe by stripe over the diagonal band
+// (myers_start.hip), over launch order slots [first, first + count): row u of each stripe buffer
+// belongs to slot first + u and holds a stripe's bottom-row deltas, 2 bits a column, hc_words a
+// row (ping-pong between hc0 and hc1).
+struct MyersStartArgs {
+    PairsView pv;
+    const PairDesc *desc;
+    const uint32_t *order;
+    uint32_t first, count;
+    const uint32_t *dist, *end;
+    uint32_t *start;
+    uint32_t *hc0, *hc1;
+    uint64_t hc_words;
+};
+
 struct AffineArgs {
     PairsView pv;
     const PairDesc *desc; // per order slot of the launch
@@ -441,6 +456,7 @@ hipError_t sf_launch_myers_striped(const sf::MyersArgs &a, hipStream_t s);
 // Lane classes (2..24 blocks) over the wave's Ukkonen band (myers_lane_ukk_kernel): for a finite k
 // below the patterns' lengths, and for the reversed pass, whose threshold is the distance.
 bool sf_myers_ukk_class(int cls);
+hipError_t sf_launch_myers_start_stripe(const sf::MyersStartArgs &a, hipStream_t s);
 hipError_t sf_launch_myers_ukk(const sf::MyersArgs &a, int maxb, hipStream_t s);
 // The 64-lane pipeline class with the shorter chain that holds a 32-lane class's patterns.
 int sf_myers_wide_class(int cls);

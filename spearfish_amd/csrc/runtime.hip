@@ -278,6 +278,12 @@ struct sf_batch {
     uint8_t *d_hcol = nullptr;
     uint64_t hcol_stride = 0, hcol_pairs = 0;
     uint32_t stripes_max = 0;
+    // The start pass of patterns over 1,536 bases (the launch order's suffix from start_first):
+    // its stripes' bottom-row deltas (2 bits a column, ping-pong), made on first use.
+    std::vector<uint32_t> morder;     // host copy of the Myers launch order
+    uint64_t start_first = 0;         // first launch slot of a pattern over 1,536 bases (n: none)
+    uint64_t start_words = 0;         // words a row of start_hc
+    uint32_t *d_start_hc = nullptr;
     // affine
     // Affine launch plans, cached per (mode, bands, shape tuning): one for score-only runs (which
     // may use the 16-bit kernel), one for runs with traceback.
@@ -384,7 +390,8 @@ struct sf_batch {
         dfree(d_dp), dfree(d_best), dfree(d_pinv), dfree(d_col), dfree(d_cut);
         dfree(d_tstart), dfree(d_nruns), dfree(d_runs), dfree(d_cig), dfree(d_runs_off), dfree(d_out_off);
         dfree(d_scan);
-        dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0), dfree(d_queues), dfree(d_wide), dfree(d_hcol);
+        dfree(d_tboff), dfree(d_tb[0]), dfree(d_tb[1]), dfree(d_toff0), dfree(d_tlen0), dfree(d_queues), dfree(d_wide), dfree(d_hcol),
+            dfree(d_start_hc);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (hipStream_t x : aux)
@@ -752,8 +759,8 @@ bool myers_use_ukk(const MyersArgs &a, int cls) {
 // The mix groups for passes whose lane classes run over the Ukkonen band: those launch alone.
 int myers_group_ukk(int cls) { return sf_myers_ukk_class(cls) ? 0 : sf_myers_mix_group(cls); }
 
-// Every Myers bin, grouped (myers_mix.hip).
-int run_myers(sf_batch *b, const MyersArgs &a) {
+// Myers bins, grouped (myers_mix.hip).
+int run_myers_bins(sf_batch *b, const MyersArgs &a, const std::vector<sf_batch::Bin> &bins) {
     auto one = [&](const sf_batch::Bin &bin, hipStream_t s) {
         MyersArgs x = a;
         x.order = b->d_morder + bin.begin;
@@ -782,8 +789,45 @@ int run_myers(sf_batch *b, const MyersArgs &a) {
     // With the Ukkonen band the lane classes launch alone (a threshold that skips no block at the
     // start of the smallest lane class leaves the plain grouping).
     const bool ukk = myers_use_ukk(a, 24);
-    return run_grouped(b, b->mbins, one, ukk ? myers_group_ukk : sf_myers_mix_group, sf_myers_blocks_for, mix,
+    return run_grouped(b, bins, one, ukk ? myers_group_ukk : sf_myers_mix_group, sf_myers_blocks_for, mix,
                        nullptr, true, 0);
+}
+
+// Every Myers bin.
+int run_myers(sf_batch *b, const MyersArgs &a) { return run_myers_bins(b, a, b->mbins); }
+
+// The start pass (a.reverse): lane classes as their bins run it (over Ukkonen's band), patterns
+// over 1,536 bases stripe by stripe over the diagonal band (myers_start.hip), one launch on the
+// batch's stream.  Under a capture without the stripe buffers made, every bin runs the pipelines'
+// full pass.
+int run_myers_start(sf_batch *b, const MyersArgs &a) {
+    const uint64_t n = b->n, q0 = b->start_first;
+    if (q0 >= n || !myers_ukk_policy() || (b->capturing && !b->d_start_hc)) return run_myers(b, a);
+    int st;
+    if (!b->d_start_hc) {
+        uint32_t tmax = 0;
+        for (uint64_t q = q0; q < n; ++q) tmax = std::max(tmax, b->tlen[b->morder[q]]);
+        b->start_words = (uint64_t)tmax / 16 + 2;
+        if ((st = dalloc(&b->d_start_hc, 2 * (n - q0) * b->start_words))) return st;
+    }
+    std::vector<sf_batch::Bin> lane;
+    for (const auto &bin : b->mbins)
+        if (bin.end <= q0) lane.push_back(bin);
+    if (!lane.empty() && (st = run_myers_bins(b, a, lane))) return st;
+    MyersStartArgs s{};
+    s.pv = a.pv;
+    s.desc = b->d_mdesc;
+    s.order = b->d_morder;
+    s.first = (uint32_t)q0;
+    s.count = (uint32_t)(n - q0);
+    s.dist = a.dist;
+    s.end = a.end;
+    s.start = a.start;
+    s.hc_words = b->start_words;
+    s.hc0 = b->d_start_hc;
+    s.hc1 = b->d_start_hc + (n - q0) * b->start_words;
+    HIP_TRY(sf_launch_myers_start_stripe(s, b->stream));
+    return SF_OK;
 }
 
 // Estimated VALU wave-instructions a row of one wave costs in shape class `cls`, shared by its
@@ -1152,6 +1196,14 @@ int batch_init(sf_batch *b, const std::vector<uint64_t> &pbase, const std::vecto
                 b->stripes_max, ((b->plen[i] + 63) / 64 + kMyersStripedBlocks - 1) / kMyersStripedBlocks);
         }
     if (b->hcol_pairs && (st = dalloc(&b->d_hcol, 2 * b->hcol_pairs * b->hcol_stride))) return st;
+    // Patterns over 1,536 bases sort last in the launch order (by blocks).
+    b->morder = order;
+    b->start_first = n;
+    for (uint64_t q = 0; q < n; ++q)
+        if (b->plen[order[q]] > 64u * 24u) {
+            b->start_first = q;
+            break;
+        }
     hipError_t e2 = hipMemcpy(b->d_poff, pbase.data(), n * 8, hipMemcpyHostToDevice);
     hipError_t e3 = hipMemcpy(b->d_toff, tbase.data(), n * 8, hipMemcpyHostToDevice);
     hipError_t e4 = hipMemcpy(b->d_plen, b->plen.data(), n * 4, hipMemcpyHostToDevice);
@@ -1370,7 +1422,7 @@ int sf_batch_myers(sf_batch *b, int mode, uint32_t k, int locate) {
         if ((st = join_aux(b))) return st; // the start pass reads the forward pass's results
         if ((st = timed_begin(b, 1, &e0))) return st;
         a.reverse = 1;
-        if ((st = run_myers(b, a))) return st;
+        if ((st = run_myers_start(b, a))) return st;
         if ((st = timed_end_open(b, 1, e0))) return st;
     }
     return SF_OK;

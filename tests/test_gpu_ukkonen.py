@@ -104,3 +104,46 @@ def test_distance_exactly_at_the_threshold(ukk):
         for i in range(len(ps)):
             want = O.locate(ps.pattern(i), ps.text(i), int(A.Mode.Infix), k)
             assert (int(got[0][i]), int(got[1][i])) == (opt(want[0]), opt(want[1])), (i, k)
+
+
+def long_pairs(seed, n):
+    rng = random.Random(seed)
+    P, T = [], []
+    for i in range(n):
+        m = rng.choice([1537, 1600, 2047, 2048, 2049, 3071, 3072, 3073, 4100, 6000, 9000, 12000,
+                        rng.randrange(1537, 12001)])
+        p = [rng.randrange(4) for _ in range(m)]
+        kind = i % 6
+        flank = lambda: [rng.randrange(4) for _ in range(rng.randrange(0, 400))]
+        if kind == 0:    # exact, inside a longer text (d = 0)
+            t = flank() + list(p) + flank()
+        elif kind == 1:  # 5% errors
+            t = flank() + mutate(rng, p, 0.05) + flank()
+        elif kind == 2:  # 12%, as the long workload's reads
+            t = mutate(rng, p, 0.12) + flank()
+        elif kind == 3:  # 40%: windows wider than a stripe
+            t = flank() + mutate(rng, p, 0.40)
+        elif kind == 4:  # unrelated (d far above 1,024: stripes start at column 1)
+            t = [rng.randrange(4) for _ in range(rng.randrange(m // 2, 2 * m))]
+        else:            # a piece of the pattern only
+            lo = rng.randrange(0, m // 2)
+            t = mutate(rng, p[lo:lo + rng.randrange(1, m // 2)], 0.03)
+        P.append(p)
+        T.append(t)
+    return A.PairSet.from_seqs(P, T)
+
+
+def test_start_over_the_diagonal_band_long_patterns(ukk):
+    # Patterns over 1,536 bases find `start` stripe by stripe over the band |i - j| <= d
+    # (myers_start.hip); against the oracle and the pipelines' full reversed pass.
+    ps = long_pairs(123, 240)
+    N.check(ukk.sf_set_myers_ukk(1))
+    dist, end, start = A.myers_batch(ps, A.Mode.Infix, U32, locate=2)
+    N.check(ukk.sf_set_myers_ukk(0))
+    full = A.myers_batch(ps, A.Mode.Infix, U32, locate=2)
+    assert (dist == full[0]).all() and (end == full[1]).all()
+    assert (start == full[2]).all(), [i for i in range(len(ps)) if start[i] != full[2][i]][:10]
+    for i in range(0, len(ps), 2):
+        p, t = ps.pattern(i), ps.text(i)
+        d, e, s = O.locate(p, t, int(A.Mode.Infix))
+        assert (int(dist[i]), int(end[i]), int(start[i])) == (opt(d), opt(e), opt(s)), (i, len(p), len(t))
