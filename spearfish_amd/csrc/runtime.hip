@@ -805,10 +805,17 @@ int run_myers_start(sf_batch *b, const MyersArgs &a) {
     if (q0 >= n || !myers_ukk_policy() || (b->capturing && !b->d_start_hc)) return run_myers(b, a);
     int st;
     if (!b->d_start_hc) {
+        // The stripe kernel keeps columns in 31 bits; its buffers are optional (without them, or
+        // for texts of 2^30 bases or more, the pipelines run the pass).
         uint32_t tmax = 0;
         for (uint64_t q = q0; q < n; ++q) tmax = std::max(tmax, b->tlen[b->morder[q]]);
+        if (tmax >= (1u << 30)) return run_myers(b, a);
         b->start_words = (uint64_t)tmax / 16 + 2;
-        if ((st = dalloc(&b->d_start_hc, 2 * (n - q0) * b->start_words))) return st;
+        if (hipMalloc((void **)&b->d_start_hc, 2 * (n - q0) * b->start_words * sizeof(uint32_t)) != hipSuccess) {
+            (void)hipGetLastError();
+            b->d_start_hc = nullptr;
+            return run_myers(b, a);
+        }
     }
     std::vector<sf_batch::Bin> lane;
     for (const auto &bin : b->mbins)
