@@ -427,11 +427,11 @@ int sf_set_persistent(int on);
 int sf_set_myers_ukk(int on);
 
 /* Tuning (no reference counterpart; results never depend on it): lanes per pair of affine runs.
- * policy 0 (default): per pair, by the launch planner's estimate (score-only runs may use 8 lanes
- * per pair, eight pairs per wave, where the 16-bit range holds); 1: every pair in its fewest-lanes
- * shape (8 wherever the 8-lane kernel's range holds, else 16 up to the 16-lane width limit, else
- * 64); -1: never 8, else as 0.  Default 0 (or SF_AFFINE_LP8 from the environment).  Applies to
- * affine runs started afterwards in this process. */
+ * policy 0 (default): per pair, by the launch planner's estimate (score-only runs may use 4 or 8
+ * lanes per pair, sixteen or eight pairs per wave, where the 16-bit range holds); 1: every pair in
+ * its 8-lane shape wherever the 8-lane kernel's range holds, else 16 up to the 16-lane width limit,
+ * else 64; 2: as 1 with the 4-lane shape first; -1: never 4 or 8, else as 0.  Default 0 (or
+ * SF_AFFINE_LP8 from the environment).  Applies to affine runs started afterwards in this process. */
 int sf_set_affine_lp8(int policy);
 
 #ifdef __cplusplus

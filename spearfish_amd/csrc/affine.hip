@@ -531,10 +531,26 @@ int sf_affine_pk8_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, 
     return top < 0x7C00 ? 8200 + 2 * k2 : 0;
 }
 
+int sf_affine_pk4_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m,
+                        uint32_t below) {
+    if (mis != 2 * ext) return 0;
+    int kp = 0; // slots a lane: the instantiated 4-lane shapes
+    for (int c : {36, 40, 48, 56, 64, 68, 72})
+        if (width <= 4u * (uint64_t)c) {
+            kp = c;
+            break;
+        }
+    if (!kp) return 0;
+    const int64_t dm = (int64_t)mat + mis;
+    const int64_t off = (int64_t)open + (dm + ext) * below + 2;
+    const int64_t top = off + dm * m + (int64_t)ext * 4 * kp;
+    return top < 0x7C00 ? 4200 + kp : 0;
+}
+
 static std::atomic<int> g_lp8_policy{[] {
     const char *e = getenv("SF_AFFINE_LP8");
     const int v = e ? atoi(e) : 0;
-    return v < -1 ? -1 : (v > 1 ? 1 : v);
+    return v < -1 ? -1 : (v > 2 ? 2 : v);
 }()};
 
 int sf_affine_lp8_policy() { return g_lp8_policy.load(std::memory_order_relaxed); }

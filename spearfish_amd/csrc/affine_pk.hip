@@ -7,6 +7,7 @@ hipError_t sf_launch_affine_packed(const AffineArgs &a, int cls, bool tb, hipStr
     const int LP = aff_lp(cls);
     dim3 grid((unsigned)(((uint64_t)a.count * LP + 255) / 256)), block(256);
     if (LP == 64) return sf_launch_affine_packed64(a, cls, tb, s);
+    if (LP == 4) return sf_launch_affine_packed4(a, cls, tb, s);
     if (LP == 32) {
         // 32 lanes a pair (two pairs a wave): traceback runs of bands over 512 slots.
         if (!tb || cls % 1000 >= 200) return hipErrorInvalidValue;

@@ -147,7 +147,7 @@ __device__ __forceinline__ void text_fields_next(uint32_t lo, uint32_t hi, uint3
 // The kernel's body for workgroup `blk` of a launch (a mixed launch runs several classes).
 template <int LP, int K2, bool M3>
 __device__ __forceinline__ void affine_pk16_body(const AffineArgs &a, uint32_t blk) {
-    static_assert(LP == 8 || LP == 16 || LP == 64, "8, 16 or 64 lanes a pair");
+    static_assert(LP == 4 || LP == 8 || LP == 16 || LP == 64, "4, 8, 16 or 64 lanes a pair");
     constexpr int SHL = LP == 64 ? kWaveShl1 : kRowShl1; // next lane up
     constexpr int S = 2 * K2;                     // slots per lane
     constexpr int BLK = 16;                       // rows per block: one pattern word
@@ -222,7 +222,7 @@ __device__ __forceinline__ void affine_pk16_body(const AffineArgs &a, uint32_t b
         // value: that lane takes the floor.)
         const uint32_t u0 = as1(U[0]);
         uint32_t un = dpp0<SHL>(u0);
-        if (LP == 8) un = l == LP - 1 ? 0u : un;
+        if (LP < 16) un = l == LP - 1 ? 0u : un;
         const u16x2 fK = as2(__builtin_amdgcn_alignbit(un, u0, 16));
         u16x2 X[K2], G[K2];
 #pragma unroll
@@ -280,7 +280,7 @@ __device__ __forceinline__ void affine_pk16_body(const AffineArgs &a, uint32_t b
 }
 
 template <int LP, int K2, bool M3>
-__global__ __launch_bounds__(256) void affine_pk16_kernel(AffineArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LP == 4 ? 2 : 1))) void affine_pk16_kernel(AffineArgs a) {
     affine_pk16_body<LP, K2, M3>(a, blockIdx.x);
 }
 

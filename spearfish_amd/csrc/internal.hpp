@@ -286,6 +286,14 @@ template <int LP>
 __device__ __forceinline__ uint32_t group_excl_umax(uint32_t x) {
     if constexpr (LP == 64) {
         return wave_excl_umax(x);
+    } else if constexpr (LP == 4) {
+        // Four groups share a DPP row: the input moves up a lane inside its group (the group's
+        // first lane takes 0), then is scanned inside the quad (quad permutes [0,0,1,2] and
+        // [0,1,0,1]; a lane reading itself is harmless under max).
+        uint32_t y = dpp0<kRowShr1>(x);
+        y = (threadIdx.x & 3) ? y : 0u;
+        y = umax(y, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)y, 0x90, 0xf, 0xf, false));
+        return umax(y, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)y, 0x44, 0xf, 0xf, false));
     } else if constexpr (LP == 8) {
         // Two groups share a DPP row.  The input moves up a lane inside its group (the group's
         // first lane takes 0), is scanned inside quads (quad permutes [0,0,1,2] and [0,1,0,1]: a
@@ -373,15 +381,20 @@ int sf_affine_pass_class(uint64_t width, bool lp16); // the same for the passes 
 // the traceback kernel.
 int sf_affine_pk16_class(int K, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m, uint32_t below,
                          bool tb, int lanes);
-// When score-only runs use the 8-lane packed kernel: 0 by estimate, 1 always, -1 never.
+// When score-only runs use the 4- and 8-lane packed kernels: 0 by estimate, 1 always 8 lanes (never
+// 4), 2 always 4 lanes first, -1 neither.
 int sf_affine_lp8_policy();
 void sf_affine_set_lp8_policy(int policy);
+// The 4-lane packed score class (4200 + slots a lane, 36..72) for a band of `width` slots, or 0.
+int sf_affine_pk4_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m,
+                        uint32_t below);
 // The 8-lane packed class (scores, three-term max) for a band of `width` slots, or 0.
 int sf_affine_pk8_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m,
                         uint32_t below);
 // The packed classes (aff_packed): the 16-bit score and traceback kernels.
 hipError_t sf_launch_affine_packed(const sf::AffineArgs &a, int cls, bool tb, hipStream_t s);
 hipError_t sf_launch_affine_packed64(const sf::AffineArgs &a, int cls, bool tb, hipStream_t s);
+hipError_t sf_launch_affine_packed4(const sf::AffineArgs &a, int cls, bool tb, hipStream_t s);
 // Score-only 64-lane classes that can share one launch, and that launch.
 bool sf_affine_mixable(int cls);
 hipError_t sf_launch_affine_mix(const sf::AffineArgs &a, const sf::MixArgs &mx, hipStream_t s);

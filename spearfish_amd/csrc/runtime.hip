@@ -878,14 +878,20 @@ double aff_chain_ns(int cls, bool tuned) {
 // as a whole where that adds little to the run (its 64-lane spread within 5% of the run's): its
 // pairs then run at a quarter to an eighth of the chain, and score-only runs launch all such
 // 64-lane classes together (run_bins).  Policy 1 keeps every pair in its fewest-lanes shape.
-// A pair's candidate shapes: 8, 16, 32 and 64 lanes (0 where not allowed), each at the smallest
-// class that holds its band and at the next class edge up (a tile of more slots a lane: chosen only
-// where the autotuner measured it cheaper).  Index kC64 is the 64-lane shape at its smallest class.
-constexpr int kCands = 7, kC64 = 5;
+// A pair's candidate shapes: 4, 8, 16, 32 and 64 lanes (0 where not allowed), each at the smallest
+// class that holds its band and, for 8, 16 and 64 lanes, at the next class edge up (a tile of more
+// slots a lane: chosen only where the autotuner measured it cheaper).  Index kC64 is the 64-lane
+// shape at its smallest class.  The fewest-lanes policies take the first candidate: 4 lanes under
+// policy 2, 8 under policy 1 (which leaves 4 lanes out).
+constexpr int kCands = 8, kC64 = 6;
 using Cands = std::array<int, kCands>;
 Cands pair_candidates(uint32_t m, uint64_t below, uint64_t width, const sf_scoring *sc, bool cigar, int lp8,
                       uint64_t lp16max) {
-    if (width > 64 * 32) return Cands{0, 0, 0, 0, 0, cigar ? 0 : kAffWide, 0}; // wider than any lane shape
+    if (width > 64 * 32) { // wider than any lane shape: the wide kernel, in the 64-lane fallback's place
+        Cands c{};
+        c[kC64] = cigar ? 0 : kAffWide;
+        return c;
+    }
     auto pk = [&](int c, int lanes) {
         if (!c) return 0;
         const int p = sf_affine_pk16_class(aff_k(c), sc->mat, sc->mis, sc->open, sc->ext, m, (uint32_t)below, cigar, lanes);
@@ -900,6 +906,9 @@ Cands pair_candidates(uint32_t m, uint64_t below, uint64_t width, const sf_scori
         const uint64_t nx = aff_slots(w16) + 1;
         if (nx <= lp16max && nx <= 16 * 32) c16n = pk(sf_affine_class(nx, true), 16);
     }
+    const int c4 = (!cigar && (lp8 == 0 || lp8 == 2))
+                       ? sf_affine_pk4_class(width, sc->mat, sc->mis, sc->open, sc->ext, m, (uint32_t)below)
+                       : 0;
     int c8 = 0, c8n = 0;
     if (!cigar && lp8 >= 0) {
         c8 = sf_affine_pk8_class(width, sc->mat, sc->mis, sc->open, sc->ext, m, (uint32_t)below);
@@ -915,7 +924,7 @@ Cands pair_candidates(uint32_t m, uint64_t below, uint64_t width, const sf_scori
     if (cigar || c64n == c64 || (c64n && aff_packed(c64n) != aff_packed(c64))) c64n = 0;
     if (cigar || c16n == c16 || (c16n && aff_packed(c16n) != aff_packed(c16))) c16n = 0;
     if (c8n == c8) c8n = 0;
-    return Cands{c8, c16, c32, c8n, c16n, c64, c64n};
+    return Cands{c4, c8, c16, c32, c8n, c16n, c64, c64n};
 }
 
 // The planner's costs of the classes one plan considers, each read once (the autotuner's lookups
@@ -1723,7 +1732,7 @@ int sf_batch_use_located(sf_batch *b, int on) {
 }
 
 int sf_set_affine_lp8(int policy) {
-    if (policy < -1 || policy > 1) return fail(SF_EINVAL, "policy must be -1, 0 or 1");
+    if (policy < -1 || policy > 2) return fail(SF_EINVAL, "policy must be -1, 0, 1 or 2");
     sf_affine_set_lp8_policy(policy);
     return SF_OK;
 }

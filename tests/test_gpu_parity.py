@@ -114,10 +114,11 @@ def test_affine_parity_set_global_band64():
     check_affine(ps, A.Mode.Global, A.Affine.ONT, band)
 
 
-@pytest.fixture(params=[0, 1], ids=["planned", "fewest-lanes"])
+@pytest.fixture(params=[0, 1, 2], ids=["planned", "fewest-lanes-8", "fewest-lanes-4"])
 def lane_policy(request):
     """Lanes per pair as the launch planner picks them (small batches: mostly 64, every pair's
-    rows outlasting the rest), or every pair in its fewest-lanes shape (8 or 16), then back."""
+    rows outlasting the rest), or every pair in its fewest-lanes shape (8 or 16; or 4 first),
+    then back."""
     lib = A.N.load()
     A.N.check(lib.sf_set_affine_lp8(request.param))
     yield request.param
@@ -252,6 +253,44 @@ def test_affine_scores_8_lane_kernel(mode, lanes8):
         P.append(p)
         T.append(t)
         B.append(rng.choice([0, 1, 3, 7, 8, 15, 16, 40, 64, 100, 127, 128, 143]))
+    if mode != A.Mode.Infix:
+        for m in (5150, 5250, 5350):
+            t = [rng.randrange(4) for _ in range(m)]
+            P.append([x if rng.random() > 0.05 else rng.randrange(4) for x in t])
+            T.append(t)
+            B.append(16)
+    ps = A.PairSet.from_seqs(P, T)
+    for sc in (A.Affine.ONT, A.Affine.HIFI, A.Affine(0, 2, 0, 1), A.Affine(3, 2, 7, 1)):
+        check_affine(ps, mode, sc, np.array(B, np.uint64), cigar=False)
+
+
+@pytest.fixture
+def lanes4():
+    """Score-only packed runs on 4 lanes per pair wherever that kernel's range holds (bands of 1
+    to 288 slots: 36 to 72 slots a lane), then the planner's choice back."""
+    lib = A.N.load()
+    A.N.check(lib.sf_set_affine_lp8(2))
+    yield
+    A.N.check(lib.sf_set_affine_lp8(0))
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_affine_scores_4_lane_kernel(mode, lanes4):
+    # Band widths over every 4-lane shape (36 .. 72 slots a lane, up to 288 slots), random and
+    # gap-heavy pairs, lengths around the value limit (m near 5,200 for ONT at band 16).
+    rng = random.Random(61 + int(mode))
+    P, T, B = [], [], []
+    nmax = 140 if mode == A.Mode.Infix else 900
+    for _ in range(500):
+        n = rng.randrange(0, nmax)
+        t = [rng.randrange(4) for _ in range(n)]
+        p = [x if rng.random() > 0.1 else rng.randrange(4) for x in t]
+        for _ in range(rng.randrange(0, 4)):
+            k = rng.randrange(len(p) + 1)
+            p = p[:k] + [rng.randrange(4) for _ in range(rng.randrange(0, 9))] + p[k + rng.randrange(0, 9):]
+        P.append(p)
+        T.append(t)
+        B.append(rng.choice([0, 1, 17, 40, 64, 71, 72, 80, 100, 111, 112, 127, 128, 135, 136, 143]))
     if mode != A.Mode.Infix:
         for m in (5150, 5250, 5350):
             t = [rng.randrange(4) for _ in range(m)]
