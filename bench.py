@@ -71,9 +71,12 @@ TB_EXT_VALU_INSTS = 1.5311e9
 
 # Issue-rate probe (sf_probe_issue): waves of 32 independent VALU instructions of one encoding.
 PROBE_KINDS = {"vop2_i32": 0, "vop3p_pk16": 1, "dpp": 2, "vop3_i32": 3}
+# The probe workload's further kinds: which instruction streams the SIMD issues two a quad-cycle.
+PROBE_PAIRING_KINDS = {"bitop3_alone": 4, "alignbit_alone": 5, "or7_alignbit1": 6, "or31_alignbit1": 7,
+                       "add_co_addc_pairs": 8}
 
 
-def issue_probe(blocks=8192, iters=20000):
+def issue_probe(blocks=8192, iters=20000, kinds=PROBE_KINDS):
     """Measured VALU issue rates on this GPU, in T lane-instructions/s, per encoding: two-operand
     32-bit ops (VOP2), packed 16-bit ops (VOP3P, the affine kernels'), DPP moves, three-operand
     32-bit ops (VOP3).  About 35 ms of GPU time per kind."""
@@ -82,7 +85,7 @@ def issue_probe(blocks=8192, iters=20000):
     lib = N.load_diag()
     ms = C.c_float()
     out = {}
-    for name, kind in PROBE_KINDS.items():
+    for name, kind in kinds.items():
         N.check_diag(lib.sf_probe_issue(kind, iters, blocks, C.byref(ms)))
         out[name] = round(blocks * 4 * iters * 32 * 64 / (ms.value * 1e-3) / 1e12, 2)
     return out
@@ -725,7 +728,7 @@ def main():
         sys.exit(2)
     if args.workload == "probe":
         # The issue-rate probe alone (the denominators of the VALU rooflines).
-        print(json.dumps({"issue_probe_t_lane_instructions_per_s": issue_probe(),
+        print(json.dumps({"issue_probe_t_lane_instructions_per_s": issue_probe(kinds={**PROBE_KINDS, **PROBE_PAIRING_KINDS}),
                           "nominal_one_wave_instruction_per_4_cycles": round(VALU_ISSUE_PEAK_T, 1)}))
         return
     if args.workload != "uniform":

@@ -27,7 +27,9 @@ int sf_selftest_lanes(int32_t *out);
 /* The VALU issue rate the kernels' rooflines are measured against.  Runs `blocks` workgroups of
  * 4 waves, each wave `iters` x 32 independent VALU instructions of one kind (0: two-operand 32-bit
  * integer ops; 1: packed 16-bit ops as in the affine kernels; 2: DPP lane moves; 3: three-operand
- * 32-bit integer ops as in the Myers step), once untimed, then once timed: *ms. */
+ * 32-bit integer ops as in the Myers step; 4: v_bitop3 alone; 5: v_alignbit alone; 6: seven v_or
+ * and one v_alignbit; 7: thirty-one v_or and one v_alignbit; 8: v_add_co / v_addc pairs through
+ * VCC, with the one wait state between them), once untimed, then once timed: *ms. */
 int sf_probe_issue(int kind, uint32_t iters, uint32_t blocks, float *ms);
 
 #ifdef __cplusplus
