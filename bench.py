@@ -62,7 +62,7 @@ VALU_INSTS_PER_CELL = {"score": 1.63724e10 / 2.46731e11, "traceback": 5.4447e10 
 # FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the same counters on a 1 GiB copy
 # (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.  Score: the
 # step's affine phase; traceback: the average of the DP's four launches a step.
-TRAFFIC_BYTES = {"score": 1_338_685_093, "traceback": 49_847_008_272}
+TRAFFIC_BYTES = {"score": 728_007_182, "traceback": 49_847_008_272}
 # Issued VALU wave-instructions of one extension pass of the tie-break workload (its four
 # affine_pass_kernel launches), from a rocprofv3 --pmc SQ_INSTS_VALU run of this script
 # (profiles/tiebreak_20k_reads_passes.md).

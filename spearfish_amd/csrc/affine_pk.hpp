@@ -45,7 +45,10 @@ __device__ __forceinline__ void pk16_result(const AffineArgs &a, const u16x2 (&V
                                             int lane, int l, int32_t m, int32_t n, int32_t below, int32_t Wd,
                                             int32_t last, int32_t off) {
     constexpr int S = 2 * K2;
-    const int32_t s0 = l * S;
+    // The lane's first slot, made opaque: its slot numbers s0 + wi are then computed here rather
+    // than kept (or spilled) across the row loop from the start-up code.
+    int32_t s0 = l * S;
+    asm volatile("" : "+v"(s0));
     const int32_t ext = a.ext;
     const bool stopped = last < m;
     int32_t score;

@@ -249,9 +249,13 @@ __device__ __forceinline__ void affine_wave_body(const AffineArgs &a, uint32_t b
         const int32_t slo = lo - b, shi = hi - b;
         int32_t bv = INT32_MIN;
         int32_t bs = INT32_MAX;
+        // The lane's first slot, made opaque: its slot numbers are then computed here rather than
+        // kept (or spilled) across the row loop from the start-up code.
+        int32_t s0r = s0;
+        asm volatile("" : "+v"(s0r));
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            const int32_t s = s0 + q;
+            const int32_t s = s0r + q;
             const int32_t h = (int32_t)(V[q] - kOff) - ext * s - rowbias;
             if (s >= slo && s <= shi && h > bv) {
                 bv = h;

@@ -466,10 +466,11 @@ def test_issue_probe_runs_every_kind():
     import ctypes as C
     lib = A.N.load_diag()
     ms = C.c_float()
-    for kind in range(4):
+    for kind in range(9):
         A.N.check_diag(lib.sf_probe_issue(kind, 200, 64, C.byref(ms)))
         assert ms.value > 0
-    assert lib.sf_probe_issue(4, 200, 64, C.byref(ms)) != 0
+    assert lib.sf_probe_issue(9, 200, 64, C.byref(ms)) != 0
+    assert lib.sf_probe_issue(-1, 200, 64, C.byref(ms)) != 0
 
 
 # Scoring values past the 16-bit range (the reference takes any i32 values, `align.rs:660-691`):
