@@ -1,7 +1,8 @@
 """GPU parity on the uniform workload at its full size: 1,000,000 pairs of 1,000 bp targets with
 queries at 6% error, band 128, exactly as the benchmark runs it.  At this size the launch planner
-picks the production shapes: the 8-lane packed affine kernels (eight pairs a wave) and the Myers
-lane kernel's full-wave path for 16-block patterns, launched alone.  Every pair's distance and
+picks the production shapes: the 4-lane packed affine score kernels (sixteen pairs a wave), the
+16-lane packed traceback kernel, and the Myers lane kernel's full-wave path for 16-block patterns,
+launched alone.  Every pair's distance and
 affine score are compared with the oracle on all host threads; ends, start columns and CIGARs on a
 strided sample.
 """
