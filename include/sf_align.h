@@ -434,6 +434,13 @@ int sf_set_myers_ukk(int on);
  * SF_AFFINE_LP8 from the environment).  Applies to affine runs started afterwards in this process. */
 int sf_set_affine_lp8(int policy);
 
+/* Tuning (no reference counterpart; results never depend on it): the passes with per-row
+ * bookkeeping (sf_affine_dp, sf_batch_affine_dp and the calls built on them) run in the 16-bit
+ * packed layout where its range holds -- as the reference runs them in 16 bits where `narrow`
+ * holds (align.rs:2381-2426) -- (1, default, or SF_AFFINE_PASS_PACKED from the environment), or
+ * in 32 bits throughout (0).  Applies to passes started afterwards in this process. */
+int sf_set_affine_pass_packed(int on);
+
 #ifdef __cplusplus
 }
 #endif

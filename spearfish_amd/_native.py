@@ -215,6 +215,7 @@ SIGNATURES = {
     "sf_tb_batch_timing": ([_vp, C.POINTER(TbTiming), _int], _int),
     "sf_set_affine_lp16_max": ([C.c_uint64], _int),
     "sf_set_affine_lp8": ([C.c_int], _int),
+    "sf_set_affine_pass_packed": ([C.c_int], _int),
     "sf_set_myers_ukk": ([C.c_int], _int),
 }
 

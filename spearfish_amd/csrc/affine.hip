@@ -547,6 +547,31 @@ int sf_affine_pk4_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, 
     return top < 0x7C00 ? 4200 + kp : 0;
 }
 
+int sf_affine_pass_pk_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m,
+                            uint32_t below) {
+    if (mis != 2 * ext) return 0;
+    int k2 = 0; // registers a lane: the instantiated packed pass shapes, 16 lanes x 2 slots each
+    for (int c : {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16})
+        if (width <= 32u * (uint64_t)c) {
+            k2 = c;
+            break;
+        }
+    if (!k2) return 0;
+    const int64_t dm = (int64_t)mat + mis;
+    const int64_t off = (int64_t)open + (dm + ext) * below + 2;
+    const int64_t top = off + dm * m + (int64_t)ext * 32 * k2;
+    if (top < 0x7C00) return 16200 + 2 * k2;
+    return top <= 65535 ? 16100 + 2 * k2 : 0;
+}
+
+static std::atomic<int> g_pass_packed{[] {
+    const char *e = getenv("SF_AFFINE_PASS_PACKED");
+    return e ? (atoi(e) != 0 ? 1 : 0) : 1;
+}()};
+
+int sf_affine_pass_packed() { return g_pass_packed.load(std::memory_order_relaxed); }
+void sf_affine_set_pass_packed(int on) { g_pass_packed.store(on ? 1 : 0, std::memory_order_relaxed); }
+
 static std::atomic<int> g_lp8_policy{[] {
     const char *e = getenv("SF_AFFINE_LP8");
     const int v = e ? atoi(e) : 0;
@@ -607,7 +632,7 @@ hipError_t sf_launch_desc(uint64_t n, const uint32_t *order, const PairsView &pv
 
 hipError_t sf_launch_pass(const PassArgs &a, int cls, bool local, hipStream_t s) {
     if (a.count == 0) return hipSuccess;
-    if (aff_packed(cls)) return hipErrorInvalidValue;
+    if (aff_packed(cls)) return local ? hipErrorInvalidValue : sf_launch_pass_packed(a, cls, s);
     const int LP = aff_lp(cls);
     dim3 grid((unsigned)(((uint64_t)a.count * LP + 255) / 256)), block(256);
     switch (cls) {

@@ -144,6 +144,79 @@ __device__ __forceinline__ void text_fields_next(uint32_t lo, uint32_t hi, uint3
         W[NF - 16 + k] = (((k < 8 ? wa : wb) >> (2 * (k % 8))) & 0x00030003u) | 0x0c000c00u;
 }
 
+// Row 0 of a pair's band in the packed layout: V (H + bias), U (the up-values the next row reads)
+// and dl (`open`, or the floor right of the band).  G = X - open (X = max(diagonal, F): H before
+// the row's horizontal gaps) is both the row's gap-open term for E and, as U = max(G, F), the next
+// row's vertical term: F never opens from E, a horizontal gap turning straight into a vertical
+// one.  With mis = 2 ext that turn never scores more than cutting its corner: a horizontal gap of
+// a and a vertical gap of b cost 2 open + ext (a + b); a - 1, a diagonal step and b - 1 cost at
+// most 2 open + ext (a + b - 2) + mis, the same, over cells the first path used.  So every H (and
+// the score) is unchanged.  Right of the band G is floored, as the up-values there
+// (align.rs:1047-1048).
+template <int K2>
+__device__ __forceinline__ void pk16_init(u16x2 (&V)[K2], u16x2 (&U)[K2], u16x2 (&dl)[K2], int32_t s0, int32_t below,
+                                          int32_t above, int32_t n, int32_t Wd, int32_t open, int32_t ext,
+                                          bool infix, int32_t off) {
+    const int32_t hi0 = above < n ? above : n;
+#pragma unroll
+    for (int r = 0; r < K2; ++r) {
+        uint32_t hv[2], dv[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int32_t s = s0 + r + h * K2;
+            const int32_t j = s - below;
+            const bool in = j >= 0 && j <= hi0;
+            hv[h] = in ? (uint32_t)(((j == 0 || infix) ? 0 : -(open + ext * j)) + ext * s + off) : 0u;
+            dv[h] = s >= Wd ? 0xFFFFu : (uint32_t)open;
+        }
+        V[r] = as2(hv[0] | (hv[1] << 16));
+        dl[r] = as2(dv[0] | (dv[1] << 16));
+        U[r] = psub(V[r], dl[r]); // row 0: the boundary's H opens the gaps, as the reference's
+    }
+}
+
+// One row of the packed recurrence: row r of the block whose 16 pattern codes are `pw` (r static
+// after unrolling, so every text field index is a constant), lane l of its group of LP lanes.
+template <int LP, int K2, bool M3>
+__device__ __forceinline__ void pk16_row(u16x2 (&V)[K2], u16x2 (&U)[K2], const u16x2 (&dl)[K2],
+                                         const uint32_t (&W)[K2 + 15], uint32_t pw, const int r, int l,
+                                         u16x2 vdm) {
+    constexpr int SHL = LP == 64 ? kWaveShl1 : kRowShl1; // next lane up
+    const uint32_t onehot = 1u << (__builtin_amdgcn_ubfe(pw, 2 * r, 2) << 3);
+    // Register K2 - 1: low half reads slot s0 + K2 (register 0, high half), high half reads slot
+    // s0 + 2 K2 (the next lane's register 0, low half).  (With 4 or 8 lanes a pair the row shift
+    // hands a group's top lane the next group's value: that lane takes the floor.)
+    const uint32_t u0 = as1(U[0]);
+    uint32_t un = dpp0<SHL>(u0);
+    if (LP < 16) un = l == LP - 1 ? 0u : un;
+    const u16x2 fK = as2(__builtin_amdgcn_alignbit(un, u0, 16));
+    u16x2 X[K2], G[K2];
+#pragma unroll
+    for (int q = 0; q < K2; ++q) {
+        const uint32_t mt = __builtin_amdgcn_perm(0u, onehot, W[q + r]); // match bits
+        const u16x2 d = V[q] + as2(mt) * vdm;
+        const u16x2 f = q < K2 - 1 ? U[q + 1] : fK; // F of this row
+        X[q] = pmax(d, f);
+        G[q] = psub(X[q], dl[q]);
+        U[q] = pmax(G[q], f); // in place: register q - 1 has read U[q] already
+    }
+    // Horizontal gaps: Pre[q] = max(G[0 .. q-1]), the lane's own prefix, is built while the group
+    // scan runs; then every register takes max(X, Pre, gaps entering the lane) with no chain
+    // after the scan.
+    u16x2 Pre[K2];
+    Pre[1 % K2] = G[0];
+#pragma unroll
+    for (int q = 2; q < K2; ++q) Pre[q] = pmax(Pre[q - 1], G[q - 1]);
+    const u16x2 tot = K2 > 1 ? pmax(Pre[K2 - 1], G[K2 - 1]) : G[0];
+    const uint32_t tt = as1(tot);
+    const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
+    const uint32_t C = group_excl_umax<LP>(tlo > thi ? tlo : thi);
+    const u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16)); // gap values entering register 0
+    V[0] = pmax(X[0], E);
+#pragma unroll
+    for (int q = 1; q < K2; ++q) V[q] = M3 ? pmax3(X[q], Pre[q], E) : pmax(X[q], pmax(Pre[q], E));
+}
+
 // LP lanes per pair: 16 (four pairs a wave) or 8 (eight pairs a wave: the per-row work that every
 // lane repeats -- scan, neighbour move, pattern code -- is shared by twice the slots).  M3: every
 // value is below 0x7C00 (checked on the host), so H takes its three terms in one max.
@@ -151,7 +224,6 @@ __device__ __forceinline__ void text_fields_next(uint32_t lo, uint32_t hi, uint3
 template <int LP, int K2, bool M3>
 __device__ __forceinline__ void affine_pk16_body(const AffineArgs &a, uint32_t blk) {
     static_assert(LP == 4 || LP == 8 || LP == 16 || LP == 64, "4, 8, 16 or 64 lanes a pair");
-    constexpr int SHL = LP == 64 ? kWaveShl1 : kRowShl1; // next lane up
     constexpr int S = 2 * K2;                     // slots per lane
     constexpr int BLK = 16;                       // rows per block: one pattern word
     constexpr int NF = K2 + BLK - 1;              // text fields of a block: q + r, q < K2, r < 16
@@ -175,30 +247,8 @@ __device__ __forceinline__ void affine_pk16_body(const AffineArgs &a, uint32_t b
     const int32_t s0 = l * S;
     const u16x2 vdm = (u16x2)((uint16_t)dm);
 
-    // G = X - open (X = max(diagonal, F): H before the row's horizontal gaps) is both the row's
-    // gap-open term for E and, as U = max(G, F), the next row's vertical term: F never opens from
-    // E, a horizontal gap turning straight into a vertical one.  With mis = 2 ext that turn never scores
-    // more than cutting its corner: a horizontal gap of a and a vertical gap of b cost
-    // 2 open + ext (a + b); a - 1, a diagonal step and b - 1 cost at most 2 open + ext (a + b - 2)
-    // + mis, the same, over cells the first path used.  So every H (and the score) is unchanged.
-    // Right of the band G is floored, as the up-values there (align.rs:1047-1048).
     u16x2 V[K2], U[K2], dl[K2];
-    const int32_t hi0 = above < n ? above : n;
-#pragma unroll
-    for (int r = 0; r < K2; ++r) {
-        uint32_t hv[2], dv[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int32_t s = s0 + r + h * K2;
-            const int32_t j = s - below;
-            const bool in = j >= 0 && j <= hi0;
-            hv[h] = in ? (uint32_t)(((j == 0 || infix) ? 0 : -(open + ext * j)) + ext * s + off) : 0u;
-            dv[h] = s >= Wd ? 0xFFFFu : (uint32_t)open;
-        }
-        V[r] = as2(hv[0] | (hv[1] << 16));
-        dl[r] = as2(dv[0] | (dv[1] << 16));
-        U[r] = psub(V[r], dl[r]); // row 0: the boundary's H opens the gaps, as the reference's
-    }
+    pk16_init<K2>(V, U, dl, s0, below, above, n, Wd, open, ext, infix, off);
     const int32_t last = valid ? (m < n + below ? m : n + below) : 0;
     int32_t rows = last, rows_all = last; // the wave's longest pair's rows, and rows every pair has
 #pragma unroll
@@ -216,43 +266,7 @@ __device__ __forceinline__ void affine_pk16_body(const AffineArgs &a, uint32_t b
     uint32_t W[NF];
     text_fields<K2>(T, -below + s0, W);
     uint32_t pw = 0; // the block's 16 pattern codes
-    // One row, r static after unrolling (so every field index is a constant).
-    auto row = [&](const int r) {
-        const uint32_t onehot = 1u << (__builtin_amdgcn_ubfe(pw, 2 * r, 2) << 3);
-        // Register K2 - 1: low half reads slot s0 + K2 (register 0, high half), high half
-        // reads slot s0 + 2 K2 (the next lane's register 0, low half).
-        // (With 8 lanes a pair the row shift hands a group's top lane the next group's
-        // value: that lane takes the floor.)
-        const uint32_t u0 = as1(U[0]);
-        uint32_t un = dpp0<SHL>(u0);
-        if (LP < 16) un = l == LP - 1 ? 0u : un;
-        const u16x2 fK = as2(__builtin_amdgcn_alignbit(un, u0, 16));
-        u16x2 X[K2], G[K2];
-#pragma unroll
-        for (int q = 0; q < K2; ++q) {
-            const uint32_t mt = __builtin_amdgcn_perm(0u, onehot, W[q + r]); // match bits
-            const u16x2 d = V[q] + as2(mt) * vdm;
-            const u16x2 f = q < K2 - 1 ? U[q + 1] : fK; // F of this row
-            X[q] = pmax(d, f);
-            G[q] = psub(X[q], dl[q]);
-            U[q] = pmax(G[q], f); // in place: register q - 1 has read U[q] already
-        }
-        // Horizontal gaps: Pre[q] = max(G[0 .. q-1]), the lane's own prefix, is built while
-        // the group scan runs; then every register takes max(X, Pre, gaps entering the lane)
-        // with no chain after the scan.
-        u16x2 Pre[K2];
-        Pre[1 % K2] = G[0];
-#pragma unroll
-        for (int q = 2; q < K2; ++q) Pre[q] = pmax(Pre[q - 1], G[q - 1]);
-        const u16x2 tot = K2 > 1 ? pmax(Pre[K2 - 1], G[K2 - 1]) : G[0];
-        const uint32_t tt = as1(tot);
-        const uint32_t tlo = tt & 0xFFFFu, thi = tt >> 16;
-        const uint32_t C = group_excl_umax<LP>(tlo > thi ? tlo : thi);
-        const u16x2 E = as2(C | ((C > tlo ? C : tlo) << 16)); // gap values entering register 0
-        V[0] = pmax(X[0], E);
-#pragma unroll
-        for (int q = 1; q < K2; ++q) V[q] = M3 ? pmax3(X[q], Pre[q], E) : pmax(X[q], pmax(Pre[q], E));
-    };
+    auto row = [&](const int r) { pk16_row<LP, K2, M3>(V, U, dl, W, pw, r, l, vdm); };
     // A block's pattern codes, and the next block's new text codes (loaded a block ahead).
     auto block_loads = [&](int32_t i0, uint32_t &nlo, uint32_t &nhi) {
         pw = codes16(P, i0 - 1);

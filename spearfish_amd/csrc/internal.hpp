@@ -376,6 +376,14 @@ hipError_t sf_launch_affine(const sf::AffineArgs &a, int K, bool tb, hipStream_t
 uint64_t sf_affine_lp16_max(); // the current 16-lane limit (slots), part of the shape choice
 int sf_affine_class(uint64_t width, bool lp16);      // 16 lanes per pair if allowed and it fits, else 64
 int sf_affine_pass_class(uint64_t width, bool lp16); // the same for the passes with bookkeeping
+// The 16-bit packed pass class (16 lanes; 16100 + slots a lane, 16200 + slots with the three-term
+// max) for a band of `width` slots, or 0 (mis != 2 ext, wider than 512 slots, or past the range).
+int sf_affine_pass_pk_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m,
+                            uint32_t below);
+// Whether passes with bookkeeping use the packed kernel where its range holds (1, default) or the
+// 32-bit one throughout (0).
+int sf_affine_pass_packed();
+void sf_affine_set_pass_packed(int on);
 // The 16-bit packed class for a pair of the 32-bit class with `lanes` (16 or 64) lanes and K slots
 // per lane, or 0 where its exact range does not hold (the scoring must have mis = 2 ext); tb: for
 // the traceback kernel.
@@ -433,6 +441,7 @@ hipError_t sf_launch_walk(const sf::WalkArgs &a, hipStream_t s);
 hipError_t sf_launch_affine_wide(const sf::WideArgs &a, hipStream_t s);
 // cls: a 32-bit affine class (16 or 64 lanes); local selects local_band_core.
 hipError_t sf_launch_pass(const sf::PassArgs &a, int cls, bool local, hipStream_t s);
+hipError_t sf_launch_pass_packed(const sf::PassArgs &a, int cls, hipStream_t s); // affine_pk_pass.hip
 // desc[w] of pair order[w], w < n (ba may be null: no band).
 hipError_t sf_launch_desc(uint64_t n, const uint32_t *order, const sf::PairsView &pv, const uint64_t *ba,
                           sf::PairDesc *desc, hipStream_t s);

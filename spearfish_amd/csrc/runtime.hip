@@ -148,6 +148,23 @@ void give_stream(hipStream_t s) {
 // Words of one traceback scratch buffer (two are used, alternately): SF_TB_BUDGET_MB, else a
 // quarter of the device's memory, at most 64 GiB.  Larger chunks run more walks at once, and the
 // walk (one dependent chain per pair) needs that concurrency.
+// A launch plan's cache key over a per-pair array (null: every pair takes `absent`): eight
+// independent multiply-xor chains, folded.  One serial chain costs a multiply's latency an
+// element, which made the key of a 10^5-pair batch a third of a millisecond of host time a call.
+uint64_t hash_words(uint64_t key, const uint64_t *v, uint64_t n, uint64_t absent) {
+    constexpr uint64_t kP = 1099511628211ull;
+    if (!v) return ((key ^ absent) * kP ^ n) * kP;
+    uint64_t h[8];
+    for (int k = 0; k < 8; ++k) h[k] = key + 0x9e3779b97f4a7c15ull * (uint64_t)(k + 1);
+    uint64_t i = 0;
+    for (; i + 8 <= n; i += 8)
+        for (int k = 0; k < 8; ++k) h[k] = (h[k] ^ v[i + k]) * kP;
+    for (; i < n; ++i) h[i & 7] = (h[i & 7] ^ v[i]) * kP;
+    uint64_t r = key ^ n;
+    for (int k = 0; k < 8; ++k) r = (r ^ h[k]) * kP;
+    return r;
+}
+
 uint64_t tb_budget_words() {
     uint64_t mb = 0;
     if (const char *e = std::getenv("SF_TB_BUDGET_MB")) mb = std::strtoull(e, nullptr, 10);
@@ -1465,10 +1482,7 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
                     ((uint64_t)(sf_affine_lp8_policy() + 1) << 4) ^ (cigar ? 0 : sf_tune_generation() << 24)) *
                    1099511628211ull;
     for (int32_t v : {sc->mat, sc->mis, sc->open, sc->ext}) key = (key ^ (uint64_t)(uint32_t)v) * 1099511628211ull;
-    if (band)
-        for (uint64_t i = 0; i < n; ++i) key = (key ^ band[i]) * 1099511628211ull;
-    else
-        key ^= 0x9e3779b97f4a7c15ull;
+    key = hash_words(key, band, n, SF_BAND_EXACT);
     if ((!pl.valid || pl.key != key) && (st = refuse_in_capture(b, "an affine run that needs a new launch plan")))
         return st;
     if (!pl.valid || pl.key != key) {
@@ -1731,6 +1745,12 @@ int sf_batch_use_located(sf_batch *b, int on) {
     return SF_OK;
 }
 
+int sf_set_affine_pass_packed(int on) {
+    if (on != 0 && on != 1) return fail(SF_EINVAL, "on must be 0 or 1");
+    sf_affine_set_pass_packed(on);
+    return SF_OK;
+}
+
 int sf_set_affine_lp8(int policy) {
     if (policy < -1 || policy > 2) return fail(SF_EINVAL, "policy must be -1, 0, 1 or 2");
     sf_affine_set_lp8_policy(policy);
@@ -1969,13 +1989,12 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
     if ((st = join_aux(b))) return st;
     const uint64_t n = b->n;
     sf_batch::AffPlan &pl = b->aplan[2];
-    uint64_t key = (0x51ed270b27f1a3c5ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8)) * 1099511628211ull;
-    // (The scoring is in the key: the plan's range check depends on it.)
+    const bool packed = sf_affine_pass_packed() != 0;
+    uint64_t key = (0x51ed270b27f1a3c5ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8) ^ ((uint64_t)packed << 62)) *
+                   1099511628211ull;
+    // (The scoring is in the key: the plan's range check and the packed classes depend on it.)
     for (int32_t v : {sc->mat, sc->mis, sc->open, sc->ext}) key = (key ^ (uint64_t)(uint32_t)v) * 1099511628211ull;
-    for (uint64_t i = 0; i < n; ++i) {
-        key = (key ^ (band ? band[i] : SF_BAND_EXACT)) * 1099511628211ull;
-        key = (key ^ (geom ? geom[i] : 0x9e3779b97f4a7c15ull)) * 1099511628211ull;
-    }
+    key = hash_words(hash_words(key, band, n, SF_BAND_EXACT), geom, n, 0x9e3779b97f4a7c15ull);
     if (!pl.valid || pl.key != key) {
         std::vector<uint64_t> ba(n);
         std::vector<int> kcls(n);
@@ -1987,6 +2006,10 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
             sf::band_geom_for(m, t, geom ? geom[i] : t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
             const uint64_t width = below + above + 1;
             kcls[i] = sf_affine_pass_class(width, width <= sf_affine_lp16_max());
+            // The 16-bit packed pass where its range holds (16 lanes, up to 512 slots).
+            if (packed && kcls[i] != 0 && width <= sf_affine_lp16_max())
+                if (const int pc = sf_affine_pass_pk_class(width, sc->mat, sc->mis, sc->open, sc->ext, m, (uint32_t)below))
+                    kcls[i] = pc;
             if (kcls[i] == 0 && width <= kAffWideMax) kcls[i] = kAffWide;
             if (kcls[i] == 0)
                 return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this library handles up to 2^26",
@@ -2004,9 +2027,7 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
     // The cuts stay on the device for the next run with the same ones (an upload waits for the
     // batch's work, which may still read the previous ones).
     if (cut) {
-        uint64_t ck = 0x243f6a8885a308d3ull;
-        for (uint64_t i = 0; i < n; ++i) ck = (ck ^ cut[i]) * 1099511628211ull;
-        ck |= 1;
+        const uint64_t ck = hash_words(0x243f6a8885a308d3ull, cut, n, 0) | 1;
         if (ck != b->cut_key) {
             if ((st = quiesce(b))) return st;
             b->cut_key = 0;
@@ -2074,10 +2095,8 @@ int sf_batch_local_band(sf_batch *b, const sf_scoring *sc, const int64_t *dlo, c
     const uint64_t n = b->n;
     sf_batch::AffPlan &pl = b->aplan[3];
     uint64_t key = (0x2545f4914f6cdd1dull ^ (sf_affine_lp16_max() << 8)) * 1099511628211ull;
-    for (uint64_t i = 0; i < n; ++i) {
-        key = (key ^ (uint64_t)dlo[i]) * 1099511628211ull;
-        key = (key ^ (uint64_t)dhi[i]) * 1099511628211ull;
-    }
+    key = hash_words(hash_words(key, reinterpret_cast<const uint64_t *>(dlo), n, 0),
+                     reinterpret_cast<const uint64_t *>(dhi), n, 0);
     if (!pl.valid || pl.key != key) {
         std::vector<uint64_t> ba(n);
         std::vector<int> kcls(n);

@@ -225,3 +225,74 @@ def test_local_band_random():
         for i in range(len(P)):
             w = O.local_band_ends(P[i], T[i], O.ONT, LO[i], HI[i], (b0, b1))
             assert int(got[i]) == w, (i, len(P[i]), len(T[i]), LO[i], HI[i], b0, b1)
+
+
+@pytest.fixture
+def pass_policy():
+    """Passes in the 16-bit packed layout where its range holds (the default), or in 32 bits."""
+    lib = A.N.load()
+
+    def use(packed):
+        A.N.check(lib.sf_set_affine_pass_packed(1 if packed else 0))
+
+    yield use
+    A.N.check(lib.sf_set_affine_pass_packed(1))
+
+
+@pytest.mark.parametrize("mode", [A.Mode.Prefix, A.Mode.Global, A.Mode.Infix])
+def test_affine_dp_packed_pass_equals_32bit_and_oracle(mode, pass_policy):
+    # The packed pass (affine_pk_pass.hip) against the 32-bit pass on the same inputs, every Dp
+    # field: bands from 0 to the packed pass's 512 slots (every class, 2 to 32 slots a lane),
+    # band layouts for other text lengths, cuts inside, at and past the text, X-drops that stop
+    # early or never, the best cell tracked or not; and against the oracle on a sample.
+    rng = random.Random(131 + int(mode))
+    maxlen = 150 if mode == A.Mode.Infix else 500
+    P, T = extension_pairs(rng, 360, maxlen)
+    n = len(P)
+    bands = np.array([rng.choice([0, 1, 7, 15, 16, 17, 31, 40, 63, 100, 127, 200, 255]) for _ in range(n)], np.uint64)
+    geoms = np.array([max(0, len(T[i]) + rng.choice([-40, -3, 0, 0, 9, 120])) for i in range(n)], np.uint64)
+    cuts = np.array([rng.choice([0, 3, len(T[i]) // 3, len(T[i]), len(T[i]) + 5, UMAX]) for i in range(n)], np.uint64)
+    ps = A.PairSet.from_seqs(P, T)
+    for sc in (ONT, A.Affine.HIFI):
+        for clip in ((0, True), (25, True), (50, False), (IMAX, True), None):
+            for use_geom, use_cut in ((False, False), (True, True), (False, True)):
+                args = (ps, mode, sc, bands, geoms if use_geom else None, clip, cuts if use_cut else None)
+                pass_policy(True)
+                got = A.affine_dp_batch(*args)
+                pass_policy(False)
+                ref = A.affine_dp_batch(*args)
+                for i in range(n):
+                    assert tuple(got[i]) == tuple(ref[i]), (i, len(P[i]), len(T[i]), int(bands[i]), sc, clip,
+                                                            use_geom, use_cut)
+                for i in range(0, n, 17):
+                    want = O.affine_dp(P[i], T[i], int(mode), (sc.mat, sc.mis, sc.open, sc.ext), int(bands[i]),
+                                       int(geoms[i]) if use_geom else None, clip,
+                                       int(cuts[i]) if use_cut else None)
+                    w = (want.score, want.score_col, want.clip_best, want.clip_i, want.clip_j, want.text_read,
+                         want.last_row, want.cut_best, want.cut_score)
+                    assert tuple(got[i]) == w, (i, sc, clip, use_geom, use_cut)
+
+
+def test_affine_dp_packed_pass_range_edges(pass_policy):
+    # Reads around the packed pass's range limits for ONT at band 16 (three-term max below
+    # 0x7C00: m near 5,200; 16 bits: m near 10,900; past it the 32-bit pass), a long match
+    # then junk, so X-drop stops late or never; equal to the 32-bit pass and to the oracle.
+    rng = random.Random(137)
+    P, T = [], []
+    for m in (5100, 5250, 5400, 10700, 10900, 11100):
+        t = [rng.randrange(4) for _ in range(m + 40)]
+        p = [x if rng.random() > 0.04 else rng.randrange(4) for x in t[:m - 200]] + \
+            [rng.randrange(4) for _ in range(200)]
+        P.append(p)
+        T.append(t)
+    ps = A.PairSet.from_seqs(P, T)
+    for clip in ((50, True), (IMAX, True)):
+        pass_policy(True)
+        got = A.affine_dp_batch(ps, A.Mode.Prefix, ONT, 16, None, clip, None)
+        pass_policy(False)
+        ref = A.affine_dp_batch(ps, A.Mode.Prefix, ONT, 16, None, clip, None)
+        for i in range(len(P)):
+            assert tuple(got[i]) == tuple(ref[i]), (i, len(P[i]), clip)
+            want = O.affine_dp(P[i], T[i], O.PREFIX, O.ONT, 16, None, clip, None)
+            assert (got[i].score, got[i].clip_best, got[i].clip_i, got[i].clip_j, got[i].last_row) == \
+                (want.score, want.clip_best, want.clip_i, want.clip_j, want.last_row), (i, clip)
