@@ -63,10 +63,10 @@ VALU_INSTS_PER_CELL = {"score": 1.63724e10 / 2.46731e11, "traceback": 5.4447e10 
 # (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.  Score: the
 # step's affine phase; traceback: the average of the DP's four launches a step.
 TRAFFIC_BYTES = {"score": 728_007_182, "traceback": 49_847_008_272}
-# Issued VALU wave-instructions of one extension pass of the tie-break workload (its two
-# affine_pk16_pass_kernel launches, 16-bit packed), from a rocprofv3 --pmc SQ_INSTS_VALU run of this
-# script (profiles/tiebreak_20k_reads_passes.md).
-TB_EXT_VALU_INSTS = 7.1239e8 + 2.5435e8
+# Issued VALU wave-instructions of one extension pass of the tie-break workload (its four
+# affine_pk16_pass_kernel launches, 16-bit packed, 8 lanes a pair), from a rocprofv3 --pmc
+# SQ_INSTS_VALU run of this script (profiles/tiebreak_20k_reads_passes.md).
+TB_EXT_VALU_INSTS = 5.2471e8 + 1.9663e8 + 9.3688e6 + 5.0622e5
 
 
 # Issue-rate probe (sf_probe_issue): waves of 32 independent VALU instructions of one encoding.
@@ -670,7 +670,7 @@ def bench_tiebreak(args, steps=None, warmup=None):
     # around the phase), bound by VALU issue.
     ext_s = tm.ext_ms / steps * 1e-3
     achieved = TB_EXT_VALU_INSTS * 64 / ext_s / 1e12
-    res["roofline"] = {"bound": "valu", "kernel": "affine_pk16_pass_kernel<3|4, true> (the extension pass, 16-bit packed, 16 lanes a pair)",
+    res["roofline"] = {"bound": "valu", "kernel": "affine_pk16_pass_kernel<8, 5..8, true> (the extension pass, 16-bit packed, 8 lanes a pair)",
                        "achieved": round(achieved, 3), "peak": round(VALU_ISSUE_PEAK_T, 1),
                        "unit": "T VALU lane-instructions/s issued (SQ_INSTS_VALU of a pass / pass time)",
                        "frac": round(achieved / VALU_ISSUE_PEAK_T, 4), "traffic": None,

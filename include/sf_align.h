@@ -437,9 +437,10 @@ int sf_set_affine_lp8(int policy);
 /* Tuning (no reference counterpart; results never depend on it): the passes with per-row
  * bookkeeping (sf_affine_dp, sf_batch_affine_dp and the calls built on them) run in the 16-bit
  * packed layout where its range holds -- as the reference runs them in 16 bits where `narrow`
- * holds (align.rs:2381-2426) -- (1, default, or SF_AFFINE_PASS_PACKED from the environment), or
- * in 32 bits throughout (0).  Applies to passes started afterwards in this process. */
-int sf_set_affine_pass_packed(int on);
+ * holds (align.rs:2381-2426) -- on 8 lanes a pair where the band fits 256 slots, else 16
+ * (policy 1, the default, or SF_AFFINE_PASS_PACKED from the environment), on 16 lanes only (2),
+ * or in 32 bits throughout (0).  Applies to passes started afterwards in this process. */
+int sf_set_affine_pass_packed(int policy);
 
 #ifdef __cplusplus
 }

@@ -547,30 +547,31 @@ int sf_affine_pk4_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, 
     return top < 0x7C00 ? 4200 + kp : 0;
 }
 
-int sf_affine_pass_pk_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m,
-                            uint32_t below) {
-    if (mis != 2 * ext) return 0;
-    int k2 = 0; // registers a lane: the instantiated packed pass shapes, 16 lanes x 2 slots each
+int sf_affine_pass_pk_class(uint64_t width, int lanes, int32_t mat, int32_t mis, int32_t open, int32_t ext,
+                            uint32_t m, uint32_t below) {
+    if (mis != 2 * ext || (lanes != 8 && lanes != 16)) return 0;
+    int k2 = 0; // registers a lane: the instantiated packed pass shapes (2 slots a register)
     for (int c : {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16})
-        if (width <= 32u * (uint64_t)c) {
+        if ((lanes == 16 || c >= 2) && width <= 2u * (uint64_t)lanes * c) {
             k2 = c;
             break;
         }
     if (!k2) return 0;
     const int64_t dm = (int64_t)mat + mis;
     const int64_t off = (int64_t)open + (dm + ext) * below + 2;
-    const int64_t top = off + dm * m + (int64_t)ext * 32 * k2;
-    if (top < 0x7C00) return 16200 + 2 * k2;
-    return top <= 65535 ? 16100 + 2 * k2 : 0;
+    const int64_t top = off + dm * m + (int64_t)ext * 2 * lanes * k2;
+    if (top < 0x7C00) return lanes * 1000 + 200 + 2 * k2;
+    return top <= 65535 ? lanes * 1000 + 100 + 2 * k2 : 0;
 }
 
 static std::atomic<int> g_pass_packed{[] {
     const char *e = getenv("SF_AFFINE_PASS_PACKED");
-    return e ? (atoi(e) != 0 ? 1 : 0) : 1;
+    const int v = e ? atoi(e) : 1;
+    return v < 0 ? 0 : (v > 2 ? 2 : v);
 }()};
 
 int sf_affine_pass_packed() { return g_pass_packed.load(std::memory_order_relaxed); }
-void sf_affine_set_pass_packed(int on) { g_pass_packed.store(on ? 1 : 0, std::memory_order_relaxed); }
+void sf_affine_set_pass_packed(int policy) { g_pass_packed.store(policy, std::memory_order_relaxed); }
 
 static std::atomic<int> g_lp8_policy{[] {
     const char *e = getenv("SF_AFFINE_LP8");

@@ -1,43 +1,55 @@
 // affine_pk_pass.hip -- the passes with per-row bookkeeping (affine_dp with a clip, X-drop stop, best
 // cell, cut or band geometry: align.rs:2580-2665 `run_core`) in the 16-bit packed layout of
-// affine_pk.hpp, 16 lanes a pair.  The reference runs these passes in 16 bits where its `narrow`
+// affine_pk.hpp, 8 or 16 lanes a pair.  The reference runs these passes in 16 bits where its `narrow`
 // bound holds (align.rs:2381-2426, `run16_avx512`); here the packed kernel's own bound decides
 // (checked on the host), and the 32-bit pass (affine.hip) takes every other pair.
 //
 // Rows are the packed score kernel's (pk16_row).  After each row:
 //   * its maximum over its band cells.  A slot's key is v + ext (SL - 1 - s) (SL = the pair's 16
-//     x 2 K2 slots): v = H + ext s + mis i + off, so the key is H + mis i + off + ext (SL - 1), one
+//     (or 8) x 2 K2 slots): v = H + ext s + mis i + off, so the key is H + mis i + off + ext (SL - 1), one
 //     offset for the whole row, positive for every band cell (v >= dm * below + 2 there) and within
 //     16 bits under the same bound as v (off + dm m + ext SL).  Slots past the band's width take key
 //     0 (one v_pk_mad: v * 1 + add, or v * 0 + 0), and only rows whose band reaches left of column
 //     0 or past column n (or past the cut, for the cut's maximum) mask further slots.
 //   * the X-drop test, the best row and its first best column, the cut's best, as the 32-bit pass
 //     keeps them (same order of tests, 32-bit arithmetic on the unbiased maximum).
-// A pair stops at its X-drop row: its lanes (one DPP row) are masked off from then on.
+// A pair stops at its X-drop row: its lanes are masked off from then on.  Every cross-lane move
+// stays inside a pair's lanes (a DPP row, or half of one for 8 lanes; the moves that could read the
+// other half take the floor), so a stopped pair beside a running one changes nothing.
 #include "affine_pk.hpp"
 
 namespace {
 
 constexpr int kRowRor1 = 0x121, kRowRor2 = 0x122, kRowRor4 = 0x124, kRowRor8 = 0x128;
+constexpr int kHalfMirror = 0x141, kQuadXor1 = 0xB1, kQuadXor2 = 0x4E; // lane i <-> 7 - i; quad [1,0,3,2]; [2,3,0,1]
 
-// Max / min of one value a lane over the 16 lanes of a DPP row (rotations: every lane ends with it).
-__device__ __forceinline__ uint32_t row16_umax(uint32_t x) {
-    x = umax(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, kRowRor8, 0xf, 0xf, true));
-    x = umax(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, kRowRor4, 0xf, 0xf, true));
-    x = umax(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, kRowRor2, 0xf, 0xf, true));
-    return umax(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, kRowRor1, 0xf, 0xf, true));
+// Max / min of one value a lane over a pair's LP lanes, every lane ending with it: 16 lanes (one
+// DPP row) by rotations; 8 lanes (half a row) by the half-row mirror, then swaps inside quads.
+template <int LP, typename Op>
+__device__ __forceinline__ int32_t group_reduce(int32_t x, Op op) {
+    if constexpr (LP == 16) {
+        x = op(x, __builtin_amdgcn_update_dpp(x, x, kRowRor8, 0xf, 0xf, true));
+        x = op(x, __builtin_amdgcn_update_dpp(x, x, kRowRor4, 0xf, 0xf, true));
+        x = op(x, __builtin_amdgcn_update_dpp(x, x, kRowRor2, 0xf, 0xf, true));
+        return op(x, __builtin_amdgcn_update_dpp(x, x, kRowRor1, 0xf, 0xf, true));
+    } else {
+        static_assert(LP == 8, "8 or 16 lanes a pair");
+        x = op(x, __builtin_amdgcn_update_dpp(x, x, kHalfMirror, 0xf, 0xf, true));
+        x = op(x, __builtin_amdgcn_update_dpp(x, x, kQuadXor1, 0xf, 0xf, true));
+        return op(x, __builtin_amdgcn_update_dpp(x, x, kQuadXor2, 0xf, 0xf, true));
+    }
 }
-__device__ __forceinline__ int32_t row16_imax(int32_t x) {
-    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor8, 0xf, 0xf, true));
-    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor4, 0xf, 0xf, true));
-    x = imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor2, 0xf, 0xf, true));
-    return imax(x, __builtin_amdgcn_update_dpp(x, x, kRowRor1, 0xf, 0xf, true));
+template <int LP>
+__device__ __forceinline__ uint32_t group_umax(uint32_t x) {
+    return (uint32_t)group_reduce<LP>((int32_t)x, [](int32_t a, int32_t b) { return (int32_t)umax((uint32_t)a, (uint32_t)b); });
 }
-__device__ __forceinline__ int32_t row16_imin(int32_t x) {
-    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor8, 0xf, 0xf, true));
-    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor4, 0xf, 0xf, true));
-    x = imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor2, 0xf, 0xf, true));
-    return imin(x, __builtin_amdgcn_update_dpp(x, x, kRowRor1, 0xf, 0xf, true));
+template <int LP>
+__device__ __forceinline__ int32_t group_imax(int32_t x) {
+    return group_reduce<LP>(x, [](int32_t a, int32_t b) { return imax(a, b); });
+}
+template <int LP>
+__device__ __forceinline__ int32_t group_imin(int32_t x) {
+    return group_reduce<LP>(x, [](int32_t a, int32_t b) { return imin(a, b); });
 }
 
 __device__ __forceinline__ u16x2 pmin(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
@@ -52,9 +64,8 @@ __device__ __forceinline__ uint32_t hmin(u16x2 v) {
     return lo < hi ? lo : hi;
 }
 
-template <int K2, bool M3>
+template <int LP, int K2, bool M3>
 __global__ __launch_bounds__(256) void affine_pk16_pass_kernel(PassArgs a) {
-    constexpr int LP = 16;
     constexpr int S = 2 * K2;       // slots a lane
     constexpr int SL = LP * S;      // slots a pair
     constexpr int BLK = 16;         // rows a block: one pattern word
@@ -129,7 +140,7 @@ __global__ __launch_bounds__(256) void affine_pk16_pass_kernel(PassArgs a) {
         u16x2 t = k[0];
 #pragma unroll
         for (int q = 1; q < K2; ++q) t = pmax(t, k[q]);
-        return row16_umax(hmax(t));
+        return group_umax<LP>(hmax(t));
     };
     // The row's bookkeeping (align.rs:2599-2628), after row i of the recurrence.
     auto book = [&](const int32_t i) {
@@ -164,7 +175,7 @@ __global__ __launch_bounds__(256) void affine_pk16_pass_kernel(PassArgs a) {
 #pragma unroll
                     for (int q = 0; q < K2; ++q)
                         cmin = pmin(cmin, pmin(psub(rv, key[q]), one) * big + sidx[q]);
-                    const int32_t sm = row16_imin((int32_t)hmin(cmin));
+                    const int32_t sm = group_imin<LP>((int32_t)hmin(cmin));
                     ci = i;
                     cj = sm + i - below;
                 }
@@ -232,8 +243,8 @@ __global__ __launch_bounds__(256) void affine_pk16_pass_kernel(PassArgs a) {
                 bs = s;
             }
         }
-        const int32_t mx = row16_imax(bv);
-        const int32_t sm = row16_imin(bv == mx ? bs : INT32_MAX);
+        const int32_t mx = group_imax<LP>(bv);
+        const int32_t sm = group_imin<LP>(bv == mx ? bs : INT32_MAX);
         score = mx;
         col = (uint32_t)(sm + (m - below));
     }
@@ -247,7 +258,7 @@ __global__ __launch_bounds__(256) void affine_pk16_pass_kernel(PassArgs a) {
             const int32_t s = sb + wi;
             if (s >= slo && s <= stop_s) lc = imax(lc, slot_h(wi));
         }
-        cut_score = imax(kNeg, row16_imax(lc));
+        cut_score = imax(kNeg, group_imax<LP>(lc));
     }
     if (valid && l == 0) {
         DpOut o;
@@ -273,16 +284,20 @@ __global__ __launch_bounds__(256) void affine_pk16_pass_kernel(PassArgs a) {
 
 hipError_t sf_launch_pass_packed(const PassArgs &a, int cls, hipStream_t s) {
     if (a.count == 0) return hipSuccess;
-    if (aff_lp(cls) != 16 || !aff_packed(cls)) return hipErrorInvalidValue;
+    const int LP = aff_lp(cls);
+    if ((LP != 8 && LP != 16) || !aff_packed(cls)) return hipErrorInvalidValue;
     const bool m3 = cls % 1000 >= 200;
-    dim3 grid((unsigned)(((uint64_t)a.count * 16 + 255) / 256)), block(256);
-    switch (aff_k(cls)) {
-#define PKP(KK)                                                                                 \
-    case 2 * KK:                                                                                \
-        if (m3) hipLaunchKernelGGL((affine_pk16_pass_kernel<KK, true>), grid, block, 0, s, a);  \
-        else hipLaunchKernelGGL((affine_pk16_pass_kernel<KK, false>), grid, block, 0, s, a);    \
+    dim3 grid((unsigned)(((uint64_t)a.count * LP + 255) / 256)), block(256);
+    switch (aff_k(cls) * 100 + LP) {
+#define PKP(LL, KK)                                                                                 \
+    case 2 * KK * 100 + LL:                                                                         \
+        if (m3) hipLaunchKernelGGL((affine_pk16_pass_kernel<LL, KK, true>), grid, block, 0, s, a);  \
+        else hipLaunchKernelGGL((affine_pk16_pass_kernel<LL, KK, false>), grid, block, 0, s, a);    \
         break;
-        PKP(1) PKP(2) PKP(3) PKP(4) PKP(5) PKP(6) PKP(7) PKP(8) PKP(10) PKP(12) PKP(14) PKP(16)
+        PKP(16, 1) PKP(16, 2) PKP(16, 3) PKP(16, 4) PKP(16, 5) PKP(16, 6) PKP(16, 7) PKP(16, 8) PKP(16, 10)
+        PKP(16, 12) PKP(16, 14) PKP(16, 16)
+        PKP(8, 2) PKP(8, 3) PKP(8, 4) PKP(8, 5) PKP(8, 6) PKP(8, 7) PKP(8, 8) PKP(8, 10) PKP(8, 12) PKP(8, 14)
+        PKP(8, 16)
 #undef PKP
     default:
         return hipErrorInvalidValue;

@@ -376,14 +376,15 @@ hipError_t sf_launch_affine(const sf::AffineArgs &a, int K, bool tb, hipStream_t
 uint64_t sf_affine_lp16_max(); // the current 16-lane limit (slots), part of the shape choice
 int sf_affine_class(uint64_t width, bool lp16);      // 16 lanes per pair if allowed and it fits, else 64
 int sf_affine_pass_class(uint64_t width, bool lp16); // the same for the passes with bookkeeping
-// The 16-bit packed pass class (16 lanes; 16100 + slots a lane, 16200 + slots with the three-term
-// max) for a band of `width` slots, or 0 (mis != 2 ext, wider than 512 slots, or past the range).
-int sf_affine_pass_pk_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, int32_t ext, uint32_t m,
-                            uint32_t below);
-// Whether passes with bookkeeping use the packed kernel where its range holds (1, default) or the
-// 32-bit one throughout (0).
+// The 16-bit packed pass class on `lanes` lanes (8 or 16; lanes * 1000 + 100 + slots a lane, + 200
+// with the three-term max) for a band of `width` slots, or 0 (mis != 2 ext, wider than the shape's
+// 32 slots a lane, or past the range).
+int sf_affine_pass_pk_class(uint64_t width, int lanes, int32_t mat, int32_t mis, int32_t open, int32_t ext,
+                            uint32_t m, uint32_t below);
+// Passes with bookkeeping: 0 the 32-bit kernel throughout; 1 (default) the packed kernel where its
+// range holds, on 8 lanes where the band fits 256 slots, else 16; 2 the packed kernel on 16 lanes.
 int sf_affine_pass_packed();
-void sf_affine_set_pass_packed(int on);
+void sf_affine_set_pass_packed(int policy);
 // The 16-bit packed class for a pair of the 32-bit class with `lanes` (16 or 64) lanes and K slots
 // per lane, or 0 where its exact range does not hold (the scoring must have mis = 2 ext); tb: for
 // the traceback kernel.

@@ -1745,9 +1745,9 @@ int sf_batch_use_located(sf_batch *b, int on) {
     return SF_OK;
 }
 
-int sf_set_affine_pass_packed(int on) {
-    if (on != 0 && on != 1) return fail(SF_EINVAL, "on must be 0 or 1");
-    sf_affine_set_pass_packed(on);
+int sf_set_affine_pass_packed(int policy) {
+    if (policy < 0 || policy > 2) return fail(SF_EINVAL, "policy must be 0, 1 or 2");
+    sf_affine_set_pass_packed(policy);
     return SF_OK;
 }
 
@@ -1989,8 +1989,8 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
     if ((st = join_aux(b))) return st;
     const uint64_t n = b->n;
     sf_batch::AffPlan &pl = b->aplan[2];
-    const bool packed = sf_affine_pass_packed() != 0;
-    uint64_t key = (0x51ed270b27f1a3c5ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8) ^ ((uint64_t)packed << 62)) *
+    const int packed = sf_affine_pass_packed();
+    uint64_t key = (0x51ed270b27f1a3c5ull ^ (uint64_t)mode ^ (sf_affine_lp16_max() << 8) ^ ((uint64_t)packed << 61)) *
                    1099511628211ull;
     // (The scoring is in the key: the plan's range check and the packed classes depend on it.)
     for (int32_t v : {sc->mat, sc->mis, sc->open, sc->ext}) key = (key ^ (uint64_t)(uint32_t)v) * 1099511628211ull;
@@ -2006,10 +2006,15 @@ int sf_batch_affine_dp(sf_batch *b, int mode, const sf_scoring *sc, const uint64
             sf::band_geom_for(m, t, geom ? geom[i] : t, mode, band ? band[i] : SF_BAND_EXACT, &below, &above);
             const uint64_t width = below + above + 1;
             kcls[i] = sf_affine_pass_class(width, width <= sf_affine_lp16_max());
-            // The 16-bit packed pass where its range holds (16 lanes, up to 512 slots).
-            if (packed && kcls[i] != 0 && width <= sf_affine_lp16_max())
-                if (const int pc = sf_affine_pass_pk_class(width, sc->mat, sc->mis, sc->open, sc->ext, m, (uint32_t)below))
-                    kcls[i] = pc;
+            // The 16-bit packed pass where its range holds: 8 lanes a pair up to 256 slots (the
+            // per-row reductions shared by twice the slots), 16 up to 512.
+            if (packed && kcls[i] != 0 && width <= sf_affine_lp16_max()) {
+                int pc = 0;
+                for (int lanes : {8, 16})
+                    if (!pc && (lanes == 16 || packed == 1))
+                        pc = sf_affine_pass_pk_class(width, lanes, sc->mat, sc->mis, sc->open, sc->ext, m, (uint32_t)below);
+                if (pc) kcls[i] = pc;
+            }
             if (kcls[i] == 0 && width <= kAffWideMax) kcls[i] = kAffWide;
             if (kcls[i] == 0)
                 return fail(SF_ERANGE, "pair %llu: band of %llu cells per row; this library handles up to 2^26",

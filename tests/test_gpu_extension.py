@@ -229,11 +229,12 @@ def test_local_band_random():
 
 @pytest.fixture
 def pass_policy():
-    """Passes in the 16-bit packed layout where its range holds (the default), or in 32 bits."""
+    """Passes in the 16-bit packed layout where its range holds, on 8 lanes where the band fits
+    256 slots (1, the default) or on 16 lanes (2), or in 32 bits (0)."""
     lib = A.N.load()
 
-    def use(packed):
-        A.N.check(lib.sf_set_affine_pass_packed(1 if packed else 0))
+    def use(policy):
+        A.N.check(lib.sf_set_affine_pass_packed(int(policy)))
 
     yield use
     A.N.check(lib.sf_set_affine_pass_packed(1))
@@ -241,8 +242,9 @@ def pass_policy():
 
 @pytest.mark.parametrize("mode", [A.Mode.Prefix, A.Mode.Global, A.Mode.Infix])
 def test_affine_dp_packed_pass_equals_32bit_and_oracle(mode, pass_policy):
-    # The packed pass (affine_pk_pass.hip) against the 32-bit pass on the same inputs, every Dp
-    # field: bands from 0 to the packed pass's 512 slots (every class, 2 to 32 slots a lane),
+    # The packed pass (affine_pk_pass.hip, 8 lanes a pair where the band fits 256 slots, and 16
+    # lanes) against the 32-bit pass on the same inputs, every Dp field: bands from 0 to the
+    # packed pass's 512 slots (every class, 2 to 32 slots a lane),
     # band layouts for other text lengths, cuts inside, at and past the text, X-drops that stop
     # early or never, the best cell tracked or not; and against the oracle on a sample.
     rng = random.Random(131 + int(mode))
@@ -257,13 +259,16 @@ def test_affine_dp_packed_pass_equals_32bit_and_oracle(mode, pass_policy):
         for clip in ((0, True), (25, True), (50, False), (IMAX, True), None):
             for use_geom, use_cut in ((False, False), (True, True), (False, True)):
                 args = (ps, mode, sc, bands, geoms if use_geom else None, clip, cuts if use_cut else None)
-                pass_policy(True)
-                got = A.affine_dp_batch(*args)
-                pass_policy(False)
+                pass_policy(0)
                 ref = A.affine_dp_batch(*args)
+                pass_policy(2)
+                got16 = A.affine_dp_batch(*args)
+                pass_policy(1)
+                got = A.affine_dp_batch(*args)
                 for i in range(n):
-                    assert tuple(got[i]) == tuple(ref[i]), (i, len(P[i]), len(T[i]), int(bands[i]), sc, clip,
-                                                            use_geom, use_cut)
+                    assert tuple(got[i]) == tuple(ref[i]) == tuple(got16[i]), (i, len(P[i]), len(T[i]),
+                                                                               int(bands[i]), sc, clip,
+                                                                               use_geom, use_cut)
                 for i in range(0, n, 17):
                     want = O.affine_dp(P[i], T[i], int(mode), (sc.mat, sc.mis, sc.open, sc.ext), int(bands[i]),
                                        int(geoms[i]) if use_geom else None, clip,
@@ -287,9 +292,9 @@ def test_affine_dp_packed_pass_range_edges(pass_policy):
         T.append(t)
     ps = A.PairSet.from_seqs(P, T)
     for clip in ((50, True), (IMAX, True)):
-        pass_policy(True)
+        pass_policy(1)
         got = A.affine_dp_batch(ps, A.Mode.Prefix, ONT, 16, None, clip, None)
-        pass_policy(False)
+        pass_policy(0)
         ref = A.affine_dp_batch(ps, A.Mode.Prefix, ONT, 16, None, clip, None)
         for i in range(len(P)):
             assert tuple(got[i]) == tuple(ref[i]), (i, len(P[i]), clip)
