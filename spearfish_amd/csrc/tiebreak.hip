@@ -265,10 +265,18 @@ struct sf_tb_batch {
     TbGroup ext, srt[3];
     sf_tb_params p{};
     bool ran = false;
+    // A batch that runs many times keeps its results' staging between calls: the extension
+    // records in pinned host memory (a copy into pageable memory runs several times slower), the
+    // per-candidate results' buffer allocated once.
+    bool resident = false;
+    sf_dp *pin_dp = nullptr;
+    uint64_t pin_n = 0;
+    std::vector<sf_extend_cut> res;
     ~sf_tb_batch() {
         for (TbGroup *g : {&ext, &srt[0], &srt[1], &srt[2]})
             if (g->b) sf_batch_destroy(g->b);
         if (own_seq) (void)hipFree(own_seq);
+        if (pin_dp) (void)hipHostFree(pin_dp);
     }
 };
 
@@ -409,12 +417,27 @@ int tb_run(sf_tb_batch &w, const sf_tb_params *p) {
 // extension pair with its segment's flags, (x, x, 0) for a sorted score (lib.rs:1442-1445).
 int tb_results(sf_tb_batch &w, sf_extend_cut *out, uint32_t *rep_out) {
     if (!w.ran) return sf_runtime_fail(SF_EINVAL, "no run on this tie-break batch");
-    std::vector<sf_extend_cut> res(w.ncand);
+    std::vector<sf_extend_cut> &res = w.res;
+    res.resize(w.ncand);
     const sf_tb_params &p = w.p;
     if (w.ext.b) {
         const size_t n = w.ext.cand.size();
-        std::vector<sf_dp> d(n);
-        if (const int r = sf_batch_dp_results(w.ext.b, d.data())) return r;
+        std::vector<sf_dp> dv;
+        sf_dp *d = nullptr;
+        if (w.resident && w.pin_n < n) {
+            if (w.pin_dp) (void)hipHostFree(w.pin_dp);
+            w.pin_dp = nullptr;
+            w.pin_n = 0;
+            if (hipHostMalloc((void **)&w.pin_dp, n * sizeof(sf_dp), hipHostMallocDefault) == hipSuccess) w.pin_n = n;
+            else w.pin_dp = nullptr; // (pageable staging below)
+        }
+        if (w.resident && w.pin_n >= n) {
+            d = w.pin_dp;
+        } else {
+            dv.resize(n);
+            d = dv.data();
+        }
+        if (const int r = sf_batch_dp_results(w.ext.b, d)) return r;
         for (size_t i = 0; i < n; ++i) {
             const sf_tb_segment &g = w.segs[w.ext.seg[i]];
             const bool has_cut = g.cut != UINT64_MAX, track = (g.flags & SF_SEG_TRACK_CUT) && has_cut;
@@ -472,6 +495,7 @@ int sf_tb_batch_create(sf_store *st, const uint8_t *reads, uint64_t reads_len, c
         return sf_runtime_fail(SF_EINVAL, "null argument");
     if (reads_len >= (1ull << 40)) return sf_runtime_fail(SF_ERANGE, "2^40 read bases or more");
     auto *w = new sf_tb_batch();
+    w->resident = true;
     w->st = st;
     // Its own buffer: the store's two parts copied (device to device) and room for the reads.
     const uint64_t words = 4 * st->nwords + 2 * ((reads_len + 15) / 16) + 1;
