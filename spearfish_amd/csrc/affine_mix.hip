@@ -1,6 +1,6 @@
-// affine_mix.hip -- one launch for several shape classes of a score-only run: the 64-lane classes
-// and the widest 16-bit packed 8- and 16-lane ones (two waves a SIMD); affine_mix1.hip holds the
-// narrower 8- and 16-lane classes.
+// affine_mix.hip -- one launch for several shape classes of a score-only run: the 64-lane classes,
+// the widest 16-bit packed 8- and 16-lane ones and the 4-lane ones (two waves a SIMD);
+// affine_mix1.hip holds the narrower 8- and 16-lane classes.
 //
 // Pairs whose rows are long run one pair a wave, and a class holding only a few of them has too few
 // waves to fill the chip: its time is its longest pair's chain of rows.  Launched one class after
@@ -30,13 +30,14 @@ bool sf_affine_mixable(int cls) {
 
 // Register groups of the score classes that share a launch: 1 for the packed 8- and 16-lane
 // classes of at most 13 registers of slot pairs a lane (at most 168 VGPRs: three waves a SIMD;
-// affine_mix1.hip), 2 for the wider ones and the 64-lane classes (affine_mix_kernel, two waves),
-// 0 for classes that launch alone (the 32-bit 16-lane kernels).
+// affine_mix1.hip), 2 for the wider ones, the 4-lane ones and the 64-lane classes
+// (affine_mix_kernel, two waves), 0 for classes that launch alone (the 32-bit 16-lane kernels).
 int sf_affine_mix_group(int cls) {
     if (!aff_packed(cls)) return sf_affine_mixable(cls) ? 2 : 0;
     const int lp = aff_lp(cls), kp = aff_k(cls), k2 = kp / 2;
     const bool m3 = cls % 1000 >= 200;
     if (kp % 2 || k2 < 1) return 0;
+    if (lp == 4) return m3 ? 2 : 0;
     if (lp == 8) return !m3 ? 0 : (k2 <= 13 ? 1 : (k2 <= 18 ? 2 : 0));
     if (lp == 16) return k2 <= 13 ? 1 : (k2 <= 16 ? 2 : 0);
     return sf_affine_mixable(cls) ? 2 : 0;

@@ -47,7 +47,8 @@ __device__ __forceinline__ void myers_mix_block(const MyersArgs &a, const MixArg
 #undef P
 }
 
-// Score classes of affine_mix_kernel: the 64-lane classes and the widest 8- and 16-lane packed ones.
+// Score classes of affine_mix_kernel: the 64-lane classes, the widest 8- and 16-lane packed ones and
+// the 4-lane ones.
 __device__ __forceinline__ void affine_mix_block(const AffineArgs &a, const MixArgs &mx, uint32_t b) {
     uint32_t s = 0;
     while (s + 1 < mx.nseg && b >= mx.blk0[s + 1]) ++s;
@@ -75,6 +76,10 @@ __device__ __forceinline__ void affine_mix_block(const AffineArgs &a, const MixA
         P8(14) P8(15) P8(16) P8(17) P8(18) P16(14) P16(15) P16(16)
 #undef P8
 #undef P16
+#define P4(KK)                                                                                   \
+    case 4200 + 2 * KK: affine_pk16_body<4, KK, true>(x, blk); break;
+        P4(18) P4(20) P4(24) P4(28) P4(32) P4(34) P4(36)
+#undef P4
     default:
         break; // the host launches only the classes above (sf_affine_mix_group)
     }
