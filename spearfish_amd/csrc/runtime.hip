@@ -1602,10 +1602,13 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         if (cb < bin.end) chunks.push_back({bin.cls, cb, bin.end});
         need_max = std::max(need_max, used);
     }
-    // The last chunk's walk runs after every DP has ended.  So the largest bin's last chunk gives
-    // up about a third of itself (its shortest patterns, whose walks are the shortest too) as a
-    // piece that runs last: the walk of the rest -- about a third of its DP time -- then overlaps
-    // the piece's DP, and only the piece's short walk is left after the last DP.
+    // The last chunk's walk runs after every DP has ended, and a walk's time is set mostly by its
+    // longest pattern (one dependent chain a pair), not by its chunk's DP time.  So the largest
+    // bin's chunks run last, after the smaller bins (wider bands or longer patterns, whose walks
+    // then overlap the large bin's DP), and its last chunk gives up about half of itself (its
+    // shortest patterns, whose walks are the shortest too) as a piece that runs last: the walk
+    // of the rest then overlaps the piece's DP, and only the piece's short walk is left after
+    // the last DP.
     if (!chunks.empty()) {
         auto words = [&](int cls, uint32_t q) {
             return (((uint64_t)b->plen[pl.order[q]] + 15) & ~15ull) * aff_tb_words_per_row(cls);
@@ -1616,16 +1619,15 @@ int sf_batch_affine(sf_batch *b, int mode, const sf_scoring *sc, const uint64_t 
         int big = chunks[0].K;
         for (auto &e : bin_words)
             if (e.second > bin_words[big]) big = e.first;
-        size_t li = 0;
-        for (size_t c = 0; c < chunks.size(); ++c)
-            if (chunks[c].K == big) li = c;
+        std::stable_partition(chunks.begin(), chunks.end(), [&](const Chunk &c) { return c.K != big; });
+        const size_t li = chunks.size() - 1;
         const Chunk last = chunks[li];
         uint64_t lw = 0;
         for (uint32_t q = last.begin; q < last.end; ++q) lw += words(last.K, q);
         if (last.end - last.begin >= 2 && lw > budget / 4) {
             uint32_t split = last.end;
             uint64_t tail = 0;
-            while (split > last.begin + 1 && tail < lw / 3) tail += words(last.K, --split);
+            while (split > last.begin + 1 && tail < lw / 2) tail += words(last.K, --split);
             uint64_t u = 0;
             for (uint32_t q = split; q < last.end; ++q) {
                 tboff[q] = u;
