@@ -320,6 +320,9 @@ __global__ __launch_bounds__(256) void affine_pass_kernel(PassArgs a) {
 constexpr int kWalkRows = 16;
 
 __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
+    // The walk's waves share SIMDs with the next chunk's DP waves, which issue every cycle they
+    // can; raised priority lets the walk's one dependent chain issue first.
+    __builtin_amdgcn_s_setprio(3);
     __shared__ uint32_t cache[256 * (kWalkRows + 1)];
     uint32_t *my = cache + threadIdx.x * (kWalkRows + 1);
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
@@ -367,7 +370,8 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
                 break;
             }
             const int32_t wd = q >> 3, b = (i - 1) >> 4;
-            if (L != cL || wd != cwd || b != cb) {
+            const bool line = L != cL || wd != cwd || b != cb;
+            if (line) {
                 const uint4 *line = reinterpret_cast<const uint4 *>(tb + ((uint64_t)(b * nwl + wd) * LP + L) * 16);
                 uint4 v[4];
 #pragma unroll
@@ -388,8 +392,11 @@ __global__ __launch_bounds__(256) void walk_kernel(WalkArgs a) {
                     pwi = (i - 1) >> 4;
                     pwv = codes16(P, pwi * 16);
                 }
+                // Re-centred on the cell's word whenever a line is loaded too (same round): the
+                // next 16 diagonal steps then stay inside the two words, and a text refill does
+                // not cost a round of its own between two line refills.
                 const int32_t tw = (j - 1) >> 4;
-                if (tw != twi && tw != twi - 1) {
+                if (tw != twi && (tw != twi - 1 || line)) {
                     twi = tw;
                     tw0 = codes16(T, tw * 16);
                     tw1 = tw > 0 ? codes16(T, (tw - 1) * 16) : 0u;
