@@ -57,12 +57,12 @@ AFFINE_TB_OPS_PER_CELL = 11 + 4
 # Issued VALU wave-instructions per DP cell on the uniform workload, from rocprofv3 --pmc
 # SQ_INSTS_VALU runs of this script (profiles/valu_issue_uniform_1e6x1000_band128.md): the
 # score step's affine phase (its 4-lane packed bins and the LP64 bin) and the traceback DP.
-VALU_INSTS_PER_CELL = {"score": 1.63724e10 / 2.46731e11, "traceback": 5.4447e10 / 2.46731e11}
+VALU_INSTS_PER_CELL = {"score": 1.62788e10 / 2.46731e11, "traceback": 5.4447e10 / 2.46731e11}
 # HBM bytes per launch of the dominant kernels on the uniform workload, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this script, scaled by the same counters on a 1 GiB copy
 # (FETCH x2.0, WRITE x1.0); profiles/affine_uniform_1e6x1000_band128_traffic.md.  Score: the
 # step's affine phase; traceback: the average of the DP's four launches a step.
-TRAFFIC_BYTES = {"score": 728_007_182, "traceback": 49_847_008_272}
+TRAFFIC_BYTES = {"score": 729_798_181, "traceback": 49_847_008_272}
 # Issued VALU wave-instructions of one extension pass of the tie-break workload (its four
 # affine_pk16_pass_kernel launches, 16-bit packed, 8 lanes a pair), from a rocprofv3 --pmc
 # SQ_INSTS_VALU run of this script (profiles/tiebreak_20k_reads_passes.md).
@@ -294,8 +294,9 @@ def bench_uniform(args):
         "affine_gcups": round(tm.affine_cells / (tm.affine_ms * 1e-3) / 1e9, 1) if tm.affine_ms else None,
         "kernel_ms_per_step": {"myers": round(tm.myers_ms / args.steps, 2),
                                "affine": round(tm.affine_ms / args.steps, 2)},
-        "roofline": roofline(tm, "score", "affine_pk16_kernel<4,34|36,true> (4 lanes x 68 or 72 slots a "
-                             "pair, 16 pairs a wave; the score step's affine phase)", AFFINE_OPS_PER_CELL, 1),
+        "roofline": roofline(tm, "score", "affine_mix_kernel: affine_pk16_kernel<4,33..36,true> bodies (4 "
+                             "lanes x 66, 68, 70 or 72 slots a pair, 16 pairs a wave) and the 64-lane bin in one "
+                             "launch; the score step's affine phase", AFFINE_OPS_PER_CELL, 1),
         "issue_probe_t": probe,
     }
     wall_c, tm_c = run(True, args.steps, 1, "with cigar")

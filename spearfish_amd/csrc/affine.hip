@@ -542,7 +542,7 @@ int sf_affine_pk4_class(uint64_t width, int32_t mat, int32_t mis, int32_t open, 
                         uint32_t below) {
     if (mis != 2 * ext) return 0;
     int kp = 0; // slots a lane: the instantiated 4-lane shapes
-    for (int c : {36, 40, 48, 56, 64, 68, 72})
+    for (int c : {36, 40, 48, 56, 64, 66, 68, 70, 72})
         if (width <= 4u * (uint64_t)c) {
             kp = c;
             break;

@@ -11,7 +11,7 @@ hipError_t sf_launch_affine_packed4(const AffineArgs &a, int cls, bool tb, hipSt
     switch (aff_k(cls)) {
 #define PK4(KK) \
     case 2 * KK: hipLaunchKernelGGL((affine_pk16_kernel<4, KK, true>), grid, block, 0, s, a); break;
-        PK4(18) PK4(20) PK4(24) PK4(28) PK4(32) PK4(34) PK4(36)
+        PK4(18) PK4(20) PK4(24) PK4(28) PK4(32) PK4(33) PK4(34) PK4(35) PK4(36)
 #undef PK4
     default:
         return hipErrorInvalidValue;

@@ -78,7 +78,7 @@ __device__ __forceinline__ void affine_mix_block(const AffineArgs &a, const MixA
 #undef P16
 #define P4(KK)                                                                                   \
     case 4200 + 2 * KK: affine_pk16_body<4, KK, true>(x, blk); break;
-        P4(18) P4(20) P4(24) P4(28) P4(32) P4(34) P4(36)
+        P4(18) P4(20) P4(24) P4(28) P4(32) P4(33) P4(34) P4(35) P4(36)
 #undef P4
     default:
         break; // the host launches only the classes above (sf_affine_mix_group)

@@ -267,7 +267,7 @@ def test_affine_scores_8_lane_kernel(mode, lanes8):
 @pytest.fixture
 def lanes4():
     """Score-only packed runs on 4 lanes per pair wherever that kernel's range holds (bands of 1
-    to 288 slots: 36 to 72 slots a lane), then the planner's choice back."""
+    to 288 slots: 36 to 72 slots a lane, 66 and 70 among them), then the planner's choice back."""
     lib = A.N.load()
     A.N.check(lib.sf_set_affine_lp8(2))
     yield
@@ -290,7 +290,9 @@ def test_affine_scores_4_lane_kernel(mode, lanes4):
             p = p[:k] + [rng.randrange(4) for _ in range(rng.randrange(0, 9))] + p[k + rng.randrange(0, 9):]
         P.append(p)
         T.append(t)
-        B.append(rng.choice([0, 1, 17, 40, 64, 71, 72, 80, 100, 111, 112, 127, 128, 135, 136, 143]))
+        # (band 128-131: 66 slots a lane; 132-135: 68; 136-139: 70; 140-143: 72, for m close to n)
+        B.append(rng.choice([0, 1, 17, 40, 64, 71, 72, 80, 100, 111, 112, 127, 128, 130, 131, 132, 135, 136,
+                             138, 139, 140, 143]))
     if mode != A.Mode.Infix:
         for m in (5150, 5250, 5350):
             t = [rng.randrange(4) for _ in range(m)]
